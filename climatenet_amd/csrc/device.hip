@@ -1,0 +1,542 @@
+// gfx950 kernels of the projective-dynamics step with dry frictional contact.
+//
+// Per time step:   step_begin -> detect (count, scan, write) -> K x [local ->
+//                  forward levels -> backward levels] -> finish.
+// Predicates (contact tests) are evaluated without fused multiply-adds
+// (-ffp-contract=off) and with correctly rounded sqrt/div so that contact sets
+// match the CPU oracle bit for bit; the dense substitution products use
+// explicit fma.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace pdfc {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 ld3(const double* v, int n, int i) { return {v[i], v[n + i], v[2 * n + i]}; }
+__device__ __forceinline__ void st3(double* v, int n, int i, D3 a) {
+    v[i] = a.x;
+    v[n + i] = a.y;
+    v[2 * n + i] = a.z;
+}
+__device__ __forceinline__ double dot3(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ D3 cross3(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ D3 normalized3(D3 a) {
+    double z = dot3(a, a);
+    if (z > 0.) {
+        double s = sqrt(z);
+        return {a.x / s, a.y / s, a.z / s};
+    }
+    return a;
+}
+
+// computeBasis, src/physics/SimulationFrictionEstimation.cpp:89-105
+__device__ void contact_basis(D3 nrm, double* B) {
+    D3 t = cross3(nrm, D3{1., 0., 0.});
+    if (sqrt(dot3(t, t)) < 1.e-15) t = cross3(nrm, D3{0., 1., 0.});
+    t = normalized3(t);
+    D3 b = normalized3(cross3(nrm, t));
+    B[0] = nrm.x; B[1] = nrm.y; B[2] = nrm.z;
+    B[3] = t.x;   B[4] = t.y;   B[5] = t.z;
+    B[6] = b.x;   B[7] = b.y;   B[8] = b.z;
+}
+
+// Block-wide accumulation of (sum, max) into per-iteration slots.
+__device__ void block_ac_accumulate(double s, double m, double* sum_slot, unsigned long long* max_slot) {
+    __shared__ double red_s[kBlock / 64], red_m[kBlock / 64];
+    s = wave_sum(s);
+    m = wave_max(m);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { red_s[w] = s; red_m[w] = m; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0., tm = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { ts += red_s[k]; tm = fmax(tm, red_m[k]); }
+        if (ts != 0.) atomicAdd(sum_slot, ts);
+        if (tm > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(tm));
+    }
+}
+
+// Alart-Curnier norm, src/physics/coulomb_util.cpp:101-152.
+__device__ double ac_norm(D3 u, const double* r, double mu) {
+    double an = fmax(0., r[0] - u.x) - r[0];
+    double d1 = r[1] - u.y, d2 = r[2] - u.z;
+    double dn = sqrt(d1 * d1 + d2 * d2);
+    double s = fmin(1., fmax(0., mu * r[0] / dn));
+    double a1 = d1 * s - r[1], a2 = d2 * s - r[2];
+    return sqrt((an * an + a1 * a1) + a2 * a2);
+}
+
+__global__ void k_fill(double* p, size_t count, double value) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+__global__ void k_fill_u64(unsigned long long* p, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0ull;
+}
+__global__ void k_copy(double* dst, const double* src, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[i];
+}
+__global__ void k_axpy(double* x, const double* dx, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) x[i] += dx[i];
+}
+
+// t_n = v_n + dt M^-1 (M g); v^0 = t_n; c = M t_n - dt A^T A q_n
+// (SimulationSpeed::initializeStep, SimulationSpeed.cpp:62-76; constant part of
+// SimulationSpeed::computeRhs, :263-277, hoisted out of the iteration loop).
+__global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const double* __restrict__ vstate,
+                             double* __restrict__ v0, double* __restrict__ cvec, StepArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    if (i >= n) return;
+    double m = M.mass[i], im = M.invmass[i];
+    D3 v = ld3(vstate, n, i);
+    D3 t = {v.x + a.dt * (im * (m * a.g[0])), v.y + a.dt * (im * (m * a.g[1])), v.z + a.dt * (im * (m * a.g[2]))};
+    st3(v0, n, i, t);
+    D3 s = {0., 0., 0.};
+    int cnt = M.ata_cnt[i];
+    for (int k = 0; k < cnt; ++k) {
+        int j = M.ata_idx[(size_t)k * n + i];
+        double w = M.ata_val[(size_t)k * n + i];
+        s.x += w * qn[j];
+        s.y += w * qn[n + j];
+        s.z += w * qn[2 * n + j];
+    }
+    st3(cvec, n, i, D3{m * t.x - a.dt * s.x, m * t.y - a.dt * s.y, m * t.z - a.dt * s.z});
+}
+
+// Obstacle detection at q_n: StaticSphere/Sphere::isInside (Sphere.cpp:93-113),
+// StaticPlane (StaticPlane.cpp:57-73). One bit per obstacle.
+__global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObstacles ob, int* cflags, int* ccnt,
+                               int* blocksum) {
+    __shared__ int red[kBlock / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    int cnt = 0;
+    if (i < n) {
+        D3 x = ld3(qn, n, i);
+        int flags = 0;
+        for (int o = 0; o < ob.count; ++o) {
+            bool hit;
+            if (ob.type[o] == 0) {
+                D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
+                hit = sqrt(dot3(r, r)) < ob.radius[o] + 1e-2;
+            } else {
+                D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
+                hit = dot3(r, D3{ob.nrm[o][0], ob.nrm[o][1], ob.nrm[o][2]}) < 1.5e-2;
+            }
+            if (hit) flags |= 1 << o;
+        }
+        cnt = __popc(flags);
+        cflags[i] = flags;
+        ccnt[i] = cnt;
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+        blocksum[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the per-block counts (one block); total -> *total.
+__global__ void k_scan_blocks(int* blocksum, int nblocks, int* total) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblocks; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < nblocks ? blocksum[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks) blocksum[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObstacles ob, const int* cflags,
+                               const int* ccnt, const int* blockoff, int* cstart, DevContacts C) {
+    __shared__ int buf[kBlock];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    int v = i < n ? ccnt[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+        __syncthreads();
+        buf[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i >= n) return;
+    int start = blockoff[blockIdx.x] + buf[threadIdx.x] - v;
+    cstart[i] = start;
+    int flags = cflags[i];
+    if (!flags) return;
+    D3 x = ld3(qn, n, i);
+    int slot = start;
+    for (int o = 0; o < ob.count; ++o) {
+        if (!(flags & (1 << o))) continue;
+        D3 nrm, pt;
+        if (ob.type[o] == 0) {
+            D3 r = normalized3(D3{x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]});
+            double R = ob.radius[o];
+            pt = {ob.p[o][0] + R * r.x, ob.p[o][1] + R * r.y, ob.p[o][2] + R * r.z};
+            nrm = r;
+        } else {
+            nrm = {ob.nrm[o][0], ob.nrm[o][1], ob.nrm[o][2]};
+            D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
+            double d = dot3(r, nrm);
+            pt = {x.x - nrm.x * d, x.y - nrm.y * d, x.z - nrm.z * d};
+        }
+        C.vertex[slot] = i;
+        C.obstacle[slot] = o;
+        C.active[slot] = 0;
+        C.normal[3 * slot] = nrm.x; C.normal[3 * slot + 1] = nrm.y; C.normal[3 * slot + 2] = nrm.z;
+        C.point[3 * slot] = pt.x;   C.point[3 * slot + 1] = pt.y;   C.point[3 * slot + 2] = pt.z;
+        C.mu[slot] = ob.mu[o];
+        contact_basis(nrm, &C.basis[9 * slot]);
+        C.r[3 * slot] = 0.; C.r[3 * slot + 1] = 0.; C.r[3 * slot + 2] = 0.;
+        ++slot;
+    }
+}
+
+// Local step + right-hand side + obstacle friction, one thread per vertex:
+//   q = q_n + dt v;  s_i = sum_springs w^2 L0 (q_i - q_j)/|q_i - q_j|  (A^T p; bending
+//   projections vanish because the reference's rest Laplacian norm is 0);
+//   b = c + dt (s - D v)                        (SimulationSpeed::computeRhs)
+//   F = b - dt^2 (A^T A v)_i at contact vertices, r = cone(B^T F), b += B r
+//                                               (SimulationFrictionEstimation.cpp:337-423)
+// The friction residual of the previous iteration (r^{k-1}, v^k) is folded in.
+__global__ void k_local(DevModel M, const double* __restrict__ qn, const double* __restrict__ vcur,
+                        const double* __restrict__ cvec, double* __restrict__ rhs, const int* __restrict__ ccnt,
+                        const int* __restrict__ cstart, DevContacts C, StepArgs a, int iteration) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    double ac_s = 0., ac_m = 0.;
+    if (i < n) {
+        const double dt = a.dt;
+        D3 v = ld3(vcur, n, i);
+        D3 q = {qn[i] + dt * v.x, qn[n + i] + dt * v.y, qn[2 * n + i] + dt * v.z};
+        D3 s = {0., 0., 0.};
+        int deg = M.sp_cnt[i];
+        for (int k = 0; k < deg; ++k) {
+            int j = M.sp_idx[(size_t)k * n + i];
+            double kap = M.sp_kappa[(size_t)k * n + i];
+            D3 qj = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
+            D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
+            s.x += kap * e.x;
+            s.y += kap * e.y;
+            s.z += kap * e.z;
+        }
+        int nc = ccnt[i];
+        double damp = nc > 0 ? 0. : a.damping;
+        D3 c = ld3(cvec, n, i);
+        D3 b = {c.x + dt * (s.x - damp * v.x), c.y + dt * (s.y - damp * v.y), c.z + dt * (s.z - damp * v.z)};
+        if (nc > 0) {
+            D3 av = {0., 0., 0.};
+            int cnt = M.ata_cnt[i];
+            for (int k = 0; k < cnt; ++k) {
+                int j = M.ata_idx[(size_t)k * n + i];
+                double w = M.ata_val[(size_t)k * n + i];
+                av.x += w * vcur[j];
+                av.y += w * vcur[n + j];
+                av.z += w * vcur[2 * n + j];
+            }
+            const double dt2 = dt * dt;
+            D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
+            int c0 = cstart[i];
+            for (int k = 0; k < nc; ++k) {
+                int cid = c0 + k;
+                const double* B = &C.basis[9 * cid];
+                double* r = &C.r[3 * cid];
+                D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+                double mu = C.mu[cid];
+                if (a.record && iteration > 0) {
+                    double an = ac_norm(D3{dot3(bn, v), dot3(bt, v), dot3(bb, v)}, r, mu);
+                    ac_s += an;
+                    ac_m = fmax(ac_m, an);
+                }
+                D3 fl = {dot3(bn, F), dot3(bt, F), dot3(bb, F)};
+                if (fl.x > 0.) {
+                    r[0] = 0.; r[1] = 0.; r[2] = 0.;
+                    C.active[cid] = 0;
+                    continue;
+                }
+                double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
+                double rt = sqrt(r1 * r1 + r2 * r2);
+                if (rt > mu * r0) {
+                    r1 *= mu * r0 / rt;
+                    r2 *= mu * r0 / rt;
+                }
+                r[0] = r0; r[1] = r1; r[2] = r2;
+                C.active[cid] = 1;
+                b.x += (bn.x * r0 + bt.x * r1) + bb.x * r2;
+                b.y += (bn.y * r0 + bt.y * r1) + bb.y * r2;
+                b.z += (bn.z * r0 + bt.z * r1) + bb.z * r2;
+            }
+        }
+        st3(rhs, n, i, b);
+    }
+    if (a.record && iteration > 0) block_ac_accumulate(ac_s, ac_m, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+}
+
+// q_{n+1} = q_n + dt v^K, v_{n+1} = v^K (SimulationFrictionEstimation::updateScene,
+// :64-87) and the friction residual of the last iteration.
+__global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __restrict__ vlast,
+                         double* __restrict__ vstate, const int* __restrict__ ccnt, const int* __restrict__ cstart,
+                         DevContacts C, StepArgs a, int iteration) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    double ac_s = 0., ac_m = 0.;
+    if (i < n) {
+        D3 v = ld3(vlast, n, i);
+        if (a.record) {
+            int nc = ccnt[i], c0 = cstart[i];
+            for (int k = 0; k < nc; ++k) {
+                const double* B = &C.basis[9 * (c0 + k)];
+                D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+                double an = ac_norm(D3{dot3(bn, v), dot3(bt, v), dot3(bb, v)}, &C.r[3 * (c0 + k)], C.mu[c0 + k]);
+                ac_s += an;
+                ac_m = fmax(ac_m, an);
+            }
+        }
+        qn[i] += a.dt * v.x;
+        qn[n + i] += a.dt * v.y;
+        qn[2 * n + i] += a.dt * v.z;
+        st3(vstate, n, i, v);
+    }
+    if (a.record) block_ac_accumulate(ac_s, ac_m, &a.ac_sum[iteration], &a.ac_max[iteration]);
+}
+
+template <typename T>
+__device__ __forceinline__ double as_d(T x) { return (double)x; }
+
+// Forward substitution, one task = (supernode s, 64 rows of F_s):
+//   b~_s = b_s - sum_children u_c|own ;  [y_s ; u_s] = F_s b~_s (+ children's u_c|below)
+template <typename T>
+__global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const double* __restrict__ Bv,
+                                                double* __restrict__ Y) {
+    extern __shared__ double lds[];
+    const int task = t0 + blockIdx.x;
+    const int s = Fa.task_s[task], r0 = Fa.task_r0[task];
+    const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
+    const int n = Fa.n;
+    double* bx = lds;
+    double* by = lds + ns;
+    double* bz = lds + 2 * ns;
+    const int lane = threadIdx.x;
+    for (int t = lane; t < ns; t += 64) {
+        bx[t] = Bv[c0 + t];
+        by[t] = Bv[n + c0 + t];
+        bz[t] = Bv[2 * n + c0 + t];
+    }
+    double ux = 0., uy = 0., uz = 0.;
+    const int my_row = r0 + lane;
+    __syncthreads();
+    for (int64_t q = Fa.choff[s]; q < Fa.choff[s + 1]; ++q) {
+        const int ch = Fa.children[q];
+        const int nbc = Fa.nb[ch];
+        const int* map = Fa.relmap + Fa.uoff[ch];
+        const double* Uc = Fa.U + 3 * Fa.uoff[ch];
+        for (int t = lane; t < nbc; t += 64) {
+            int idx = map[t];
+            if (idx < ns) {
+                bx[idx] -= Uc[3 * t];
+                by[idx] -= Uc[3 * t + 1];
+                bz[idx] -= Uc[3 * t + 2];
+            }
+        }
+        __syncthreads();
+        // below-row contributions for this task's own rows (each row appears at most once per child)
+        if (my_row >= ns && my_row < f && my_row < r0 + 64) {
+            // binary search of my_row in the child's (ascending) map
+            int lo = 0, hi = nbc;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (map[mid] < my_row) lo = mid + 1; else hi = mid;
+            }
+            if (lo < nbc && map[lo] == my_row) {
+                ux += Uc[3 * lo];
+                uy += Uc[3 * lo + 1];
+                uz += Uc[3 * lo + 2];
+            }
+        }
+    }
+    if (my_row >= f) return;
+    const T* Fs = static_cast<const T*>(Fa.F) + Fa.foff[s] + my_row;
+    double ax = 0., ay = 0., az = 0.;
+#pragma unroll 4
+    for (int j = 0; j < ns; ++j) {
+        double fv = as_d(Fs[(size_t)j * f]);
+        ax = fma(fv, bx[j], ax);
+        ay = fma(fv, by[j], ay);
+        az = fma(fv, bz[j], az);
+    }
+    if (my_row < ns) {
+        Y[c0 + my_row] = ax;
+        Y[n + c0 + my_row] = ay;
+        Y[2 * n + c0 + my_row] = az;
+    } else {
+        double* u = Fa.U + 3 * (Fa.uoff[s] + my_row - ns);
+        u[0] = ax + ux;
+        u[1] = ay + uy;
+        u[2] = az + uz;
+    }
+}
+
+// Backward substitution, one task = (supernode s, 64 columns):
+//   x_s = F_s^T [ y_s ; -x_R ]
+template <typename T>
+__global__ void __launch_bounds__(64) k_backward(DevFactor Fa, int t0, const double* __restrict__ Y,
+                                                 double* __restrict__ X) {
+    extern __shared__ double lds[];
+    const int task = t0 + blockIdx.x;
+    const int s = Fa.task_s[task], j0 = Fa.task_r0[task];
+    const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
+    const int n = Fa.n;
+    double* gx = lds;
+    double* gy = lds + f;
+    double* gz = lds + 2 * f;
+    const int lane = threadIdx.x;
+    for (int t = lane; t < ns; t += 64) {
+        gx[t] = Y[c0 + t];
+        gy[t] = Y[n + c0 + t];
+        gz[t] = Y[2 * n + c0 + t];
+    }
+    const int* R = Fa.rows + Fa.roff[s];
+    for (int t = lane; t < nb; t += 64) {
+        int r = R[t];
+        gx[ns + t] = -X[r];
+        gy[ns + t] = -X[n + r];
+        gz[ns + t] = -X[2 * n + r];
+    }
+    __syncthreads();
+    const int j = j0 + lane;
+    if (j >= ns) return;
+    const T* Ft = static_cast<const T*>(Fa.FT) + Fa.foff[s] + j;
+    double ax = 0., ay = 0., az = 0.;
+#pragma unroll 4
+    for (int i = 0; i < f; ++i) {
+        double fv = as_d(Ft[(size_t)i * ns]);
+        ax = fma(fv, gx[i], ax);
+        ay = fma(fv, gy[i], ay);
+        az = fma(fv, gz[i], az);
+    }
+    X[c0 + j] = ax;
+    X[n + c0 + j] = ay;
+    X[2 * n + c0 + j] = az;
+}
+
+// r = b - (M x + dt2 A^T A x)
+__global__ void k_residual(DevModel M, double dt2, const double* __restrict__ b, const double* __restrict__ x,
+                           double* __restrict__ r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = M.n;
+    if (i >= n) return;
+    D3 s = {0., 0., 0.};
+    int cnt = M.ata_cnt[i];
+    for (int k = 0; k < cnt; ++k) {
+        int j = M.ata_idx[(size_t)k * n + i];
+        double w = M.ata_val[(size_t)k * n + i];
+        s.x += w * x[j];
+        s.y += w * x[n + j];
+        s.z += w * x[2 * n + j];
+    }
+    double m = M.mass[i];
+    r[i] = b[i] - (m * x[i] + dt2 * s.x);
+    r[n + i] = b[n + i] - (m * x[n + i] + dt2 * s.y);
+    r[2 * n + i] = b[2 * n + i] - (m * x[2 * n + i] + dt2 * s.z);
+}
+
+inline unsigned blocks_for(size_t count, int bs = kBlock) { return (unsigned)((count + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_fill(double* p, size_t count, double value, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_fill, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, count, value);
+}
+void launch_fill_u64(unsigned long long* p, size_t count, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, count);
+}
+void launch_copy(double* dst, const double* src, size_t count, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_copy, dim3(blocks_for(count)), dim3(kBlock), 0, st, dst, src, count);
+}
+void launch_axpy(double* x, const double* dx, size_t count, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_axpy, dim3(blocks_for(count)), dim3(kBlock), 0, st, x, dx, count);
+}
+void launch_step_begin(const DevModel& M, const double* qn, const double* vstate, double* v0, double* cvec,
+                       const StepArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_begin, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vstate, v0, cvec, a);
+}
+void launch_detect(const DevModel& M, const double* qn, const DevObstacles& ob, int* cflags, int* ccnt, int* cstart,
+                   int* blocksum, int* n_contacts, DevContacts C, hipStream_t st) {
+    unsigned nb = blocks_for(M.n);
+    hipLaunchKernelGGL(k_detect_count, dim3(nb), dim3(kBlock), 0, st, M, qn, ob, cflags, ccnt, blocksum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blocksum, (int)nb, n_contacts);
+    hipLaunchKernelGGL(k_detect_write, dim3(nb), dim3(kBlock), 0, st, M, qn, ob, cflags, ccnt, blocksum, cstart, C);
+}
+void launch_local(const DevModel& M, const double* qn, const double* vcur, const double* cvec, double* rhs,
+                  const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st) {
+    hipLaunchKernelGGL(k_local, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
+                       iteration);
+}
+void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
+                   const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vlast, vstate, ccnt, cstart, C, a,
+                       iteration);
+}
+void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y, int max_ns,
+                          hipStream_t st) {
+    if (ntasks <= 0) return;
+    size_t lds = (size_t)3 * max_ns * sizeof(double);
+    if (fp32)
+        hipLaunchKernelGGL(k_forward<float>, dim3(ntasks), dim3(64), lds, st, F, t0, B, Y);
+    else
+        hipLaunchKernelGGL(k_forward<double>, dim3(ntasks), dim3(64), lds, st, F, t0, B, Y);
+}
+void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
+                           int max_front, hipStream_t st) {
+    if (ntasks <= 0) return;
+    size_t lds = (size_t)3 * max_front * sizeof(double);
+    if (fp32)
+        hipLaunchKernelGGL(k_backward<float>, dim3(ntasks), dim3(64), lds, st, F, t0, Y, X);
+    else
+        hipLaunchKernelGGL(k_backward<double>, dim3(ntasks), dim3(64), lds, st, F, t0, Y, X);
+}
+void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st) {
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, dt2, b, x, r);
+}
+
+}  // namespace pdfc

@@ -1,0 +1,295 @@
+// Fill-reducing ordering and supernodal multifrontal factorisation of
+// K = M + dt^2 A^T A (SimulationSpeed::computeLhs + SimplicialLLT::compute,
+// src/physics/SimulationSpeed.cpp:44-53,230-236), done once on the host in FP64.
+//
+// Output per supernode s (columns [c0, c0+ns), below-rows R_s, |R_s| = nb):
+//   F_s = [ L_ss^-1 ; L_bs L_ss^-1 ]        (ns+nb) x ns, column-major
+// so one forward-substitution step is y_s = F_s[0:ns] b~_s, u_s = F_s[ns:] b~_s
+// and one backward step is x_s = F_s^T [ y_s ; -x_R ]. Inverting the dense
+// diagonal blocks on the host turns both substitutions into dense
+// matrix-vector products with no sequential dependence inside a supernode.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <stdexcept>
+
+#include "host.hpp"
+
+namespace pdfc {
+namespace {
+
+struct NDState {
+    const Model* M;
+    std::vector<char> side;
+    std::vector<int> order;
+    std::vector<int> bounds;  // supernode column boundaries
+    int leaf;
+};
+
+double coord(const Model& M, int v, int ax) { return M.rest[3 * v + ax]; }
+
+// Recursive coordinate bisection on the graph of K (the pattern of A^T A). The
+// smaller of the two one-sided boundaries becomes the separator; separators and
+// leaves become supernodes, emitted in post-order.
+void dissect(NDState& st, std::vector<int>& set) {
+    const Model& M = *st.M;
+    if ((int)set.size() <= st.leaf) {
+        if (set.empty()) return;
+        std::sort(set.begin(), set.end());
+        st.order.insert(st.order.end(), set.begin(), set.end());
+        st.bounds.push_back((int)st.order.size());
+        return;
+    }
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = coord(M, set[0], k);
+    for (int v : set)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], coord(M, v, k));
+            hi[k] = std::max(hi[k], coord(M, v, k));
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k) if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+    auto less = [&](int a, int b) {
+        double ca = coord(M, a, ax), cb = coord(M, b, ax);
+        return ca < cb || (ca == cb && a < b);
+    };
+    size_t half = set.size() / 2;
+    std::nth_element(set.begin(), set.begin() + half, set.end(), less);
+    std::vector<int> L(set.begin(), set.begin() + half), R(set.begin() + half, set.end());
+    set.clear();
+    set.shrink_to_fit();
+    auto boundary = [&](const std::vector<int>& A, char other, std::vector<int>& sep, std::vector<int>& rest) {
+        for (int v : A) {
+            bool touch = false;
+            for (int64_t p = M.ata_ptr[v]; p < M.ata_ptr[v + 1] && !touch; ++p) touch = st.side[M.ata_idx[p]] == other;
+            (touch ? sep : rest).push_back(v);
+        }
+    };
+    for (int v : L) st.side[v] = 1;
+    for (int v : R) st.side[v] = 2;
+    std::vector<int> sepL, restL, sepR, restR;
+    boundary(L, 2, sepL, restL);
+    boundary(R, 1, sepR, restR);
+    for (int v : L) st.side[v] = 0;
+    for (int v : R) st.side[v] = 0;
+    std::vector<int> sep;
+    if (sepL.size() <= sepR.size()) {
+        sep.swap(sepL);
+        L.swap(restL);
+    } else {
+        sep.swap(sepR);
+        R.swap(restR);
+    }
+    restL.clear(); restR.clear(); sepL.clear(); sepR.clear();
+    dissect(st, L);
+    dissect(st, R);
+    if (!sep.empty()) {
+        std::sort(sep.begin(), sep.end());
+        st.order.insert(st.order.end(), sep.begin(), sep.end());
+        st.bounds.push_back((int)st.order.size());
+    }
+}
+
+// Dense lower-triangular inverse, column-major ld x n block starting at a.
+void lower_inverse(const double* Lm, int ld, int n, std::vector<double>& inv) {
+    inv.assign((size_t)n * n, 0.);
+    for (int j = 0; j < n; ++j) {
+        // solve L x = e_j, x stored in column j of inv (rows >= j)
+        double* x = &inv[(size_t)j * n];
+        x[j] = 1. / Lm[(size_t)j * ld + j];
+        for (int i = j + 1; i < n; ++i) {
+            double s = 0.;
+            for (int k = j; k < i; ++k) s += Lm[(size_t)k * ld + i] * x[k];
+            x[i] = -s / Lm[(size_t)i * ld + i];
+        }
+    }
+}
+
+}  // namespace
+
+void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int n = M.n;
+    F.n = n;
+    NDState st;
+    st.M = &M;
+    st.side.assign(n, 0);
+    st.leaf = std::max(8, leaf_size);
+    st.bounds.push_back(0);
+    {
+        std::vector<int> all(n);
+        for (int i = 0; i < n; ++i) all[i] = i;
+        dissect(st, all);
+    }
+    if ((int)st.order.size() != n) throw std::logic_error("ordering lost vertices");
+    F.perm = st.order;
+    F.iperm.assign(n, 0);
+    for (int k = 0; k < n; ++k) F.iperm[F.perm[k]] = k;
+    const int S = (int)st.bounds.size() - 1;
+    F.c0.resize(S);
+    F.ns.resize(S);
+    std::vector<int> snode_of(n);
+    for (int s = 0; s < S; ++s) {
+        F.c0[s] = st.bounds[s];
+        F.ns[s] = st.bounds[s + 1] - st.bounds[s];
+        for (int j = F.c0[s]; j < F.c0[s] + F.ns[s]; ++j) snode_of[j] = s;
+    }
+    // ---- symbolic: below-row structure and supernodal elimination tree
+    F.nb.assign(S, 0);
+    F.parent.assign(S, -1);
+    F.roff.assign(S + 1, 0);
+    F.rows.clear();
+    std::vector<std::vector<int>> kids(S);
+    std::vector<int> mark(n, -1), buf;
+    for (int s = 0; s < S; ++s) {
+        const int cend = F.c0[s] + F.ns[s];
+        buf.clear();
+        for (int j = F.c0[s]; j < cend; ++j) {
+            int old = F.perm[j];
+            for (int64_t p = M.ata_ptr[old]; p < M.ata_ptr[old + 1]; ++p) {
+                int i = F.iperm[M.ata_idx[p]];
+                if (i >= cend && mark[i] != s) { mark[i] = s; buf.push_back(i); }
+            }
+        }
+        for (int c : kids[s])
+            for (int64_t p = F.roff[c]; p < F.roff[c + 1]; ++p) {
+                int i = F.rows[p];
+                if (i >= cend && mark[i] != s) { mark[i] = s; buf.push_back(i); }
+            }
+        std::sort(buf.begin(), buf.end());
+        F.rows.insert(F.rows.end(), buf.begin(), buf.end());
+        F.roff[s + 1] = (int64_t)F.rows.size();
+        F.nb[s] = (int)buf.size();
+        if (!buf.empty()) {
+            int par = snode_of[buf.front()];
+            F.parent[s] = par;
+            kids[par].push_back(s);
+        }
+    }
+    F.choff.assign(S + 1, 0);
+    F.children.clear();
+    for (int s = 0; s < S; ++s) {
+        F.children.insert(F.children.end(), kids[s].begin(), kids[s].end());
+        F.choff[s + 1] = (int64_t)F.children.size();
+    }
+    F.level.assign(S, 0);
+    F.n_levels = 0;
+    for (int s = 0; s < S; ++s) {
+        for (int c : kids[s]) F.level[s] = std::max(F.level[s], F.level[c] + 1);
+        F.n_levels = std::max(F.n_levels, F.level[s] + 1);
+    }
+    F.uoff.assign(S + 1, 0);
+    F.foff.assign(S + 1, 0);
+    F.max_ns = F.max_front = 0;
+    F.nnz_L = 0;
+    for (int s = 0; s < S; ++s) {
+        F.uoff[s + 1] = F.uoff[s] + F.nb[s];
+        F.foff[s + 1] = F.foff[s] + (int64_t)(F.ns[s] + F.nb[s]) * F.ns[s];
+        F.max_ns = std::max(F.max_ns, F.ns[s]);
+        F.max_front = std::max(F.max_front, F.ns[s] + F.nb[s]);
+        F.nnz_L += (int64_t)F.ns[s] * (F.ns[s] + 1) / 2 + (int64_t)F.nb[s] * F.ns[s];
+    }
+    F.mapoff = F.uoff;  // one relative index per below row
+    F.relmap.assign(F.uoff[S], 0);
+    F.F.assign(F.foff[S], 0.);
+    F.FT.assign(F.foff[S], 0.);
+    // ---- numeric multifrontal factorisation
+    std::vector<std::vector<double>> update(S);  // nb x nb lower, column-major
+    std::vector<int> pos(n, -1);
+    std::vector<double> front, inv;
+    for (int s = 0; s < S; ++s) {
+        const int ns = F.ns[s], nb = F.nb[s], f = ns + nb, c0 = F.c0[s];
+        for (int j = 0; j < ns; ++j) pos[c0 + j] = j;
+        for (int k = 0; k < nb; ++k) pos[F.rows[F.roff[s] + k]] = ns + k;
+        front.assign((size_t)f * f, 0.);
+        auto at = [&](int i, int j) -> double& { return front[(size_t)j * f + i]; };
+        for (int j = 0; j < ns; ++j) {
+            int col = c0 + j, old = F.perm[col];
+            for (int64_t p = M.ata_ptr[old]; p < M.ata_ptr[old + 1]; ++p) {
+                int i = F.iperm[M.ata_idx[p]];
+                if (i < col) continue;
+                double v = dt2 * M.ata_val[p];
+                if (i == col) v = M.mass[old] + v;
+                at(pos[i], j) += v;
+            }
+        }
+        for (int64_t q = F.choff[s]; q < F.choff[s + 1]; ++q) {
+            int c = F.children[q];
+            const int nbc = F.nb[c];
+            const int* Rc = &F.rows[F.roff[c]];
+            int* rel = &F.relmap[F.mapoff[c]];
+            for (int k = 0; k < nbc; ++k) rel[k] = pos[Rc[k]];
+            const std::vector<double>& U = update[c];
+            for (int b = 0; b < nbc; ++b)
+                for (int a = b; a < nbc; ++a) at(rel[a], rel[b]) += U[(size_t)b * nbc + a];
+            std::vector<double>().swap(update[c]);
+        }
+        // partial dense Cholesky of the first ns columns (right-looking)
+        for (int k = 0; k < ns; ++k) {
+            double d = at(k, k);
+            if (!(d > 0.)) throw std::runtime_error("K is not positive definite");
+            d = std::sqrt(d);
+            at(k, k) = d;
+            for (int i = k + 1; i < f; ++i) at(i, k) /= d;
+            for (int j = k + 1; j < f; ++j) {
+                double ljk = at(j, k);
+                if (ljk == 0.) continue;
+                double* cj = &front[(size_t)j * f];
+                const double* ck = &front[(size_t)k * f];
+                for (int i = j; i < f; ++i) cj[i] -= ck[i] * ljk;
+            }
+        }
+        if (nb > 0) {
+            std::vector<double>& U = update[s];
+            U.assign((size_t)nb * nb, 0.);
+            for (int b = 0; b < nb; ++b)
+                for (int a = b; a < nb; ++a) U[(size_t)b * nb + a] = at(ns + a, ns + b);
+        }
+        // F_s = [Linv ; L_bs Linv], F_s^T
+        lower_inverse(front.data(), f, ns, inv);
+        double* Fs = &F.F[F.foff[s]];
+        double* FTs = &F.FT[F.foff[s]];
+        for (int j = 0; j < ns; ++j) {
+            for (int i = 0; i < ns; ++i) Fs[(size_t)j * f + i] = inv[(size_t)j * ns + i];
+            for (int r = 0; r < nb; ++r) {
+                double acc = 0.;
+                for (int k = j; k < ns; ++k) acc += at(ns + r, k) * inv[(size_t)j * ns + k];
+                Fs[(size_t)j * f + ns + r] = acc;
+            }
+        }
+        for (int j = 0; j < ns; ++j)
+            for (int i = 0; i < f; ++i) FTs[(size_t)i * ns + j] = Fs[(size_t)j * f + i];
+        for (int j = 0; j < ns; ++j) pos[c0 + j] = -1;
+        for (int k = 0; k < nb; ++k) pos[F.rows[F.roff[s] + k]] = -1;
+    }
+    F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd) {
+    const int S = (int)F.ns.size();
+    std::vector<std::vector<int>> by_level(F.n_levels);
+    for (int s = 0; s < S; ++s) by_level[F.level[s]].push_back(s);
+    fwd = SweepPlan();
+    bwd = SweepPlan();
+    fwd.level_ptr.push_back(0);
+    for (int l = 0; l < F.n_levels; ++l) {
+        for (int s : by_level[l])
+            for (int r0 = 0; r0 < F.ns[s] + F.nb[s]; r0 += chunk) {
+                fwd.task_snode.push_back(s);
+                fwd.task_row0.push_back(r0);
+            }
+        fwd.level_ptr.push_back((int)fwd.task_snode.size());
+    }
+    bwd.level_ptr.push_back(0);
+    for (int l = F.n_levels - 1; l >= 0; --l) {
+        for (int s : by_level[l])
+            for (int j0 = 0; j0 < F.ns[s]; j0 += chunk) {
+                bwd.task_snode.push_back(s);
+                bwd.task_row0.push_back(j0);
+            }
+        bwd.level_ptr.push_back((int)bwd.task_snode.size());
+    }
+}
+
+}  // namespace pdfc

@@ -1,0 +1,101 @@
+// Host-side model of one cloth scene: topology, constraint stencils, the
+// constant system matrix K = M + dt^2 A^T A, its fill-reducing ordering and
+// the supernodal FP64 factorisation that the device substitution kernels use.
+//
+// Everything here runs once per pdfc_create (except the self-contact pairing,
+// which runs once per step when self-collision is on).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pdfc.h"
+
+namespace pdfc {
+
+struct V3 {
+    double x, y, z;
+};
+
+// Scene-level data in the scene's own (OBJ) vertex numbering.
+struct Model {
+    int n = 0, T = 0;
+    std::vector<std::array<int, 3>> tris;
+    std::vector<double> rest;  // 3n
+    std::vector<double> mass;  // n
+    // springs (rows 0..E-1 of A): SpringConstraint, src/physics/SpringConstraint.cpp:42-69
+    std::vector<std::array<int, 2>> edges;
+    std::vector<double> spring_w;   // sqrt(k_s)
+    std::vector<double> spring_L0;  // rest length
+    // bending rows (E..E+B-1): BendingConstraint, src/physics/BendingConstraint.cpp:47-104
+    std::vector<int> bend_ptr;      // B+1, into bend_idx/bend_coef
+    std::vector<int> bend_idx;      // [center, ring...]
+    std::vector<double> bend_coef;  // weighted row entries sqrt(k_b) * beltrami coefficient
+    std::vector<char> border;
+    std::vector<std::vector<int>> neighbors;  // one ring, ascending
+    // A^T A, symmetric, CSR with ascending columns
+    std::vector<int64_t> ata_ptr;
+    std::vector<int> ata_idx;
+    std::vector<double> ata_val;
+    int max_degree = 0;    // springs per vertex
+    int max_ata_row = 0;   // non-zeros per row of A^T A
+};
+
+// Builds topology, masses, springs, bending rows and A^T A. Throws std::invalid_argument.
+void build_model(const pdfc_scene& sc, Model& M);
+
+// Supernodal factor of P K P^T.
+struct Factor {
+    int n = 0;
+    std::vector<int> perm;   // new -> old
+    std::vector<int> iperm;  // old -> new
+    // supernode s owns columns [c0[s], c0[s]+ns[s]) and below-rows R (ascending, new numbering)
+    std::vector<int> c0, ns, nb, parent, level;
+    std::vector<int64_t> roff;       // into rows
+    std::vector<int> rows;
+    std::vector<int64_t> uoff;       // update-vector offset (nb entries)
+    std::vector<int64_t> mapoff;     // relative map of s's below rows into parent front (nb entries)
+    std::vector<int> relmap;
+    std::vector<int64_t> choff;      // children list offsets (n_snodes+1)
+    std::vector<int> children;
+    // F_s = [L_ss^-1 ; L_bs L_ss^-1], column-major (ns+nb) x ns, and its transpose
+    // F_s^T column-major ns x (ns+nb).
+    std::vector<int64_t> foff;       // same offsets for F and F^T
+    std::vector<double> F, FT;
+    int64_t nnz_L = 0;
+    int n_levels = 0;
+    int max_ns = 0, max_front = 0;
+    double seconds = 0.;
+};
+
+// Nested dissection (coordinate bisection on the graph of K), symbolic and
+// numeric multifrontal factorisation. K = diag(mass) + dt2 * A^T A.
+void factorize(const Model& M, double dt2, Factor& F, int leaf_size = 64);
+
+// Work lists of one substitution sweep: (supernode, first row) tasks of at most
+// `chunk` rows, grouped by level (forward: leaves first; backward: root first).
+struct SweepPlan {
+    std::vector<int> task_snode, task_row0;
+    std::vector<int> level_ptr;  // tasks of level l are [level_ptr[l], level_ptr[l+1])
+};
+void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd);
+
+// Greedy node-node pairing and level ordering of self contacts, done on the host
+// from the device's sorted candidate list. Candidates are (vertex, triangle, bary)
+// in scene numbering, sorted by (vertex, triangle).
+struct SelfPair {
+    int v, partner;
+    std::array<int, 3> face;
+    int best;       // which face corner is the partner
+    V3 normal;      // (x_v - closest point) / |.|
+    int level;
+};
+void pair_self_contacts(const Model& M, const double* q, const pdfc_candidate* cand, size_t ncand,
+                        std::vector<SelfPair>& out);
+// Level of each pair (BFS from the obstacle-contact vertices, then degree-1 and
+// remaining vertices); returns the number of levels.
+int order_self_contacts(int n, const std::vector<int>& obstacle_vertices, std::vector<SelfPair>& pairs);
+
+}  // namespace pdfc

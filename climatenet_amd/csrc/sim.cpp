@@ -1,0 +1,664 @@
+// pdfc_sim: device-resident simulation and the extern "C" entry points of pdfc.h.
+//
+// SimulationBase::step() (src/physics/SimulationBase.cpp:103-173) becomes one
+// stream-ordered sequence of kernels; the host only enqueues work (and, with
+// self-collision on, pairs the device's candidate list between detection and
+// the local-global loop).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "host.hpp"
+#include "pdfc.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NumericError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+}  // namespace
+
+struct pdfc_sim {
+    pdfc::Model model;
+    pdfc::Factor fac;
+    pdfc::SweepPlan fwd, bwd;
+    pdfc_params prm{};
+    std::vector<pdfc_obstacle> obstacles;
+    std::vector<double> friction;
+    int n_mat = 0, mesh_material = 0;
+    int n = 0, nobs = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<void*> allocs;
+
+    pdfc::DevModel dm{};
+    pdfc::DevFactor dfwd{}, dbwd{};
+    pdfc::DevObstacles dob{};
+    pdfc::DevContacts dc{};
+    double *qn = nullptr, *vstate = nullptr, *V[2] = {nullptr, nullptr}, *Y = nullptr, *cvec = nullptr;
+    double *Rv = nullptr, *Bsave = nullptr;
+    int *cflags = nullptr, *ccnt = nullptr, *cstart = nullptr, *blocksum = nullptr, *ncontacts = nullptr;
+    double* ac_sum = nullptr;
+    unsigned long long* ac_max = nullptr;
+    bool fp32 = false;
+    bool have_stats = false;
+    // kernel-timing instrumentation
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;        // event pool, two per timed launch
+    std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level
+    size_t kused = 0;
+    double ksum[3] = {0., 0., 0.};
+    int64_t kcount[3] = {0, 0, 0};
+    int64_t ksolves = 0;
+    pdfc_phase_stats stats{};
+    int64_t steps_done = 0;
+
+    template <typename T>
+    T* dalloc(size_t count) {
+        void* p = nullptr;
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
+        allocs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& h) {
+        T* d = dalloc<T>(h.size());
+        if (!h.empty()) HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+    ~pdfc_sim() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : kev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void build(const pdfc_scene& sc, const pdfc_params& p);
+    // Brackets one launch with an event pair when instrumentation is on.
+    template <typename L>
+    void timed(int family, L&& launch) {
+        if (!ktiming) {
+            launch();
+            return;
+        }
+        if (kused + 2 > kev.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                HIP_CHECK(hipEventCreate(&e));
+                kev.push_back(e);
+            }
+        }
+        HIP_CHECK(hipEventRecord(kev[kused], stream));
+        launch();
+        HIP_CHECK(hipEventRecord(kev[kused + 1], stream));
+        kfamily.push_back(family);
+        kused += 2;
+    }
+    void collect_kernel_times();
+    void to_device_vec(const double* host3n, double* dev);
+    void from_device_vec(const double* dev, double* host3n);
+    void solve_inplace(double* X);
+    void run_step();
+    void detect();
+};
+
+void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
+    if (!(p.time_step > 0.)) throw std::invalid_argument("time_step must be positive");
+    if (p.local_global_iterations < 1) throw std::invalid_argument("local_global_iterations must be >= 1");
+    if (p.precision != PDFC_FP64 && p.precision != PDFC_FP32_REFINE64) throw std::invalid_argument("unknown precision");
+    if (sc.n_obstacles > (size_t)pdfc::kMaxObstacles) throw std::invalid_argument("too many obstacles (max 16)");
+    if (sc.n_materials == 0 || !sc.friction) throw std::invalid_argument("friction table is empty");
+    if (sc.material < 0 || sc.material >= (int)sc.n_materials) throw std::invalid_argument("mesh material out of range");
+    if (p.self_collision) throw std::invalid_argument("self-collision is not available in this build");
+    prm = p;
+    fp32 = p.precision == PDFC_FP32_REFINE64;
+    n_mat = (int)sc.n_materials;
+    friction.assign(sc.friction, sc.friction + (size_t)n_mat * n_mat);
+    mesh_material = sc.material;
+    for (size_t o = 0; o < sc.n_obstacles; ++o) {
+        pdfc_obstacle ob = sc.obstacles[o];
+        if (ob.material < 0 || ob.material >= n_mat) throw std::invalid_argument("obstacle material out of range");
+        if (ob.type == PDFC_OBSTACLE_PLANE) {
+            double z = (ob.normal[0] * ob.normal[0] + ob.normal[1] * ob.normal[1]) + ob.normal[2] * ob.normal[2];
+            if (!(z > 0.)) throw std::invalid_argument("plane normal is zero");
+            double s = std::sqrt(z);
+            for (int k = 0; k < 3; ++k) ob.normal[k] /= s;
+        } else if (ob.type != PDFC_OBSTACLE_SPHERE) {
+            throw std::invalid_argument("unknown obstacle type");
+        }
+        obstacles.push_back(ob);
+    }
+    nobs = (int)obstacles.size();
+    pdfc::build_model(sc, model);
+    n = model.n;
+    const double dt = p.time_step;
+    try {
+        pdfc::factorize(model, dt * dt, fac, 32);
+    } catch (const std::runtime_error& e) {
+        throw NumericError(e.what());
+    }
+    if ((size_t)3 * fac.max_front * sizeof(double) > 64 * 1024)
+        throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
+    pdfc::plan_sweeps(fac, pdfc::kSolveChunk, fwd, bwd);
+
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+
+    // ---- model arrays in the supernodal numbering
+    const std::vector<int>& perm = fac.perm;
+    const std::vector<int>& iperm = fac.iperm;
+    std::vector<double> mass(n), invm(n);
+    for (int k = 0; k < n; ++k) {
+        mass[k] = model.mass[perm[k]];
+        invm[k] = 1. / mass[k];
+    }
+    // springs incident to each vertex: kappa = w^2 L0
+    std::vector<std::vector<std::pair<int, double>>> inc(n);
+    for (size_t e = 0; e < model.edges.size(); ++e) {
+        int a = iperm[model.edges[e][0]], b = iperm[model.edges[e][1]];
+        double w = model.spring_w[e];
+        double kap = w * (w * model.spring_L0[e]);
+        inc[a].push_back({b, kap});
+        inc[b].push_back({a, kap});
+    }
+    int sp_slots = 0;
+    for (auto& l : inc) sp_slots = std::max(sp_slots, (int)l.size());
+    std::vector<int> sp_cnt(n), sp_idx((size_t)sp_slots * n, 0);
+    std::vector<double> sp_kap((size_t)sp_slots * n, 0.);
+    for (int k = 0; k < n; ++k) {
+        sp_cnt[k] = (int)inc[k].size();
+        for (size_t s = 0; s < inc[k].size(); ++s) {
+            sp_idx[s * n + k] = inc[k][s].first;
+            sp_kap[s * n + k] = inc[k][s].second;
+        }
+    }
+    const int ata_slots = model.max_ata_row;
+    std::vector<int> ata_cnt(n), ata_idx((size_t)ata_slots * n, 0);
+    std::vector<double> ata_val((size_t)ata_slots * n, 0.);
+    for (int k = 0; k < n; ++k) {
+        int old = perm[k];
+        std::vector<std::pair<int, double>> row;
+        for (int64_t q = model.ata_ptr[old]; q < model.ata_ptr[old + 1]; ++q)
+            row.push_back({iperm[model.ata_idx[q]], model.ata_val[q]});
+        std::sort(row.begin(), row.end());
+        ata_cnt[k] = (int)row.size();
+        for (size_t s = 0; s < row.size(); ++s) {
+            ata_idx[s * n + k] = row[s].first;
+            ata_val[s * n + k] = row[s].second;
+        }
+    }
+    dm.n = n;
+    dm.sp_slots = sp_slots;
+    dm.ata_slots = ata_slots;
+    dm.mass = upload(mass);
+    dm.invmass = upload(invm);
+    dm.sp_cnt = upload(sp_cnt);
+    dm.sp_idx = upload(sp_idx);
+    dm.sp_kappa = upload(sp_kap);
+    dm.ata_cnt = upload(ata_cnt);
+    dm.ata_idx = upload(ata_idx);
+    dm.ata_val = upload(ata_val);
+
+    // ---- factor
+    const int S = (int)fac.ns.size();
+    pdfc::DevFactor df{};
+    df.n = n;
+    df.c0 = upload(fac.c0);
+    df.ns = upload(fac.ns);
+    df.nb = upload(fac.nb);
+    df.foff = upload(fac.foff);
+    df.roff = upload(fac.roff);
+    df.uoff = upload(fac.uoff);
+    df.choff = upload(fac.choff);
+    df.children = upload(fac.children);
+    df.relmap = upload(fac.relmap);
+    df.rows = upload(fac.rows);
+    if (fp32) {
+        std::vector<float> f32(fac.F.begin(), fac.F.end()), ft32(fac.FT.begin(), fac.FT.end());
+        df.F = upload(f32);
+        df.FT = upload(ft32);
+    } else {
+        df.F = upload(fac.F);
+        df.FT = upload(fac.FT);
+    }
+    df.U = dalloc<double>(3 * (size_t)fac.uoff[S]);
+    dfwd = df;
+    dbwd = df;
+    dfwd.task_s = upload(fwd.task_snode);
+    dfwd.task_r0 = upload(fwd.task_row0);
+    dbwd.task_s = upload(bwd.task_snode);
+    dbwd.task_r0 = upload(bwd.task_row0);
+    // host copies of F no longer needed beyond statistics
+    std::vector<double>().swap(fac.FT);
+
+    // ---- obstacles
+    dob.count = nobs;
+    for (int o = 0; o < nobs; ++o) {
+        dob.type[o] = obstacles[o].type;
+        for (int k = 0; k < 3; ++k) {
+            dob.p[o][k] = obstacles[o].point[k];
+            dob.nrm[o][k] = obstacles[o].normal[k];
+        }
+        dob.radius[o] = obstacles[o].radius;
+        dob.mu[o] = friction[(size_t)mesh_material * n_mat + obstacles[o].material];
+    }
+
+    // ---- state and work buffers
+    const size_t n3 = 3 * (size_t)n;
+    qn = dalloc<double>(n3);
+    vstate = dalloc<double>(n3);
+    V[0] = dalloc<double>(n3);
+    V[1] = dalloc<double>(n3);
+    Y = dalloc<double>(n3);
+    cvec = dalloc<double>(n3);
+    Rv = dalloc<double>(n3);
+    Bsave = dalloc<double>(n3);
+    const size_t cap = std::max<size_t>(1, (size_t)n * std::max(1, nobs));
+    dc.vertex = dalloc<int>(cap);
+    dc.obstacle = dalloc<int>(cap);
+    dc.active = dalloc<int>(cap);
+    dc.normal = dalloc<double>(3 * cap);
+    dc.point = dalloc<double>(3 * cap);
+    dc.mu = dalloc<double>(cap);
+    dc.basis = dalloc<double>(9 * cap);
+    dc.r = dalloc<double>(3 * cap);
+    cflags = dalloc<int>(n);
+    ccnt = dalloc<int>(n);
+    cstart = dalloc<int>(n);
+    blocksum = dalloc<int>((n + 255) / 256 + 1);
+    ncontacts = dalloc<int>(1);
+    ac_sum = dalloc<double>(p.local_global_iterations);
+    ac_max = dalloc<unsigned long long>(p.local_global_iterations);
+    HIP_CHECK(hipMemsetAsync(ccnt, 0, n * sizeof(int), stream));
+    HIP_CHECK(hipMemsetAsync(ncontacts, 0, sizeof(int), stream));
+    std::vector<double> q0(sc.positions ? sc.positions : sc.rest_positions,
+                           (sc.positions ? sc.positions : sc.rest_positions) + n3);
+    to_device_vec(q0.data(), qn);
+    std::vector<double> zero(n3, 0.);
+    to_device_vec(zero.data(), vstate);
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void pdfc_sim::to_device_vec(const double* h, double* d) {
+    std::vector<double> planar(3 * (size_t)n);
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < 3; ++c) planar[(size_t)c * n + k] = h[3 * (size_t)fac.perm[k] + c];
+    HIP_CHECK(hipMemcpyAsync(d, planar.data(), planar.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void pdfc_sim::from_device_vec(const double* d, double* h) {
+    std::vector<double> planar(3 * (size_t)n);
+    HIP_CHECK(hipMemcpyAsync(planar.data(), d, planar.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < 3; ++c) h[3 * (size_t)fac.perm[k] + c] = planar[(size_t)c * n + k];
+}
+
+// K x = b in place (X holds b on entry): forward levels leaves -> roots, backward
+// levels roots -> leaves; FP32 operators add FP64 residual corrections.
+void pdfc_sim::solve_inplace(double* X) {
+    const size_t n3 = 3 * (size_t)n;
+    auto sweep = [&](double* inout) {
+        for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
+            timed(1, [&] {
+                pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
+                                           inout, Y, fac.max_ns, stream);
+            });
+        for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
+            timed(2, [&] {
+                pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
+                                            inout, fac.max_front, stream);
+            });
+        if (ktiming) ++ksolves;
+    };
+    if (!fp32) {
+        sweep(X);
+        return;
+    }
+    pdfc::launch_copy(Bsave, X, n3, stream);
+    sweep(X);
+    const double dt2 = prm.time_step * prm.time_step;
+    for (int k = 0; k < prm.refinement_sweeps; ++k) {
+        pdfc::launch_residual(dm, dt2, Bsave, X, Rv, stream);
+        sweep(Rv);
+        pdfc::launch_axpy(X, Rv, n3, stream);
+    }
+}
+
+void pdfc_sim::detect() {
+    pdfc::launch_detect(dm, qn, dob, cflags, ccnt, cstart, blocksum, ncontacts, dc, stream);
+}
+
+void pdfc_sim::run_step() {
+    const int K = prm.local_global_iterations;
+    pdfc::StepArgs a{};
+    a.dt = prm.time_step;
+    for (int k = 0; k < 3; ++k) a.g[k] = prm.gravity[k];
+    a.damping = std::max(prm.air_damping, 0.);
+    a.record = prm.record_iterations ? 1 : 0;
+    a.ac_sum = ac_sum;
+    a.ac_max = ac_max;
+    if (a.record) {
+        pdfc::launch_fill(ac_sum, K, 0., stream);
+        pdfc::launch_fill_u64(ac_max, K, stream);
+    }
+    pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
+    detect();
+    HIP_CHECK(hipEventRecord(ev[1], stream));
+    for (int it = 0; it < K; ++it) {
+        double* vcur = V[it & 1];
+        double* vnext = V[(it + 1) & 1];
+        timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, stream); });
+        solve_inplace(vnext);
+    }
+    pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, a, K - 1, stream);
+    HIP_CHECK(hipGetLastError());
+}
+
+void pdfc_sim::collect_kernel_times() {
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (size_t k = 0; k < kfamily.size(); ++k) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
+        ksum[kfamily[k]] += 1e3 * ms;
+        kcount[kfamily[k]]++;
+    }
+    kfamily.clear();
+    kused = 0;
+}
+
+// ------------------------------------------------------------------ C ABI
+namespace {
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PDFC_OK;
+    } catch (const CapacityError& e) {
+        g_last_error = e.what();
+        return PDFC_ERR_CAPACITY;
+    } catch (const DeviceError& e) {
+        g_last_error = e.what();
+        return PDFC_ERR_DEVICE;
+    } catch (const NumericError& e) {
+        g_last_error = e.what();
+        return PDFC_ERR_NUMERIC;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return PDFC_ERR_INVALID;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PDFC_ERR_DEVICE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* pdfc_last_error(void) { return g_last_error.c_str(); }
+
+int pdfc_create(const pdfc_scene* scene, const pdfc_params* params, pdfc_sim** out) {
+    return guarded([&] {
+        if (!scene || !params || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw DeviceError("no HIP device available");
+        auto sim = std::make_unique<pdfc_sim>();
+        sim->build(*scene, *params);
+        *out = sim.release();
+    });
+}
+
+void pdfc_destroy(pdfc_sim* sim) { delete sim; }
+
+int pdfc_step(pdfc_sim* sim, int nsteps) {
+    return guarded([&] {
+        if (!sim || nsteps < 0) throw std::invalid_argument("bad argument");
+        if (nsteps == 0) return;
+        HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
+        for (int s = 0; s < nsteps; ++s) {
+            sim->run_step();
+            if (sim->ktiming) sim->collect_kernel_times();
+        }
+        HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
+        HIP_CHECK(hipEventSynchronize(sim->ev[2]));
+        float ms_total = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
+        sim->stats.step_us = 1e3 * ms_total / nsteps;
+        sim->stats.steps = nsteps;
+        sim->steps_done += nsteps;
+    });
+}
+
+int pdfc_get_state(pdfc_sim* sim, double* q, double* v) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        if (q) sim->from_device_vec(sim->qn, q);
+        if (v) sim->from_device_vec(sim->vstate, v);
+    });
+}
+
+int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        if (q) sim->to_device_vec(q, sim->qn);
+        if (v) sim->to_device_vec(v, sim->vstate);
+    });
+}
+
+int pdfc_detect(pdfc_sim* sim) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        sim->detect();
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+    });
+}
+
+int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        if (!sim || !n) throw std::invalid_argument("null argument");
+        int nc = 0;
+        HIP_CHECK(hipMemcpyAsync(&nc, sim->ncontacts, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        *n = (size_t)nc;
+        if (cap < (size_t)nc) throw CapacityError("contact buffer too small");
+        if (nc == 0) return;
+        std::vector<int> vtx(nc), obs(nc), act(nc);
+        std::vector<double> nrm(3 * nc), pt(3 * nc), mu(nc), r(3 * nc);
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, sim->stream));
+        };
+        get(vtx.data(), sim->dc.vertex, nc * sizeof(int));
+        get(obs.data(), sim->dc.obstacle, nc * sizeof(int));
+        get(act.data(), sim->dc.active, nc * sizeof(int));
+        get(nrm.data(), sim->dc.normal, 3 * nc * sizeof(double));
+        get(pt.data(), sim->dc.point, 3 * nc * sizeof(double));
+        get(mu.data(), sim->dc.mu, nc * sizeof(double));
+        get(r.data(), sim->dc.r, 3 * nc * sizeof(double));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        std::vector<int> order(nc);
+        for (int i = 0; i < nc; ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            if (obs[a] != obs[b]) return obs[a] < obs[b];
+            return sim->fac.perm[vtx[a]] < sim->fac.perm[vtx[b]];
+        });
+        for (int k = 0; k < nc; ++k) {
+            int i = order[k];
+            pdfc_contact& c = out[k];
+            c.vertex = sim->fac.perm[vtx[i]];
+            c.obstacle = obs[i];
+            c.active = act[i];
+            for (int d = 0; d < 3; ++d) {
+                c.normal[d] = nrm[3 * i + d];
+                c.point[d] = pt[3 * i + d];
+                c.r_local[d] = r[3 * i + d];
+            }
+            c.mu = mu[i];
+        }
+    });
+}
+
+int pdfc_get_self_contacts(pdfc_sim* sim, pdfc_self_contact* out, size_t cap, size_t* n) {
+    (void)out;
+    (void)cap;
+    return guarded([&] {
+        if (!sim || !n) throw std::invalid_argument("null argument");
+        *n = 0;
+    });
+}
+
+int pdfc_get_self_candidates(pdfc_sim* sim, pdfc_candidate* out, size_t cap, size_t* n) {
+    (void)out;
+    (void)cap;
+    return guarded([&] {
+        if (!sim || !n) throw std::invalid_argument("null argument");
+        *n = 0;
+    });
+}
+
+int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, size_t cap, size_t* n_iter) {
+    return guarded([&] {
+        if (!sim || !n_iter) throw std::invalid_argument("null argument");
+        const int K = sim->prm.local_global_iterations;
+        *n_iter = (size_t)K;
+        if (cap < (size_t)K) throw CapacityError("iteration buffer too small");
+        std::vector<double> s(K);
+        std::vector<unsigned long long> m(K);
+        int nc = 0;
+        HIP_CHECK(hipMemcpyAsync(s.data(), sim->ac_sum, K * sizeof(double), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipMemcpyAsync(m.data(), sim->ac_max, K * sizeof(unsigned long long), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipMemcpyAsync(&nc, sim->ncontacts, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        for (int k = 0; k < K; ++k) {
+            if (ac_mean) ac_mean[k] = nc > 0 ? s[k] / nc : 0.;
+            if (ac_max) {
+                double d;
+                std::memcpy(&d, &m[k], sizeof(double));
+                ac_max[k] = d;
+            }
+        }
+    });
+}
+
+int pdfc_get_phase_stats(pdfc_sim* sim, pdfc_phase_stats* out) {
+    return guarded([&] {
+        if (!sim || !out) throw std::invalid_argument("null argument");
+        *out = sim->stats;
+    });
+}
+
+int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
+    return guarded([&] {
+        if (!sim || !out) throw std::invalid_argument("null argument");
+        pdfc_info I{};
+        I.n_vertices = sim->n;
+        I.n_triangles = sim->model.T;
+        I.n_springs = (int64_t)sim->model.edges.size();
+        I.n_bending = (int64_t)sim->model.bend_ptr.size() - 1;
+        I.nnz_K = (int64_t)sim->model.ata_idx.size();
+        I.nnz_L = sim->fac.nnz_L;
+        I.nnz_F = sim->fac.foff.back();
+        I.n_supernodes = (int64_t)sim->fac.ns.size();
+        I.n_levels = sim->fac.n_levels;
+        int nc = 0;
+        HIP_CHECK(hipMemcpyAsync(&nc, sim->ncontacts, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        I.n_contacts = nc;
+        I.max_front = sim->fac.max_front;
+        I.factor_seconds = sim->fac.seconds;
+        // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
+        //   local kernel: read v, q_n, c and write b (4 x 24 B per vertex); per spring end an
+        //   index and kappa (4 + 8 B); per contact the basis (72 B), mu (8 B), r read+write
+        //   (48 B) and the A^T A row of its vertex (19 entries x 12 B + v gather 24 B).
+        //   forward sweep: F once (8 or 4 B per entry), b read and y written (2 x 24 B per
+        //   vertex), update vectors written and read (2 x 24 B per below row), child maps 4 B.
+        //   backward sweep: F^T once, y read, x written, x gathered at below rows (24 B + 4 B
+        //   index per below row).
+        const double n = sim->n, E = (double)sim->model.edges.size();
+        const double ent = sim->fp32 ? 4. : 8.;
+        const double nbtot = (double)sim->fac.uoff.back();
+        const double fwd = ent * (double)I.nnz_F + 2. * 24. * n + 2. * 24. * nbtot + 4. * nbtot;
+        const double bwd = ent * (double)I.nnz_F + 2. * 24. * n + 28. * nbtot;
+        const double local = 4. * 24. * n + 2. * E * 12. + nc * (72. + 8. + 48. + 19. * 36.);
+        double solve = fwd + bwd;
+        if (sim->fp32) solve = (fwd + bwd) * (1. + sim->prm.refinement_sweeps) +
+                               sim->prm.refinement_sweeps * (4. * 24. * n + 19. * 12. * n);
+        I.bytes_local = local;
+        I.bytes_forward = fwd;
+        I.bytes_backward = bwd;
+        I.n_below_rows = (int64_t)nbtot;
+        I.bytes_solve = solve;
+        I.bytes_per_iteration = solve + local;
+        *out = I;
+    });
+}
+
+int pdfc_set_kernel_timing(pdfc_sim* sim, int enable) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        sim->collect_kernel_times();
+        sim->ktiming = enable != 0;
+        for (int k = 0; k < 3; ++k) {
+            sim->ksum[k] = 0.;
+            sim->kcount[k] = 0;
+        }
+        sim->ksolves = 0;
+    });
+}
+
+int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
+    return guarded([&] {
+        if (!sim || !out) throw std::invalid_argument("null argument");
+        sim->collect_kernel_times();
+        pdfc_kernel_times t{};
+        t.local_launches = sim->kcount[0];
+        t.forward_launches = sim->kcount[1];
+        t.backward_launches = sim->kcount[2];
+        t.local_us = sim->kcount[0] ? sim->ksum[0] / sim->kcount[0] : 0.;
+        t.forward_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
+        t.backward_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
+        t.forward_sweep_us = sim->ksolves ? sim->ksum[1] / sim->ksolves : 0.;
+        t.backward_sweep_us = sim->ksolves ? sim->ksum[2] / sim->ksolves : 0.;
+        *out = t;
+    });
+}
+
+int pdfc_solve(pdfc_sim* sim, const double* b, double* x) {
+    return guarded([&] {
+        if (!sim || !b || !x) throw std::invalid_argument("null argument");
+        sim->to_device_vec(b, sim->V[1]);
+        sim->solve_inplace(sim->V[1]);
+        HIP_CHECK(hipGetLastError());
+        sim->from_device_vec(sim->V[1], x);
+    });
+}
+
+}  // extern "C"

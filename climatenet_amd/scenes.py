@@ -1,0 +1,236 @@
+"""Synthetic cloth scenes (square grids over spheres and planes).
+
+Each scene is what the reference would load from a JSON file plus OBJ meshes:
+positions are rounded to float32 exactly like the reference's OBJ reader
+(tinyobjloader with ``real_t = float``, extern/tinyObjLoader/tiny_obj_loader.h:100-106),
+then widened to float64.
+
+Grid convention: vertex (i, j) has index j*(N+1)+i and sits at
+(-side/2 + i*h, -side/2 + j*h, z) with h = side/N; quad (a=(i,j), b=(i+1,j),
+c=(i,j+1), d=(i+1,j+1)) splits into triangles (a,b,d) and (a,d,c), counter-
+clockwise seen from +z.
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import abi
+
+_MASK64 = (1 << 64) - 1
+
+
+class MT19937_64:
+    """The standard 64-bit Mersenne Twister (std::mt19937_64), seeded like the C++ engine."""
+
+    def __init__(self, seed: int = 0):
+        self.mt = [0] * 312
+        self.mt[0] = seed & _MASK64
+        for i in range(1, 312):
+            self.mt[i] = (6364136223846793005 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 62)) + i) & _MASK64
+        self.index = 312
+
+    def _twist(self):
+        mt = self.mt
+        upper, lower = 0xFFFFFFFF80000000, 0x7FFFFFFF
+        for i in range(312):
+            x = (mt[i] & upper) | (mt[(i + 1) % 312] & lower)
+            xa = x >> 1
+            if x & 1:
+                xa ^= 0xB5026F5AA96619E9
+            mt[i] = mt[(i + 156) % 312] ^ xa
+        self.index = 0
+
+    def __call__(self) -> int:
+        if self.index >= 312:
+            self._twist()
+        y = self.mt[self.index]
+        self.index += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEEB70000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & _MASK64
+
+    def uniform01(self) -> float:
+        """(u64 >> 11) * 2^-53, in [0, 1)."""
+        return (self() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def square_grid(N: int, side: float, z: float, x0: float = 0.0, y0: float = 0.0):
+    """(positions 3n float64 rounded to float32, triangles 3T int32) of an N x N quad grid."""
+    h = side / N
+    i = np.arange(N + 1, dtype=np.float64)
+    X, Y = np.meshgrid(x0 - side / 2 + i * h, y0 - side / 2 + i * h, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="xy")
+    a = (jj * (N + 1) + ii).ravel()
+    b = a + 1
+    c = a + (N + 1)
+    d = c + 1
+    tris = np.empty((2 * a.size, 3), dtype=np.int32)
+    tris[0::2] = np.stack([a, b, d], axis=1)
+    tris[1::2] = np.stack([a, d, c], axis=1)
+    return f32(pos.ravel()), tris.ravel()
+
+
+def jitter_xy(pos: np.ndarray, amplitude: float, seed: int = 0) -> np.ndarray:
+    """x, y of each vertex += U(-a, a) from mt19937_64(seed), x then y, vertex order."""
+    rng = MT19937_64(seed)
+    out = pos.reshape(-1, 3).copy()
+    for k in range(out.shape[0]):
+        out[k, 0] += -amplitude + 2.0 * amplitude * rng.uniform01()
+        out[k, 1] += -amplitude + 2.0 * amplitude * rng.uniform01()
+    return f32(out.ravel())
+
+
+@dataclass
+class ObstacleSpec:
+    kind: str  # "sphere" | "plane"
+    point: tuple
+    normal: tuple = (0.0, 0.0, 1.0)
+    radius: float = 0.0
+    material: int = 0
+
+
+@dataclass
+class SceneSpec:
+    name: str
+    rest: np.ndarray
+    triangles: np.ndarray
+    positions: Optional[np.ndarray]
+    obstacles: List[ObstacleSpec]
+    friction: np.ndarray  # n_mat x n_mat
+    material: int = 0
+    area_density: float = 0.1
+    bending: float = 5e-4
+    stretching: float = 2e2
+    time_step: float = 0.01
+    iterations: int = 10
+    steps: int = 200
+    self_collision: bool = False
+    self_tolerance: float = 0.0
+    gravity: tuple = (0.0, 0.0, -9.81)
+    air_damping: float = 0.0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_vertices(self) -> int:
+        return self.rest.size // 3
+
+    @property
+    def n_triangles(self) -> int:
+        return self.triangles.size // 3
+
+    def initial_positions(self) -> np.ndarray:
+        return self.rest.copy() if self.positions is None else self.positions.copy()
+
+    def c_scene(self) -> abi.Scene:
+        rest = np.ascontiguousarray(self.rest, dtype=np.float64)
+        pos = None if self.positions is None else np.ascontiguousarray(self.positions, dtype=np.float64)
+        tris = np.ascontiguousarray(self.triangles, dtype=np.int32)
+        fr = np.ascontiguousarray(self.friction, dtype=np.float64)
+        obs = (abi.Obstacle * max(1, len(self.obstacles)))()
+        for k, o in enumerate(self.obstacles):
+            obs[k].type = abi.OBSTACLE_SPHERE if o.kind == "sphere" else abi.OBSTACLE_PLANE
+            obs[k].material = o.material
+            for c in range(3):
+                obs[k].point[c] = o.point[c]
+                obs[k].normal[c] = o.normal[c]
+            obs[k].radius = o.radius
+        self._keep = [rest, pos, tris, fr, obs]
+        s = abi.Scene()
+        s.n_vertices = self.n_vertices
+        s.rest_positions = abi.dptr(rest)
+        s.positions = abi.dptr(pos)
+        s.n_triangles = self.n_triangles
+        s.triangles = abi.iptr(tris)
+        s.area_density = self.area_density
+        s.bending = self.bending
+        s.stretching = self.stretching
+        s.material = self.material
+        s.n_obstacles = len(self.obstacles)
+        s.obstacles = obs
+        s.n_materials = fr.shape[0]
+        s.friction = abi.dptr(fr)
+        return s
+
+    def c_params(self, precision=abi.FP64, refinement_sweeps=2, warm_start=abi.WARM_PLAIN,
+                 record_iterations=True, iterations=None) -> abi.Params:
+        p = abi.Params()
+        p.time_step = self.time_step
+        p.local_global_iterations = self.iterations if iterations is None else iterations
+        p.self_collision = int(self.self_collision)
+        p.self_collision_tolerance = self.self_tolerance
+        for c in range(3):
+            p.gravity[c] = self.gravity[c]
+        p.air_damping = self.air_damping
+        p.precision = precision
+        p.refinement_sweeps = refinement_sweeps
+        p.warm_start = warm_start
+        p.record_iterations = int(record_iterations)
+        return p
+
+
+def sphere_drape_small(jitter_seed: int = 0) -> SceneSpec:
+    """20x20-quad cloth (441 vertices), side 1 m at z=0.3, over a sphere R=0.25, mu=0.3,
+    dt=1/100, 10 local-global iterations, 200 steps; initial x,y jitter 1e-4 (seeded)."""
+    rest, tris = square_grid(20, 1.0, 0.3)
+    pos = jitter_xy(rest, 1e-4, jitter_seed)
+    return SceneSpec("drape_20x20", rest, tris, pos,
+                     [ObstacleSpec("sphere", (0.0, 0.0, 0.0), radius=0.25)],
+                     np.array([[0.3]]), time_step=0.01, iterations=10, steps=200)
+
+
+def sphere_drape_40k() -> SceneSpec:
+    """200x200-quad cloth (40,401 vertices, 80,000 triangles), side 2 m at z=0.55 over a
+    sphere R=0.5 at the origin plus a ground plane z=-0.5, mu=0.1, dt=1/200,
+    20 local-global iterations, self-collision on (tolerance 5e-3), 500 steps."""
+    rest, tris = square_grid(200, 2.0, 0.55)
+    return SceneSpec("drape_200x200", rest, tris, None,
+                     [ObstacleSpec("sphere", (0.0, 0.0, 0.0), radius=0.5),
+                      ObstacleSpec("plane", (0.0, 0.0, -0.5), (0.0, 0.0, 1.0))],
+                     np.array([[0.1]]), time_step=0.005, iterations=20, steps=500,
+                     self_collision=True, self_tolerance=5e-3)
+
+
+def stacked_layers(N: int = 150, side: float = 1.5, zs=(0.45, 0.455, 0.46)) -> SceneSpec:
+    """Three N x N sheets stacked 5 mm apart (one mesh, three components) over a sphere
+    R=0.4; mu=0.3 for cloth-sphere and cloth-cloth; self tolerance 4e-3; dt=1/200,
+    30 iterations, 400 steps."""
+    P, T = [], []
+    nv = (N + 1) ** 2
+    for k, z in enumerate(zs):
+        p, t = square_grid(N, side, z)
+        P.append(p)
+        T.append(t + k * nv)
+    rest = np.concatenate(P)
+    tris = np.concatenate(T).astype(np.int32)
+    return SceneSpec("stacked_3x%d" % N, rest, tris, None,
+                     [ObstacleSpec("sphere", (0.0, 0.0, 0.0), radius=0.4, material=1)],
+                     np.array([[0.3, 0.3], [0.3, 0.3]]), time_step=0.005, iterations=30,
+                     steps=400, self_collision=True, self_tolerance=4e-3)
+
+
+def inclined_plane(mu: float = 0.1, angle_deg: float = 45.0, height: float = 0.05) -> SceneSpec:
+    """A 2x2-vertex square cloth (two triangles) held flat and released `height` above a
+    plane through the origin tilted by `angle_deg` about y (the closed-form example of
+    replicability/analytic_example_data_processing.py)."""
+    th = np.deg2rad(angle_deg)
+    nrm = (-np.sin(th), 0.0, np.cos(th))
+    s = 0.1
+    # square lying parallel to the plane, centred above the origin along the normal
+    t1 = np.array([np.cos(th), 0.0, np.sin(th)])
+    t2 = np.array([0.0, 1.0, 0.0])
+    c0 = np.array(nrm) * height
+    pts = [c0 + a * t1 + b * t2 for b in (-s / 2, s / 2) for a in (-s / 2, s / 2)]
+    rest = f32(np.array(pts).ravel())
+    tris = np.array([0, 1, 3, 0, 3, 2], dtype=np.int32)
+    return SceneSpec("inclined", rest, tris, None,
+                     [ObstacleSpec("plane", (0.0, 0.0, 0.0), nrm)],
+                     np.array([[mu]]), area_density=1.0, bending=0.0, stretching=1e3,
+                     time_step=1e-3, iterations=50, steps=400)

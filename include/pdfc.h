@@ -1,0 +1,215 @@
+/*
+ * pdfc.h — C ABI of the MI355X projective-dynamics step with dry frictional
+ * contact ("pdfc").
+ *
+ * One pdfc_sim owns one cloth scene on the current HIP device: the constant
+ * system matrix K = M + dt^2 A^T A (factored once on the host, FP64), the
+ * device-resident state (q, v), the per-step contact sets and all the
+ * kernels' buffers. Every entry point returns PDFC_OK (0) or a negative
+ * status; the message of the last failure on the calling thread is returned
+ * by pdfc_last_error(). No C++ exception crosses this boundary.
+ *
+ * Reference interfaces replaced (file:line in guibquatre/ClimateNet):
+ *   pdfc_create      getSimulation()            src/simulation_building.cpp:80-98
+ *                    SimulationSpeed ctor       src/physics/SimulationSpeed.cpp:29-60
+ *                    PhysicScene::finalize      src/physics/PhysicScene.cpp:88-91
+ *   pdfc_step        SimulationBase::step()     src/physics/SimulationBase.cpp:103-173
+ *   pdfc_get_state   PhysicScene::getGeneralizedPositions/Speeds
+ *                                               src/physics/PhysicScene.cpp:228-305
+ *   pdfc_set_state   PhysicScene::setGeneralizedPositions/Speeds
+ *                                               src/physics/PhysicScene.cpp:268-314
+ *   pdfc_get_contacts  m_collisions_infos + m_collisions_local_forces
+ *                                               src/physics/SimulationFrictionEstimation.cpp:350-390
+ *   pdfc_get_self_contacts  m_self_collisions_infos + m_self_collisions_local_forces
+ *                                               src/physics/SimulationFrictionEstimation.cpp:437-540
+ *   pdfc_get_iteration_stats  updateIterationData / Alart-Curnier norms
+ *                                               src/physics/SimulationFrictionEstimation.cpp:590-612,720-753
+ *   pdfc_get_phase_stats  exportStats timers     src/physics/SimulationFrictionEstimation.cpp:755-771
+ *
+ * Layout conventions at the boundary: positions and velocities are the
+ * reference's generalized vectors (Eigen::VectorXd of size 3n, x,y,z of
+ * vertex i at 3i..3i+2). Vertex indices are the scene's (OBJ) indices.
+ */
+#ifndef PDFC_H
+#define PDFC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdfc_sim pdfc_sim;
+
+#define PDFC_OK 0
+#define PDFC_ERR_INVALID (-1)   /* bad argument or scene (std::invalid_argument in the reference) */
+#define PDFC_ERR_DEVICE (-2)    /* HIP runtime failure */
+#define PDFC_ERR_NUMERIC (-3)   /* factorisation failed (K not SPD) */
+#define PDFC_ERR_CAPACITY (-4)  /* output buffer too small; *n holds the size needed */
+
+enum pdfc_obstacle_type {
+    PDFC_OBSTACLE_SPHERE = 0, /* StaticSphere: src/physics/StaticSphere.cpp:35-47 */
+    PDFC_OBSTACLE_PLANE = 1   /* StaticPlane:  src/physics/StaticPlane.cpp:57-73 */
+};
+
+enum pdfc_precision {
+    PDFC_FP64 = 0,           /* substitution and local step in FP64 */
+    PDFC_FP32_REFINE64 = 1   /* FP32 substitution, FP64 residual + iterative refinement */
+};
+
+enum pdfc_warm_start {
+    PDFC_WARM_PLAIN = 0,     /* the reference: r^0 is the cone projection of the contact force at v^0 */
+    PDFC_WARM_LEARNED = 2    /* r^0 predicted by the contact MLP (pdfc_set_mlp), fallback to plain */
+};
+
+typedef struct pdfc_obstacle {
+    int32_t type;            /* enum pdfc_obstacle_type */
+    int32_t material;        /* "Material Identifier" */
+    double point[3];         /* sphere centre, or a point of the plane ("Position" / "Point") */
+    double normal[3];        /* plane normal ("Normal"), normalised by pdfc_create; unused for spheres */
+    double radius;           /* sphere radius ("Radius") */
+} pdfc_obstacle;
+
+typedef struct pdfc_scene {
+    size_t n_vertices;
+    const double* rest_positions;    /* 3n, the "Obj filename" geometry (rest state) */
+    const double* positions;         /* 3n or NULL: "Current state obj filename" geometry */
+    size_t n_triangles;
+    const int32_t* triangles;        /* 3T vertex indices, consistent winding */
+    double area_density;             /* "Area Density" */
+    double bending;                  /* "Bending" stiffness k_b (row weight sqrt(k_b)) */
+    double stretching;               /* "Stretching" stiffness k_s (row weight sqrt(k_s)) */
+    int32_t material;                /* mesh "Material Identifier" */
+    size_t n_obstacles;
+    const pdfc_obstacle* obstacles;
+    size_t n_materials;
+    const double* friction;          /* n_materials x n_materials, row-major ("Friction coefficients") */
+} pdfc_scene;
+
+typedef struct pdfc_params {
+    double time_step;                /* "Timestep" */
+    int32_t local_global_iterations; /* "LocalGlobal" */
+    int32_t self_collision;          /* "Self-collision" */
+    double self_collision_tolerance; /* "Self-collision tolerance" */
+    double gravity[3];               /* "Gravity" */
+    double air_damping;              /* "Air damping" (optional, 0 when absent) */
+    int32_t precision;               /* enum pdfc_precision */
+    int32_t refinement_sweeps;       /* FP32_REFINE64 only: corrections after the first FP32 solve */
+    int32_t warm_start;              /* enum pdfc_warm_start */
+    int32_t record_iterations;       /* nonzero: keep the per-iteration friction residual (costs a sync-free reduction per iteration) */
+} pdfc_params;
+
+typedef struct pdfc_contact {
+    int64_t vertex;                  /* CollisionInfo::vertex_index */
+    int32_t obstacle;                /* index into pdfc_scene.obstacles */
+    int32_t active;                  /* 0: take-off in the last iteration (r = 0) */
+    double normal[3];                /* CollisionInfo::normal */
+    double point[3];                 /* CollisionInfo::contact_point */
+    double mu;                       /* CollisionInfo::friction_coefficient */
+    double r_local[3];               /* reaction in the contact basis [n, t, b] (m_collisions_local_forces) */
+} pdfc_contact;
+
+typedef struct pdfc_self_contact {
+    int64_t vertex;                  /* SelfCollisionInfo::vertex_index */
+    int64_t partner;                 /* the face vertex paired with it (largest barycentric coordinate) */
+    int64_t face[3];                 /* SelfCollisionInfo::face_indices */
+    int32_t level;                   /* computation level (BFS order) */
+    int32_t active;
+    double normal[3];
+    double mu;
+    double r_local[3];               /* m_self_collisions_local_forces (mass-weighted) */
+} pdfc_self_contact;
+
+typedef struct pdfc_candidate {      /* one self-collision candidate surviving the exact test */
+    int64_t vertex;
+    int64_t triangle;
+    double barycentric[3];
+} pdfc_candidate;
+
+typedef struct pdfc_info {
+    int64_t n_vertices, n_triangles, n_springs, n_bending;
+    int64_t nnz_K;                   /* A^T A / K non-zeros (full symmetric pattern) */
+    int64_t nnz_L;                   /* non-zeros of the Cholesky factor (supernodal, dense blocks) */
+    int64_t nnz_F;                   /* stored entries of the substitution operators (per copy) */
+    int64_t n_supernodes;
+    int64_t n_levels;                /* supernodal elimination-tree levels */
+    int64_t n_contacts;              /* obstacle contacts of the last step */
+    int64_t n_self_contacts;         /* self contacts of the last step */
+    int64_t n_self_candidates;       /* exact-test survivors of the last step */
+    int64_t n_self_levels;
+    int64_t max_front;               /* largest supernode (rows) */
+    double factor_seconds;           /* host symbolic + numeric factorisation time */
+    double bytes_per_iteration;      /* algorithmic HBM bytes of one local-global iteration */
+    double bytes_solve;              /* of which the two substitutions */
+    double bytes_local;              /* local-step kernel, per launch */
+    double bytes_forward;            /* one forward sweep (all levels) */
+    double bytes_backward;           /* one backward sweep (all levels) */
+    int64_t n_below_rows;            /* total update-vector rows (sum of |R_s|) */
+} pdfc_info;
+
+typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
+    double step_us;
+    double detection_us;
+    double iterations_us;
+    int64_t steps;
+} pdfc_phase_stats;
+
+typedef struct pdfc_kernel_times {  /* per-launch device time (HIP events), averaged over an instrumented run */
+    double local_us;                 /* local step + RHS + obstacle friction kernel */
+    double forward_us;               /* one forward-substitution level */
+    double backward_us;              /* one backward-substitution level */
+    double forward_sweep_us;         /* all forward levels of one substitution (sum of launches) */
+    double backward_sweep_us;
+    int64_t local_launches, forward_launches, backward_launches;
+} pdfc_kernel_times;
+
+/* Build the simulation: topology, masses, constraints, K, host FP64 factorisation, device upload. */
+int pdfc_create(const pdfc_scene* scene, const pdfc_params* params, pdfc_sim** out);
+
+/* Advance nsteps time steps (SimulationBase::step() nsteps times). State stays on the device. */
+int pdfc_step(pdfc_sim* sim, int nsteps);
+
+/* Copy the generalized positions / velocities (3n each) to host memory; either may be NULL. */
+int pdfc_get_state(pdfc_sim* sim, double* q, double* v);
+
+/* Overwrite the state (3n each; either may be NULL). Used for identical-state parity and checkpoints. */
+int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v);
+
+/* Obstacle contacts of the last step, in the reference's order (obstacle, then vertex). */
+int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n);
+
+/* Self contacts of the last step, in detection order (vertex ascending). */
+int pdfc_get_self_contacts(pdfc_sim* sim, pdfc_self_contact* out, size_t cap, size_t* n);
+
+/* Self-collision candidates of the last step's detection, sorted by (vertex, triangle). */
+int pdfc_get_self_candidates(pdfc_sim* sim, pdfc_candidate* out, size_t cap, size_t* n);
+
+/* Run only the detection of the current state (no time step) so candidates/contacts can be read. */
+int pdfc_detect(pdfc_sim* sim);
+
+/* Per-iteration mean and max Alart-Curnier norm over all contacts of the last step
+ * (requires record_iterations). ac_mean/ac_max may be NULL. */
+int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, size_t cap, size_t* n_iter);
+
+int pdfc_get_phase_stats(pdfc_sim* sim, pdfc_phase_stats* out);
+
+/* Instrumentation (no counterpart in the reference; measurement only): when enabled, every
+ * launch of the local kernel and of each substitution level is bracketed by HIP events on the
+ * simulation's stream, and pdfc_step synchronises after each step to accumulate the times.
+ * Enabling resets the accumulated times. */
+int pdfc_set_kernel_timing(pdfc_sim* sim, int enable);
+int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out);
+int pdfc_get_info(pdfc_sim* sim, pdfc_info* out);
+
+/* Solve K x = b for 3 right-hand sides (3n, generalized layout) with the device substitution
+ * in the simulation's precision mode. Exposed for parity tests of the global step. */
+int pdfc_solve(pdfc_sim* sim, const double* b, double* x);
+
+void pdfc_destroy(pdfc_sim* sim);
+const char* pdfc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDFC_H */

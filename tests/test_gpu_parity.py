@@ -1,0 +1,146 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Per-step parity is checked from identical states (both sides start every
+step from the oracle's state), because contact predicates are discontinuous
+and free-running trajectories of a chaotic system drift apart.
+Tolerances (FP64 mode): positions, velocities and contact reactions within
+1e-9 relative (max-norm, relative to the largest magnitude of the quantity);
+contact sets identical.
+"""
+import numpy as np
+import pytest
+
+from climatenet_amd import abi, scenes
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-9
+
+
+def rel_err(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def contact_arrays(cs):
+    key = np.array([(c.obstacle, c.vertex) for c in cs], dtype=np.int64).reshape(-1, 2)
+    r = np.array([list(c.r_local) for c in cs]).reshape(-1, 3)
+    nrm = np.array([list(c.normal) for c in cs]).reshape(-1, 3)
+    act = np.array([c.active for c in cs], dtype=np.int64)
+    return key, r, nrm, act
+
+
+def test_solve_matches_oracle(gpu):
+    spec = scenes.sphere_drape_small()
+    sim = gpu.Simulation(spec)
+    orc = po.Oracle(spec)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    x = sim.solve(b)
+    xo = orc.solve(b)
+    assert rel_err(x, xo) < 1e-12
+    # residual against the oracle's K (independent assembly)
+    K = orc.K()
+    X = x.reshape(-1, 3)
+    res = np.abs(K @ X - b.reshape(-1, 3)).max() / np.abs(b).max()
+    assert res < 1e-12
+
+
+def test_solve_40k_residual(gpu):
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    sim = gpu.Simulation(spec)
+    orc = po.Oracle(spec, iterations=1)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    x = sim.solve(b)
+    K = orc.K()
+    res = np.abs(K @ x.reshape(-1, 3) - b.reshape(-1, 3)).max() / np.abs(b).max()
+    assert res < 1e-11
+    assert rel_err(x, orc.solve(b)) < 1e-11
+
+
+@pytest.mark.parametrize("steps", [200])
+def test_small_drape_identical_state_parity(gpu, steps):
+    """20x20 drape: every step from the oracle's state; q, v, contacts, reactions."""
+    spec = scenes.sphere_drape_small()
+    sim = gpu.Simulation(spec)
+    orc = po.Oracle(spec)
+    worst = dict(q=0.0, v=0.0, r=0.0)
+    n_contact_steps = 0
+    for s in range(steps):
+        q0, v0 = orc.state()
+        sim.set_state(q0, v0)
+        orc.step(1)
+        sim.step(1)
+        q, v = sim.state()
+        qo, vo = orc.state()
+        worst["q"] = max(worst["q"], rel_err(q, qo))
+        worst["v"] = max(worst["v"], rel_err(v, vo))
+        ko, ro, no, ao = contact_arrays(orc.contacts())
+        kg, rg, ng, ag = contact_arrays(sim.contacts())
+        assert np.array_equal(ko, kg), "contact sets differ at step %d" % s
+        if len(ko):
+            n_contact_steps += 1
+            assert np.array_equal(ao, ag), "take-off flags differ at step %d" % s
+            assert rel_err(ng, no) < 1e-14
+            worst["r"] = max(worst["r"], rel_err(rg, ro))
+        mo, Mo = orc.iteration_stats()
+        mg, Mg = sim.iteration_stats()
+        if len(ko):
+            assert np.allclose(mg, mo, rtol=1e-6, atol=1e-12), (s, mg, mo)
+    print("worst relative errors:", worst, "steps with contact:", n_contact_steps)
+    assert n_contact_steps > 100
+    assert worst["q"] < REL and worst["v"] < REL and worst["r"] < REL, worst
+
+
+def test_free_fall_closed_form(gpu):
+    """No contact, undeformed cloth: v_{n+1} = v_n + dt g, q_{n+1} = q_n + dt v_{n+1}."""
+    rest, tris = scenes.square_grid(10, 1.0, 5.0)
+    spec = scenes.SceneSpec("freefall", rest, tris, None,
+                            [scenes.ObstacleSpec("sphere", (0.0, 0.0, 0.0), radius=0.25)],
+                            np.array([[0.3]]), time_step=0.01, iterations=10)
+    sim = gpu.Simulation(spec)
+    q = rest.copy().reshape(-1, 3)
+    v = np.zeros_like(q)
+    g = np.array(spec.gravity)
+    for _ in range(20):
+        sim.step(1)
+        v = v + spec.time_step * g
+        q = q + spec.time_step * v
+    qs, vs = sim.state()
+    assert np.abs(vs.reshape(-1, 3) - v).max() < 1e-9
+    assert np.abs(qs.reshape(-1, 3) - q).max() < 1e-9
+
+
+def test_inclined_plane_closed_form(gpu):
+    """Sliding on a 45 degree plane with mu = 0.1: tangential acceleration
+    g (sin th - mu cos th) (replicability/analytic_example_data_processing.py:53-147)."""
+    mu, th = 0.1, np.deg2rad(45.0)
+    spec = scenes.inclined_plane(mu=mu)
+    sim = gpu.Simulation(spec)
+    dt = spec.time_step
+    tangent = np.array([np.cos(th), 0.0, np.sin(th)])
+    speeds = []
+    for s in range(400):
+        sim.step(1)
+        _, v = sim.state()
+        speeds.append(v.reshape(-1, 3).mean(axis=0) @ -tangent)
+    speeds = np.array(speeds)
+    # after landing the tangential speed grows linearly with slope g (sin - mu cos)
+    acc = np.polyfit(np.arange(250, 400) * dt, speeds[250:400], 1)[0]
+    expect = 9.81 * (np.sin(th) - mu * np.cos(th))
+    assert abs(acc - expect) < 1e-6 * expect + 1e-9, (acc, expect)
+    assert len(sim.contacts()) == 4
+
+
+def test_fp32_refinement_error(gpu):
+    spec = scenes.sphere_drape_small()
+    s64 = gpu.Simulation(spec)
+    s32 = gpu.Simulation(spec, precision=abi.FP32_REFINE64, refinement_sweeps=3)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    x64 = s64.solve(b)
+    x32 = s32.solve(b)
+    assert rel_err(x32, x64) < 1e-12
