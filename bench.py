@@ -1,0 +1,108 @@
+"""Benchmark: steps/s of the 40k-vertex cloth drape on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one SimulationBase::step() of the 200x200-quad drape (40,401
+vertices, 80,000 triangles, sphere R=0.5 + ground plane, mu=0.1, dt=1/200,
+20 local-global iterations) with state resident in HBM. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(spec, budget_s=20.0):
+    """The CPU oracle (reference algorithm and OpenMP structure) on a bounded sample."""
+    from oracle import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc = po.Oracle(spec)
+    orc.step(2)  # warm-up into the first steps of the same trajectory
+    t0 = time.perf_counter()
+    orc.step(1)
+    t1 = time.perf_counter() - t0
+    k = max(2, min(100, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    orc.step(k)
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "steps 4..%d of the same drape, %d OpenMP threads, factorisation excluded" % (3 + k, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", choices=["fp64", "fp32r"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from climatenet_amd import abi, native, scenes
+
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False  # self-collision detection is not in this build yet
+    prec = abi.FP64 if args.precision == "fp64" else abi.FP32_REFINE64
+    sim = native.Simulation(spec, precision=prec, refinement_sweeps=2, record_iterations=True)
+    sim.step(args.warmup)
+    t0 = time.perf_counter()
+    sim.step(args.steps)  # returns after the stream is synchronised
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+
+    # kernel times: instrumented continuation of the same run (events around every launch)
+    sim.set_kernel_timing(True)
+    sim.step(args.timing_steps)
+    kt = sim.kernel_times()
+    sim.set_kernel_timing(False)
+    info = sim.info()
+    K = spec.iterations
+    fam = {
+        "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
+        "k_forward": (kt["forward_us"], info["bytes_forward"] / max(1, info["n_levels"]), kt["forward_launches"]),
+        "k_backward": (kt["backward_us"], info["bytes_backward"] / max(1, info["n_levels"]), kt["backward_launches"]),
+    }
+    dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
+    us, bytes_per_launch, _ = fam[dom]
+    achieved = bytes_per_launch / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(us, 3), "algorithmic_bytes_per_launch": bytes_per_launch}
+
+    out = {
+        "metric": "steps/s on 40k-vertex cloth drape",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == abi.FP64 else "f32+f64-refine",
+        "data": "synthetic",
+        "config": {"workload": "drape_200x200: 40,401 vertices, sphere R=0.5 + plane, mu=0.1, dt=1/200, "
+                               "20 local-global iterations, self-collision off",
+                   "iterations": K, "n_vertices": info["n_vertices"], "nnz_L": info["nnz_L"],
+                   "nnz_F": info["nnz_F"], "levels": info["n_levels"], "contacts_last_step": info["n_contacts"]},
+        "roofline": roofline,
+        "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(spec)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -47,7 +47,7 @@ class MT19937_64:
         y = self.mt[self.index]
         self.index += 1
         y ^= (y >> 29) & 0x5555555555555555
-        y ^= (y << 17) & 0x71D67FFFEEB70000
+        y ^= (y << 17) & 0x71D67FFFEDA60000
         y ^= (y << 37) & 0xFFF7EEE000000000
         y ^= y >> 43
         return y & _MASK64
