@@ -241,7 +241,7 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
 // The friction residual of the previous iteration (r^{k-1}, v^k) is folded in.
 __global__ void k_local(DevModel M, const double* __restrict__ qn, const double* __restrict__ vcur,
                         const double* __restrict__ cvec, double* __restrict__ rhs, const int* __restrict__ ccnt,
-                        const int* __restrict__ cstart, DevContacts C, StepArgs a, int iteration) {
+                        const int* __restrict__ cstart, DevContacts C, StepArgs a, int iteration, LocalSelf ls) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
     double ac_s = 0., ac_m = 0.;
@@ -261,10 +261,11 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
             s.z += kap * e.z;
         }
         int nc = ccnt[i];
-        double damp = nc > 0 ? 0. : a.damping;
+        const int slot = ls.selfmode ? ls.vslot[i] : -1;
+        double damp = (nc > 0 || slot >= 0) ? 0. : a.damping;
         D3 c = ld3(cvec, n, i);
         D3 b = {c.x + dt * (s.x - damp * v.x), c.y + dt * (s.y - damp * v.y), c.z + dt * (s.z - damp * v.z)};
-        if (nc > 0) {
+        if (nc > 0 || slot >= 0) {
             D3 av = {0., 0., 0.};
             int cnt = M.ata_cnt[i];
             for (int k = 0; k < cnt; ++k) {
@@ -276,6 +277,13 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
             }
             const double dt2 = dt * dt;
             D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
+            D3 Fs = F;  // what the obstacle contacts see
+            if (slot >= 0) {
+                ls.Fc[3 * slot] = F.x;
+                ls.Fc[3 * slot + 1] = F.y;
+                ls.Fc[3 * slot + 2] = F.z;
+                Fs = {F.x + ls.SCFc[3 * slot], F.y + ls.SCFc[3 * slot + 1], F.z + ls.SCFc[3 * slot + 2]};
+            }
             int c0 = cstart[i];
             for (int k = 0; k < nc; ++k) {
                 int cid = c0 + k;
@@ -288,7 +296,7 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
                     ac_s += an;
                     ac_m = fmax(ac_m, an);
                 }
-                D3 fl = {dot3(bn, F), dot3(bt, F), dot3(bb, F)};
+                D3 fl = {dot3(bn, Fs), dot3(bt, Fs), dot3(bb, Fs)};
                 if (fl.x > 0.) {
                     r[0] = 0.; r[1] = 0.; r[2] = 0.;
                     C.active[cid] = 0;
@@ -302,9 +310,19 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
                 }
                 r[0] = r0; r[1] = r1; r[2] = r2;
                 C.active[cid] = 1;
-                b.x += (bn.x * r0 + bt.x * r1) + bb.x * r2;
-                b.y += (bn.y * r0 + bt.y * r1) + bb.y * r2;
-                b.z += (bn.z * r0 + bt.z * r1) + bb.z * r2;
+                D3 g = {(bn.x * r0 + bt.x * r1) + bb.x * r2, (bn.y * r0 + bt.y * r1) + bb.y * r2,
+                        (bn.z * r0 + bt.z * r1) + bb.z * r2};
+                if (slot >= 0) {
+                    // self-collision mode: the last active contact of the vertex is kept and
+                    // added after the self-contact levels
+                    ls.CFn[3 * slot] = g.x;
+                    ls.CFn[3 * slot + 1] = g.y;
+                    ls.CFn[3 * slot + 2] = g.z;
+                } else {
+                    b.x += g.x;
+                    b.y += g.y;
+                    b.z += g.z;
+                }
             }
         }
         st3(rhs, n, i, b);
@@ -508,9 +526,10 @@ void launch_detect(const DevModel& M, const double* qn, const DevObstacles& ob, 
     hipLaunchKernelGGL(k_detect_write, dim3(nb), dim3(kBlock), 0, st, M, qn, ob, cflags, ccnt, blocksum, cstart, C);
 }
 void launch_local(const DevModel& M, const double* qn, const double* vcur, const double* cvec, double* rhs,
-                  const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st) {
+                  const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,
+                  const LocalSelf& ls, hipStream_t st) {
     hipLaunchKernelGGL(k_local, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
-                       iteration);
+                       iteration, ls);
 }
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st) {

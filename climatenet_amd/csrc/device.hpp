@@ -10,6 +10,8 @@
 
 #include <cstdint>
 
+#include "host.hpp"
+
 namespace pdfc {
 
 constexpr int kMaxObstacles = 16;
@@ -78,15 +80,91 @@ struct StepArgs {
     unsigned long long* ac_max;  // per iteration, bit pattern of a non-negative double
 };
 
+// Self-collision detection buffers (scene numbering for vertices and triangles).
+struct DevSelfDetect {
+    int n, T;
+    const double* q;        // positions, supernodal planar layout
+    const int* iperm;       // scene vertex -> supernodal index
+    const int* tris;        // 3T scene vertex indices
+    double tol, tol_sq, t11;  // tol = sqrt(tol*tol) of the parameters, t11 = 1.1 tol
+    double* tbox;           // 6T: lo xyz, hi xyz
+    unsigned long long* ext_bits;  // largest box extent (bit pattern)
+    int P;                  // hash buckets (power of two)
+    int* bucket_cnt;        // P + 1
+    int* bucket_fill;       // P
+    int* bucket_off;        // P + 1
+    int* bucket_tris;       // T
+    int* cnt_full;
+    int* cnt_filt;
+    int* off_full;
+    int* off_filt;
+    int* totals;            // [0] candidates, [1] filtered candidates, [2] triangles hashed
+    int* scan_sums;
+    int* scan_sums2;
+    int* full_v;
+    int* full_t;
+    double* full_bary;
+    SelfCandidate* filt;
+    int cap_full, cap_filt;
+    int write_full;         // also write the complete candidate list
+};
+
+// Self contacts of the current step, sorted by computation level.
+struct DevSelf {
+    int nsc, nslots;
+    const int* a;           // contact vertex (supernodal index)
+    const int* b;           // paired face vertex
+    const int* sa;          // slot of a in the involved-vertex buffers
+    const int* sb;
+    const double* basis;    // 9 per contact: n, t, b
+    const double* mu;
+    double* r;              // local reaction (mass-weighted)
+    int* active;
+    double* rem;            // force applied in the previous iteration (3 per contact)
+    double* addv;           // force of this iteration
+    double* Fc;             // force estimate F at the involved vertices (3 per slot)
+    double* CFc;            // obstacle contact forces, current / next buffer
+    double* CFn;
+    double* SCFc;           // self contact forces, current / next buffer
+    double* SCFn;
+    const int* slot_vertex; // slot -> supernodal vertex
+    const int* lvl_ptr;     // contacts of level l: [lvl_ptr[l], lvl_ptr[l+1])
+    const int* grp_ptr;     // vertex groups of level l
+    const int* grp_slot;
+    const int* ent_ptr;     // entries of group g
+    const int* ent_contact;
+    const int* ent_sign;    // +1: contact vertex, -1: paired vertex
+};
+
 void launch_fill(double* p, size_t count, double value, hipStream_t st);
+void launch_fill_int(int* p, int count, int value, hipStream_t st);
+void launch_copy_int(int* dst, const int* src, int count, hipStream_t st);
+void launch_scan_int(int* a, int count, int* sums, int* total, int* tail, hipStream_t st);
+void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st);
+void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st);
+void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st);
+void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
+                       const StepArgs& a, int iteration, hipStream_t st);
+void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st);
+void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st);
+void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, size_t count, hipStream_t st);
 void launch_step_begin(const DevModel& M, const double* qn, const double* vstate, double* v0, double* cvec,
                        const StepArgs& a, hipStream_t st);
 void launch_detect(const DevModel& M, const double* qn, const DevObstacles& ob, int* cflags, int* ccnt,
                    int* cstart, int* blocksum, int* n_contacts, DevContacts C, hipStream_t st);
+// Self-mode arguments of the local kernel (selfmode = 0: obstacle reactions go
+// straight into b; 1: they go to the involved-vertex buffers, see DevSelf).
+struct LocalSelf {
+    int selfmode;
+    const int* vslot;       // n: slot or -1
+    double* Fc;
+    double* CFn;
+    const double* SCFc;
+};
 void launch_local(const DevModel& M, const double* qn, const double* vcur, const double* cvec, double* rhs,
                   const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,
-                  hipStream_t st);
+                  const LocalSelf& ls, hipStream_t st);
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st);
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.

@@ -82,20 +82,28 @@ struct SweepPlan {
 };
 void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd);
 
-// Greedy node-node pairing and level ordering of self contacts, done on the host
-// from the device's sorted candidate list. Candidates are (vertex, triangle, bary)
-// in scene numbering, sorted by (vertex, triangle).
+// One self-collision candidate that survived the exact distance test and the
+// pairing pre-filter (vertex outside the face, no zero barycentric coordinate).
+// Written by the device, read by the host pairing: layout shared with the kernels.
+struct SelfCandidate {
+    int v, t;          // scene vertex and triangle index
+    double bary[3];    // barycentric coordinates of the closest point
+    double normal[3];  // (x_v - closest point) / |.|
+};
+
 struct SelfPair {
     int v, partner;
     std::array<int, 3> face;
     int best;       // which face corner is the partner
-    V3 normal;      // (x_v - closest point) / |.|
+    V3 normal;
     int level;
 };
-void pair_self_contacts(const Model& M, const double* q, const pdfc_candidate* cand, size_t ncand,
-                        std::vector<SelfPair>& out);
-// Level of each pair (BFS from the obstacle-contact vertices, then degree-1 and
-// remaining vertices); returns the number of levels.
-int order_self_contacts(int n, const std::vector<int>& obstacle_vertices, std::vector<SelfPair>& pairs);
+// Greedy node-node pairing over candidates sorted by (vertex, triangle).
+void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out);
+// Computation levels: BFS from the obstacle-contact vertices, then from degree-1 and
+// remaining vertices in ascending order. Fills pair.level and the per-level lists
+// (contact indices in BFS order); returns the number of levels.
+int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPair>& pairs,
+                        std::vector<std::vector<int>>& levels);
 
 }  // namespace pdfc

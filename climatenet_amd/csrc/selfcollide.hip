@@ -1,0 +1,519 @@
+// Self-collision on gfx950: detection (hash grid + exact closest-point test) and
+// the level-scheduled self-contact friction of the local-global iteration.
+//
+// Detection restates getMeshPotentialTriangleCollision / checkMeshAllCollision
+// (src/geometry/geometry_util.cpp:229-327): closed boxes, vertex box x +- 1.1 tol,
+// triangle bounding box, exact test |cp - x|^2 < tol^2 with Ericson's closest
+// point (geometry_util.cpp:506-616, early exit against the plain tolerance).
+// Candidate lists come out sorted by (vertex, triangle) in scene numbering and
+// match the CPU oracle bit for bit (-ffp-contract=off, correctly rounded div/sqrt).
+//
+// Friction restates SimulationFrictionEstimation::computeRhs self-contact levels
+// (src/physics/SimulationFrictionEstimation.cpp:431-587) with deterministic
+// per-vertex accumulation in the reference's serial order.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace pdfc {
+namespace {
+
+constexpr int kB = 256;
+
+struct P3 {
+    double x, y, z;
+};
+__device__ __forceinline__ P3 at_perm(const double* q, int n, int p) { return {q[p], q[n + p], q[2 * n + p]}; }
+__device__ __forceinline__ P3 psub(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ P3 padd(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ P3 pscale(double s, P3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double pdot(P3 a, P3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+__device__ __forceinline__ double cell_size(const DevSelfDetect& D) {
+    double E = __longlong_as_double((long long)*D.ext_bits);
+    double c = E * (1. + 1e-9) + 2. * D.t11 * (1. + 1e-9);
+    return c > 1e-12 ? c : 1e-12;
+}
+__device__ __forceinline__ long long cell_of(double x, double c) { return (long long)floor(x / c); }
+__device__ __forceinline__ unsigned cell_hash(long long ix, long long iy, long long iz, int P) {
+    unsigned h = ((unsigned)ix * 73856093u) ^ ((unsigned)iy * 19349663u) ^ ((unsigned)iz * 83492791u);
+    return h & (unsigned)(P - 1);
+}
+
+// Ericson closest point, barycentric output (same operation order as the CPU oracle).
+__device__ void closest_point(P3 p, P3 a, P3 b, P3 c, double tol2, double* bary, P3& cp) {
+    P3 ap = psub(p, a), ab = psub(b, a), ac = psub(c, a);
+    if ((pdot(ap, ap) - pdot(ab, ab)) - pdot(ac, ac) > tol2) { cp = a; bary[0] = 1; bary[1] = 0; bary[2] = 0; return; }
+    double d1 = pdot(ab, ap), d2 = pdot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { cp = a; bary[0] = 1; bary[1] = 0; bary[2] = 0; return; }
+    P3 bp = psub(p, b);
+    double d3 = pdot(ab, bp), d4 = pdot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { cp = b; bary[0] = 0; bary[1] = 1; bary[2] = 0; return; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        cp = padd(a, pscale(v, ab)); bary[0] = 1 - v; bary[1] = v; bary[2] = 0; return;
+    }
+    P3 cpv = psub(p, c);
+    double d5 = pdot(ab, cpv), d6 = pdot(ac, cpv);
+    if (d6 >= 0.0 && d5 <= d6) { cp = c; bary[0] = 0; bary[1] = 0; bary[2] = 1; return; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        cp = padd(a, pscale(w, ac)); bary[0] = 1 - w; bary[1] = 0; bary[2] = w; return;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        cp = padd(b, pscale(w, psub(c, b))); bary[0] = 0; bary[1] = 1 - w; bary[2] = w; return;
+    }
+    double denom = 1.0 / ((va + vb) + vc);
+    double v = vb * denom, w = vc * denom;
+    cp = padd(padd(a, pscale(v, ab)), pscale(w, ac));
+    bary[0] = (1. - v) - w; bary[1] = v; bary[2] = w;
+}
+
+__global__ void k_fill_int(int* p, int count, int value) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+__global__ void k_copy_int(int* dst, const int* src, int count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[i];
+}
+
+// Triangle bounding boxes and the largest box extent (for the cell size).
+__global__ void k_tri_box(DevSelfDetect D) {
+    __shared__ double red[kB / 64];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double ext = 0.;
+    if (t < D.T) {
+        P3 a = at_perm(D.q, D.n, D.iperm[D.tris[3 * t]]);
+        P3 b = at_perm(D.q, D.n, D.iperm[D.tris[3 * t + 1]]);
+        P3 c = at_perm(D.q, D.n, D.iperm[D.tris[3 * t + 2]]);
+        double lo[3] = {fmin(fmin(a.x, b.x), c.x), fmin(fmin(a.y, b.y), c.y), fmin(fmin(a.z, b.z), c.z)};
+        double hi[3] = {fmax(fmax(a.x, b.x), c.x), fmax(fmax(a.y, b.y), c.y), fmax(fmax(a.z, b.z), c.z)};
+        for (int k = 0; k < 3; ++k) {
+            D.tbox[6 * t + k] = lo[k];
+            D.tbox[6 * t + 3 + k] = hi[k];
+            ext = fmax(ext, hi[k] - lo[k]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) ext = fmax(ext, __shfl_xor(ext, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ext;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
+        atomicMax(D.ext_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_tri_count(DevSelfDetect D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.T) return;
+    double c = cell_size(D);
+    unsigned h = cell_hash(cell_of(D.tbox[6 * t], c), cell_of(D.tbox[6 * t + 1], c), cell_of(D.tbox[6 * t + 2], c), D.P);
+    atomicAdd(&D.bucket_cnt[h], 1);
+}
+
+__global__ void k_tri_scatter(DevSelfDetect D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.T) return;
+    double c = cell_size(D);
+    unsigned h = cell_hash(cell_of(D.tbox[6 * t], c), cell_of(D.tbox[6 * t + 1], c), cell_of(D.tbox[6 * t + 2], c), D.P);
+    int slot = atomicAdd(&D.bucket_fill[h], 1);
+    D.bucket_tris[D.bucket_off[h] + slot] = t;
+}
+
+// pass 0: count survivors per vertex; pass 1: write them (sorted by triangle).
+__global__ void k_vertex_query(DevSelfDetect D, int pass) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= D.n) return;
+    const double c = cell_size(D);
+    const double E = __longlong_as_double((long long)*D.ext_bits) * (1. + 1e-9);
+    P3 x = at_perm(D.q, D.n, D.iperm[v]);
+    const double lo[3] = {x.x - D.t11, x.y - D.t11, x.z - D.t11};
+    const double hi[3] = {x.x + D.t11, x.y + D.t11, x.z + D.t11};
+    long long c0[3], c1[3];
+    bool sane = isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+    for (int k = 0; k < 3; ++k) {
+        c0[k] = sane ? cell_of(lo[k] - E, c) : 0;
+        c1[k] = sane ? cell_of(hi[k], c) : -1;
+        // the query range is never longer than one cell, so it spans at most 3 cells
+        if (c1[k] - c0[k] > 2) c1[k] = c0[k] - 1;
+    }
+    int nfull = 0, nfilt = 0;
+    int wf = 0, wp = 0;
+    const int base_f = pass ? D.off_full[v] : 0;
+    const int base_p = pass ? D.off_filt[v] : 0;
+    for (long long ix = c0[0]; ix <= c1[0]; ++ix)
+        for (long long iy = c0[1]; iy <= c1[1]; ++iy)
+            for (long long iz = c0[2]; iz <= c1[2]; ++iz) {
+                unsigned h = cell_hash(ix, iy, iz, D.P);
+                int b0 = D.bucket_off[h], b1 = D.bucket_off[h + 1];
+                for (int q = b0; q < b1; ++q) {
+                    int t = D.bucket_tris[q];
+                    const double* bx = &D.tbox[6 * t];
+                    if (cell_of(bx[0], c) != ix || cell_of(bx[1], c) != iy || cell_of(bx[2], c) != iz) continue;
+                    if (!(bx[0] <= hi[0] && lo[0] <= bx[3] && bx[1] <= hi[1] && lo[1] <= bx[4] && bx[2] <= hi[2] &&
+                          lo[2] <= bx[5]))
+                        continue;
+                    int ia = D.tris[3 * t], ib = D.tris[3 * t + 1], ic = D.tris[3 * t + 2];
+                    P3 a = at_perm(D.q, D.n, D.iperm[ia]);
+                    P3 b = at_perm(D.q, D.n, D.iperm[ib]);
+                    P3 cc = at_perm(D.q, D.n, D.iperm[ic]);
+                    double bary[3];
+                    P3 cp;
+                    closest_point(x, a, b, cc, D.tol, bary, cp);
+                    P3 d = psub(cp, x);
+                    if (!(pdot(d, d) < D.tol_sq)) continue;
+                    bool keep = !(ia == v || ib == v || ic == v || bary[0] == 0 || bary[1] == 0 || bary[2] == 0);
+                    if (pass == 0) {
+                        ++nfull;
+                        nfilt += keep;
+                        continue;
+                    }
+                    if (D.write_full) {
+                        int k = base_f + wf++;
+                        if (k < D.cap_full) {
+                            D.full_v[k] = v;
+                            D.full_t[k] = t;
+                            D.full_bary[3 * k] = bary[0];
+                            D.full_bary[3 * k + 1] = bary[1];
+                            D.full_bary[3 * k + 2] = bary[2];
+                        }
+                    }
+                    if (keep) {
+                        int k = base_p + wp++;
+                        if (k < D.cap_filt) {
+                            SelfCandidate& o = D.filt[k];
+                            o.v = v;
+                            o.t = t;
+                            o.bary[0] = bary[0];
+                            o.bary[1] = bary[1];
+                            o.bary[2] = bary[2];
+                            // normal = (x - cp) / |x - cp| with cp = a w0 + b w1 + c w2
+                            P3 cpb = {(a.x * bary[0] + b.x * bary[1]) + cc.x * bary[2],
+                                      (a.y * bary[0] + b.y * bary[1]) + cc.y * bary[2],
+                                      (a.z * bary[0] + b.z * bary[1]) + cc.z * bary[2]};
+                            P3 nv = psub(x, cpb);
+                            double z = pdot(nv, nv);
+                            if (z > 0.) {
+                                double s = sqrt(z);
+                                nv = {nv.x / s, nv.y / s, nv.z / s};
+                            }
+                            o.normal[0] = nv.x;
+                            o.normal[1] = nv.y;
+                            o.normal[2] = nv.z;
+                        }
+                    }
+                }
+            }
+    if (pass == 0) {
+        D.cnt_full[v] = nfull;
+        D.cnt_filt[v] = nfilt;
+        return;
+    }
+    // insertion sort of this vertex's segments by triangle index
+    if (D.write_full && base_f + wf <= D.cap_full) {
+        for (int i = base_f + 1; i < base_f + wf; ++i) {
+            int t = D.full_t[i];
+            double b0 = D.full_bary[3 * i], b1 = D.full_bary[3 * i + 1], b2 = D.full_bary[3 * i + 2];
+            int j = i - 1;
+            while (j >= base_f && D.full_t[j] > t) {
+                D.full_t[j + 1] = D.full_t[j];
+                D.full_bary[3 * (j + 1)] = D.full_bary[3 * j];
+                D.full_bary[3 * (j + 1) + 1] = D.full_bary[3 * j + 1];
+                D.full_bary[3 * (j + 1) + 2] = D.full_bary[3 * j + 2];
+                --j;
+            }
+            D.full_t[j + 1] = t;
+            D.full_bary[3 * (j + 1)] = b0;
+            D.full_bary[3 * (j + 1) + 1] = b1;
+            D.full_bary[3 * (j + 1) + 2] = b2;
+        }
+    }
+    if (base_p + wp <= D.cap_filt) {
+        for (int i = base_p + 1; i < base_p + wp; ++i) {
+            SelfCandidate key = D.filt[i];
+            int j = i - 1;
+            while (j >= base_p && D.filt[j].t > key.t) {
+                D.filt[j + 1] = D.filt[j];
+                --j;
+            }
+            D.filt[j + 1] = key;
+        }
+    }
+}
+
+// Bitset (permuted numbering) of vertices with at least one obstacle contact.
+__global__ void k_seed_bits(const int* ccnt, int n, unsigned long long* bits) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool on = i < n && ccnt[i] > 0;
+    unsigned long long m = __ballot(on);
+    if ((threadIdx.x & 63) == 0 && i < n) bits[i >> 6] = m;
+}
+
+// Block-wise exclusive scan in place; per-block totals into sums.
+__global__ void k_scan_local(int* a, int count, int* sums) {
+    __shared__ int buf[kB];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = i < count ? a[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kB; off <<= 1) {
+        int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+        __syncthreads();
+        buf[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < count) a[i] = buf[threadIdx.x] - v;
+    if (threadIdx.x == kB - 1) sums[blockIdx.x] = buf[kB - 1];
+}
+__global__ void k_scan_sums(int* sums, int nblocks, int* total) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblocks; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < nblocks ? sums[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks) sums[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+__global__ void k_scan_add(int* a, int count, const int* sums, const int* total, int* tail) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] += sums[blockIdx.x];
+    if (tail && i == 0) *tail = *total;
+}
+
+// ---------------------------------------------------------------- friction
+__device__ __forceinline__ P3 ld_slot(const double* A, int s) { return {A[3 * s], A[3 * s + 1], A[3 * s + 2]}; }
+__device__ __forceinline__ void st_slot(double* A, int s, P3 v) {
+    A[3 * s] = v.x;
+    A[3 * s + 1] = v.y;
+    A[3 * s + 2] = v.z;
+}
+
+__device__ double ac_norm3(P3 u, const double* r, double mu) {
+    double an = fmax(0., r[0] - u.x) - r[0];
+    double d1 = r[1] - u.y, d2 = r[2] - u.z;
+    double dn = sqrt(d1 * d1 + d2 * d2);
+    double s = fmin(1., fmax(0., mu * r[0] / dn));
+    double a1 = d1 * s - r[1], a2 = d2 * s - r[2];
+    return sqrt((an * an + a1 * a1) + a2 * a2);
+}
+
+__device__ void ac_block(double s, double m, double* sum_slot, unsigned long long* max_slot) {
+    __shared__ double rs[kB / 64], rm[kB / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        m = fmax(m, __shfl_xor(m, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        rs[threadIdx.x >> 6] = s;
+        rm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0., tm = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            ts += rs[k];
+            tm = fmax(tm, rm[k]);
+        }
+        if (ts != 0.) atomicAdd(sum_slot, ts);
+        if (tm > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(tm));
+    }
+}
+
+// One level: new reaction of every self contact of the level from the state at the
+// level's start (computed-then-applied, as in the reference's two parallel loops).
+__global__ void k_self_level(DevSelf S, int l, const double* __restrict__ v, int n, const double* __restrict__ mass,
+                             StepArgs a, int iteration) {
+    int s = S.lvl_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
+    double acs = 0., acm = 0.;
+    if (s < S.lvl_ptr[l + 1]) {
+        const double* B = &S.basis[9 * s];
+        P3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+        double* r = &S.r[3 * s];
+        const int pa = S.a[s], pb = S.b[s];
+        if (a.record && iteration > 0) {
+            P3 u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
+            acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, r, S.mu[s]);
+            acm = acs;
+        }
+        const int sa = S.sa[s], sb = S.sb[s];
+        P3 rem = ld_slot(S.rem, s);
+        P3 fA = psub(padd(padd(padd(ld_slot(S.Fc, sa), ld_slot(S.CFn, sa)), ld_slot(S.SCFc, sa)), ld_slot(S.SCFn, sa)), rem);
+        P3 fB = padd(padd(padd(padd(ld_slot(S.Fc, sb), ld_slot(S.CFn, sb)), ld_slot(S.SCFc, sb)), ld_slot(S.SCFn, sb)), rem);
+        const double mA = mass[pa], mB = mass[pb];
+        P3 f = psub(pscale(1. / mA, fA), pscale(1. / mB, fB));
+        P3 fl = {pdot(bn, f), pdot(bt, f), pdot(bb, f)};
+        P3 add = {0., 0., 0.};
+        if (fl.x > 0.) {
+            r[0] = 0.; r[1] = 0.; r[2] = 0.;
+            S.active[s] = 0;
+        } else {
+            double mu = S.mu[s];
+            double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
+            double rt = sqrt(r1 * r1 + r2 * r2);
+            if (rt > mu * r0) {
+                r1 *= mu * r0 / rt;
+                r2 *= mu * r0 / rt;
+            }
+            double w = (mA * mB) / (mA + mB);
+            r0 *= w; r1 *= w; r2 *= w;
+            r[0] = r0; r[1] = r1; r[2] = r2;
+            S.active[s] = 1;
+            add = {(bn.x * r0 + bt.x * r1) + bb.x * r2, (bn.y * r0 + bt.y * r1) + bb.y * r2,
+                   (bn.z * r0 + bt.z * r1) + bb.z * r2};
+        }
+        st_slot(S.addv, s, add);
+    }
+    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+}
+
+// Apply one level: per touched vertex, in the level's contact order:
+//   SCF_cur -= sign * previous force, SCF_next += sign * new force.
+__global__ void k_self_apply(DevSelf S, int l) {
+    int g = S.grp_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.grp_ptr[l + 1]) return;
+    const int slot = S.grp_slot[g];
+    P3 cur = ld_slot(S.SCFc, slot), nxt = ld_slot(S.SCFn, slot);
+    for (int e = S.ent_ptr[g]; e < S.ent_ptr[g + 1]; ++e) {
+        int s = S.ent_contact[e];
+        P3 prev = ld_slot(S.rem, s), add = ld_slot(S.addv, s);
+        if (S.ent_sign[e] > 0) {
+            cur = psub(cur, prev);
+            nxt = padd(nxt, add);
+        } else {
+            cur = padd(cur, prev);
+            nxt = psub(nxt, add);
+        }
+    }
+    st_slot(S.SCFc, slot, cur);
+    st_slot(S.SCFn, slot, nxt);
+}
+
+// b += CF_next + SCF_next at the involved vertices; clear the buffers that become
+// "next" after the swap.
+__global__ void k_self_finalize(DevSelf S, double* __restrict__ rhs, int n) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S.nslots) return;
+    int v = S.slot_vertex[s];
+    P3 c = ld_slot(S.CFn, s), sc = ld_slot(S.SCFn, s);
+    rhs[v] += c.x + sc.x;
+    rhs[n + v] += c.y + sc.y;
+    rhs[2 * n + v] += c.z + sc.z;
+    st_slot(S.CFc, s, P3{0., 0., 0.});
+    st_slot(S.SCFc, s, P3{0., 0., 0.});
+}
+
+__global__ void k_self_ac_final(DevSelf S, const double* __restrict__ v, int n, StepArgs a, int iteration) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double acs = 0.;
+    if (s < S.nsc) {
+        const double* B = &S.basis[9 * s];
+        P3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+        P3 u = psub(at_perm(v, n, S.a[s]), at_perm(v, n, S.b[s]));
+        acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, &S.r[3 * s], S.mu[s]);
+    }
+    ac_block(acs, acs, &a.ac_sum[iteration], &a.ac_max[iteration]);
+}
+
+// Start of step: slot map on, buffers zeroed. End of step: slot map off.
+__global__ void k_self_prepare(DevSelf S, int* vslot) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S.nslots) {
+        vslot[S.slot_vertex[s]] = s;
+        st_slot(S.CFc, s, P3{0., 0., 0.});
+        st_slot(S.CFn, s, P3{0., 0., 0.});
+        st_slot(S.SCFc, s, P3{0., 0., 0.});
+        st_slot(S.SCFn, s, P3{0., 0., 0.});
+        st_slot(S.Fc, s, P3{0., 0., 0.});
+    }
+    if (s < S.nsc) {
+        st_slot(S.rem, s, P3{0., 0., 0.});
+        st_slot(S.addv, s, P3{0., 0., 0.});
+        S.r[3 * s] = 0.; S.r[3 * s + 1] = 0.; S.r[3 * s + 2] = 0.;
+        S.active[s] = 0;
+    }
+}
+__global__ void k_self_release(DevSelf S, int* vslot) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S.nslots) vslot[S.slot_vertex[s]] = -1;
+}
+
+inline unsigned nblk(long long c) { return (unsigned)((c + kB - 1) / kB); }
+
+}  // namespace
+
+void launch_fill_int(int* p, int count, int value, hipStream_t st) {
+    if (count > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(count)), dim3(kB), 0, st, p, count, value);
+}
+
+void launch_copy_int(int* dst, const int* src, int count, hipStream_t st) {
+    if (count > 0) hipLaunchKernelGGL(k_copy_int, dim3(nblk(count)), dim3(kB), 0, st, dst, src, count);
+}
+
+void launch_scan_int(int* a, int count, int* sums, int* total, int* tail, hipStream_t st) {
+    unsigned nb = nblk(count);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kB), 0, st, a, count, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, (int)nb, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kB), 0, st, a, count, sums, total, tail);
+}
+
+void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, reinterpret_cast<int*>(D.ext_bits), 2, 0);
+    hipLaunchKernelGGL(k_tri_box, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    launch_fill_int(D.bucket_cnt, D.P + 1, 0, st);
+    launch_fill_int(D.bucket_fill, D.P, 0, st);
+    hipLaunchKernelGGL(k_tri_count, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    // bucket_off = exclusive scan of bucket_cnt (P+1 entries, the last one becomes T)
+    launch_copy_int(D.bucket_off, D.bucket_cnt, D.P + 1, st);
+    launch_scan_int(D.bucket_off, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
+    hipLaunchKernelGGL(k_tri_scatter, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    hipLaunchKernelGGL(k_vertex_query, dim3(nblk(D.n)), dim3(kB), 0, st, D, 0);
+    launch_copy_int(D.off_full, D.cnt_full, D.n, st);
+    launch_copy_int(D.off_filt, D.cnt_filt, D.n, st);
+    launch_scan_int(D.off_full, D.n, D.scan_sums, D.totals + 0, nullptr, st);
+    launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
+    hipLaunchKernelGGL(k_vertex_query, dim3(nblk(D.n)), dim3(kB), 0, st, D, 1);
+    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits);
+}
+
+void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st) {
+    int c = S.nslots > S.nsc ? S.nslots : S.nsc;
+    if (c > 0) hipLaunchKernelGGL(k_self_prepare, dim3(nblk(c)), dim3(kB), 0, st, S, vslot);
+}
+void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st) {
+    if (S.nslots > 0) hipLaunchKernelGGL(k_self_release, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, vslot);
+}
+void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
+                       const StepArgs& a, int iteration, hipStream_t st) {
+    if (count > 0) hipLaunchKernelGGL(k_self_level, dim3(nblk(count)), dim3(kB), 0, st, S, l, v, n, mass, a, iteration);
+}
+void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st) {
+    if (groups > 0) hipLaunchKernelGGL(k_self_apply, dim3(nblk(groups)), dim3(kB), 0, st, S, l);
+}
+void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st) {
+    if (S.nslots > 0) hipLaunchKernelGGL(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);
+}
+void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st) {
+    if (S.nsc > 0) hipLaunchKernelGGL(k_self_ac_final, dim3(nblk(S.nsc)), dim3(kB), 0, st, S, v, n, a, iteration);
+}
+
+}  // namespace pdfc

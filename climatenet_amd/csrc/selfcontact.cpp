@@ -1,0 +1,121 @@
+// Host half of self-collision handling, run once per step on the device's
+// filtered candidate list:
+//   greedy node-node pairing   src/geometry/geometry_util.cpp:94-206
+//   computation levels (BFS)   src/physics/SimulationFrictionEstimation.cpp:161-293
+// Both are order-dependent and sequential in the reference.
+#include <algorithm>
+
+#include "host.hpp"
+
+namespace pdfc {
+namespace {
+
+// Position in the face of the first face vertex that appears in `list`, 3 if none
+// (isTriangleVertexIndexIn, geometry_util.cpp:213-227).
+int first_face_vertex_in(const std::array<int, 3>& face, const std::vector<int>& list) {
+    for (int k = 0; k < 3; ++k)
+        if (std::find(list.begin(), list.end(), face[k]) != list.end()) return k;
+    return 3;
+}
+
+}  // namespace
+
+void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out) {
+    out.clear();
+    // registered partners of each vertex touched in this step
+    std::vector<std::vector<int>> registered(M.n);
+    std::vector<int> touched;
+    for (size_t k = 0; k < ncand; ++k) {
+        const SelfCandidate& c = cand[k];
+        const int v = c.v;
+        const std::array<int, 3>& face = M.tris[c.t];
+        // the device list holds only candidates with v outside the face and no zero coordinate
+        if (first_face_vertex_in(face, registered[v]) < 3) continue;
+        bool ok[3] = {true, true, true};
+        for (int u : M.neighbors[v]) {
+            int j = first_face_vertex_in(face, registered[u]);
+            if (j < 3) ok[j] = false;
+        }
+        int best = 3;
+        double bv = 0.;
+        for (int j = 0; j < 3; ++j)
+            if (ok[j] && c.bary[j] > bv) {
+                best = j;
+                bv = c.bary[j];
+            }
+        if (best >= 3) continue;
+        const int partner = face[best];
+        if (registered[partner].empty()) touched.push_back(partner);
+        if (registered[v].empty()) touched.push_back(v);
+        registered[partner].push_back(v);
+        registered[v].push_back(partner);
+        SelfPair p;
+        p.v = v;
+        p.partner = partner;
+        p.face = face;
+        p.best = best;
+        p.normal = {c.normal[0], c.normal[1], c.normal[2]};
+        p.level = -1;
+        out.push_back(p);
+    }
+}
+
+int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPair>& pairs,
+                        std::vector<std::vector<int>>& levels) {
+    levels.clear();
+    const int ns = (int)pairs.size();
+    if (ns == 0) return 0;
+    // adjacency of the self-contact graph, only for vertices that appear in it
+    std::vector<int> deg(n, 0);
+    for (const auto& p : pairs) {
+        deg[p.v]++;
+        deg[p.partner]++;
+    }
+    std::vector<int> ptr(n + 1, 0);
+    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + deg[i];
+    std::vector<std::pair<int, int>> adj(ptr[n]);
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int s = 0; s < ns; ++s) {
+            adj[fill[pairs[s].v]++] = {pairs[s].partner, s};
+            adj[fill[pairs[s].partner]++] = {pairs[s].v, s};
+        }
+    }
+    std::vector<char> visited(ns, 0);
+    std::vector<int> last, next;
+    auto bfs = [&](std::vector<int> start) {
+        size_t lvl = 0;
+        last.swap(start);
+        while (!last.empty()) {
+            if (levels.size() <= lvl) levels.emplace_back();
+            next.clear();
+            for (int u : last)
+                for (int q = ptr[u]; q < ptr[u + 1]; ++q) {
+                    int s = adj[q].second;
+                    if (visited[s]) continue;
+                    visited[s] = 1;
+                    next.push_back(adj[q].first);
+                    levels[lvl].push_back(s);
+                }
+            last.swap(next);
+            ++lvl;
+        }
+    };
+    auto all_visited = [&](int v) {
+        for (int q = ptr[v]; q < ptr[v + 1]; ++q)
+            if (!visited[adj[q].second]) return false;
+        return true;
+    };
+    bfs(seeds);
+    for (int v = 0; v < n; ++v)
+        if (deg[v] == 1 && !all_visited(v)) bfs({v});
+    for (int v = 0; v < n; ++v)
+        if (deg[v] > 1 && !all_visited(v)) bfs({v});
+    // drop empty trailing levels produced by the last BFS frontier
+    while (!levels.empty() && levels.back().empty()) levels.pop_back();
+    for (size_t l = 0; l < levels.size(); ++l)
+        for (int s : levels[l]) pairs[s].level = (int)l;
+    return (int)levels.size();
+}
+
+}  // namespace pdfc

@@ -78,6 +78,41 @@ struct pdfc_sim {
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
 
+    // ---- self-collision (per-step detection on the device, pairing on the host)
+    bool self_on = false;
+    double self_mu = 0.;
+    pdfc::DevSelfDetect sd{};
+    unsigned long long* seed_bits = nullptr;
+    int* vslot = nullptr;
+    std::vector<int> host_totals = std::vector<int>(3, 0);
+    std::vector<unsigned long long> host_bits;
+    std::vector<pdfc::SelfCandidate> host_filt;
+    std::vector<pdfc::SelfPair> pairs;             // pairing order
+    std::vector<std::vector<int>> levels;          // pair indices per level
+    std::vector<int> sorted_pair;                  // device order -> pair index
+    std::vector<int> up_int;                       // staging of the per-step upload
+    std::vector<double> up_dbl;
+    int* d_self_int = nullptr;
+    double* d_self_dbl = nullptr;
+    size_t cap_self_int = 0, cap_self_dbl = 0;
+    double* d_self_work = nullptr;                 // r, rem, addv (per contact) + Fc, CF x2, SCF x2 (per slot)
+    int* d_self_active = nullptr;
+    size_t cap_self_contacts = 0, cap_self_slots = 0;
+    pdfc::DevSelf ds{};
+    int nsc = 0, n_self_levels = 0;
+    std::vector<int> lvl_counts, grp_counts;       // per level
+    bool keep_full_candidates = false;
+    int n_full_candidates = 0;
+
+    void free_ptr(void* p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(p);
+    }
+    void self_detect_and_pair(bool write_full);
+    void self_upload();
+
     template <typename T>
     T* dalloc(size_t count) {
         void* p = nullptr;
@@ -136,7 +171,8 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     if (sc.n_obstacles > (size_t)pdfc::kMaxObstacles) throw std::invalid_argument("too many obstacles (max 16)");
     if (sc.n_materials == 0 || !sc.friction) throw std::invalid_argument("friction table is empty");
     if (sc.material < 0 || sc.material >= (int)sc.n_materials) throw std::invalid_argument("mesh material out of range");
-    if (p.self_collision) throw std::invalid_argument("self-collision is not available in this build");
+    if (p.self_collision && !(p.self_collision_tolerance >= 0.))
+        throw std::invalid_argument("self-collision tolerance must be non-negative");
     prm = p;
     fp32 = p.precision == PDFC_FP32_REFINE64;
     n_mat = (int)sc.n_materials;
@@ -298,6 +334,46 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     ac_max = dalloc<unsigned long long>(p.local_global_iterations);
     HIP_CHECK(hipMemsetAsync(ccnt, 0, n * sizeof(int), stream));
     HIP_CHECK(hipMemsetAsync(ncontacts, 0, sizeof(int), stream));
+    self_on = p.self_collision != 0;
+    if (self_on) {
+        self_mu = friction[(size_t)mesh_material * n_mat + mesh_material];
+        const int T = model.T;
+        std::vector<int> tri_flat(3 * (size_t)T);
+        for (int t = 0; t < T; ++t)
+            for (int k = 0; k < 3; ++k) tri_flat[3 * (size_t)t + k] = model.tris[t][k];
+        sd.n = n;
+        sd.T = T;
+        sd.iperm = upload(fac.iperm);
+        sd.tris = upload(tri_flat);
+        // PhysicScene::getSelfCollisionsInfo passes sqrt(tol^2) (PhysicScene.cpp:183-202)
+        sd.tol = std::sqrt(p.self_collision_tolerance * p.self_collision_tolerance);
+        sd.tol_sq = sd.tol * sd.tol;
+        sd.t11 = 1.1 * sd.tol;
+        sd.tbox = dalloc<double>(6 * (size_t)T);
+        sd.ext_bits = dalloc<unsigned long long>(1);
+        int P = 1;
+        while (P < 2 * T) P <<= 1;
+        sd.P = P;
+        sd.bucket_cnt = dalloc<int>(P + 1);
+        sd.bucket_fill = dalloc<int>(P);
+        sd.bucket_off = dalloc<int>(P + 1);
+        sd.bucket_tris = dalloc<int>(T);
+        sd.cnt_full = dalloc<int>(n);
+        sd.cnt_filt = dalloc<int>(n);
+        sd.off_full = dalloc<int>(n);
+        sd.off_filt = dalloc<int>(n);
+        sd.totals = dalloc<int>(3);
+        int nbl = (std::max(P + 1, n) + 255) / 256 + 1;
+        sd.scan_sums = dalloc<int>(nbl);
+        sd.scan_sums2 = dalloc<int>(nbl);
+        sd.cap_full = 0;  // allocated on demand (pdfc_detect)
+        sd.cap_filt = std::max(1024, n);
+        sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
+        seed_bits = dalloc<unsigned long long>(n / 64 + 1);
+        host_bits.resize(n / 64 + 1);
+        vslot = dalloc<int>(n);
+        pdfc::launch_fill_int(vslot, n, -1, stream);
+    }
     std::vector<double> q0(sc.positions ? sc.positions : sc.rest_positions,
                            (sc.positions ? sc.positions : sc.rest_positions) + n3);
     to_device_vec(q0.data(), qn);
@@ -357,6 +433,203 @@ void pdfc_sim::detect() {
     pdfc::launch_detect(dm, qn, dob, cflags, ccnt, cstart, blocksum, ncontacts, dc, stream);
 }
 
+// Device detection, host pairing and level ordering, upload of the step's self
+// contacts (geometry_util.cpp:94-206, SimulationFrictionEstimation.cpp:107-293).
+void pdfc_sim::self_detect_and_pair(bool write_full) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        sd.q = qn;
+        sd.write_full = write_full ? 1 : 0;
+        pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
+        HIP_CHECK(hipMemcpyAsync(host_totals.data(), sd.totals, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(host_bits.data(), seed_bits, host_bits.size() * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        bool grow = false;
+        if (host_totals[1] > sd.cap_filt) {
+            free_ptr(sd.filt);
+            sd.cap_filt = host_totals[1] + host_totals[1] / 2 + 1024;
+            sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
+            grow = true;
+        }
+        if (write_full && host_totals[0] > sd.cap_full) {
+            free_ptr(sd.full_v);
+            free_ptr(sd.full_t);
+            free_ptr(sd.full_bary);
+            sd.cap_full = host_totals[0] + host_totals[0] / 4 + 1024;
+            sd.full_v = dalloc<int>(sd.cap_full);
+            sd.full_t = dalloc<int>(sd.cap_full);
+            sd.full_bary = dalloc<double>(3 * (size_t)sd.cap_full);
+            grow = true;
+        }
+        if (!grow) break;
+        if (attempt == 2) throw DeviceError("self-collision candidate buffers could not be sized");
+    }
+    n_full_candidates = host_totals[0];
+    const int nf = host_totals[1];
+    host_filt.resize(nf);
+    if (nf > 0) {
+        HIP_CHECK(hipMemcpyAsync(host_filt.data(), sd.filt, nf * sizeof(pdfc::SelfCandidate), hipMemcpyDeviceToHost,
+                                 stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    pdfc::pair_self_contacts(model, host_filt.data(), host_filt.size(), pairs);
+    std::vector<int> seeds;
+    for (int i = 0; i < n; ++i)
+        if (host_bits[i >> 6] >> (i & 63) & 1ull) seeds.push_back(fac.perm[i]);
+    n_self_levels = pdfc::order_self_contacts(n, seeds, pairs, levels);
+    nsc = (int)pairs.size();
+    if (nsc > 0) self_upload();
+}
+
+// Packs the self contacts sorted by level, the involved-vertex slots and the
+// per-level apply groups, and uploads them in one copy per type.
+void pdfc_sim::self_upload() {
+    const int L = (int)levels.size();
+    sorted_pair.clear();
+    for (const auto& lv : levels) sorted_pair.insert(sorted_pair.end(), lv.begin(), lv.end());
+    // involved vertices: obstacle-contact vertices, then contact and paired vertices
+    std::vector<int> slot_vertex;
+    std::vector<int> slot_of(n, -1);
+    for (int i = 0; i < n; ++i)
+        if (host_bits[i >> 6] >> (i & 63) & 1ull) {
+            slot_of[i] = (int)slot_vertex.size();
+            slot_vertex.push_back(i);
+        }
+    std::vector<int> A(nsc), B(nsc), SA(nsc), SB(nsc);
+    std::vector<double> basis(9 * (size_t)nsc), mu(nsc, self_mu);
+    for (int j = 0; j < nsc; ++j) {
+        const pdfc::SelfPair& p = pairs[sorted_pair[j]];
+        A[j] = fac.iperm[p.v];
+        B[j] = fac.iperm[p.partner];
+        for (int x : {A[j], B[j]})
+            if (slot_of[x] < 0) {
+                slot_of[x] = (int)slot_vertex.size();
+                slot_vertex.push_back(x);
+            }
+        SA[j] = slot_of[A[j]];
+        SB[j] = slot_of[B[j]];
+        // computeBasis (SimulationFrictionEstimation.cpp:89-105)
+        const pdfc::V3 nr = p.normal;
+        auto cross = [](pdfc::V3 a, pdfc::V3 b) {
+            return pdfc::V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+        };
+        auto nrm = [](pdfc::V3 a) {
+            double z = (a.x * a.x + a.y * a.y) + a.z * a.z;
+            if (z > 0.) {
+                double q = std::sqrt(z);
+                return pdfc::V3{a.x / q, a.y / q, a.z / q};
+            }
+            return a;
+        };
+        pdfc::V3 t = cross(nr, pdfc::V3{1., 0., 0.});
+        if (std::sqrt((t.x * t.x + t.y * t.y) + t.z * t.z) < 1.e-15) t = cross(nr, pdfc::V3{0., 1., 0.});
+        t = nrm(t);
+        pdfc::V3 b = nrm(cross(nr, t));
+        double* Bj = &basis[9 * (size_t)j];
+        Bj[0] = nr.x; Bj[1] = nr.y; Bj[2] = nr.z;
+        Bj[3] = t.x;  Bj[4] = t.y;  Bj[5] = t.z;
+        Bj[6] = b.x;  Bj[7] = b.y;  Bj[8] = b.z;
+    }
+    const int nslots = (int)slot_vertex.size();
+    // per-level groups: each touched slot once, entries in the level's contact order
+    std::vector<int> lvl_ptr(L + 1, 0), grp_ptr(L + 1, 0), grp_slot, ent_ptr(1, 0), ent_contact, ent_sign;
+    std::vector<int> gidx(nslots, -1);
+    lvl_counts.assign(L, 0);
+    grp_counts.assign(L, 0);
+    int j0 = 0;
+    for (int l = 0; l < L; ++l) {
+        const int cnt = (int)levels[l].size();
+        lvl_ptr[l + 1] = lvl_ptr[l] + cnt;
+        std::vector<std::vector<std::pair<int, int>>> groups;
+        std::vector<int> gs;
+        for (int j = j0; j < j0 + cnt; ++j)
+            for (int side = 0; side < 2; ++side) {
+                int sl = side == 0 ? SA[j] : SB[j];
+                if (gidx[sl] < 0) {
+                    gidx[sl] = (int)groups.size();
+                    groups.emplace_back();
+                    gs.push_back(sl);
+                }
+                groups[gidx[sl]].push_back({j, side == 0 ? 1 : -1});
+            }
+        for (size_t g = 0; g < groups.size(); ++g) {
+            grp_slot.push_back(gs[g]);
+            for (auto& e : groups[g]) {
+                ent_contact.push_back(e.first);
+                ent_sign.push_back(e.second);
+            }
+            ent_ptr.push_back((int)ent_contact.size());
+            gidx[gs[g]] = -1;
+        }
+        grp_ptr[l + 1] = grp_ptr[l] + (int)groups.size();
+        lvl_counts[l] = cnt;
+        grp_counts[l] = (int)groups.size();
+        j0 += cnt;
+    }
+    // staging: ints then doubles
+    up_int.clear();
+    auto put = [&](const std::vector<int>& v) {
+        size_t off = up_int.size();
+        up_int.insert(up_int.end(), v.begin(), v.end());
+        return off;
+    };
+    size_t oA = put(A), oB = put(B), oSA = put(SA), oSB = put(SB), oL = put(lvl_ptr), oG = put(grp_ptr),
+           oGS = put(grp_slot), oEP = put(ent_ptr), oEC = put(ent_contact), oES = put(ent_sign), oSV = put(slot_vertex);
+    up_dbl.clear();
+    up_dbl.insert(up_dbl.end(), basis.begin(), basis.end());
+    up_dbl.insert(up_dbl.end(), mu.begin(), mu.end());
+    if (up_int.size() > cap_self_int) {
+        free_ptr(d_self_int);
+        cap_self_int = up_int.size() * 2;
+        d_self_int = dalloc<int>(cap_self_int);
+    }
+    if (up_dbl.size() > cap_self_dbl) {
+        free_ptr(d_self_dbl);
+        cap_self_dbl = up_dbl.size() * 2;
+        d_self_dbl = dalloc<double>(cap_self_dbl);
+    }
+    if ((size_t)nsc > cap_self_contacts || (size_t)nslots > cap_self_slots) {
+        free_ptr(d_self_work);
+        free_ptr(d_self_active);
+        cap_self_contacts = std::max(cap_self_contacts, (size_t)nsc * 2);
+        cap_self_slots = std::max(cap_self_slots, (size_t)nslots * 2);
+        d_self_work = dalloc<double>(9 * cap_self_contacts + 15 * cap_self_slots);
+        d_self_active = dalloc<int>(cap_self_contacts);
+    }
+    HIP_CHECK(hipMemcpyAsync(d_self_int, up_int.data(), up_int.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_self_dbl, up_dbl.data(), up_dbl.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    pdfc::DevSelf S{};
+    S.nsc = nsc;
+    S.nslots = nslots;
+    S.a = d_self_int + oA;
+    S.b = d_self_int + oB;
+    S.sa = d_self_int + oSA;
+    S.sb = d_self_int + oSB;
+    S.lvl_ptr = d_self_int + oL;
+    S.grp_ptr = d_self_int + oG;
+    S.grp_slot = d_self_int + oGS;
+    S.ent_ptr = d_self_int + oEP;
+    S.ent_contact = d_self_int + oEC;
+    S.ent_sign = d_self_int + oES;
+    S.slot_vertex = d_self_int + oSV;
+    S.basis = d_self_dbl;
+    S.mu = d_self_dbl + 9 * (size_t)nsc;
+    double* w = d_self_work;
+    S.r = w;
+    S.rem = w + 3 * cap_self_contacts;
+    S.addv = w + 6 * cap_self_contacts;
+    double* ws = w + 9 * cap_self_contacts;
+    S.Fc = ws;
+    S.CFc = ws + 3 * cap_self_slots;
+    S.CFn = ws + 6 * cap_self_slots;
+    S.SCFc = ws + 9 * cap_self_slots;
+    S.SCFn = ws + 12 * cap_self_slots;
+    S.active = d_self_active;
+    ds = S;
+    // the staging vectors must outlive the asynchronous copies
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
 void pdfc_sim::run_step() {
     const int K = prm.local_global_iterations;
     pdfc::StepArgs a{};
@@ -372,14 +645,39 @@ void pdfc_sim::run_step() {
     }
     pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
     detect();
+    nsc = 0;
+    if (self_on) {
+        self_detect_and_pair(keep_full_candidates);
+        if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
+    }
     HIP_CHECK(hipEventRecord(ev[1], stream));
+    pdfc::LocalSelf ls{};
+    ls.selfmode = nsc > 0 ? 1 : 0;
+    ls.vslot = vslot;
     for (int it = 0; it < K; ++it) {
         double* vcur = V[it & 1];
         double* vnext = V[(it + 1) & 1];
-        timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, stream); });
+        ls.Fc = ds.Fc;
+        ls.CFn = ds.CFn;
+        ls.SCFc = ds.SCFc;
+        timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
+        if (nsc > 0) {
+            for (int l = 0; l < n_self_levels; ++l) {
+                pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
+                pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+            }
+            pdfc::launch_self_finalize(ds, vnext, n, stream);
+            std::swap(ds.CFc, ds.CFn);
+            std::swap(ds.SCFc, ds.SCFn);
+            std::swap(ds.rem, ds.addv);
+        }
         solve_inplace(vnext);
     }
     pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, a, K - 1, stream);
+    if (nsc > 0) {
+        pdfc::launch_self_ac_final(ds, V[K & 1], n, a, K - 1, stream);
+        pdfc::launch_self_release(ds, vslot, stream);
+    }
     HIP_CHECK(hipGetLastError());
 }
 
@@ -478,6 +776,7 @@ int pdfc_detect(pdfc_sim* sim) {
     return guarded([&] {
         if (!sim) throw std::invalid_argument("null sim");
         sim->detect();
+        if (sim->self_on) sim->self_detect_and_pair(true);
         HIP_CHECK(hipStreamSynchronize(sim->stream));
     });
 }
@@ -527,20 +826,62 @@ int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n) {
 }
 
 int pdfc_get_self_contacts(pdfc_sim* sim, pdfc_self_contact* out, size_t cap, size_t* n) {
-    (void)out;
-    (void)cap;
     return guarded([&] {
         if (!sim || !n) throw std::invalid_argument("null argument");
-        *n = 0;
+        const int ns = sim->nsc;
+        *n = (size_t)ns;
+        if (cap < (size_t)ns) throw CapacityError("self-contact buffer too small");
+        if (ns == 0) return;
+        std::vector<double> r(3 * (size_t)ns);
+        std::vector<int> act(ns);
+        HIP_CHECK(hipMemcpyAsync(r.data(), sim->ds.r, r.size() * sizeof(double), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipMemcpyAsync(act.data(), sim->ds.active, ns * sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        for (int j = 0; j < ns; ++j) {
+            const int p = sim->sorted_pair[j];
+            const pdfc::SelfPair& sp = sim->pairs[p];
+            pdfc_self_contact& o = out[p];
+            o.vertex = sp.v;
+            o.partner = sp.partner;
+            for (int k = 0; k < 3; ++k) {
+                o.face[k] = sp.face[k];
+                o.r_local[k] = r[3 * (size_t)j + k];
+            }
+            o.normal[0] = sp.normal.x;
+            o.normal[1] = sp.normal.y;
+            o.normal[2] = sp.normal.z;
+            o.level = sp.level;
+            o.active = act[j];
+            o.mu = sim->self_mu;
+        }
     });
 }
 
 int pdfc_get_self_candidates(pdfc_sim* sim, pdfc_candidate* out, size_t cap, size_t* n) {
-    (void)out;
-    (void)cap;
     return guarded([&] {
         if (!sim || !n) throw std::invalid_argument("null argument");
-        *n = 0;
+        if (!sim->self_on) {
+            *n = 0;
+            return;
+        }
+        if (sim->sd.cap_full == 0 || sim->n_full_candidates > sim->sd.cap_full)
+            throw std::invalid_argument("no complete candidate list: call pdfc_detect first");
+        const int nc = sim->n_full_candidates;
+        *n = (size_t)nc;
+        if (cap < (size_t)nc) throw CapacityError("candidate buffer too small");
+        if (nc == 0) return;
+        std::vector<int> v(nc), t(nc);
+        std::vector<double> b(3 * (size_t)nc);
+        HIP_CHECK(hipMemcpyAsync(v.data(), sim->sd.full_v, nc * sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipMemcpyAsync(t.data(), sim->sd.full_t, nc * sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+        HIP_CHECK(hipMemcpyAsync(b.data(), sim->sd.full_bary, b.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        for (int i = 0; i < nc; ++i) {
+            out[i].vertex = v[i];
+            out[i].triangle = t[i];
+            for (int k = 0; k < 3; ++k) out[i].barycentric[k] = b[3 * (size_t)i + k];
+        }
     });
 }
 
@@ -558,7 +899,8 @@ int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, siz
         HIP_CHECK(hipMemcpyAsync(&nc, sim->ncontacts, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         for (int k = 0; k < K; ++k) {
-            if (ac_mean) ac_mean[k] = nc > 0 ? s[k] / nc : 0.;
+            const int tot = nc + sim->nsc;
+            if (ac_mean) ac_mean[k] = tot > 0 ? s[k] / tot : 0.;
             if (ac_max) {
                 double d;
                 std::memcpy(&d, &m[k], sizeof(double));
@@ -592,6 +934,9 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         HIP_CHECK(hipMemcpyAsync(&nc, sim->ncontacts, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         I.n_contacts = nc;
+        I.n_self_contacts = sim->nsc;
+        I.n_self_candidates = sim->n_full_candidates;
+        I.n_self_levels = sim->n_self_levels;
         I.max_front = sim->fac.max_front;
         I.factor_seconds = sim->fac.seconds;
         // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
