@@ -50,7 +50,29 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cpp_api(verbose)
     return LIB
+
+
+CPP_SOURCES = ["json.cpp", "scene_building.cpp", "simulation.cpp"]
+
+
+def build_cpp_api(verbose=False):
+    """The C++ host interface (reference API mirror) over the C ABI, and the driver."""
+    cpp = os.path.join(HERE, "cpp")
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + cpp]
+    rp = "-Wl,-rpath,$ORIGIN"
+    api = os.path.join(LIBDIR, "libpdfc_cpp.so")
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + [os.path.join(cpp, s) for s in CPP_SOURCES]
+        + ["-L" + LIBDIR, "-lpdfc", rp, "-o", api],
+        ["g++", "-O2", "-std=c++17", "-Wall"] + inc + [os.path.join(cpp, "main.cpp"), "-L" + LIBDIR, "-lpdfc_cpp",
+                                                      "-lpdfc", rp, "-o", os.path.join(LIBDIR, "pdfc_main")],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,101 @@
+// getSimulation and the GPU-backed SimulationFrictionEstimation
+// (src/simulation_building.cpp:80-98, src/physics/SimulationFrictionEstimation.cpp).
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <stdexcept>
+
+#include "ProjectiveFriction.hpp"
+
+namespace {
+void fail_stop(const char* where) {
+    std::cerr << "pdfc: " << where << " failed: " << pdfc_last_error() << std::endl;
+    std::abort();
+}
+}  // namespace
+
+SimulationFrictionEstimationHIP::SimulationFrictionEstimationHIP(PhysicScene& scene, const SimulationParameters& p)
+    : m_scene(scene), m_iterations(p.local_global_iterations) {
+    auto mesh = scene.getMesh(0);
+    const auto& rest = mesh->getRestPositions();
+    const auto& tris = mesh->getTriangles();
+    std::vector<double> rest_flat, cur_flat = scene.getGeneralizedPositions();
+    for (const auto& x : rest) rest_flat.insert(rest_flat.end(), x.begin(), x.end());
+    std::vector<int32_t> tri_flat;
+    for (const auto& t : tris) tri_flat.insert(tri_flat.end(), t.begin(), t.end());
+    const auto& table = scene.getFrictionCoefficientTable();
+    std::vector<double> fr;
+    for (const auto& row : table) fr.insert(fr.end(), row.begin(), row.end());
+    const auto& obs = scene.getObstacles();
+    pdfc_scene sc{};
+    sc.n_vertices = rest.size();
+    sc.rest_positions = rest_flat.data();
+    sc.positions = cur_flat.data();
+    sc.n_triangles = tris.size();
+    sc.triangles = tri_flat.data();
+    sc.area_density = mesh->getPhysicParameters().area_density;
+    sc.bending = mesh->getPhysicParameters().bend;
+    sc.stretching = mesh->getPhysicParameters().stretch;
+    sc.material = (int32_t)mesh->getMaterialIdentifier();
+    sc.n_obstacles = obs.size();
+    sc.obstacles = obs.data();
+    sc.n_materials = table.size();
+    sc.friction = fr.data();
+    pdfc_params prm{};
+    prm.time_step = p.time_step;
+    prm.local_global_iterations = (int32_t)p.local_global_iterations;
+    prm.self_collision = p.self_collision ? 1 : 0;
+    prm.self_collision_tolerance = p.self_collision_tolerance;
+    for (int k = 0; k < 3; ++k) prm.gravity[k] = p.gravity[k];
+    prm.air_damping = p.air_damping;
+    prm.precision = p.precision;
+    prm.refinement_sweeps = p.refinement_sweeps;
+    prm.warm_start = p.warm_start;
+    prm.record_iterations = 1;
+    int rc = pdfc_create(&sc, &prm, &m_sim);
+    if (rc != PDFC_OK) {
+        if (rc == PDFC_ERR_INVALID) throw std::invalid_argument(pdfc_last_error());
+        throw std::runtime_error(pdfc_last_error());
+    }
+    std::vector<double> v0 = scene.getGeneralizedSpeeds();
+    if (pdfc_set_state(m_sim, nullptr, v0.data()) != PDFC_OK) throw std::runtime_error(pdfc_last_error());
+    pdfc_info info{};
+    if (pdfc_get_info(m_sim, &info) == PDFC_OK) m_factor_us = 1e6 * info.factor_seconds;
+    m_current_step_error.assign(m_iterations, 0.);
+    m_current_step_duration.assign(m_iterations, IterationDuration::zero());
+    m_current_step_number_of_collisions.assign(m_iterations, 0);
+}
+
+SimulationFrictionEstimationHIP::~SimulationFrictionEstimationHIP() { pdfc_destroy(m_sim); }
+
+void SimulationFrictionEstimationHIP::step() noexcept {
+    if (pdfc_step(m_sim, 1) != PDFC_OK) fail_stop("step");
+    pdfc_phase_stats ps{};
+    pdfc_info info{};
+    if (pdfc_get_phase_stats(m_sim, &ps) != PDFC_OK || pdfc_get_info(m_sim, &info) != PDFC_OK) fail_stop("stats");
+    m_step_times.push_back(ps.step_us);
+    if (info.n_contacts > 0) m_collision_numbers.push_back((double)info.n_contacts);
+    if (info.n_self_contacts > 0) m_self_collision_numbers.push_back((double)info.n_self_contacts);
+    std::fill(m_current_step_number_of_collisions.begin(), m_current_step_number_of_collisions.end(),
+              (std::size_t)info.n_contacts);
+    if (m_mirror) {
+        const std::size_t n3 = 3 * m_scene.getNumberOfVertices();
+        m_q.resize(n3);
+        m_v.resize(n3);
+        if (pdfc_get_state(m_sim, m_q.data(), m_v.data()) != PDFC_OK) fail_stop("get_state");
+        m_scene.setGeneralizedPositions(m_q);
+        m_scene.setGeneralizedSpeeds(m_v);
+    }
+}
+
+// Section names of SimulationFrictionEstimation::exportStats (src/physics/SimulationFrictionEstimation.cpp:755-771).
+void SimulationFrictionEstimationHIP::exportStats(std::ofstream& of) const noexcept {
+    computeAndWriteStats("Cholesky computation time (µs)", of, {m_factor_us});
+    computeAndWriteStats("Step time (µs)", of, m_step_times);
+    computeAndWriteStats("Number of collisions", of, m_collision_numbers);
+    computeAndWriteStats("Number of self collisions", of, m_self_collision_numbers);
+}
+
+std::unique_ptr<SimulationBase> getSimulation(PhysicScene& scene, const SimulationParameters& params) {
+    return std::make_unique<SimulationFrictionEstimationHIP>(scene, params);
+}

@@ -1,0 +1,47 @@
+// Test program for the C++ host interface: reads a scene JSON and prints what the
+// builders produced; with a second argument "sim" it also builds the simulation and
+// runs the given number of steps, printing the final positions of the log vertices.
+#include <cstdlib>
+#include <iostream>
+#include <string>
+
+#include "ProjectiveFriction.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    try {
+        json::Value doc = getJSONDocumentFromFilename(argv[1]);
+        PhysicScene scene = getSceneFromJSONDocument(doc);
+        SimulationParameters p = getSimulationParametersFromJSONDocument(doc);
+        StepCallBacks cb = getLogCallBacksFromJSONDocument(doc);
+        auto m = scene.getMesh(0);
+        std::cout << "vertices " << m->getNumberOfVertices() << "\n";
+        std::cout << "triangles " << m->getNumberOfTriangles() << "\n";
+        std::cout << "obstacles " << scene.getObstacles().size() << "\n";
+        std::cout << "materials " << scene.getFrictionCoefficientTable().size() << "\n";
+        std::cout.precision(17);
+        std::cout << "timestep " << p.time_step << "\n";
+        std::cout << "iterations " << p.local_global_iterations << "\n";
+        std::cout << "self " << p.self_collision << " " << p.self_collision_tolerance << "\n";
+        std::cout << "damping " << p.air_damping << "\n";
+        auto q = m->getVertexPosition(1);
+        std::cout << "v1 " << q[0] << " " << q[1] << " " << q[2] << "\n";
+        if (argc >= 4 && std::string(argv[2]) == "sim") {
+            auto sim = getSimulation(scene, p);
+            int steps = std::atoi(argv[3]);
+            for (int s = 0; s < steps; ++s) {
+                sim->step();
+                cb((size_t)s, scene);
+            }
+            auto x = scene.getMesh(0)->getVertexPosition(0);
+            std::cout << "final0 " << x[0] << " " << x[1] << " " << x[2] << "\n";
+        }
+    } catch (const std::invalid_argument& e) {
+        std::cout << "invalid_argument: " << e.what() << "\n";
+        return 3;
+    } catch (const std::exception& e) {
+        std::cout << "error: " << e.what() << "\n";
+        return 4;
+    }
+    return 0;
+}
