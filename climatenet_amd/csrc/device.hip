@@ -361,12 +361,16 @@ __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __re
 template <typename T>
 __device__ __forceinline__ double as_d(T x) { return (double)x; }
 
-// Forward substitution, one task = (supernode s, 64 rows of F_s):
+// Forward substitution, one task = (supernode s, 64 rows of F_s), W waves:
 //   b~_s = b_s - sum_children u_c|own ;  [y_s ; u_s] = F_s b~_s (+ children's u_c|below)
-template <typename T>
-__global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const double* __restrict__ Bv,
-                                                double* __restrict__ Y) {
+// The reduction over the ns columns is split across the W waves (wave w takes
+// columns w, w+W, ...) so each wave has a short chain of independent loads; the
+// partial sums are combined in LDS in wave order (deterministic).
+template <typename T, int W>
+__global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const double* __restrict__ Bv,
+                                                    double* __restrict__ Y) {
     extern __shared__ double lds[];
+    __shared__ double part[W > 1 ? W : 1][3][64];
     const int task = t0 + blockIdx.x;
     const int s = Fa.task_s[task], r0 = Fa.task_r0[task];
     const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
@@ -374,8 +378,8 @@ __global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const doub
     double* bx = lds;
     double* by = lds + ns;
     double* bz = lds + 2 * ns;
-    const int lane = threadIdx.x;
-    for (int t = lane; t < ns; t += 64) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < ns; t += 64 * W) {
         bx[t] = Bv[c0 + t];
         by[t] = Bv[n + c0 + t];
         bz[t] = Bv[2 * n + c0 + t];
@@ -388,7 +392,7 @@ __global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const doub
         const int nbc = Fa.nb[ch];
         const int* map = Fa.relmap + Fa.uoff[ch];
         const double* Uc = Fa.U + 3 * Fa.uoff[ch];
-        for (int t = lane; t < nbc; t += 64) {
+        for (int t = tid; t < nbc; t += 64 * W) {
             int idx = map[t];
             if (idx < ns) {
                 bx[idx] -= Uc[3 * t];
@@ -396,10 +400,8 @@ __global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const doub
                 bz[idx] -= Uc[3 * t + 2];
             }
         }
-        __syncthreads();
-        // below-row contributions for this task's own rows (each row appears at most once per child)
-        if (my_row >= ns && my_row < f && my_row < r0 + 64) {
-            // binary search of my_row in the child's (ascending) map
+        // below-row contributions for this task's rows (each row appears at most once per child)
+        if (wave == 0 && my_row >= ns && my_row < f) {
             int lo = 0, hi = nbc;
             while (lo < hi) {
                 int mid = (lo + hi) >> 1;
@@ -411,17 +413,35 @@ __global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const doub
                 uz += Uc[3 * lo + 2];
             }
         }
+        __syncthreads();
+    }
+    double ax = 0., ay = 0., az = 0.;
+    if (my_row < f) {
+        const T* Fs = static_cast<const T*>(Fa.F) + Fa.foff[s] + my_row;
+#pragma unroll 8
+        for (int j = wave; j < ns; j += W) {
+            double fv = as_d(Fs[(size_t)j * f]);
+            ax = fma(fv, bx[j], ax);
+            ay = fma(fv, by[j], ay);
+            az = fma(fv, bz[j], az);
+        }
+    }
+    if (W > 1) {
+        part[wave][0][lane] = ax;
+        part[wave][1][lane] = ay;
+        part[wave][2][lane] = az;
+        __syncthreads();
+        if (wave != 0) return;
+        ax = part[0][0][lane];
+        ay = part[0][1][lane];
+        az = part[0][2][lane];
+        for (int w = 1; w < W; ++w) {
+            ax += part[w][0][lane];
+            ay += part[w][1][lane];
+            az += part[w][2][lane];
+        }
     }
     if (my_row >= f) return;
-    const T* Fs = static_cast<const T*>(Fa.F) + Fa.foff[s] + my_row;
-    double ax = 0., ay = 0., az = 0.;
-#pragma unroll 4
-    for (int j = 0; j < ns; ++j) {
-        double fv = as_d(Fs[(size_t)j * f]);
-        ax = fma(fv, bx[j], ax);
-        ay = fma(fv, by[j], ay);
-        az = fma(fv, bz[j], az);
-    }
     if (my_row < ns) {
         Y[c0 + my_row] = ax;
         Y[n + c0 + my_row] = ay;
@@ -434,12 +454,13 @@ __global__ void __launch_bounds__(64) k_forward(DevFactor Fa, int t0, const doub
     }
 }
 
-// Backward substitution, one task = (supernode s, 64 columns):
-//   x_s = F_s^T [ y_s ; -x_R ]
-template <typename T>
-__global__ void __launch_bounds__(64) k_backward(DevFactor Fa, int t0, const double* __restrict__ Y,
-                                                 double* __restrict__ X) {
+// Backward substitution, one task = (supernode s, 64 columns), W waves:
+//   x_s = F_s^T [ y_s ; -x_R ]      (reduction over the f = ns + nb rows split across waves)
+template <typename T, int W>
+__global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const double* __restrict__ Y,
+                                                     double* __restrict__ X) {
     extern __shared__ double lds[];
+    __shared__ double part[W > 1 ? W : 1][3][64];
     const int task = t0 + blockIdx.x;
     const int s = Fa.task_s[task], j0 = Fa.task_r0[task];
     const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
@@ -447,14 +468,14 @@ __global__ void __launch_bounds__(64) k_backward(DevFactor Fa, int t0, const dou
     double* gx = lds;
     double* gy = lds + f;
     double* gz = lds + 2 * f;
-    const int lane = threadIdx.x;
-    for (int t = lane; t < ns; t += 64) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < ns; t += 64 * W) {
         gx[t] = Y[c0 + t];
         gy[t] = Y[n + c0 + t];
         gz[t] = Y[2 * n + c0 + t];
     }
     const int* R = Fa.rows + Fa.roff[s];
-    for (int t = lane; t < nb; t += 64) {
+    for (int t = tid; t < nb; t += 64 * W) {
         int r = R[t];
         gx[ns + t] = -X[r];
         gy[ns + t] = -X[n + r];
@@ -462,16 +483,33 @@ __global__ void __launch_bounds__(64) k_backward(DevFactor Fa, int t0, const dou
     }
     __syncthreads();
     const int j = j0 + lane;
-    if (j >= ns) return;
-    const T* Ft = static_cast<const T*>(Fa.FT) + Fa.foff[s] + j;
     double ax = 0., ay = 0., az = 0.;
-#pragma unroll 4
-    for (int i = 0; i < f; ++i) {
-        double fv = as_d(Ft[(size_t)i * ns]);
-        ax = fma(fv, gx[i], ax);
-        ay = fma(fv, gy[i], ay);
-        az = fma(fv, gz[i], az);
+    if (j < ns) {
+        const T* Ft = static_cast<const T*>(Fa.FT) + Fa.foff[s] + j;
+#pragma unroll 8
+        for (int i = wave; i < f; i += W) {
+            double fv = as_d(Ft[(size_t)i * ns]);
+            ax = fma(fv, gx[i], ax);
+            ay = fma(fv, gy[i], ay);
+            az = fma(fv, gz[i], az);
+        }
     }
+    if (W > 1) {
+        part[wave][0][lane] = ax;
+        part[wave][1][lane] = ay;
+        part[wave][2][lane] = az;
+        __syncthreads();
+        if (wave != 0) return;
+        ax = part[0][0][lane];
+        ay = part[0][1][lane];
+        az = part[0][2][lane];
+        for (int w = 1; w < W; ++w) {
+            ax += part[w][0][lane];
+            ay += part[w][1][lane];
+            az += part[w][2][lane];
+        }
+    }
+    if (j >= ns) return;
     X[c0 + j] = ax;
     X[n + c0 + j] = ay;
     X[2 * n + c0 + j] = az;
@@ -536,23 +574,56 @@ void launch_finish(const DevModel& M, double* qn, const double* vlast, double* v
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vlast, vstate, ccnt, cstart, C, a,
                        iteration);
 }
+// Waves per task: the reduction length (ns forward, ns+nb backward) divided into
+// chains of about 8 loads, at most 16 waves (1024 threads).
+static int waves_for(int kmax) {
+    int w = 1;
+    while (w < 16 && w * 8 < kmax) w <<= 1;
+    return w;
+}
+
+template <typename T>
+static void forward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* B,
+                             double* Y) {
+    switch (W) {
+        case 1: hipLaunchKernelGGL((k_forward<T, 1>), g, dim3(64), lds, st, F, t0, B, Y); break;
+        case 2: hipLaunchKernelGGL((k_forward<T, 2>), g, dim3(128), lds, st, F, t0, B, Y); break;
+        case 4: hipLaunchKernelGGL((k_forward<T, 4>), g, dim3(256), lds, st, F, t0, B, Y); break;
+        case 8: hipLaunchKernelGGL((k_forward<T, 8>), g, dim3(512), lds, st, F, t0, B, Y); break;
+        default: hipLaunchKernelGGL((k_forward<T, 16>), g, dim3(1024), lds, st, F, t0, B, Y); break;
+    }
+}
+template <typename T>
+static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* Y,
+                              double* X) {
+    switch (W) {
+        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X); break;
+        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X); break;
+        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X); break;
+        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X); break;
+        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X); break;
+    }
+}
+
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y, int max_ns,
-                          hipStream_t st) {
+                          int kmax, hipStream_t st) {
     if (ntasks <= 0) return;
     size_t lds = (size_t)3 * max_ns * sizeof(double);
+    int W = waves_for(kmax);
     if (fp32)
-        hipLaunchKernelGGL(k_forward<float>, dim3(ntasks), dim3(64), lds, st, F, t0, B, Y);
+        forward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, B, Y);
     else
-        hipLaunchKernelGGL(k_forward<double>, dim3(ntasks), dim3(64), lds, st, F, t0, B, Y);
+        forward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, B, Y);
 }
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, hipStream_t st) {
+                           int max_front, int kmax, hipStream_t st) {
     if (ntasks <= 0) return;
     size_t lds = (size_t)3 * max_front * sizeof(double);
+    int W = waves_for(kmax);
     if (fp32)
-        hipLaunchKernelGGL(k_backward<float>, dim3(ntasks), dim3(64), lds, st, F, t0, Y, X);
+        backward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, Y, X);
     else
-        hipLaunchKernelGGL(k_backward<double>, dim3(ntasks), dim3(64), lds, st, F, t0, Y, X);
+        backward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, Y, X);
 }
 void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st) {
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, dt2, b, x, r);

@@ -168,10 +168,11 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st);
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
+// kmax: longest reduction among the level's tasks (ns forward, ns + nb backward).
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y,
-                          int max_ns, hipStream_t st);
+                          int max_ns, int kmax, hipStream_t st);
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, hipStream_t st);
+                           int max_front, int kmax, hipStream_t st);
 // r = b - K x with K = diag(mass) + dt2 * A^T A
 void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st);
 void launch_axpy(double* x, const double* dx, size_t count, hipStream_t st);

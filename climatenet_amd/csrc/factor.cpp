@@ -274,21 +274,29 @@ void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd) {
     bwd = SweepPlan();
     fwd.level_ptr.push_back(0);
     for (int l = 0; l < F.n_levels; ++l) {
-        for (int s : by_level[l])
+        int kmax = 0;
+        for (int s : by_level[l]) {
+            kmax = std::max(kmax, F.ns[s]);
             for (int r0 = 0; r0 < F.ns[s] + F.nb[s]; r0 += chunk) {
                 fwd.task_snode.push_back(s);
                 fwd.task_row0.push_back(r0);
             }
+        }
         fwd.level_ptr.push_back((int)fwd.task_snode.size());
+        fwd.level_kmax.push_back(kmax);
     }
     bwd.level_ptr.push_back(0);
     for (int l = F.n_levels - 1; l >= 0; --l) {
-        for (int s : by_level[l])
+        int kmax = 0;
+        for (int s : by_level[l]) {
+            kmax = std::max(kmax, F.ns[s] + F.nb[s]);
             for (int j0 = 0; j0 < F.ns[s]; j0 += chunk) {
                 bwd.task_snode.push_back(s);
                 bwd.task_row0.push_back(j0);
             }
+        }
         bwd.level_ptr.push_back((int)bwd.task_snode.size());
+        bwd.level_kmax.push_back(kmax);
     }
 }
 

@@ -79,6 +79,7 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size = 64);
 struct SweepPlan {
     std::vector<int> task_snode, task_row0;
     std::vector<int> level_ptr;  // tasks of level l are [level_ptr[l], level_ptr[l+1])
+    std::vector<int> level_kmax; // longest reduction of the level's tasks
 };
 void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd);
 

@@ -22,9 +22,13 @@ int first_face_vertex_in(const std::array<int, 3>& face, const std::vector<int>&
 
 void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out) {
     out.clear();
-    // registered partners of each vertex touched in this step
-    std::vector<std::vector<int>> registered(M.n);
-    std::vector<int> touched;
+    // registered partners of each vertex touched in this step; the lists persist across
+    // calls and only the touched ones are cleared
+    static thread_local std::vector<std::vector<int>> registered;
+    static thread_local std::vector<int> touched;
+    if ((int)registered.size() < M.n) registered.resize(M.n);
+    for (int v : touched) registered[v].clear();
+    touched.clear();
     for (size_t k = 0; k < ncand; ++k) {
         const SelfCandidate& c = cand[k];
         const int v = c.v;

@@ -200,7 +200,8 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     } catch (const std::runtime_error& e) {
         throw NumericError(e.what());
     }
-    if ((size_t)3 * fac.max_front * sizeof(double) > 64 * 1024)
+    // dynamic LDS (right-hand side of one supernode) + the 24 KiB partial-sum buffer
+    if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
     pdfc::plan_sweeps(fac, pdfc::kSolveChunk, fwd, bwd);
 
@@ -406,12 +407,12 @@ void pdfc_sim::solve_inplace(double* X) {
         for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
             timed(1, [&] {
                 pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
-                                           inout, Y, fac.max_ns, stream);
+                                           inout, Y, fac.max_ns, fwd.level_kmax[l], stream);
             });
         for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
             timed(2, [&] {
                 pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
-                                            inout, fac.max_front, stream);
+                                            inout, fac.max_front, bwd.level_kmax[l], stream);
             });
         if (ktiming) ++ksolves;
     };
