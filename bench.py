@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32r"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--no-self-collision", action="store_true",
+                    help="disable self-collision (not the benchmark configuration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -51,7 +53,8 @@ def main():
     from climatenet_amd import abi, native, scenes
 
     spec = scenes.sphere_drape_40k()
-    spec.self_collision = False  # self-collision detection is not in this build yet
+    if args.no_self_collision:
+        spec.self_collision = False
     prec = abi.FP64 if args.precision == "fp64" else abi.FP32_REFINE64
     sim = native.Simulation(spec, precision=prec, refinement_sweeps=2, record_iterations=True)
     sim.step(args.warmup)
@@ -69,8 +72,9 @@ def main():
     K = spec.iterations
     fam = {
         "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
-        "k_forward": (kt["forward_us"], info["bytes_forward"] / max(1, info["n_levels"]), kt["forward_launches"]),
-        "k_backward": (kt["backward_us"], info["bytes_backward"] / max(1, info["n_levels"]), kt["backward_launches"]),
+        "k_forward": (kt["forward_us"], info["bytes_forward"] / max(1, info["n_levels_swept"]), kt["forward_launches"]),
+        "k_backward": (kt["backward_us"], info["bytes_backward"] / max(1, info["n_levels_swept"]), kt["backward_launches"]),
+        "k_top": (kt["top_us"], info["bytes_top"], kt["top_launches"]),
     }
     dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
     us, bytes_per_launch, _ = fam[dom]
@@ -93,9 +97,11 @@ def main():
         "dtype": "f64" if prec == abi.FP64 else "f32+f64-refine",
         "data": "synthetic",
         "config": {"workload": "drape_200x200: 40,401 vertices, sphere R=0.5 + plane, mu=0.1, dt=1/200, "
-                               "20 local-global iterations, self-collision off",
+                               "20 local-global iterations, self-collision %s" % ("on (tol 5e-3)" if spec.self_collision else "off"),
+                   "first_step": args.warmup, "self_contacts_last_step": info["n_self_contacts"],
                    "iterations": K, "n_vertices": info["n_vertices"], "nnz_L": info["nnz_L"],
-                   "nnz_F": info["nnz_F"], "levels": info["n_levels"], "contacts_last_step": info["n_contacts"]},
+                   "nnz_F": info["nnz_F"], "levels": info["n_levels"], "levels_swept": info["n_levels_swept"],
+                   "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"]},
         "roofline": roofline,
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
     }

@@ -120,6 +120,9 @@ class Info(C.Structure):
         ("bytes_forward", C.c_double),
         ("bytes_backward", C.c_double),
         ("n_below_rows", C.c_int64),
+        ("bytes_top", C.c_double),
+        ("top_m", C.c_int64),
+        ("n_levels_swept", C.c_int64),
     ]
 
 
@@ -137,11 +140,13 @@ class KernelTimes(C.Structure):
         ("local_us", C.c_double),
         ("forward_us", C.c_double),
         ("backward_us", C.c_double),
+        ("top_us", C.c_double),
         ("forward_sweep_us", C.c_double),
         ("backward_sweep_us", C.c_double),
         ("local_launches", C.c_int64),
         ("forward_launches", C.c_int64),
         ("backward_launches", C.c_int64),
+        ("top_launches", C.c_int64),
     ]
 
 

@@ -515,6 +515,95 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
     X[2 * n + c0 + j] = az;
 }
 
+// ---------------------------------------------------------------- top block
+__global__ void k_top_gather(DevTop T, const double* __restrict__ U, const double* __restrict__ B, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T.m) return;
+    int row = T.rows[t];
+    double bx = B[row], by = B[n + row], bz = B[2 * n + row];
+    for (int q = T.cptr[t]; q < T.cptr[t + 1]; ++q) {
+        const double* u = U + 3 * (size_t)T.cidx[q];
+        bx -= u[0];
+        by -= u[1];
+        bz -= u[2];
+    }
+    T.bt[t] = bx;
+    T.bt[T.m + t] = by;
+    T.bt[2 * T.m + t] = bz;
+}
+
+// Partial products of S^-1 b~: workgroup (row block of 64, reduction chunk), 16 waves
+// splitting the chunk; b~ entries are wave-uniform loads.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_top_matvec(DevTop Tp) {
+    __shared__ double part[16][3][64];
+    const int m = Tp.m;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = blockIdx.x * 64 + lane;
+    const int klen = (m + Tp.kchunks - 1) / Tp.kchunks;
+    const int k0 = blockIdx.y * klen, k1 = min(m, k0 + klen);
+    double ax = 0., ay = 0., az = 0.;
+    if (row < m) {
+        const T* S = static_cast<const T*>(Tp.Sinv) + row;
+#pragma unroll 8
+        for (int k = k0 + wave; k < k1; k += 16) {
+            double f = (double)S[(size_t)k * m];
+            ax = fma(f, Tp.bt[k], ax);
+            ay = fma(f, Tp.bt[m + k], ay);
+            az = fma(f, Tp.bt[2 * m + k], az);
+        }
+    }
+    part[wave][0][lane] = ax;
+    part[wave][1][lane] = ay;
+    part[wave][2][lane] = az;
+    __syncthreads();
+    if (wave != 0 || row >= m) return;
+    for (int w = 1; w < 16; ++w) {
+        ax += part[w][0][lane];
+        ay += part[w][1][lane];
+        az += part[w][2][lane];
+    }
+    double* P = Tp.part + (size_t)blockIdx.y * 3 * m;
+    P[row] = ax;
+    P[m + row] = ay;
+    P[2 * m + row] = az;
+}
+
+__global__ void k_top_finish(DevTop T, double* __restrict__ X, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T.m) return;
+    double ax = 0., ay = 0., az = 0.;
+    for (int c = 0; c < T.kchunks; ++c) {
+        const double* P = T.part + (size_t)c * 3 * T.m;
+        ax += P[t];
+        ay += P[T.m + t];
+        az += P[2 * T.m + t];
+    }
+    int row = T.rows[t];
+    X[row] = ax;
+    X[n + row] = ay;
+    X[2 * n + row] = az;
+}
+
+__global__ void k_unit3(double* B, int n, const int* rows, int m, int j) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n) B[i] = 0.;
+}
+__global__ void k_unit3_set(double* B, int n, const int* rows, int m, int j) {
+    int c = threadIdx.x;
+    if (c < 3 && j + c < m) B[(size_t)c * n + rows[j + c]] = 1.;
+}
+__global__ void k_take_cols(const double* X, int n, const int* rows, int m, int j, double* S) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    for (int c = 0; c < 3; ++c)
+        if (j + c < m) S[(size_t)(j + c) * m + t] = X[(size_t)c * n + rows[t]];
+}
+__global__ void k_to_float(const double* src, float* dst, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = (float)src[i];
+}
+
 // r = b - (M x + dt2 A^T A x)
 __global__ void k_residual(DevModel M, double dt2, const double* __restrict__ b, const double* __restrict__ x,
                            double* __restrict__ r) {
@@ -624,6 +713,26 @@ void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, co
         backward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, Y, X);
     else
         backward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, Y, X);
+}
+void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st) {
+    if (T.m <= 0) return;
+    hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
+    dim3 g((T.m + 63) / 64, T.kchunks);
+    if (fp32)
+        hipLaunchKernelGGL(k_top_matvec<float>, g, dim3(1024), 0, st, T);
+    else
+        hipLaunchKernelGGL(k_top_matvec<double>, g, dim3(1024), 0, st, T);
+    hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
+}
+void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st) {
+    hipLaunchKernelGGL(k_unit3, dim3(blocks_for(3 * (size_t)n)), dim3(kBlock), 0, st, B, n, rows, m, j);
+    hipLaunchKernelGGL(k_unit3_set, dim3(1), dim3(64), 0, st, B, n, rows, m, j);
+}
+void launch_take_cols(const double* X, int n, const int* rows, int m, int j, double* S, hipStream_t st) {
+    hipLaunchKernelGGL(k_take_cols, dim3(blocks_for(m)), dim3(kBlock), 0, st, X, n, rows, m, j, S);
+}
+void launch_to_float(const double* src, float* dst, size_t count, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(k_to_float, dim3(blocks_for(count)), dim3(kBlock), 0, st, src, dst, count);
 }
 void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st) {
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, dt2, b, x, r);

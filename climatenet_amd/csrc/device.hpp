@@ -173,6 +173,25 @@ void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, con
                           int max_ns, int kmax, hipStream_t st);
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
                            int max_front, int kmax, hipStream_t st);
+// Dense top block of the substitution (see TopPlan).
+struct DevTop {
+    int m;
+    int kchunks;            // reduction split of the matvec across workgroups
+    const int* rows;        // m supernodal indices
+    const int* cptr;        // m + 1
+    const int* cidx;        // update-vector entries to subtract
+    const void* Sinv;       // m x m, double or float
+    double* bt;             // 3 m: right-hand side of the top block (planar)
+    double* part;           // kchunks * 3 * m partial products
+};
+// x_T = S^-1 (b_T - lower updates): gather, split matvec, ordered reduction + scatter.
+void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st);
+// Unit right-hand sides e_{rows[j+c]} in plane c (c = 0..2) and extraction of the
+// solution's top rows into columns j..j+2 of S^-1 (build of the top block).
+void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st);
+void launch_take_cols(const double* X, int n, const int* rows, int m, int j, double* S, hipStream_t st);
+void launch_to_float(const double* src, float* dst, size_t count, hipStream_t st);
+
 // r = b - K x with K = diag(mass) + dt2 * A^T A
 void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st);
 void launch_axpy(double* x, const double* dx, size_t count, hipStream_t st);

@@ -266,14 +266,51 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
     F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd) {
+void plan_top(const Factor& F, double max_bytes, int entry_bytes, TopPlan& top) {
     const int S = (int)F.ns.size();
+    std::vector<int64_t> cols_at(F.n_levels + 1, 0);
+    for (int s = 0; s < S; ++s) cols_at[F.level[s]] += F.ns[s];
+    int64_t m = 0;
+    int best = F.n_levels;
+    for (int L = F.n_levels - 1; L >= 0; --L) {
+        m += cols_at[L];
+        if ((double)m * (double)m * entry_bytes <= max_bytes) best = L;
+        else break;
+    }
+    top = TopPlan();
+    top.level = best;
+    std::vector<int> tpos(F.n, -1);
+    for (int s = 0; s < S; ++s)
+        if (F.level[s] >= best)
+            for (int j = 0; j < F.ns[s]; ++j) top.rows.push_back(F.c0[s] + j);
+    std::sort(top.rows.begin(), top.rows.end());
+    top.m = (int)top.rows.size();
+    for (int t = 0; t < top.m; ++t) tpos[top.rows[t]] = t;
+    std::vector<std::vector<int>> contrib(top.m);
+    for (int c = 0; c < S; ++c) {
+        if (F.level[c] >= best || F.parent[c] < 0 || F.level[F.parent[c]] < best) continue;
+        for (int k = 0; k < F.nb[c]; ++k) {
+            int t = tpos[F.rows[F.roff[c] + k]];
+            if (t < 0) throw std::logic_error("update row outside the top block");
+            contrib[t].push_back((int)(F.uoff[c] + k));
+        }
+    }
+    top.cptr.assign(top.m + 1, 0);
+    for (int t = 0; t < top.m; ++t) {
+        top.cptr[t + 1] = top.cptr[t] + (int)contrib[t].size();
+        top.cidx.insert(top.cidx.end(), contrib[t].begin(), contrib[t].end());
+    }
+}
+
+void plan_sweeps(const Factor& F, int chunk, int top_level, SweepPlan& fwd, SweepPlan& bwd) {
+    const int S = (int)F.ns.size();
+    const int L = std::min(top_level, F.n_levels);
     std::vector<std::vector<int>> by_level(F.n_levels);
     for (int s = 0; s < S; ++s) by_level[F.level[s]].push_back(s);
     fwd = SweepPlan();
     bwd = SweepPlan();
     fwd.level_ptr.push_back(0);
-    for (int l = 0; l < F.n_levels; ++l) {
+    for (int l = 0; l < L; ++l) {
         int kmax = 0;
         for (int s : by_level[l]) {
             kmax = std::max(kmax, F.ns[s]);
@@ -286,7 +323,7 @@ void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd) {
         fwd.level_kmax.push_back(kmax);
     }
     bwd.level_ptr.push_back(0);
-    for (int l = F.n_levels - 1; l >= 0; --l) {
+    for (int l = L - 1; l >= 0; --l) {
         int kmax = 0;
         for (int s : by_level[l]) {
             kmax = std::max(kmax, F.ns[s] + F.nb[s]);

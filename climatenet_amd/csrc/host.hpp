@@ -81,7 +81,23 @@ struct SweepPlan {
     std::vector<int> level_ptr;  // tasks of level l are [level_ptr[l], level_ptr[l+1])
     std::vector<int> level_kmax; // longest reduction of the level's tasks
 };
-void plan_sweeps(const Factor& F, int chunk, SweepPlan& fwd, SweepPlan& bwd);
+// Levels below `top_level` only; supernodes at levels >= top_level form the dense top block.
+void plan_sweeps(const Factor& F, int chunk, int top_level, SweepPlan& fwd, SweepPlan& bwd);
+
+// Dense top block: the columns T of all supernodes at levels >= level. Both
+// substitutions over T collapse into x_T = S^-1 b~_T with S the Schur complement
+// of K on T (S^-1 = (K^-1)_TT), where b~_T subtracts the update vectors of the
+// lower supernodes whose parent is in T.
+struct TopPlan {
+    int level = 0;             // first collapsed level; == n_levels when there is no top block
+    int m = 0;
+    std::vector<int> rows;     // supernodal indices of T, ascending
+    std::vector<int> cptr;     // m + 1
+    std::vector<int> cidx;     // update-vector entries (index into U) to subtract from each top row
+};
+// Largest top block with m^2 * entry_bytes <= max_bytes (at least one level collapsed
+// when possible; the whole matrix for small meshes).
+void plan_top(const Factor& F, double max_bytes, int entry_bytes, TopPlan& top);
 
 // One self-collision candidate that survived the exact distance test and the
 // pairing pre-filter (vertex outside the face, no zero barycentric coordinate).

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -67,13 +68,21 @@ struct pdfc_sim {
     unsigned long long* ac_max = nullptr;
     bool fp32 = false;
     bool have_stats = false;
+    // dense top block of the substitution
+    pdfc::TopPlan top;
+    pdfc::DevTop dtop{};
+    bool use_top = false;
+    int64_t nnzF_lower = 0;
+    int* task_buf = nullptr;
+    void upload_sweeps(int top_level);
+    void build_top();
     // kernel-timing instrumentation
     bool ktiming = false;
     std::vector<hipEvent_t> kev;        // event pool, two per timed launch
-    std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level
+    std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level, 3 top block
     size_t kused = 0;
-    double ksum[3] = {0., 0., 0.};
-    int64_t kcount[3] = {0, 0, 0};
+    double ksum[4] = {0., 0., 0., 0.};
+    int64_t kcount[4] = {0, 0, 0, 0};
     int64_t ksolves = 0;
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
@@ -203,7 +212,11 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     // dynamic LDS (right-hand side of one supernode) + the 24 KiB partial-sum buffer
     if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
-    pdfc::plan_sweeps(fac, pdfc::kSolveChunk, fwd, bwd);
+    {
+        double top_mb = 64.;
+        if (const char* e = std::getenv("PDFC_TOP_MB")) top_mb = std::atof(e);
+        pdfc::plan_top(fac, top_mb * 1048576., fp32 ? 4 : 8, top);
+    }
 
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
@@ -288,10 +301,7 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     df.U = dalloc<double>(3 * (size_t)fac.uoff[S]);
     dfwd = df;
     dbwd = df;
-    dfwd.task_s = upload(fwd.task_snode);
-    dfwd.task_r0 = upload(fwd.task_row0);
-    dbwd.task_s = upload(bwd.task_snode);
-    dbwd.task_r0 = upload(bwd.task_row0);
+    upload_sweeps(fac.n_levels);
     // host copies of F no longer needed beyond statistics
     std::vector<double>().swap(fac.FT);
 
@@ -381,6 +391,61 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     std::vector<double> zero(n3, 0.);
     to_device_vec(zero.data(), vstate);
     HIP_CHECK(hipStreamSynchronize(stream));
+    if (top.level < fac.n_levels && top.m > 0) {
+        build_top();
+        upload_sweeps(top.level);
+        use_top = true;
+    }
+    nnzF_lower = 0;
+    for (size_t s2 = 0; s2 < fac.ns.size(); ++s2)
+        if (!use_top || fac.level[s2] < top.level) nnzF_lower += fac.foff[s2 + 1] - fac.foff[s2];
+}
+
+// Task lists of the levels below top_level (all levels when there is no top block).
+void pdfc_sim::upload_sweeps(int top_level) {
+    pdfc::plan_sweeps(fac, pdfc::kSolveChunk, top_level, fwd, bwd);
+    if (task_buf) free_ptr(task_buf);
+    std::vector<int> all;
+    all.insert(all.end(), fwd.task_snode.begin(), fwd.task_snode.end());
+    all.insert(all.end(), fwd.task_row0.begin(), fwd.task_row0.end());
+    all.insert(all.end(), bwd.task_snode.begin(), bwd.task_snode.end());
+    all.insert(all.end(), bwd.task_row0.begin(), bwd.task_row0.end());
+    task_buf = upload(all);
+    size_t nf = fwd.task_snode.size(), nbk = bwd.task_snode.size();
+    dfwd.task_s = task_buf;
+    dfwd.task_r0 = task_buf + nf;
+    dbwd.task_s = task_buf + 2 * nf;
+    dbwd.task_r0 = task_buf + 2 * nf + nbk;
+}
+
+// S^-1 = (K^-1)_TT, one column triple per solve with the full sweeps.
+void pdfc_sim::build_top() {
+    const int m = top.m;
+    dtop.m = m;
+    dtop.rows = upload(top.rows);
+    dtop.cptr = upload(top.cptr);
+    dtop.cidx = upload(top.cidx);
+    dtop.bt = dalloc<double>(3 * (size_t)m);
+    const int rowblocks = (m + 63) / 64;
+    dtop.kchunks = std::max(1, std::min(16, 512 / std::max(1, rowblocks)));
+    dtop.part = dalloc<double>((size_t)dtop.kchunks * 3 * m);
+    double* S64 = dalloc<double>((size_t)m * m);
+    for (int j = 0; j < m; j += 3) {
+        pdfc::launch_unit3(V[1], n, dtop.rows, m, j, stream);
+        solve_inplace(V[1]);
+        pdfc::launch_take_cols(V[1], n, dtop.rows, m, j, S64, stream);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (fp32) {
+        float* S32 = dalloc<float>((size_t)m * m);
+        pdfc::launch_to_float(S64, S32, (size_t)m * m, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        free_ptr(S64);
+        dtop.Sinv = S32;
+    } else {
+        dtop.Sinv = S64;
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 void pdfc_sim::to_device_vec(const double* h, double* d) {
@@ -409,6 +474,7 @@ void pdfc_sim::solve_inplace(double* X) {
                 pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
                                            inout, Y, fac.max_ns, fwd.level_kmax[l], stream);
             });
+        if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream); });
         for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
             timed(2, [&] {
                 pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
@@ -951,12 +1017,18 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         const double n = sim->n, E = (double)sim->model.edges.size();
         const double ent = sim->fp32 ? 4. : 8.;
         const double nbtot = (double)sim->fac.uoff.back();
-        const double fwd = ent * (double)I.nnz_F + 2. * 24. * n + 2. * 24. * nbtot + 4. * nbtot;
-        const double bwd = ent * (double)I.nnz_F + 2. * 24. * n + 28. * nbtot;
+        const double fwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 2. * 24. * nbtot + 4. * nbtot;
+        const double bwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 28. * nbtot;
+        const double mt = sim->use_top ? (double)sim->top.m : 0.;
+        // top block: S^-1 once, b_T in, x_T out, gathered update entries (index + 24 B)
+        const double topb = ent * mt * mt + 2. * 24. * mt + 28. * (double)sim->top.cidx.size() * (sim->use_top ? 1. : 0.);
         const double local = 4. * 24. * n + 2. * E * 12. + nc * (72. + 8. + 48. + 19. * 36.);
-        double solve = fwd + bwd;
-        if (sim->fp32) solve = (fwd + bwd) * (1. + sim->prm.refinement_sweeps) +
+        double solve = fwd + bwd + topb;
+        if (sim->fp32) solve = solve * (1. + sim->prm.refinement_sweeps) +
                                sim->prm.refinement_sweeps * (4. * 24. * n + 19. * 12. * n);
+        I.bytes_top = topb;
+        I.top_m = (int64_t)mt;
+        I.n_levels_swept = (int64_t)(sim->fwd.level_ptr.size() - 1);
         I.bytes_local = local;
         I.bytes_forward = fwd;
         I.bytes_backward = bwd;
@@ -972,7 +1044,7 @@ int pdfc_set_kernel_timing(pdfc_sim* sim, int enable) {
         if (!sim) throw std::invalid_argument("null sim");
         sim->collect_kernel_times();
         sim->ktiming = enable != 0;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 4; ++k) {
             sim->ksum[k] = 0.;
             sim->kcount[k] = 0;
         }
@@ -991,6 +1063,8 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         t.local_us = sim->kcount[0] ? sim->ksum[0] / sim->kcount[0] : 0.;
         t.forward_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
         t.backward_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
+        t.top_us = sim->kcount[3] ? sim->ksum[3] / sim->kcount[3] : 0.;
+        t.top_launches = sim->kcount[3];
         t.forward_sweep_us = sim->ksolves ? sim->ksum[1] / sim->ksolves : 0.;
         t.backward_sweep_us = sim->ksolves ? sim->ksum[2] / sim->ksolves : 0.;
         *out = t;

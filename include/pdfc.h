@@ -146,6 +146,9 @@ typedef struct pdfc_info {
     double bytes_forward;            /* one forward sweep (all levels) */
     double bytes_backward;           /* one backward sweep (all levels) */
     int64_t n_below_rows;            /* total update-vector rows (sum of |R_s|) */
+    double bytes_top;                /* dense top block of the substitution, per solve */
+    int64_t top_m;                   /* rows of the dense top block (0: none) */
+    int64_t n_levels_swept;          /* supernode levels swept per substitution (below the top block) */
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
@@ -159,9 +162,10 @@ typedef struct pdfc_kernel_times {  /* per-launch device time (HIP events), aver
     double local_us;                 /* local step + RHS + obstacle friction kernel */
     double forward_us;               /* one forward-substitution level */
     double backward_us;              /* one backward-substitution level */
+    double top_us;                   /* dense top block (gather + matvec + scatter) */
     double forward_sweep_us;         /* all forward levels of one substitution (sum of launches) */
     double backward_sweep_us;
-    int64_t local_launches, forward_launches, backward_launches;
+    int64_t local_launches, forward_launches, backward_launches, top_launches;
 } pdfc_kernel_times;
 
 /* Build the simulation: topology, masses, constraints, K, host FP64 factorisation, device upload. */

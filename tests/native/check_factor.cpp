@@ -96,6 +96,12 @@ int main(int argc, char** argv) {
             bn += b[3 * i + c] * b[3 * i + c];
         }
     int maxlvl_tasks = 0;
+    for (double mb : {16., 64., 256.}) {
+        TopPlan top;
+        plan_top(F, mb * 1048576., 8, top);
+        printf("top budget %.0f MB: level %d of %d, m=%d, contributions=%zu\n", mb, top.level, F.n_levels, top.m,
+               top.cidx.size());
+    }
     printf("n=%d supernodes=%d levels=%d nnzL=%lld nnzF=%lld max_ns=%d max_front=%d factor_s=%.3f resid=%.3e\n",
            n, S, F.n_levels, (long long)F.nnz_L, (long long)F.foff[S], F.max_ns, F.max_front, F.seconds, std::sqrt(rn / bn));
     (void)maxlvl_tasks;

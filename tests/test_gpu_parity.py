@@ -31,9 +31,22 @@ def contact_arrays(cs):
     return key, r, nrm, act
 
 
-def test_solve_matches_oracle(gpu):
+@pytest.fixture(params=["top", "levels"])
+def solve_mode(request, monkeypatch):
+    """'top': default dense top block (the whole matrix for small meshes);
+    'levels': supernodal sweeps only (PDFC_TOP_MB=0)."""
+    if request.param == "levels":
+        monkeypatch.setenv("PDFC_TOP_MB", "0")
+    else:
+        monkeypatch.delenv("PDFC_TOP_MB", raising=False)
+    return request.param
+
+
+def test_solve_matches_oracle(gpu, solve_mode):
     spec = scenes.sphere_drape_small()
     sim = gpu.Simulation(spec)
+    info = sim.info()
+    assert (info["top_m"] > 0) == (solve_mode == "top")
     orc = po.Oracle(spec)
     rng = np.random.default_rng(0)
     b = rng.standard_normal(3 * spec.n_vertices)
@@ -47,10 +60,14 @@ def test_solve_matches_oracle(gpu):
     assert res < 1e-12
 
 
-def test_solve_40k_residual(gpu):
+def test_solve_40k_residual(gpu, solve_mode):
     spec = scenes.sphere_drape_40k()
     spec.self_collision = False
     sim = gpu.Simulation(spec)
+    info = sim.info()
+    assert (info["top_m"] > 0) == (solve_mode == "top")
+    if solve_mode == "top":
+        assert info["n_levels_swept"] < info["n_levels"]
     orc = po.Oracle(spec, iterations=1)
     rng = np.random.default_rng(1)
     b = rng.standard_normal(3 * spec.n_vertices)
@@ -62,7 +79,7 @@ def test_solve_40k_residual(gpu):
 
 
 @pytest.mark.parametrize("steps", [200])
-def test_small_drape_identical_state_parity(gpu, steps):
+def test_small_drape_identical_state_parity(gpu, steps, solve_mode):
     """20x20 drape: every step from the oracle's state; q, v, contacts, reactions."""
     spec = scenes.sphere_drape_small()
     sim = gpu.Simulation(spec)
@@ -135,7 +152,7 @@ def test_inclined_plane_closed_form(gpu):
     assert len(sim.contacts()) == 4
 
 
-def test_fp32_refinement_error(gpu):
+def test_fp32_refinement_error(gpu, solve_mode):
     spec = scenes.sphere_drape_small()
     s64 = gpu.Simulation(spec)
     s32 = gpu.Simulation(spec, precision=abi.FP32_REFINE64, refinement_sweeps=3)
@@ -144,3 +161,15 @@ def test_fp32_refinement_error(gpu):
     x64 = s64.solve(b)
     x32 = s32.solve(b)
     assert rel_err(x32, x64) < 1e-12
+
+
+def test_fp32_refinement_error_40k(gpu, solve_mode):
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    s64 = gpu.Simulation(spec)
+    s32 = gpu.Simulation(spec, precision=abi.FP32_REFINE64, refinement_sweeps=2)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    err = rel_err(s32.solve(b), s64.solve(b))
+    print("fp32 + 2 refinement sweeps, max rel error vs fp64:", err)
+    assert err < 1e-10
