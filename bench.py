@@ -62,6 +62,7 @@ def main():
     sim.step(args.steps)  # returns after the stream is synchronised
     elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed
+    phases = sim.phase_stats()
 
     # kernel times: instrumented continuation of the same run (events around every launch)
     sim.set_kernel_timing(True)
@@ -101,9 +102,13 @@ def main():
                    "first_step": args.warmup, "self_contacts_last_step": info["n_self_contacts"],
                    "iterations": K, "n_vertices": info["n_vertices"], "nnz_L": info["nnz_L"],
                    "nnz_F": info["nnz_F"], "levels": info["n_levels"], "levels_swept": info["n_levels_swept"],
-                   "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"]},
+                   "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"],
+                   "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
+                   "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
+                   "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
+        "self_collision_host_us_per_step": round(phases["detection_us"], 1),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec)

@@ -123,6 +123,9 @@ class Info(C.Structure):
         ("bytes_top", C.c_double),
         ("top_m", C.c_int64),
         ("n_levels_swept", C.c_int64),
+        ("waves_forward", C.c_int32 * 32),
+        ("waves_backward", C.c_int32 * 32),
+        ("autotune_ms", C.c_double),
     ]
 
 

@@ -371,53 +371,42 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
                                                     double* __restrict__ Y) {
     extern __shared__ double lds[];
     __shared__ double part[W > 1 ? W : 1][3][64];
-    const int task = t0 + blockIdx.x;
-    const int s = Fa.task_s[task], r0 = Fa.task_r0[task];
-    const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
+    const TaskDesc d = Fa.desc[t0 + blockIdx.x];
+    const int ns = d.ns, f = d.f, c0 = d.c0, r0 = d.r0;
     const int n = Fa.n;
     double* bx = lds;
     double* by = lds + ns;
     double* bz = lds + 2 * ns;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int* pp = Fa.pull_ptr + d.pull_base;
+    // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
-        bx[t] = Bv[c0 + t];
-        by[t] = Bv[n + c0 + t];
-        bz[t] = Bv[2 * n + c0 + t];
+        double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
+        for (int q = pp[t]; q < pp[t + 1]; ++q) {
+            const double* u = Fa.U + 3 * (size_t)Fa.pull_idx[q];
+            x -= u[0];
+            y -= u[1];
+            z -= u[2];
+        }
+        bx[t] = x;
+        by[t] = y;
+        bz[t] = z;
     }
-    double ux = 0., uy = 0., uz = 0.;
+    // this task's below rows: children's entries to pass on
     const int my_row = r0 + lane;
-    __syncthreads();
-    for (int64_t q = Fa.choff[s]; q < Fa.choff[s + 1]; ++q) {
-        const int ch = Fa.children[q];
-        const int nbc = Fa.nb[ch];
-        const int* map = Fa.relmap + Fa.uoff[ch];
-        const double* Uc = Fa.U + 3 * Fa.uoff[ch];
-        for (int t = tid; t < nbc; t += 64 * W) {
-            int idx = map[t];
-            if (idx < ns) {
-                bx[idx] -= Uc[3 * t];
-                by[idx] -= Uc[3 * t + 1];
-                bz[idx] -= Uc[3 * t + 2];
-            }
+    double ux = 0., uy = 0., uz = 0.;
+    if (wave == 0 && my_row >= ns && my_row < f) {
+        for (int q = pp[my_row]; q < pp[my_row + 1]; ++q) {
+            const double* u = Fa.U + 3 * (size_t)Fa.pull_idx[q];
+            ux += u[0];
+            uy += u[1];
+            uz += u[2];
         }
-        // below-row contributions for this task's rows (each row appears at most once per child)
-        if (wave == 0 && my_row >= ns && my_row < f) {
-            int lo = 0, hi = nbc;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (map[mid] < my_row) lo = mid + 1; else hi = mid;
-            }
-            if (lo < nbc && map[lo] == my_row) {
-                ux += Uc[3 * lo];
-                uy += Uc[3 * lo + 1];
-                uz += Uc[3 * lo + 2];
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
-        const T* Fs = static_cast<const T*>(Fa.F) + Fa.foff[s] + my_row;
+        const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
 #pragma unroll 8
         for (int j = wave; j < ns; j += W) {
             double fv = as_d(Fs[(size_t)j * f]);
@@ -447,7 +436,7 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
         Y[n + c0 + my_row] = ay;
         Y[2 * n + c0 + my_row] = az;
     } else {
-        double* u = Fa.U + 3 * (Fa.uoff[s] + my_row - ns);
+        double* u = Fa.U + 3 * (d.uoff + my_row - ns);
         u[0] = ax + ux;
         u[1] = ay + uy;
         u[2] = az + uz;
@@ -461,9 +450,8 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
                                                      double* __restrict__ X) {
     extern __shared__ double lds[];
     __shared__ double part[W > 1 ? W : 1][3][64];
-    const int task = t0 + blockIdx.x;
-    const int s = Fa.task_s[task], j0 = Fa.task_r0[task];
-    const int ns = Fa.ns[s], nb = Fa.nb[s], f = ns + nb, c0 = Fa.c0[s];
+    const TaskDesc d = Fa.desc[t0 + blockIdx.x];
+    const int ns = d.ns, nb = d.nb, f = d.f, c0 = d.c0, j0 = d.r0;
     const int n = Fa.n;
     double* gx = lds;
     double* gy = lds + f;
@@ -474,7 +462,7 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
         gy[t] = Y[n + c0 + t];
         gz[t] = Y[2 * n + c0 + t];
     }
-    const int* R = Fa.rows + Fa.roff[s];
+    const int* R = Fa.rows + d.roff;
     for (int t = tid; t < nb; t += 64 * W) {
         int r = R[t];
         gx[ns + t] = -X[r];
@@ -485,7 +473,7 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
     const int j = j0 + lane;
     double ax = 0., ay = 0., az = 0.;
     if (j < ns) {
-        const T* Ft = static_cast<const T*>(Fa.FT) + Fa.foff[s] + j;
+        const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
 #pragma unroll 8
         for (int i = wave; i < f; i += W) {
             double fv = as_d(Ft[(size_t)i * ns]);
@@ -665,7 +653,7 @@ void launch_finish(const DevModel& M, double* qn, const double* vlast, double* v
 }
 // Waves per task: the reduction length (ns forward, ns+nb backward) divided into
 // chains of about 8 loads, at most 16 waves (1024 threads).
-static int waves_for(int kmax) {
+int default_waves(int kmax) {
     int w = 1;
     while (w < 16 && w * 8 < kmax) w <<= 1;
     return w;
@@ -695,20 +683,18 @@ static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const D
 }
 
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y, int max_ns,
-                          int kmax, hipStream_t st) {
+                          int W, hipStream_t st) {
     if (ntasks <= 0) return;
     size_t lds = (size_t)3 * max_ns * sizeof(double);
-    int W = waves_for(kmax);
     if (fp32)
         forward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, B, Y);
     else
         forward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, B, Y);
 }
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int kmax, hipStream_t st) {
+                           int max_front, int W, hipStream_t st) {
     if (ntasks <= 0) return;
     size_t lds = (size_t)3 * max_front * sizeof(double);
-    int W = waves_for(kmax);
     if (fp32)
         backward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, Y, X);
     else

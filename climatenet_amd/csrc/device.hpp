@@ -51,6 +51,17 @@ struct DevModel {
     const double* ata_val;
 };
 
+// One substitution task (supernode s, rows or columns r0 .. r0+63), everything the
+// kernel needs in one 64-byte load.
+struct TaskDesc {
+    int s, r0, ns, nb;
+    int c0, f, pull_base, pad;
+    int64_t foff;
+    int64_t uoff;
+    int64_t roff;
+    int64_t spare;
+};
+
 // Supernodal substitution operators (see factor.cpp).
 struct DevFactor {
     int n;
@@ -69,6 +80,9 @@ struct DevFactor {
     double* U;           // 3 * total below rows, interleaved
     const int* task_s;
     const int* task_r0;
+    const TaskDesc* desc;   // per task of this sweep
+    const int* pull_ptr;    // per supernode, f + 1 entries from pull_base: children's update
+    const int* pull_idx;    //   entries (index into U) contributing to each front row, child order
 };
 
 struct StepArgs {
@@ -107,7 +121,16 @@ struct DevSelfDetect {
     SelfCandidate* filt;
     int cap_full, cap_filt;
     int write_full;         // also write the complete candidate list
+    // single-pass path (filtered list only): triangle data sorted by bucket and a
+    // fixed number of candidate slots per vertex
+    double* bk_box;         // 6 per bucket slot
+    double* bk_pos;         // 9 per bucket slot: the three vertex positions
+    int* bk_vid;            // 3 per bucket slot: scene vertex indices
+    int* bk_tri;            // triangle index per bucket slot
+    SelfCandidate* vslots;  // n * kSlotsPerVertex
+    int* overflow;          // set when a vertex has more candidates than slots
 };
+constexpr int kSlotsPerVertex = 8;
 
 // Self contacts of the current step, sorted by computation level.
 struct DevSelf {
@@ -141,6 +164,10 @@ void launch_fill_int(int* p, int count, int value, hipStream_t st);
 void launch_copy_int(int* dst, const int* src, int count, hipStream_t st);
 void launch_scan_int(int* a, int count, int* sums, int* total, int* tail, hipStream_t st);
 void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st);
+// Single pass: filtered candidates only (the hot path); falls back to launch_self_detect
+// when *overflow is set.
+void launch_self_detect_filtered(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits,
+                                 hipStream_t st);
 void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st);
 void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st);
 void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
@@ -168,11 +195,13 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st);
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
-// kmax: longest reduction among the level's tasks (ns forward, ns + nb backward).
+// waves: 1, 2, 4, 8 or 16 waves per task splitting the reduction (ns forward, ns + nb backward).
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y,
-                          int max_ns, int kmax, hipStream_t st);
+                          int max_ns, int waves, hipStream_t st);
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int kmax, hipStream_t st);
+                           int max_front, int waves, hipStream_t st);
+// Heuristic wave count: chains of about 8 loads, at most 16 waves.
+int default_waves(int kmax);
 // Dense top block of the substitution (see TopPlan).
 struct DevTop {
     int m;

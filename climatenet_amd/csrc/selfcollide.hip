@@ -247,6 +247,126 @@ __global__ void k_vertex_query(DevSelfDetect D, int pass) {
     }
 }
 
+// Bucket-sorted copy of the triangle data (box, vertex positions, vertex ids).
+__global__ void k_tri_scatter_packed(DevSelfDetect D) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.T) return;
+    double c = cell_size(D);
+    unsigned h = cell_hash(cell_of(D.tbox[6 * t], c), cell_of(D.tbox[6 * t + 1], c), cell_of(D.tbox[6 * t + 2], c), D.P);
+    int q = D.bucket_off[h] + atomicAdd(&D.bucket_fill[h], 1);
+    D.bk_tri[q] = t;
+    for (int k = 0; k < 6; ++k) D.bk_box[6 * q + k] = D.tbox[6 * t + k];
+    for (int k = 0; k < 3; ++k) {
+        int v = D.tris[3 * t + k];
+        D.bk_vid[3 * q + k] = v;
+        P3 p = at_perm(D.q, D.n, D.iperm[v]);
+        D.bk_pos[9 * q + 3 * k] = p.x;
+        D.bk_pos[9 * q + 3 * k + 1] = p.y;
+        D.bk_pos[9 * q + 3 * k + 2] = p.z;
+    }
+}
+
+// One pass over the grid: filtered candidates into the vertex's slots. Eight lanes per
+// vertex, one grid cell each (a query box spans 2 x 2 x 2 cells, rarely up to 3 per
+// axis); slot order is arbitrary, the compaction sorts by triangle.
+constexpr int kLanesPerVertex = 8;
+__global__ void k_vertex_query_slots(DevSelfDetect D) {
+    __shared__ int slot_count[kB / kLanesPerVertex];
+    const int local_v = threadIdx.x / kLanesPerVertex, sub = threadIdx.x % kLanesPerVertex;
+    const int v = blockIdx.x * (kB / kLanesPerVertex) + local_v;
+    if (sub == 0) slot_count[local_v] = 0;
+    __syncthreads();
+    if (v < D.n) {
+        const double c = cell_size(D);
+        const double E = __longlong_as_double((long long)*D.ext_bits) * (1. + 1e-9);
+        P3 x = at_perm(D.q, D.n, D.iperm[v]);
+        const double lo[3] = {x.x - D.t11, x.y - D.t11, x.z - D.t11};
+        const double hi[3] = {x.x + D.t11, x.y + D.t11, x.z + D.t11};
+        long long c0[3], span[3];
+        bool sane = isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+        for (int k = 0; k < 3; ++k) {
+            c0[k] = sane ? cell_of(lo[k] - E, c) : 0;
+            span[k] = sane ? cell_of(hi[k], c) - c0[k] + 1 : 0;
+            if (span[k] > 3 || span[k] < 0) span[k] = 0;
+        }
+        const int ncell = (int)(span[0] * span[1] * span[2]);
+        SelfCandidate* out = D.vslots + (size_t)v * kSlotsPerVertex;
+        for (int ci = sub; ci < ncell; ci += kLanesPerVertex) {
+            const long long ix = c0[0] + ci % span[0];
+            const long long iy = c0[1] + (ci / span[0]) % span[1];
+            const long long iz = c0[2] + ci / (span[0] * span[1]);
+            unsigned h = cell_hash(ix, iy, iz, D.P);
+            int b0 = D.bucket_off[h], b1 = D.bucket_off[h + 1];
+            for (int q = b0; q < b1; ++q) {
+                const double* bx = &D.bk_box[6 * q];
+                if (cell_of(bx[0], c) != ix || cell_of(bx[1], c) != iy || cell_of(bx[2], c) != iz) continue;
+                if (!(bx[0] <= hi[0] && lo[0] <= bx[3] && bx[1] <= hi[1] && lo[1] <= bx[4] && bx[2] <= hi[2] &&
+                      lo[2] <= bx[5]))
+                    continue;
+                const int* vid = &D.bk_vid[3 * q];
+                if (vid[0] == v || vid[1] == v || vid[2] == v) continue;  // never kept by the pairing
+                const double* pp = &D.bk_pos[9 * q];
+                P3 a = {pp[0], pp[1], pp[2]}, b = {pp[3], pp[4], pp[5]}, cc = {pp[6], pp[7], pp[8]};
+                double bary[3];
+                P3 cp;
+                closest_point(x, a, b, cc, D.tol, bary, cp);
+                P3 d = psub(cp, x);
+                if (!(pdot(d, d) < D.tol_sq)) continue;
+                if (bary[0] == 0 || bary[1] == 0 || bary[2] == 0) continue;
+                int k = atomicAdd(&slot_count[local_v], 1);
+                if (k >= kSlotsPerVertex) {
+                    atomicOr(D.overflow, 1);
+                    continue;
+                }
+                SelfCandidate& o = out[k];
+                o.v = v;
+                o.t = D.bk_tri[q];
+                o.bary[0] = bary[0];
+                o.bary[1] = bary[1];
+                o.bary[2] = bary[2];
+                P3 cpb = {(a.x * bary[0] + b.x * bary[1]) + cc.x * bary[2],
+                          (a.y * bary[0] + b.y * bary[1]) + cc.y * bary[2],
+                          (a.z * bary[0] + b.z * bary[1]) + cc.z * bary[2]};
+                P3 nv = psub(x, cpb);
+                double z = pdot(nv, nv);
+                if (z > 0.) {
+                    double sq = sqrt(z);
+                    nv = {nv.x / sq, nv.y / sq, nv.z / sq};
+                }
+                o.normal[0] = nv.x;
+                o.normal[1] = nv.y;
+                o.normal[2] = nv.z;
+            }
+        }
+    }
+    __syncthreads();
+    if (v < D.n && sub == 0) {
+        int cnt = slot_count[local_v];
+        D.cnt_filt[v] = cnt < kSlotsPerVertex ? cnt : kSlotsPerVertex;
+    }
+}
+
+// Compaction of the slots, sorted by triangle index.
+__global__ void k_compact_slots(DevSelfDetect D) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= D.n) return;
+    const int cnt = D.cnt_filt[v];
+    const int base = D.off_filt[v];
+    if (cnt == 0 || base + cnt > D.cap_filt) return;
+    const SelfCandidate* in = D.vslots + (size_t)v * kSlotsPerVertex;
+    int order[kSlotsPerVertex];
+    for (int i = 0; i < cnt; ++i) order[i] = i;
+    for (int i = 1; i < cnt; ++i) {
+        int k = order[i], j = i - 1;
+        while (j >= 0 && in[order[j]].t > in[k].t) {
+            order[j + 1] = order[j];
+            --j;
+        }
+        order[j + 1] = k;
+    }
+    for (int i = 0; i < cnt; ++i) D.filt[base + i] = in[order[i]];
+}
+
 // Bitset (permuted numbering) of vertices with at least one obstacle contact.
 __global__ void k_seed_bits(const int* ccnt, int n, unsigned long long* bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -492,6 +612,25 @@ void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long l
     launch_scan_int(D.off_full, D.n, D.scan_sums, D.totals + 0, nullptr, st);
     launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
     hipLaunchKernelGGL(k_vertex_query, dim3(nblk(D.n)), dim3(kB), 0, st, D, 1);
+    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits);
+}
+
+void launch_self_detect_filtered(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, reinterpret_cast<int*>(D.ext_bits), 2, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, D.overflow, 1, 0);
+    hipLaunchKernelGGL(k_tri_box, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    launch_fill_int(D.bucket_cnt, D.P + 1, 0, st);
+    launch_fill_int(D.bucket_fill, D.P, 0, st);
+    hipLaunchKernelGGL(k_tri_count, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    launch_copy_int(D.bucket_off, D.bucket_cnt, D.P + 1, st);
+    launch_scan_int(D.bucket_off, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
+    hipLaunchKernelGGL(k_tri_scatter_packed, dim3(nblk(D.T)), dim3(kB), 0, st, D);
+    hipLaunchKernelGGL(k_vertex_query_slots, dim3((unsigned)((D.n + kB / kLanesPerVertex - 1) / (kB / kLanesPerVertex))),
+                       dim3(kB), 0, st, D);
+    launch_copy_int(D.off_filt, D.cnt_filt, D.n, st);
+    launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
+    hipLaunchKernelGGL(k_compact_slots, dim3(nblk(D.n)), dim3(kB), 0, st, D);
     hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits);
 }
 

@@ -74,6 +74,10 @@ struct pdfc_sim {
     bool use_top = false;
     int64_t nnzF_lower = 0;
     int* task_buf = nullptr;
+    pdfc::TaskDesc* desc_buf = nullptr;
+    std::vector<int> fwd_waves, bwd_waves;  // waves per task, per level (autotuned)
+    double autotune_ms = 0.;
+    void autotune_waves();
     void upload_sweeps(int top_level);
     void build_top();
     // kernel-timing instrumentation
@@ -111,6 +115,8 @@ struct pdfc_sim {
     int nsc = 0, n_self_levels = 0;
     std::vector<int> lvl_counts, grp_counts;       // per level
     bool keep_full_candidates = false;
+    size_t filt_guess = 4096;
+    double self_host_us = 0.;   // host time in detection round trips + pairing, accumulated per pdfc_step
     int n_full_candidates = 0;
 
     void free_ptr(void* p) {
@@ -380,6 +386,12 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
         sd.cap_full = 0;  // allocated on demand (pdfc_detect)
         sd.cap_filt = std::max(1024, n);
         sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
+        sd.bk_box = dalloc<double>(6 * (size_t)T);
+        sd.bk_pos = dalloc<double>(9 * (size_t)T);
+        sd.bk_vid = dalloc<int>(3 * (size_t)T);
+        sd.bk_tri = dalloc<int>(T);
+        sd.vslots = dalloc<pdfc::SelfCandidate>((size_t)n * pdfc::kSlotsPerVertex);
+        sd.overflow = dalloc<int>(1);
         seed_bits = dalloc<unsigned long long>(n / 64 + 1);
         host_bits.resize(n / 64 + 1);
         vslot = dalloc<int>(n);
@@ -396,6 +408,10 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
         upload_sweeps(top.level);
         use_top = true;
     }
+    {
+        const char* e = std::getenv("PDFC_AUTOTUNE");
+        if (!e || std::atoi(e) != 0) autotune_waves();
+    }
     nnzF_lower = 0;
     for (size_t s2 = 0; s2 < fac.ns.size(); ++s2)
         if (!use_top || fac.level[s2] < top.level) nnzF_lower += fac.foff[s2 + 1] - fac.foff[s2];
@@ -411,11 +427,122 @@ void pdfc_sim::upload_sweeps(int top_level) {
     all.insert(all.end(), bwd.task_snode.begin(), bwd.task_snode.end());
     all.insert(all.end(), bwd.task_row0.begin(), bwd.task_row0.end());
     task_buf = upload(all);
+    fwd_waves.clear();
+    bwd_waves.clear();
+    for (int k : fwd.level_kmax) fwd_waves.push_back(pdfc::default_waves(k));
+    for (int k : bwd.level_kmax) bwd_waves.push_back(pdfc::default_waves(k));
     size_t nf = fwd.task_snode.size(), nbk = bwd.task_snode.size();
     dfwd.task_s = task_buf;
     dfwd.task_r0 = task_buf + nf;
     dbwd.task_s = task_buf + 2 * nf;
     dbwd.task_r0 = task_buf + 2 * nf + nbk;
+    // pull lists: for every front row of every supernode, the children's update entries
+    // that land on it, in child order (the order the reference-style extend-add uses)
+    const int S = (int)fac.ns.size();
+    std::vector<int> base(S), pull_ptr, pull_idx;
+    std::vector<std::vector<int>> rowlist;
+    for (int s2 = 0; s2 < S; ++s2) {
+        const int f = fac.ns[s2] + fac.nb[s2];
+        rowlist.assign(f, {});
+        for (int64_t q = fac.choff[s2]; q < fac.choff[s2 + 1]; ++q) {
+            int c = fac.children[q];
+            for (int k = 0; k < fac.nb[c]; ++k) rowlist[fac.relmap[fac.mapoff[c] + k]].push_back((int)(fac.uoff[c] + k));
+        }
+        base[s2] = (int)pull_ptr.size();
+        pull_ptr.push_back((int)pull_idx.size());
+        for (int r = 0; r < f; ++r) {
+            pull_idx.insert(pull_idx.end(), rowlist[r].begin(), rowlist[r].end());
+            pull_ptr.push_back((int)pull_idx.size());
+        }
+    }
+    auto descs = [&](const pdfc::SweepPlan& P) {
+        std::vector<pdfc::TaskDesc> D(P.task_snode.size());
+        for (size_t t = 0; t < D.size(); ++t) {
+            int s2 = P.task_snode[t];
+            pdfc::TaskDesc& d = D[t];
+            d.s = s2;
+            d.r0 = P.task_row0[t];
+            d.ns = fac.ns[s2];
+            d.nb = fac.nb[s2];
+            d.c0 = fac.c0[s2];
+            d.f = fac.ns[s2] + fac.nb[s2];
+            d.pull_base = base[s2];
+            d.pad = 0;
+            d.foff = fac.foff[s2];
+            d.uoff = fac.uoff[s2];
+            d.roff = fac.roff[s2];
+            d.spare = 0;
+        }
+        return D;
+    };
+    if (!dfwd.pull_ptr) {
+        dfwd.pull_ptr = dbwd.pull_ptr = upload(pull_ptr);
+        dfwd.pull_idx = dbwd.pull_idx = upload(pull_idx);
+    }
+    if (desc_buf) free_ptr(desc_buf);
+    std::vector<pdfc::TaskDesc> df = descs(fwd), db = descs(bwd);
+    df.insert(df.end(), db.begin(), db.end());
+    desc_buf = upload(df);
+    dfwd.desc = desc_buf;
+    dbwd.desc = desc_buf + nf;
+}
+
+// Chooses the waves per task of every substitution level by timing each level alone
+// with 1, 2, 4, 8 and 16 waves on the device (median of 5 launches). The buffers hold
+// valid data from the top-block build or zeros; only the timing matters here.
+void pdfc_sim::autotune_waves() {
+    auto t0 = std::chrono::steady_clock::now();
+    pdfc::launch_fill(V[1], 3 * (size_t)n, 0., stream);
+    pdfc::launch_fill(Y, 3 * (size_t)n, 0., stream);
+    pdfc::launch_fill(dfwd.U, 3 * (size_t)fac.uoff.back(), 0., stream);
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    auto time_it = [&](auto&& launch) {
+        std::vector<float> t;
+        launch();  // warm
+        for (int r = 0; r < 5; ++r) {
+            HIP_CHECK(hipEventRecord(a, stream));
+            launch();
+            HIP_CHECK(hipEventRecord(b, stream));
+            HIP_CHECK(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+            t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return t[2];
+    };
+    const int cand[5] = {1, 2, 4, 8, 16};
+    for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l) {
+        float best = 1e30f;
+        for (int W : cand) {
+            float ms = time_it([&] {
+                pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l], V[1],
+                                           Y, fac.max_ns, W, stream);
+            });
+            if (ms < best) {
+                best = ms;
+                fwd_waves[l] = W;
+            }
+        }
+    }
+    for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l) {
+        float best = 1e30f;
+        for (int W : cand) {
+            float ms = time_it([&] {
+                pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
+                                            V[1], fac.max_front, W, stream);
+            });
+            if (ms < best) {
+                best = ms;
+                bwd_waves[l] = W;
+            }
+        }
+    }
+    HIP_CHECK(hipEventDestroy(a));
+    HIP_CHECK(hipEventDestroy(b));
+    autotune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // S^-1 = (K^-1)_TT, one column triple per solve with the full sweeps.
@@ -472,13 +599,13 @@ void pdfc_sim::solve_inplace(double* X) {
         for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
             timed(1, [&] {
                 pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
-                                           inout, Y, fac.max_ns, fwd.level_kmax[l], stream);
+                                           inout, Y, fac.max_ns, fwd_waves[l], stream);
             });
         if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream); });
         for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
             timed(2, [&] {
                 pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
-                                            inout, fac.max_front, bwd.level_kmax[l], stream);
+                                            inout, fac.max_front, bwd_waves[l], stream);
             });
         if (ktiming) ++ksolves;
     };
@@ -503,14 +630,34 @@ void pdfc_sim::detect() {
 // Device detection, host pairing and level ordering, upload of the step's self
 // contacts (geometry_util.cpp:94-206, SimulationFrictionEstimation.cpp:107-293).
 void pdfc_sim::self_detect_and_pair(bool write_full) {
-    for (int attempt = 0; attempt < 3; ++attempt) {
+    auto t_host0 = std::chrono::steady_clock::now();
+    // One synchronisation in the common case: counts, seed bits and a guessed prefix of
+    // the filtered list come back together; a second copy only if the guess was short.
+    size_t guess = std::min<size_t>((size_t)sd.cap_filt, (size_t)filt_guess);
+    if (host_filt.size() < guess) host_filt.resize(guess);
+    bool single_pass = !write_full;
+    for (int attempt = 0; attempt < 4; ++attempt) {
         sd.q = qn;
         sd.write_full = write_full ? 1 : 0;
-        pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
+        if (single_pass)
+            pdfc::launch_self_detect_filtered(sd, ccnt, seed_bits, stream);
+        else
+            pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
         HIP_CHECK(hipMemcpyAsync(host_totals.data(), sd.totals, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        int overflow = 0;
+        if (single_pass)
+            HIP_CHECK(hipMemcpyAsync(&overflow, sd.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipMemcpyAsync(host_bits.data(), seed_bits, host_bits.size() * sizeof(unsigned long long),
                                  hipMemcpyDeviceToHost, stream));
+        if (guess > 0)
+            HIP_CHECK(hipMemcpyAsync(host_filt.data(), sd.filt, guess * sizeof(pdfc::SelfCandidate),
+                                     hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+        if (single_pass && overflow) {  // a vertex has more candidates than slots: two-pass path
+            single_pass = false;
+            guess = 0;
+            continue;
+        }
         bool grow = false;
         if (host_totals[1] > sd.cap_filt) {
             free_ptr(sd.filt);
@@ -529,23 +676,27 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
             grow = true;
         }
         if (!grow) break;
-        if (attempt == 2) throw DeviceError("self-collision candidate buffers could not be sized");
+        if (attempt == 3) throw DeviceError("self-collision candidate buffers could not be sized");
+        guess = 0;
     }
-    n_full_candidates = host_totals[0];
-    const int nf = host_totals[1];
-    host_filt.resize(nf);
-    if (nf > 0) {
-        HIP_CHECK(hipMemcpyAsync(host_filt.data(), sd.filt, nf * sizeof(pdfc::SelfCandidate), hipMemcpyDeviceToHost,
-                                 stream));
+    if (!write_full) n_full_candidates = -1;
+    else n_full_candidates = host_totals[0];
+    const size_t nf = (size_t)host_totals[1];
+    if (nf > guess) {
+        host_filt.resize(nf);
+        HIP_CHECK(hipMemcpyAsync(host_filt.data() + guess, sd.filt + guess, (nf - guess) * sizeof(pdfc::SelfCandidate),
+                                 hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
-    pdfc::pair_self_contacts(model, host_filt.data(), host_filt.size(), pairs);
+    filt_guess = nf + nf / 2 + 256;
+    pdfc::pair_self_contacts(model, host_filt.data(), nf, pairs);
     std::vector<int> seeds;
     for (int i = 0; i < n; ++i)
         if (host_bits[i >> 6] >> (i & 63) & 1ull) seeds.push_back(fac.perm[i]);
     n_self_levels = pdfc::order_self_contacts(n, seeds, pairs, levels);
     nsc = (int)pairs.size();
     if (nsc > 0) self_upload();
+    self_host_us += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t_host0).count();
 }
 
 // Packs the self contacts sorted by level, the involved-vertex slots and the
@@ -809,6 +960,7 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         if (!sim || nsteps < 0) throw std::invalid_argument("bad argument");
         if (nsteps == 0) return;
         HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
+        sim->self_host_us = 0.;
         for (int s = 0; s < nsteps; ++s) {
             sim->run_step();
             if (sim->ktiming) sim->collect_kernel_times();
@@ -818,6 +970,7 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         float ms_total = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
         sim->stats.step_us = 1e3 * ms_total / nsteps;
+        sim->stats.detection_us = sim->self_host_us / nsteps;
         sim->stats.steps = nsteps;
         sim->steps_done += nsteps;
     });
@@ -931,7 +1084,7 @@ int pdfc_get_self_candidates(pdfc_sim* sim, pdfc_candidate* out, size_t cap, siz
             *n = 0;
             return;
         }
-        if (sim->sd.cap_full == 0 || sim->n_full_candidates > sim->sd.cap_full)
+        if (sim->n_full_candidates < 0 || sim->sd.cap_full == 0 || sim->n_full_candidates > sim->sd.cap_full)
             throw std::invalid_argument("no complete candidate list: call pdfc_detect first");
         const int nc = sim->n_full_candidates;
         *n = (size_t)nc;
@@ -1002,7 +1155,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         I.n_contacts = nc;
         I.n_self_contacts = sim->nsc;
-        I.n_self_candidates = sim->n_full_candidates;
+        I.n_self_candidates = sim->n_full_candidates >= 0 ? sim->n_full_candidates : sim->host_totals[1];
         I.n_self_levels = sim->n_self_levels;
         I.max_front = sim->fac.max_front;
         I.factor_seconds = sim->fac.seconds;
@@ -1027,6 +1180,9 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         if (sim->fp32) solve = solve * (1. + sim->prm.refinement_sweeps) +
                                sim->prm.refinement_sweeps * (4. * 24. * n + 19. * 12. * n);
         I.bytes_top = topb;
+        for (size_t l = 0; l < sim->fwd_waves.size() && l < 32; ++l) I.waves_forward[l] = sim->fwd_waves[l];
+        for (size_t l = 0; l < sim->bwd_waves.size() && l < 32; ++l) I.waves_backward[l] = sim->bwd_waves[l];
+        I.autotune_ms = sim->autotune_ms;
         I.top_m = (int64_t)mt;
         I.n_levels_swept = (int64_t)(sim->fwd.level_ptr.size() - 1);
         I.bytes_local = local;

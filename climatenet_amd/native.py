@@ -136,7 +136,11 @@ class Simulation:
     def info(self):
         I = abi.Info()
         _check(lib().pdfc_get_info(self.h, C.byref(I)))
-        return {f: getattr(I, f) for f, _ in abi.Info._fields_}
+        out = {}
+        for f, _ in abi.Info._fields_:
+            v = getattr(I, f)
+            out[f] = list(v) if hasattr(v, "__len__") else v
+        return out
 
     def phase_stats(self):
         P = abi.PhaseStats()

@@ -136,7 +136,8 @@ typedef struct pdfc_info {
     int64_t n_levels;                /* supernodal elimination-tree levels */
     int64_t n_contacts;              /* obstacle contacts of the last step */
     int64_t n_self_contacts;         /* self contacts of the last step */
-    int64_t n_self_candidates;       /* exact-test survivors of the last step */
+    int64_t n_self_candidates;       /* exact-test survivors of the last detection (after pdfc_detect: all of
+                                        them; after pdfc_step: those that pass the pairing pre-filter) */
     int64_t n_self_levels;
     int64_t max_front;               /* largest supernode (rows) */
     double factor_seconds;           /* host symbolic + numeric factorisation time */
@@ -149,11 +150,14 @@ typedef struct pdfc_info {
     double bytes_top;                /* dense top block of the substitution, per solve */
     int64_t top_m;                   /* rows of the dense top block (0: none) */
     int64_t n_levels_swept;          /* supernode levels swept per substitution (below the top block) */
+    int32_t waves_forward[32];       /* waves per task chosen for each forward level (autotuned) */
+    int32_t waves_backward[32];      /* same for the backward levels (root side first) */
+    double autotune_ms;              /* time spent choosing them at creation */
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
     double step_us;
-    double detection_us;
+    double detection_us;             /* host time per step of the self-collision round trip and pairing */
     double iterations_us;
     int64_t steps;
 } pdfc_phase_stats;
