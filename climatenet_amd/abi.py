@@ -5,6 +5,8 @@ C++ host layer fills). Field order and types must match the header exactly.
 """
 import ctypes as C
 
+import numpy as np
+
 PDFC_OK = 0
 PDFC_ERR_INVALID = -1
 PDFC_ERR_DEVICE = -2
@@ -18,7 +20,12 @@ FP64 = 0
 FP32_REFINE64 = 1
 
 WARM_PLAIN = 0
+WARM_PREVIOUS = 1
 WARM_LEARNED = 2
+MLP_FEATURES = 12
+MLP_OUTPUTS = 3
+MLP_PARAMS = 10448
+MLP_NORM = 27
 
 
 class Obstacle(C.Structure):
@@ -61,6 +68,11 @@ class Params(C.Structure):
         ("refinement_sweeps", C.c_int32),
         ("warm_start", C.c_int32),
         ("record_iterations", C.c_int32),
+        ("residual_tolerance", C.c_double),
+        ("max_iterations", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("warm_hold", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -135,6 +147,11 @@ class PhaseStats(C.Structure):
         ("detection_us", C.c_double),
         ("iterations_us", C.c_double),
         ("steps", C.c_int64),
+        ("iterations", C.c_int64),
+        ("iterations_to_tolerance", C.c_int64),
+        ("contact_steps", C.c_int64),
+        ("converged_steps", C.c_int64),
+        ("fallbacks", C.c_int64),
     ]
 
 
@@ -162,3 +179,11 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def fptr(a):
+    """Pointer to a C-contiguous float32 numpy array (or NULL for None)."""
+    if a is None:
+        return C.POINTER(C.c_float)()
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))

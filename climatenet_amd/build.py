@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpdfc.so")
-SOURCES = ["device.hip", "selfcollide.hip", "sim.cpp", "model.cpp", "factor.cpp", "selfcontact.cpp"]
+SOURCES = ["device.hip", "selfcollide.hip", "mlp.hip", "sim.cpp", "model.cpp", "factor.cpp", "selfcontact.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PDFC_OFFLOAD_ARCH", "gfx950")
 

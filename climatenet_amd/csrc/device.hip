@@ -124,6 +124,17 @@ __global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const do
     st3(cvec, n, i, D3{m * t.x - a.dt * s.x, m * t.y - a.dt * s.y, m * t.z - a.dt * s.z});
 }
 
+// Obstacles vertex i meets: all of them, or those of its scene when scenes are batched.
+__device__ __forceinline__ void obstacle_range(const DevObstacles& ob, int i, int& o0, int& o1) {
+    o0 = 0;
+    o1 = ob.count;
+    if (ob.vinst) {
+        const int k = ob.vinst[i];
+        o0 = ob.obs_ptr[k];
+        o1 = ob.obs_ptr[k + 1];
+    }
+}
+
 // Obstacle detection at q_n: StaticSphere/Sphere::isInside (Sphere.cpp:93-113),
 // StaticPlane (StaticPlane.cpp:57-73). One bit per obstacle.
 __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObstacles ob, int* cflags, int* ccnt,
@@ -135,16 +146,16 @@ __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObs
     if (i < n) {
         D3 x = ld3(qn, n, i);
         int flags = 0;
-        for (int o = 0; o < ob.count; ++o) {
+        int o0, o1;
+        obstacle_range(ob, i, o0, o1);
+        for (int o = o0; o < o1; ++o) {
             bool hit;
-            if (ob.type[o] == 0) {
-                D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
+            D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
+            if (ob.type[o] == 0)
                 hit = sqrt(dot3(r, r)) < ob.radius[o] + 1e-2;
-            } else {
-                D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
-                hit = dot3(r, D3{ob.nrm[o][0], ob.nrm[o][1], ob.nrm[o][2]}) < 1.5e-2;
-            }
-            if (hit) flags |= 1 << o;
+            else
+                hit = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]}) < 1.5e-2;
+            if (hit) flags |= 1 << (o - o0);
         }
         cnt = __popc(flags);
         cflags[i] = flags;
@@ -206,17 +217,20 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
     if (!flags) return;
     D3 x = ld3(qn, n, i);
     int slot = start;
-    for (int o = 0; o < ob.count; ++o) {
-        if (!(flags & (1 << o))) continue;
+    int o0, o1;
+    obstacle_range(ob, i, o0, o1);
+    for (int o = o0; o < o1; ++o) {
+        if (!(flags & (1 << (o - o0)))) continue;
         D3 nrm, pt;
+        const D3 c = {ob.p[3 * o], ob.p[3 * o + 1], ob.p[3 * o + 2]};
         if (ob.type[o] == 0) {
-            D3 r = normalized3(D3{x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]});
+            D3 r = normalized3(D3{x.x - c.x, x.y - c.y, x.z - c.z});
             double R = ob.radius[o];
-            pt = {ob.p[o][0] + R * r.x, ob.p[o][1] + R * r.y, ob.p[o][2] + R * r.z};
+            pt = {c.x + R * r.x, c.y + R * r.y, c.z + R * r.z};
             nrm = r;
         } else {
-            nrm = {ob.nrm[o][0], ob.nrm[o][1], ob.nrm[o][2]};
-            D3 r = {x.x - ob.p[o][0], x.y - ob.p[o][1], x.z - ob.p[o][2]};
+            nrm = {ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]};
+            D3 r = {x.x - c.x, x.y - c.y, x.z - c.z};
             double d = dot3(r, nrm);
             pt = {x.x - nrm.x * d, x.y - nrm.y * d, x.z - nrm.z * d};
         }
@@ -230,6 +244,39 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
         C.r[3 * slot] = 0.; C.r[3 * slot + 1] = 0.; C.r[3 * slot + 2] = 0.;
         ++slot;
     }
+}
+
+// (A^T p)_i at q = q_n + dt v: sum over the incident springs of w^2 L0 (q_i - q_j)/|q_i - q_j|.
+__device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __restrict__ qn,
+                                         const double* __restrict__ vcur, double dt, int i, D3 q) {
+    const int n = M.n;
+    D3 s = {0., 0., 0.};
+    int deg = M.sp_cnt[i];
+    for (int k = 0; k < deg; ++k) {
+        int j = M.sp_idx[(size_t)k * n + i];
+        double kap = M.sp_kappa[(size_t)k * n + i];
+        D3 qj = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
+        D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
+        s.x += kap * e.x;
+        s.y += kap * e.y;
+        s.z += kap * e.z;
+    }
+    return s;
+}
+
+// (A^T A v)_i
+__device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restrict__ vcur, int i) {
+    const int n = M.n;
+    D3 av = {0., 0., 0.};
+    int cnt = M.ata_cnt[i];
+    for (int k = 0; k < cnt; ++k) {
+        int j = M.ata_idx[(size_t)k * n + i];
+        double w = M.ata_val[(size_t)k * n + i];
+        av.x += w * vcur[j];
+        av.y += w * vcur[n + j];
+        av.z += w * vcur[2 * n + j];
+    }
+    return av;
 }
 
 // Local step + right-hand side + obstacle friction, one thread per vertex:
@@ -249,32 +296,14 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
         const double dt = a.dt;
         D3 v = ld3(vcur, n, i);
         D3 q = {qn[i] + dt * v.x, qn[n + i] + dt * v.y, qn[2 * n + i] + dt * v.z};
-        D3 s = {0., 0., 0.};
-        int deg = M.sp_cnt[i];
-        for (int k = 0; k < deg; ++k) {
-            int j = M.sp_idx[(size_t)k * n + i];
-            double kap = M.sp_kappa[(size_t)k * n + i];
-            D3 qj = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
-            D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
-            s.x += kap * e.x;
-            s.y += kap * e.y;
-            s.z += kap * e.z;
-        }
+        D3 s = spring_sum(M, qn, vcur, dt, i, q);
         int nc = ccnt[i];
         const int slot = ls.selfmode ? ls.vslot[i] : -1;
         double damp = (nc > 0 || slot >= 0) ? 0. : a.damping;
         D3 c = ld3(cvec, n, i);
         D3 b = {c.x + dt * (s.x - damp * v.x), c.y + dt * (s.y - damp * v.y), c.z + dt * (s.z - damp * v.z)};
         if (nc > 0 || slot >= 0) {
-            D3 av = {0., 0., 0.};
-            int cnt = M.ata_cnt[i];
-            for (int k = 0; k < cnt; ++k) {
-                int j = M.ata_idx[(size_t)k * n + i];
-                double w = M.ata_val[(size_t)k * n + i];
-                av.x += w * vcur[j];
-                av.y += w * vcur[n + j];
-                av.z += w * vcur[2 * n + j];
-            }
+            D3 av = ata_row(M, vcur, i);
             const double dt2 = dt * dt;
             D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
             D3 Fs = F;  // what the obstacle contacts see
@@ -297,12 +326,26 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
                     ac_m = fmax(ac_m, an);
                 }
                 D3 fl = {dot3(bn, Fs), dot3(bt, Fs), dot3(bb, Fs)};
-                if (fl.x > 0.) {
+                double r0, r1, r2;
+                bool warm = false;
+                if (a.warm_mode != 0) {  // the host clears warm_mode once the hold ends
+                    const float* y = a.warm_y + (size_t)cid * a.warm_stride;
+                    const double m = M.mass[i];
+                    r0 = m * (double)y[0];
+                    r1 = m * (double)y[1];
+                    r2 = m * (double)y[2];
+                    warm = a.warm_mode == 2 || r0 != 0. || r1 != 0. || r2 != 0.;
+                }
+                if (warm ? !(r0 > 0.) : fl.x > 0.) {
                     r[0] = 0.; r[1] = 0.; r[2] = 0.;
                     C.active[cid] = 0;
                     continue;
                 }
-                double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
+                if (!warm) {
+                    r0 = -fl.x;
+                    r1 = -fl.y;
+                    r2 = -fl.z;
+                }
                 double rt = sqrt(r1 * r1 + r2 * r2);
                 if (rt > mu * r0) {
                     r1 *= mu * r0 / rt;
@@ -334,7 +377,7 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
 // :64-87) and the friction residual of the last iteration.
 __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __restrict__ vlast,
                          double* __restrict__ vstate, const int* __restrict__ ccnt, const int* __restrict__ cstart,
-                         DevContacts C, StepArgs a, int iteration) {
+                         DevContacts C, DevObstacles ob, StepArgs a, int iteration) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
     double ac_s = 0., ac_m = 0.;
@@ -354,8 +397,79 @@ __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __re
         qn[n + i] += a.dt * v.y;
         qn[2 * n + i] += a.dt * v.z;
         st3(vstate, n, i, v);
+        if (a.prev_g) {
+            int o0, o1;
+            obstacle_range(ob, i, o0, o1);
+            for (int o = 0; o < o1 - o0; ++o) {
+                float* g = a.prev_g + 3 * ((size_t)o * n + i);
+                g[0] = 0.f; g[1] = 0.f; g[2] = 0.f;
+            }
+            int nc = ccnt[i], c0 = cstart[i];
+            for (int k = 0; k < nc; ++k) {
+                const int cid = c0 + k;
+                if (!C.active[cid]) continue;
+                const double* B = &C.basis[9 * cid];
+                const double* r = &C.r[3 * cid];
+                float* g = a.prev_g + 3 * ((size_t)(C.obstacle[cid] - o0) * n + i);
+                g[0] = (float)((B[0] * r[0] + B[3] * r[1]) + B[6] * r[2]);
+                g[1] = (float)((B[1] * r[0] + B[4] * r[1]) + B[7] * r[2]);
+                g[2] = (float)((B[2] * r[0] + B[5] * r[1]) + B[8] * r[2]);
+            }
+        }
     }
     if (a.record) block_ac_accumulate(ac_s, ac_m, &a.ac_sum[iteration], &a.ac_max[iteration]);
+}
+
+// Warm-start features of each obstacle contact at the start of the step (v = v^0):
+// relative velocity and previous reaction in the contact basis, normal, mu, gap at q_n,
+// and the normal contact force of the plain warm start, the last two per unit mass.
+__global__ void k_warm_features(DevModel M, const double* __restrict__ qn, const double* __restrict__ v0,
+                                const double* __restrict__ cvec, const int* __restrict__ ncontacts, DevContacts C,
+                                DevObstacles ob, StepArgs a, float* __restrict__ feat) {
+    const int cid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cid >= *ncontacts) return;
+    const int n = M.n;
+    const int i = C.vertex[cid], o = C.obstacle[cid];
+    const double dt = a.dt;
+    const double m = M.mass[i], im = 1. / m;
+    const double* B = &C.basis[9 * cid];
+    D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+    D3 v = ld3(v0, n, i);
+    D3 x = ld3(qn, n, i);
+    D3 q = {x.x + dt * v.x, x.y + dt * v.y, x.z + dt * v.z};
+    D3 s = spring_sum(M, qn, v0, dt, i, q);
+    D3 c = ld3(cvec, n, i);
+    D3 av = ata_row(M, v0, i);
+    const double dt2 = dt * dt;
+    D3 F = {c.x + dt * s.x - dt2 * av.x, c.y + dt * s.y - dt2 * av.y, c.z + dt * s.z - dt2 * av.z};
+    double gap;
+    {
+        D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
+        if (ob.type[o] == 0)
+            gap = sqrt(dot3(r, r)) - ob.radius[o];
+        else
+            gap = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]});
+    }
+    D3 g = {0., 0., 0.};
+    if (a.prev_g) {
+        int o0, o1;
+        obstacle_range(ob, i, o0, o1);
+        const float* pg = a.prev_g + 3 * ((size_t)(o - o0) * n + i);
+        g = {(double)pg[0], (double)pg[1], (double)pg[2]};
+    }
+    float* f = feat + (size_t)cid * kWarmFeatures;
+    f[0] = (float)dot3(bn, v);
+    f[1] = (float)dot3(bt, v);
+    f[2] = (float)dot3(bb, v);
+    f[3] = (float)bn.x;
+    f[4] = (float)bn.y;
+    f[5] = (float)bn.z;
+    f[6] = (float)(im * dot3(bn, g));
+    f[7] = (float)(im * dot3(bt, g));
+    f[8] = (float)(im * dot3(bb, g));
+    f[9] = (float)C.mu[cid];
+    f[10] = (float)gap;
+    f[11] = (float)(im * dot3(bn, F));
 }
 
 template <typename T>
@@ -647,9 +761,17 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
                        iteration, ls);
 }
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
-                   const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vlast, vstate, ccnt, cstart, C, a,
-                       iteration);
+                   const int* cstart, DevContacts C, const DevObstacles& ob, const StepArgs& a, int iteration,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vlast, vstate, ccnt, cstart, C, ob,
+                       a, iteration);
+}
+void launch_warm_features(const DevModel& M, const double* qn, const double* v0, const double* cvec,
+                          const int* ncontacts, int cap, DevContacts C, const DevObstacles& ob, const StepArgs& a,
+                          float* feat, hipStream_t st) {
+    if (cap > 0)
+        hipLaunchKernelGGL(k_warm_features, dim3(blocks_for(cap)), dim3(kBlock), 0, st, M, qn, v0, cvec, ncontacts, C,
+                           ob, a, feat);
 }
 // Waves per task: the reduction length (ns forward, ns+nb backward) divided into
 // chains of about 8 loads, at most 16 waves (1024 threads).

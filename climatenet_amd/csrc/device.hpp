@@ -14,16 +14,20 @@
 
 namespace pdfc {
 
-constexpr int kMaxObstacles = 16;
+constexpr int kMaxObstacles = 16;  // per scene (one contact bit each)
 constexpr int kSolveChunk = 64;  // rows per substitution task (one wave)
 
+// Obstacles in device memory. With batched scenes a vertex meets only the
+// obstacles of its own scene.
 struct DevObstacles {
     int count;
-    int type[kMaxObstacles];
-    double p[kMaxObstacles][3];
-    double nrm[kMaxObstacles][3];
-    double radius[kMaxObstacles];
-    double mu[kMaxObstacles];
+    const int* type;
+    const double* p;       // 3 per obstacle: sphere centre or plane point
+    const double* nrm;     // 3 per obstacle: plane normal
+    const double* radius;
+    const double* mu;      // friction coefficient against the scene's cloth
+    const int* vinst;      // per vertex: its scene; null when there is one scene
+    const int* obs_ptr;    // per scene k: obstacles [obs_ptr[k], obs_ptr[k+1])
 };
 
 // Obstacle contact record (structure of arrays, capacity n * n_obstacles).
@@ -92,6 +96,12 @@ struct StepArgs {
     int record;        // accumulate Alart-Curnier norms
     double* ac_sum;    // per iteration
     unsigned long long* ac_max;  // per iteration, bit pattern of a non-negative double
+    // warm start of obstacle contacts in the iterations launched with warm_mode != 0:
+    // r = m_i * warm_y[cid * warm_stride + 0..2] (contact basis, per unit mass), projected on the cone
+    int warm_mode;     // 0 plain, 1 previous step (plain where it is zero), 2 learned
+    int warm_stride;
+    const float* warm_y;
+    float* prev_g;     // end of step: world reaction per (scene obstacle slot, vertex), 3 * n each; null: not kept
 };
 
 // Self-collision detection buffers (scene numbering for vertices and triangles).
@@ -193,7 +203,13 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
                   const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,
                   const LocalSelf& ls, hipStream_t st);
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
-                   const int* cstart, DevContacts C, const StepArgs& a, int iteration, hipStream_t st);
+                   const int* cstart, DevContacts C, const DevObstacles& ob, const StepArgs& a, int iteration,
+                   hipStream_t st);
+// Per obstacle contact: the 12 warm-start features (layout of pdfc_get_warm_data) at v^0.
+constexpr int kWarmFeatures = 12;
+void launch_warm_features(const DevModel& M, const double* qn, const double* v0, const double* cvec,
+                          const int* ncontacts, int cap, DevContacts C, const DevObstacles& ob, const StepArgs& a,
+                          float* feat, hipStream_t st);
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
 // waves: 1, 2, 4, 8 or 16 waves per task splitting the reduction (ns forward, ns + nb backward).
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y,

@@ -118,9 +118,30 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
     st.leaf = std::max(8, leaf_size);
     st.bounds.push_back(0);
     {
-        std::vector<int> all(n);
-        for (int i = 0; i < n; ++i) all[i] = i;
-        dissect(st, all);
+        // Connected components of the graph of K (independent cloth pieces, batched
+        // instances) are dissected one after another, so each component gets the
+        // ordering and supernodes it would get alone and the elimination forest has
+        // one tree per component. Components are numbered by their smallest vertex.
+        std::vector<int> comp(n, -1), set, queue;
+        for (int s0 = 0; s0 < n; ++s0) {
+            if (comp[s0] >= 0) continue;
+            set.clear();
+            queue.assign(1, s0);
+            comp[s0] = s0;
+            for (size_t h = 0; h < queue.size(); ++h) {
+                int v = queue[h];
+                set.push_back(v);
+                for (int64_t p = M.ata_ptr[v]; p < M.ata_ptr[v + 1]; ++p) {
+                    int u = M.ata_idx[p];
+                    if (comp[u] < 0) {
+                        comp[u] = s0;
+                        queue.push_back(u);
+                    }
+                }
+            }
+            std::sort(set.begin(), set.end());
+            dissect(st, set);
+        }
     }
     if ((int)st.order.size() != n) throw std::logic_error("ordering lost vertices");
     F.perm = st.order;

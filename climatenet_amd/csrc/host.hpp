@@ -45,6 +45,8 @@ struct Model {
 
 // Builds topology, masses, springs, bending rows and A^T A. Throws std::invalid_argument.
 void build_model(const pdfc_scene& sc, Model& M);
+// Disjoint union of independently built scenes (batched instances).
+void merge_models(const std::vector<Model>& parts, Model& M);
 
 // Supernodal factor of P K P^T.
 struct Factor {
@@ -115,6 +117,9 @@ struct SelfPair {
     V3 normal;
     int level;
 };
+// Contact basis rows [n, t, b] (computeBasis, SimulationFrictionEstimation.cpp:89-105).
+void host_contact_basis(const double n[3], double B[9]);
+
 // Greedy node-node pairing over candidates sorted by (vertex, triangle).
 void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out);
 // Computation levels: BFS from the obstacle-contact vertices, then from degree-1 and

@@ -192,3 +192,42 @@ void build_model(const pdfc_scene& sc, Model& M) {
 }
 
 }  // namespace pdfc
+
+namespace pdfc {
+
+// Disjoint union of independent scenes: vertices, triangles, constraint rows and
+// A^T A of part k follow those of parts 0..k-1 (block-diagonal K).
+void merge_models(const std::vector<Model>& parts, Model& M) {
+    M = Model();
+    for (const Model& P : parts) {
+        const int v0 = M.n;
+        for (const auto& t : P.tris) M.tris.push_back({t[0] + v0, t[1] + v0, t[2] + v0});
+        M.rest.insert(M.rest.end(), P.rest.begin(), P.rest.end());
+        M.mass.insert(M.mass.end(), P.mass.begin(), P.mass.end());
+        for (const auto& e : P.edges) M.edges.push_back({e[0] + v0, e[1] + v0});
+        M.spring_w.insert(M.spring_w.end(), P.spring_w.begin(), P.spring_w.end());
+        M.spring_L0.insert(M.spring_L0.end(), P.spring_L0.begin(), P.spring_L0.end());
+        if (M.bend_ptr.empty()) M.bend_ptr.push_back(0);
+        const int b0 = (int)M.bend_idx.size();
+        for (size_t b = 1; b < P.bend_ptr.size(); ++b) M.bend_ptr.push_back(P.bend_ptr[b] + b0);
+        for (int x : P.bend_idx) M.bend_idx.push_back(x + v0);
+        M.bend_coef.insert(M.bend_coef.end(), P.bend_coef.begin(), P.bend_coef.end());
+        M.border.insert(M.border.end(), P.border.begin(), P.border.end());
+        for (const auto& nb : P.neighbors) {
+            std::vector<int> r(nb);
+            for (int& x : r) x += v0;
+            M.neighbors.push_back(std::move(r));
+        }
+        if (M.ata_ptr.empty()) M.ata_ptr.push_back(0);
+        const int64_t a0 = (int64_t)M.ata_idx.size();
+        for (size_t i = 1; i < P.ata_ptr.size(); ++i) M.ata_ptr.push_back(P.ata_ptr[i] + a0);
+        for (int x : P.ata_idx) M.ata_idx.push_back(x + v0);
+        M.ata_val.insert(M.ata_val.end(), P.ata_val.begin(), P.ata_val.end());
+        M.max_degree = std::max(M.max_degree, P.max_degree);
+        M.max_ata_row = std::max(M.max_ata_row, P.max_ata_row);
+        M.n += P.n;
+        M.T += P.T;
+    }
+}
+
+}  // namespace pdfc

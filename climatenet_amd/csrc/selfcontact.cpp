@@ -4,6 +4,7 @@
 //   computation levels (BFS)   src/physics/SimulationFrictionEstimation.cpp:161-293
 // Both are order-dependent and sequential in the reference.
 #include <algorithm>
+#include <cmath>
 
 #include "host.hpp"
 
@@ -120,6 +121,30 @@ int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPa
     for (size_t l = 0; l < levels.size(); ++l)
         for (int s : levels[l]) pairs[s].level = (int)l;
     return (int)levels.size();
+}
+
+}  // namespace pdfc
+
+namespace pdfc {
+
+void host_contact_basis(const double nr[3], double B[9]) {
+    auto cross = [](V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+    auto nrm = [](V3 a) {
+        double z = (a.x * a.x + a.y * a.y) + a.z * a.z;
+        if (z > 0.) {
+            double q = std::sqrt(z);
+            return V3{a.x / q, a.y / q, a.z / q};
+        }
+        return a;
+    };
+    const V3 n{nr[0], nr[1], nr[2]};
+    V3 t = cross(n, V3{1., 0., 0.});
+    if (std::sqrt((t.x * t.x + t.y * t.y) + t.z * t.z) < 1.e-15) t = cross(n, V3{0., 1., 0.});
+    t = nrm(t);
+    V3 b = nrm(cross(n, t));
+    B[0] = n.x; B[1] = n.y; B[2] = n.z;
+    B[3] = t.x; B[4] = t.y; B[5] = t.z;
+    B[6] = b.x; B[7] = b.y; B[8] = b.z;
 }
 
 }  // namespace pdfc

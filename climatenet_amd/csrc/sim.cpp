@@ -19,6 +19,7 @@
 
 #include "device.hpp"
 #include "host.hpp"
+#include "mlp.hpp"
 #include "pdfc.h"
 
 namespace {
@@ -44,12 +45,30 @@ struct CapacityError : std::runtime_error {
 
 }  // namespace
 
+void pdfc::set_last_error(const char* msg) { g_last_error = msg; }
+
+// Adaptive iteration. Residuals come back in chunks of kCheckChunk iterations, one chunk
+// in flight while the next is enqueued. A warm-start hold ends once the residual has not
+// dropped by kReleaseFactor over kReleaseWindow iterations; after the hold, a learned warm
+// start is abandoned for the plain one (the step restarts) when the residual has not
+// dropped by kStagnationFactor over kStagnationWindow iterations.
+constexpr int kCheckChunk = 8;
+constexpr int kReleaseWindow = 50;
+constexpr double kReleaseFactor = 0.9;
+constexpr int kStagnationWindow = 200;
+constexpr double kStagnationFactor = 0.9;
+
 struct pdfc_sim {
     pdfc::Model model;
     pdfc::Factor fac;
     pdfc::SweepPlan fwd, bwd;
     pdfc_params prm{};
     std::vector<pdfc_obstacle> obstacles;
+    std::vector<double> obs_mu;         // per obstacle, against its scene's cloth
+    std::vector<int> obs_ptr;           // per scene: its obstacles
+    std::vector<int> scene_v0;          // per scene: first vertex
+    int n_scenes = 1, max_local_obs = 0;
+    size_t cap_contacts = 1;
     std::vector<double> friction;
     int n_mat = 0, mesh_material = 0;
     int n = 0, nobs = 0;
@@ -90,6 +109,29 @@ struct pdfc_sim {
     int64_t ksolves = 0;
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
+
+    // ---- adaptive iteration count and warm starts
+    int ac_cap = 0;                 // entries of ac_sum / ac_max
+    int last_records = 0;           // iterations recorded in ac_sum by the last step
+    double* host_ac = nullptr;      // pinned: per-iteration residual sums read back in the adaptive mode
+    int* host_nc = nullptr;         // pinned: obstacle contact count of the step
+    hipEvent_t ev_ac[2] = {nullptr, nullptr};
+    bool given_pending = false;     // pdfc_set_warm_reactions: warm start of the next step
+    int given_hold = 1;
+    bool warm_data = false;         // features and previous reactions are kept
+    float* feat = nullptr;          // 12 per contact
+    float* warm_out = nullptr;      // 3 per contact: network output
+    float* prev_g = nullptr;        // 3n per scene obstacle slot
+    int last_nc = 0;
+    pdfc_mlp* mlp = nullptr;
+
+    // ---- one step captured as a HIP graph and replayed (steps with no host round trip)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_capturable() const {
+        return prm.use_graph && !self_on && !(prm.residual_tolerance > 0.) && !warm_data && !ktiming && !given_pending;
+    }
+    void graph_step();
 
     // ---- self-collision (per-step detection on the device, pairing on the host)
     bool self_on = false;
@@ -143,6 +185,12 @@ struct pdfc_sim {
     }
     ~pdfc_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (host_ac) (void)hipHostFree(host_ac);
+        if (host_nc) (void)hipHostFree(host_nc);
+        for (auto& e : ev_ac)
+            if (e) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -150,7 +198,7 @@ struct pdfc_sim {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    void build(const pdfc_scene& sc, const pdfc_params& p);
+    void build(const pdfc_scene* scenes, size_t count, const pdfc_params& p);
     // Brackets one launch with an event pair when instrumentation is on.
     template <typename L>
     void timed(int family, L&& launch) {
@@ -177,38 +225,65 @@ struct pdfc_sim {
     void solve_inplace(double* X);
     void run_step();
     void detect();
+    void launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls);
 };
 
-void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
+void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& p) {
+    if (count == 0) throw std::invalid_argument("no scene");
+    const pdfc_scene& sc = scenes[0];
     if (!(p.time_step > 0.)) throw std::invalid_argument("time_step must be positive");
     if (p.local_global_iterations < 1) throw std::invalid_argument("local_global_iterations must be >= 1");
     if (p.precision != PDFC_FP64 && p.precision != PDFC_FP32_REFINE64) throw std::invalid_argument("unknown precision");
-    if (sc.n_obstacles > (size_t)pdfc::kMaxObstacles) throw std::invalid_argument("too many obstacles (max 16)");
-    if (sc.n_materials == 0 || !sc.friction) throw std::invalid_argument("friction table is empty");
-    if (sc.material < 0 || sc.material >= (int)sc.n_materials) throw std::invalid_argument("mesh material out of range");
     if (p.self_collision && !(p.self_collision_tolerance >= 0.))
         throw std::invalid_argument("self-collision tolerance must be non-negative");
+    if (!(p.residual_tolerance >= 0.)) throw std::invalid_argument("residual_tolerance must be non-negative");
+    if (p.residual_tolerance > 0. && p.max_iterations < 1)
+        throw std::invalid_argument("max_iterations must be >= 1 with a residual tolerance");
+    if (p.warm_start != PDFC_WARM_PLAIN && p.warm_start != PDFC_WARM_PREVIOUS && p.warm_start != PDFC_WARM_LEARNED)
+        throw std::invalid_argument("unknown warm start");
+    if (count > 1 && p.self_collision) throw std::invalid_argument("self-collision is not available for batched scenes");
     prm = p;
     fp32 = p.precision == PDFC_FP32_REFINE64;
     n_mat = (int)sc.n_materials;
     friction.assign(sc.friction, sc.friction + (size_t)n_mat * n_mat);
     mesh_material = sc.material;
-    for (size_t o = 0; o < sc.n_obstacles; ++o) {
-        pdfc_obstacle ob = sc.obstacles[o];
-        if (ob.material < 0 || ob.material >= n_mat) throw std::invalid_argument("obstacle material out of range");
-        if (ob.type == PDFC_OBSTACLE_PLANE) {
-            double z = (ob.normal[0] * ob.normal[0] + ob.normal[1] * ob.normal[1]) + ob.normal[2] * ob.normal[2];
-            if (!(z > 0.)) throw std::invalid_argument("plane normal is zero");
-            double s = std::sqrt(z);
-            for (int k = 0; k < 3; ++k) ob.normal[k] /= s;
-        } else if (ob.type != PDFC_OBSTACLE_SPHERE) {
-            throw std::invalid_argument("unknown obstacle type");
+    obs_ptr.assign(1, 0);
+    for (size_t k = 0; k < count; ++k) {
+        const pdfc_scene& S = scenes[k];
+        if (S.n_obstacles > (size_t)pdfc::kMaxObstacles) throw std::invalid_argument("too many obstacles (max 16)");
+        if (S.n_materials == 0 || !S.friction) throw std::invalid_argument("friction table is empty");
+        if (S.material < 0 || S.material >= (int)S.n_materials) throw std::invalid_argument("mesh material out of range");
+        for (size_t o = 0; o < S.n_obstacles; ++o) {
+            pdfc_obstacle ob = S.obstacles[o];
+            if (ob.material < 0 || ob.material >= (int)S.n_materials)
+                throw std::invalid_argument("obstacle material out of range");
+            if (ob.type == PDFC_OBSTACLE_PLANE) {
+                double z = (ob.normal[0] * ob.normal[0] + ob.normal[1] * ob.normal[1]) + ob.normal[2] * ob.normal[2];
+                if (!(z > 0.)) throw std::invalid_argument("plane normal is zero");
+                double s = std::sqrt(z);
+                for (int c = 0; c < 3; ++c) ob.normal[c] /= s;
+            } else if (ob.type != PDFC_OBSTACLE_SPHERE) {
+                throw std::invalid_argument("unknown obstacle type");
+            }
+            obstacles.push_back(ob);
+            obs_mu.push_back(S.friction[(size_t)S.material * S.n_materials + ob.material]);
         }
-        obstacles.push_back(ob);
+        obs_ptr.push_back((int)obstacles.size());
+        max_local_obs = std::max(max_local_obs, (int)S.n_obstacles);
     }
     nobs = (int)obstacles.size();
-    pdfc::build_model(sc, model);
+    n_scenes = (int)count;
+    if (count == 1) {
+        pdfc::build_model(sc, model);
+    } else {
+        std::vector<pdfc::Model> parts(count);
+        for (size_t k = 0; k < count; ++k) pdfc::build_model(scenes[k], parts[k]);
+        pdfc::merge_models(parts, model);
+    }
     n = model.n;
+    scene_v0.assign(1, 0);
+    for (size_t k = 0; k < count; ++k) scene_v0.push_back(scene_v0.back() + (int)scenes[k].n_vertices);
+    cap_contacts = std::max<size_t>(1, (size_t)n * std::max(1, max_local_obs));
     const double dt = p.time_step;
     try {
         pdfc::factorize(model, dt * dt, fac, 32);
@@ -312,15 +387,31 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     std::vector<double>().swap(fac.FT);
 
     // ---- obstacles
-    dob.count = nobs;
-    for (int o = 0; o < nobs; ++o) {
-        dob.type[o] = obstacles[o].type;
-        for (int k = 0; k < 3; ++k) {
-            dob.p[o][k] = obstacles[o].point[k];
-            dob.nrm[o][k] = obstacles[o].normal[k];
+    {
+        std::vector<int> type(nobs);
+        std::vector<double> pt(3 * (size_t)nobs), nr(3 * (size_t)nobs), rad(nobs);
+        for (int o = 0; o < nobs; ++o) {
+            type[o] = obstacles[o].type;
+            for (int k = 0; k < 3; ++k) {
+                pt[3 * o + k] = obstacles[o].point[k];
+                nr[3 * o + k] = obstacles[o].normal[k];
+            }
+            rad[o] = obstacles[o].radius;
         }
-        dob.radius[o] = obstacles[o].radius;
-        dob.mu[o] = friction[(size_t)mesh_material * n_mat + obstacles[o].material];
+        dob.count = nobs;
+        dob.type = upload(type);
+        dob.p = upload(pt);
+        dob.nrm = upload(nr);
+        dob.radius = upload(rad);
+        dob.mu = upload(obs_mu);
+        dob.vinst = nullptr;
+        dob.obs_ptr = upload(obs_ptr);
+        if (n_scenes > 1) {
+            std::vector<int> vinst(n);
+            for (int k = 0; k < n_scenes; ++k)
+                for (int v = scene_v0[k]; v < scene_v0[k + 1]; ++v) vinst[fac.iperm[v]] = k;
+            dob.vinst = upload(vinst);
+        }
     }
 
     // ---- state and work buffers
@@ -333,7 +424,7 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     cvec = dalloc<double>(n3);
     Rv = dalloc<double>(n3);
     Bsave = dalloc<double>(n3);
-    const size_t cap = std::max<size_t>(1, (size_t)n * std::max(1, nobs));
+    const size_t cap = cap_contacts;
     dc.vertex = dalloc<int>(cap);
     dc.obstacle = dalloc<int>(cap);
     dc.active = dalloc<int>(cap);
@@ -347,8 +438,17 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
     cstart = dalloc<int>(n);
     blocksum = dalloc<int>((n + 255) / 256 + 1);
     ncontacts = dalloc<int>(1);
-    ac_sum = dalloc<double>(p.local_global_iterations);
-    ac_max = dalloc<unsigned long long>(p.local_global_iterations);
+    ac_cap = std::max(p.local_global_iterations, p.residual_tolerance > 0. ? p.max_iterations : 0);
+    ac_sum = dalloc<double>(ac_cap);
+    ac_max = dalloc<unsigned long long>(ac_cap);
+    HIP_CHECK(hipHostMalloc((void**)&host_ac, (size_t)ac_cap * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&host_nc, sizeof(int), hipHostMallocDefault));
+    for (auto& e : ev_ac) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    warm_data = p.warm_start != PDFC_WARM_PLAIN || p.residual_tolerance > 0.;
+    feat = dalloc<float>((size_t)pdfc::kWarmFeatures * cap);
+    warm_out = dalloc<float>(3 * cap);
+    prev_g = dalloc<float>(3 * (size_t)n * std::max(1, max_local_obs));
+    HIP_CHECK(hipMemsetAsync(prev_g, 0, 3 * (size_t)n * std::max(1, max_local_obs) * sizeof(float), stream));
     HIP_CHECK(hipMemsetAsync(ccnt, 0, n * sizeof(int), stream));
     HIP_CHECK(hipMemsetAsync(ncontacts, 0, sizeof(int), stream));
     self_on = p.self_collision != 0;
@@ -397,8 +497,12 @@ void pdfc_sim::build(const pdfc_scene& sc, const pdfc_params& p) {
         vslot = dalloc<int>(n);
         pdfc::launch_fill_int(vslot, n, -1, stream);
     }
-    std::vector<double> q0(sc.positions ? sc.positions : sc.rest_positions,
-                           (sc.positions ? sc.positions : sc.rest_positions) + n3);
+    std::vector<double> q0;
+    q0.reserve(n3);
+    for (size_t k = 0; k < count; ++k) {
+        const double* src = scenes[k].positions ? scenes[k].positions : scenes[k].rest_positions;
+        q0.insert(q0.end(), src, src + 3 * scenes[k].n_vertices);
+    }
     to_device_vec(q0.data(), qn);
     std::vector<double> zero(n3, 0.);
     to_device_vec(zero.data(), vstate);
@@ -726,27 +830,8 @@ void pdfc_sim::self_upload() {
             }
         SA[j] = slot_of[A[j]];
         SB[j] = slot_of[B[j]];
-        // computeBasis (SimulationFrictionEstimation.cpp:89-105)
-        const pdfc::V3 nr = p.normal;
-        auto cross = [](pdfc::V3 a, pdfc::V3 b) {
-            return pdfc::V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-        };
-        auto nrm = [](pdfc::V3 a) {
-            double z = (a.x * a.x + a.y * a.y) + a.z * a.z;
-            if (z > 0.) {
-                double q = std::sqrt(z);
-                return pdfc::V3{a.x / q, a.y / q, a.z / q};
-            }
-            return a;
-        };
-        pdfc::V3 t = cross(nr, pdfc::V3{1., 0., 0.});
-        if (std::sqrt((t.x * t.x + t.y * t.y) + t.z * t.z) < 1.e-15) t = cross(nr, pdfc::V3{0., 1., 0.});
-        t = nrm(t);
-        pdfc::V3 b = nrm(cross(nr, t));
-        double* Bj = &basis[9 * (size_t)j];
-        Bj[0] = nr.x; Bj[1] = nr.y; Bj[2] = nr.z;
-        Bj[3] = t.x;  Bj[4] = t.y;  Bj[5] = t.z;
-        Bj[6] = b.x;  Bj[7] = b.y;  Bj[8] = b.z;
+        const double nr[3] = {p.normal.x, p.normal.y, p.normal.z};
+        pdfc::host_contact_basis(nr, &basis[9 * (size_t)j]);
     }
     const int nslots = (int)slot_vertex.size();
     // per-level groups: each touched slot once, entries in the level's contact order
@@ -848,55 +933,200 @@ void pdfc_sim::self_upload() {
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+// One local-global iteration: local step (+ self-contact levels) and the global solve. The
+// friction residual of the previous iteration is complete once its local kernels have run.
+void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) {
+    double* vcur = V[it & 1];
+    double* vnext = V[(it + 1) & 1];
+    ls.Fc = ds.Fc;
+    ls.CFn = ds.CFn;
+    ls.SCFc = ds.SCFc;
+    timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
+    if (nsc > 0) {
+        for (int l = 0; l < n_self_levels; ++l) {
+            pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
+            pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+        }
+        pdfc::launch_self_finalize(ds, vnext, n, stream);
+        std::swap(ds.CFc, ds.CFn);
+        std::swap(ds.SCFc, ds.SCFn);
+        std::swap(ds.rem, ds.addv);
+    }
+    solve_inplace(vnext);
+}
+
 void pdfc_sim::run_step() {
-    const int K = prm.local_global_iterations;
+    const bool adaptive = prm.residual_tolerance > 0.;
     pdfc::StepArgs a{};
     a.dt = prm.time_step;
     for (int k = 0; k < 3; ++k) a.g[k] = prm.gravity[k];
     a.damping = std::max(prm.air_damping, 0.);
-    a.record = prm.record_iterations ? 1 : 0;
+    a.record = (prm.record_iterations || adaptive) ? 1 : 0;
     a.ac_sum = ac_sum;
     a.ac_max = ac_max;
+    const bool given = given_pending;
+    given_pending = false;
+    const bool features = warm_data || given;
+    a.prev_g = features ? prev_g : nullptr;
+    int mode = prm.warm_start;
+    int hold = std::max(1, prm.warm_hold);
+    if (given) {
+        mode = PDFC_WARM_PREVIOUS;
+        hold = std::max(1, given_hold);
+    }
+    if (mode == PDFC_WARM_LEARNED && !mlp)
+        throw std::invalid_argument("learned warm start without a network (pdfc_set_mlp)");
     if (a.record) {
-        pdfc::launch_fill(ac_sum, K, 0., stream);
-        pdfc::launch_fill_u64(ac_max, K, stream);
+        pdfc::launch_fill(ac_sum, ac_cap, 0., stream);
+        pdfc::launch_fill_u64(ac_max, ac_cap, stream);
     }
     pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
     detect();
+    const bool need_nc = adaptive || features;
+    if (need_nc) HIP_CHECK(hipMemcpyAsync(host_nc, ncontacts, sizeof(int), hipMemcpyDeviceToHost, stream));
     nsc = 0;
     if (self_on) {
         self_detect_and_pair(keep_full_candidates);
         if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
     }
-    HIP_CHECK(hipEventRecord(ev[1], stream));
+    int nc = -1;
+    if (need_nc) {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        nc = *host_nc;
+        last_nc = nc;
+    }
+    if (features && nc > 0) {
+        pdfc::launch_warm_features(dm, qn, V[0], cvec, ncontacts, (int)std::min<size_t>(cap_contacts, nc), dc, dob, a,
+                                   feat, stream);
+        if (mode == PDFC_WARM_LEARNED) pdfc::launch_mlp_forward(*pdfc_mlp_device(mlp), feat, nc, warm_out, stream);
+    }
+    if (nc == 0) mode = PDFC_WARM_PLAIN;
+    if (mode == PDFC_WARM_PREVIOUS) {
+        a.warm_y = feat + 6;
+        a.warm_stride = pdfc::kWarmFeatures;
+    } else if (mode == PDFC_WARM_LEARNED) {
+        a.warm_y = warm_out;
+        a.warm_stride = 3;
+    }
     pdfc::LocalSelf ls{};
     ls.selfmode = nsc > 0 ? 1 : 0;
     ls.vslot = vslot;
-    for (int it = 0; it < K; ++it) {
-        double* vcur = V[it & 1];
-        double* vnext = V[(it + 1) & 1];
-        ls.Fc = ds.Fc;
-        ls.CFn = ds.CFn;
-        ls.SCFc = ds.SCFc;
-        timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
-        if (nsc > 0) {
-            for (int l = 0; l < n_self_levels; ++l) {
-                pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
-                pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+
+    const int total = std::max(nc, 0) + nsc;
+    const bool check = adaptive && total > 0;
+    int limit = check ? prm.max_iterations : prm.local_global_iterations;
+    int base = 0;         // iterations spent on an abandoned learned warm start
+    int to_tol = -1;      // iterations to reach the tolerance (counting abandoned ones)
+    bool fell_back = false;
+    int it = 0;
+    for (;;) {
+        int hold_end = mode != PDFC_WARM_PLAIN ? hold : 0;  // iterations < hold_end use the warm reaction
+        int copied = 0;                   // residuals [0, copied) requested from the device
+        int pend_lo = 0, pend_hi = 0, pend_ev = 0;
+        bool pending = false, stop = false, restart = false;
+        // Scan residuals [lo, hi) (now on the host): tolerance, end of the hold, stagnation.
+        auto scan = [&](int lo, int hi) {
+            for (int j = lo; j < hi && !stop; ++j) {
+                const double mean = host_ac[j] / total;
+                host_ac[j] = mean;
+                if (mean < prm.residual_tolerance) {
+                    to_tol = base + j + 1;
+                    stop = true;
+                } else if (j < hold_end && j >= kReleaseWindow && mean > kReleaseFactor * host_ac[j - kReleaseWindow]) {
+                    hold_end = std::min(hold_end, it + 1);  // the hold ends with the next launch
+                } else if (mode == PDFC_WARM_LEARNED && j - kStagnationWindow >= hold_end &&
+                           base + it + 1 < prm.max_iterations &&
+                           mean > kStagnationFactor * host_ac[j - kStagnationWindow]) {
+                    restart = true;
+                    stop = true;
+                }
             }
-            pdfc::launch_self_finalize(ds, vnext, n, stream);
-            std::swap(ds.CFc, ds.CFn);
-            std::swap(ds.SCFc, ds.SCFn);
-            std::swap(ds.rem, ds.addv);
+        };
+        for (it = 0; it < limit && !stop; ++it) {
+            a.warm_mode = it < hold_end ? (mode == PDFC_WARM_LEARNED ? 2 : 1) : 0;
+            launch_iteration(it, a, ls);
+            if (!check || (it + 1) % kCheckChunk != 0) continue;
+            // residuals [copied, it) are complete once iteration it's local kernels have run
+            const int ev = pending ? 1 - pend_ev : 0;
+            HIP_CHECK(hipMemcpyAsync(host_ac + copied, ac_sum + copied, (it - copied) * sizeof(double),
+                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipEventRecord(ev_ac[ev], stream));
+            if (pending) {
+                HIP_CHECK(hipEventSynchronize(ev_ac[pend_ev]));
+                scan(pend_lo, pend_hi);
+            }
+            pending = true;
+            pend_lo = copied;
+            pend_hi = it;
+            pend_ev = ev;
+            copied = it;
         }
-        solve_inplace(vnext);
+        const int K = it;  // iterations launched in this attempt
+        if (check && !stop) {
+            // the last residuals: wait for everything enqueued so far
+            if (K - 1 > copied)
+                HIP_CHECK(hipMemcpyAsync(host_ac + copied, ac_sum + copied, (K - 1 - copied) * sizeof(double),
+                                         hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            if (pending) scan(pend_lo, pend_hi);
+            if (!stop) scan(copied, K - 1);
+        } else if (pending) {
+            HIP_CHECK(hipEventSynchronize(ev_ac[pend_ev]));
+        }
+        if (!restart) break;
+        // plain warm start from the beginning of the step (same detection and contact sets)
+        fell_back = true;
+        base += K;
+        limit = std::max(1, prm.max_iterations - base);
+        mode = PDFC_WARM_PLAIN;
+        a.warm_mode = 0;
+        a.warm_y = nullptr;
+        pdfc::launch_fill(ac_sum, ac_cap, 0., stream);
+        pdfc::launch_fill_u64(ac_max, ac_cap, stream);
+        pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
+        if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
     }
-    pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, a, K - 1, stream);
+    const int K = it;
+    last_records = K;
+    pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, dob, a, K - 1, stream);
     if (nsc > 0) {
         pdfc::launch_self_ac_final(ds, V[K & 1], n, a, K - 1, stream);
         pdfc::launch_self_release(ds, vslot, stream);
     }
+    if (check && to_tol < 0) {
+        HIP_CHECK(hipMemcpyAsync(host_ac + (K - 1), ac_sum + (K - 1), sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (host_ac[K - 1] / total < prm.residual_tolerance) to_tol = base + K;
+    }
     HIP_CHECK(hipGetLastError());
+    stats.iterations += base + K;
+    if (total > 0) {
+        stats.contact_steps += 1;
+        if (adaptive) stats.iterations_to_tolerance += to_tol > 0 ? to_tol : prm.max_iterations;
+        if (to_tol > 0) stats.converged_steps += 1;
+    }
+    if (fell_back) stats.fallbacks += 1;
+}
+
+// The first call captures run_step() on the stream (kernels only: no copies, memsets or
+// host synchronisation on this path); every call replays the graph once.
+void pdfc_sim::graph_step() {
+    if (!graph_exec) {
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_step();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        HIP_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        stats.iterations -= prm.local_global_iterations;  // counted again by the replay below
+    }
+    HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+    stats.iterations += prm.local_global_iterations;
 }
 
 void pdfc_sim::collect_kernel_times() {
@@ -948,7 +1178,19 @@ int pdfc_create(const pdfc_scene* scene, const pdfc_params* params, pdfc_sim** o
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw DeviceError("no HIP device available");
         auto sim = std::make_unique<pdfc_sim>();
-        sim->build(*scene, *params);
+        sim->build(scene, 1, *params);
+        *out = sim.release();
+    });
+}
+
+int pdfc_create_batch(const pdfc_scene* scenes, size_t count, const pdfc_params* params, pdfc_sim** out) {
+    return guarded([&] {
+        if (!scenes || !params || !out || count == 0) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw DeviceError("no HIP device available");
+        auto sim = std::make_unique<pdfc_sim>();
+        sim->build(scenes, count, *params);
         *out = sim.release();
     });
 }
@@ -961,8 +1203,14 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         if (nsteps == 0) return;
         HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
         sim->self_host_us = 0.;
+        sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
+        sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
+        const bool graph = sim->graph_capturable();
         for (int s = 0; s < nsteps; ++s) {
-            sim->run_step();
+            if (graph)
+                sim->graph_step();
+            else
+                sim->run_step();
             if (sim->ktiming) sim->collect_kernel_times();
         }
         HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
@@ -989,6 +1237,10 @@ int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
         if (!sim) throw std::invalid_argument("null sim");
         if (q) sim->to_device_vec(q, sim->qn);
         if (v) sim->to_device_vec(v, sim->vstate);
+        if (sim->prev_g)  // a new state starts a new trajectory: no previous reactions
+            HIP_CHECK(hipMemsetAsync(sim->prev_g, 0, 3 * (size_t)sim->n * std::max(1, sim->max_local_obs) * sizeof(float),
+                                     sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
     });
 }
 
@@ -1108,7 +1360,7 @@ int pdfc_get_self_candidates(pdfc_sim* sim, pdfc_candidate* out, size_t cap, siz
 int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, size_t cap, size_t* n_iter) {
     return guarded([&] {
         if (!sim || !n_iter) throw std::invalid_argument("null argument");
-        const int K = sim->prm.local_global_iterations;
+        const int K = sim->last_records > 0 ? sim->last_records : sim->prm.local_global_iterations;
         *n_iter = (size_t)K;
         if (cap < (size_t)K) throw CapacityError("iteration buffer too small");
         std::vector<double> s(K);
@@ -1126,6 +1378,78 @@ int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, siz
                 std::memcpy(&d, &m[k], sizeof(double));
                 ac_max[k] = d;
             }
+        }
+    });
+}
+
+int pdfc_set_warm_reactions(pdfc_sim* sim, const pdfc_contact* contacts, size_t n, int hold) {
+    return guarded([&] {
+        if (!sim || (n && !contacts)) throw std::invalid_argument("null argument");
+        const int nv = sim->n;
+        const size_t slots = std::max(1, sim->max_local_obs);
+        std::vector<float> g(3 * (size_t)nv * slots, 0.f);
+        for (size_t k = 0; k < n; ++k) {
+            const pdfc_contact& c = contacts[k];
+            if (c.vertex < 0 || c.vertex >= nv || c.obstacle < 0 || c.obstacle >= sim->nobs)
+                throw std::invalid_argument("contact out of range");
+            int scene = (int)(std::upper_bound(sim->scene_v0.begin(), sim->scene_v0.end(), (int)c.vertex) -
+                              sim->scene_v0.begin()) - 1;
+            const int slot = c.obstacle - sim->obs_ptr[scene];
+            if (slot < 0 || slot >= sim->obs_ptr[scene + 1] - sim->obs_ptr[scene])
+                throw std::invalid_argument("obstacle of another scene");
+            double B[9];
+            pdfc::host_contact_basis(c.normal, B);
+            const size_t at = 3 * ((size_t)slot * nv + sim->fac.iperm[c.vertex]);
+            for (int d = 0; d < 3; ++d)
+                g[at + d] = (float)((B[d] * c.r_local[0] + B[3 + d] * c.r_local[1]) + B[6 + d] * c.r_local[2]);
+        }
+        HIP_CHECK(hipMemcpy(sim->prev_g, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice));
+        sim->given_pending = true;
+        sim->given_hold = std::max(1, hold);
+    });
+}
+
+int pdfc_set_mlp(pdfc_sim* sim, pdfc_mlp* mlp) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        sim->mlp = mlp;
+    });
+}
+
+int pdfc_get_warm_data(pdfc_sim* sim, float* features, float* targets, size_t cap, size_t* n) {
+    return guarded([&] {
+        if (!sim || !n) throw std::invalid_argument("null argument");
+        if (!sim->warm_data) throw std::invalid_argument("warm-start data needs a warm start or a residual tolerance");
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        const int nc = sim->last_nc;
+        *n = (size_t)nc;
+        if (cap < (size_t)nc) throw CapacityError("warm-start data buffer too small");
+        if (nc == 0) return;
+        std::vector<int> vtx(nc), obs(nc);
+        std::vector<double> r(3 * (size_t)nc);
+        std::vector<float> f((size_t)pdfc::kWarmFeatures * nc);
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, sim->stream));
+        };
+        get(vtx.data(), sim->dc.vertex, nc * sizeof(int));
+        get(obs.data(), sim->dc.obstacle, nc * sizeof(int));
+        get(r.data(), sim->dc.r, 3 * (size_t)nc * sizeof(double));
+        get(f.data(), sim->feat, f.size() * sizeof(float));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        std::vector<int> order(nc);
+        for (int i = 0; i < nc; ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            if (obs[a] != obs[b]) return obs[a] < obs[b];
+            return sim->fac.perm[vtx[a]] < sim->fac.perm[vtx[b]];
+        });
+        for (int k = 0; k < nc; ++k) {
+            const int i = order[k];
+            const double m = sim->model.mass[sim->fac.perm[vtx[i]]];
+            if (features)
+                std::memcpy(features + (size_t)k * pdfc::kWarmFeatures, &f[(size_t)i * pdfc::kWarmFeatures],
+                            pdfc::kWarmFeatures * sizeof(float));
+            if (targets)
+                for (int d = 0; d < 3; ++d) targets[3 * (size_t)k + d] = (float)(r[3 * (size_t)i + d] / m);
         }
     });
 }

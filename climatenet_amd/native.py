@@ -14,10 +14,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpdfc.so")
 _LIB = None
 
 EXPORTS = [
-    "pdfc_create", "pdfc_step", "pdfc_get_state", "pdfc_set_state", "pdfc_get_contacts",
+    "pdfc_create", "pdfc_create_batch", "pdfc_step", "pdfc_get_state", "pdfc_set_state", "pdfc_get_contacts",
     "pdfc_get_self_contacts", "pdfc_get_self_candidates", "pdfc_detect", "pdfc_get_iteration_stats",
     "pdfc_get_phase_stats", "pdfc_get_info", "pdfc_solve", "pdfc_destroy", "pdfc_last_error",
-    "pdfc_set_kernel_timing", "pdfc_get_kernel_times",
+    "pdfc_set_kernel_timing", "pdfc_get_kernel_times", "pdfc_set_mlp", "pdfc_get_warm_data",
+    "pdfc_set_warm_reactions",
+    "pdfc_mlp_create", "pdfc_mlp_destroy", "pdfc_mlp_set_params", "pdfc_mlp_get_params",
+    "pdfc_mlp_forward", "pdfc_mlp_gradient", "pdfc_mlp_train",
 ]
 
 
@@ -37,6 +40,7 @@ def lib():
         SZ = C.POINTER(C.c_size_t)
         D = C.POINTER(C.c_double)
         L.pdfc_create.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.Params), C.POINTER(P)]
+        L.pdfc_create_batch.argtypes = [C.POINTER(abi.Scene), C.c_size_t, C.POINTER(abi.Params), C.POINTER(P)]
         L.pdfc_step.argtypes = [P, C.c_int]
         L.pdfc_get_state.argtypes = [P, D, D]
         L.pdfc_set_state.argtypes = [P, D, D]
@@ -52,6 +56,18 @@ def lib():
         L.pdfc_get_kernel_times.argtypes = [P, C.POINTER(abi.KernelTimes)]
         L.pdfc_destroy.argtypes = [P]
         L.pdfc_destroy.restype = None
+        F = C.POINTER(C.c_float)
+        L.pdfc_set_mlp.argtypes = [P, P]
+        L.pdfc_set_warm_reactions.argtypes = [P, C.POINTER(abi.Contact), C.c_size_t, C.c_int]
+        L.pdfc_get_warm_data.argtypes = [P, F, F, C.c_size_t, SZ]
+        L.pdfc_mlp_create.argtypes = [C.POINTER(P), C.c_uint64]
+        L.pdfc_mlp_destroy.argtypes = [P]
+        L.pdfc_mlp_destroy.restype = None
+        L.pdfc_mlp_set_params.argtypes = [P, F, F]
+        L.pdfc_mlp_get_params.argtypes = [P, F, F]
+        L.pdfc_mlp_forward.argtypes = [P, F, C.c_size_t, F]
+        L.pdfc_mlp_gradient.argtypes = [P, F, F, C.c_size_t, F, F]
+        L.pdfc_mlp_train.argtypes = [P, F, F, C.c_size_t, C.c_int, C.c_int, C.c_float, C.c_uint64, F]
         L.pdfc_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB
@@ -66,16 +82,42 @@ class Simulation:
     """One device-resident simulation (SimulationFrictionEstimation on the GPU)."""
 
     def __init__(self, spec, precision=abi.FP64, refinement_sweeps=2, warm_start=abi.WARM_PLAIN,
-                 record_iterations=True, iterations=None):
+                 record_iterations=True, iterations=None, residual_tolerance=0.0, max_iterations=1000,
+                 mlp=None, use_graph=False, warm_hold=1):
         self.spec = spec
         self._scene = spec.c_scene()
         self._params = spec.c_params(precision=precision, refinement_sweeps=refinement_sweeps,
                                      warm_start=warm_start, record_iterations=record_iterations,
-                                     iterations=iterations)
+                                     iterations=iterations, residual_tolerance=residual_tolerance,
+                                     max_iterations=max_iterations, use_graph=use_graph, warm_hold=warm_hold)
         h = C.c_void_p()
         _check(lib().pdfc_create(C.byref(self._scene), C.byref(self._params), C.byref(h)))
         self.h = h
         self.n = spec.n_vertices
+        self._mlp = None
+        if mlp is not None:
+            self.set_mlp(mlp)
+
+    def set_mlp(self, mlp):
+        """Attach a ContactMLP (kept alive by this object) for the learned warm start."""
+        _check(lib().pdfc_set_mlp(self.h, None if mlp is None else mlp.h))
+        self._mlp = mlp
+
+    def set_warm_reactions(self, contacts, hold=1):
+        """Warm start of the next step from pdfc_contact records (e.g. a converged step's)."""
+        arr = (abi.Contact * max(1, len(contacts)))(*contacts)
+        _check(lib().pdfc_set_warm_reactions(self.h, arr, len(contacts), int(hold)))
+
+    def warm_data(self):
+        """(features [C x 12], targets [C x 3]) of the last step's obstacle contacts."""
+        n = C.c_size_t()
+        code = lib().pdfc_get_warm_data(self.h, None, None, 0, C.byref(n))
+        if code not in (abi.PDFC_OK, abi.PDFC_ERR_CAPACITY):
+            _check(code)
+        x = np.zeros((max(1, n.value), abi.MLP_FEATURES), dtype=np.float32)
+        y = np.zeros((max(1, n.value), abi.MLP_OUTPUTS), dtype=np.float32)
+        _check(lib().pdfc_get_warm_data(self.h, abi.fptr(x), abi.fptr(y), n.value, C.byref(n)))
+        return x[: n.value], y[: n.value]
 
     def close(self):
         if getattr(self, "h", None):
@@ -160,3 +202,92 @@ class Simulation:
         x = np.zeros_like(b)
         _check(lib().pdfc_solve(self.h, abi.dptr(b), abi.dptr(x)))
         return x
+
+
+class BatchSimulation(Simulation):
+    """Independent scenes advanced together (one block-diagonal system, one kernel sequence
+    per step). Per-vertex arrays concatenate the scenes in order; ``ranges[k]`` is scene k's
+    vertex range."""
+
+    def __init__(self, specs, precision=abi.FP64, refinement_sweeps=2, warm_start=abi.WARM_PLAIN,
+                 record_iterations=True, iterations=None, residual_tolerance=0.0, max_iterations=1000,
+                 use_graph=False):
+        if not specs:
+            raise ValueError("no scene")
+        for sp in specs[1:]:
+            if sp.time_step != specs[0].time_step or sp.iterations != specs[0].iterations:
+                raise ValueError("batched scenes share the time step and iteration count")
+        self.specs = list(specs)
+        self.spec = specs[0]
+        self._scenes = (abi.Scene * len(specs))()
+        for k, sp in enumerate(specs):
+            self._scenes[k] = sp.c_scene()
+        self._params = specs[0].c_params(precision=precision, refinement_sweeps=refinement_sweeps,
+                                         warm_start=warm_start, record_iterations=record_iterations,
+                                         iterations=iterations, residual_tolerance=residual_tolerance,
+                                         max_iterations=max_iterations, use_graph=use_graph)
+        h = C.c_void_p()
+        _check(lib().pdfc_create_batch(self._scenes, len(specs), C.byref(self._params), C.byref(h)))
+        self.h = h
+        off = np.cumsum([0] + [sp.n_vertices for sp in specs])
+        self.ranges = [(int(off[k]), int(off[k + 1])) for k in range(len(specs))]
+        self.n = int(off[-1])
+        self._mlp = None
+
+
+class ContactMLP:
+    """The contact network of the learned warm start (12 -> 64 -> 64 -> 64 -> 3, SiLU, BF16 MFMA)."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        _check(lib().pdfc_mlp_create(C.byref(h), seed))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pdfc_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        z = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        if p is not None and p.size != abi.MLP_PARAMS:
+            raise ValueError("expected %d parameters" % abi.MLP_PARAMS)
+        if z is not None and z.size != abi.MLP_NORM:
+            raise ValueError("expected %d normalisation values" % abi.MLP_NORM)
+        _check(lib().pdfc_mlp_set_params(self.h, abi.fptr(p), abi.fptr(z)))
+
+    def params(self):
+        p = np.zeros(abi.MLP_PARAMS, dtype=np.float32)
+        z = np.zeros(abi.MLP_NORM, dtype=np.float32)
+        _check(lib().pdfc_mlp_get_params(self.h, abi.fptr(p), abi.fptr(z)))
+        return p, z
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, abi.MLP_FEATURES)
+        y = np.zeros((x.shape[0], abi.MLP_OUTPUTS), dtype=np.float32)
+        _check(lib().pdfc_mlp_forward(self.h, abi.fptr(x), x.shape[0], abi.fptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, abi.MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, abi.MLP_OUTPUTS)
+        g = np.zeros(abi.MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().pdfc_mlp_gradient(self.h, abi.fptr(x), abi.fptr(t), x.shape[0], abi.fptr(g),
+                                       C.cast(C.byref(loss), C.POINTER(C.c_float))))
+        return g, float(loss.value)
+
+    def train(self, x, t, epochs, batch=512, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, abi.MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, abi.MLP_OUTPUTS)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().pdfc_mlp_train(self.h, abi.fptr(x), abi.fptr(t), x.shape[0], epochs, batch, lr, seed,
+                                    abi.fptr(losses)))
+        return losses

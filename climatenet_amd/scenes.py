@@ -160,7 +160,8 @@ class SceneSpec:
         return s
 
     def c_params(self, precision=abi.FP64, refinement_sweeps=2, warm_start=abi.WARM_PLAIN,
-                 record_iterations=True, iterations=None) -> abi.Params:
+                 record_iterations=True, iterations=None, residual_tolerance=0.0,
+                 max_iterations=1000, use_graph=False, warm_hold=1) -> abi.Params:
         p = abi.Params()
         p.time_step = self.time_step
         p.local_global_iterations = self.iterations if iterations is None else iterations
@@ -173,6 +174,10 @@ class SceneSpec:
         p.refinement_sweeps = refinement_sweeps
         p.warm_start = warm_start
         p.record_iterations = int(record_iterations)
+        p.residual_tolerance = residual_tolerance
+        p.max_iterations = max_iterations
+        p.use_graph = int(use_graph)
+        p.warm_hold = warm_hold
         return p
 
 
@@ -234,3 +239,24 @@ def inclined_plane(mu: float = 0.1, angle_deg: float = 45.0, height: float = 0.0
                      [ObstacleSpec("plane", (0.0, 0.0, 0.0), nrm)],
                      np.array([[mu]]), area_density=1.0, bending=0.0, stretching=1e3,
                      time_step=1e-3, iterations=50, steps=400)
+
+
+def batched_instances(count: int = 64, seed: int = 0, sizes=(32, 128)):
+    """Independent 1 m cloths of N x N quads, N ~ U{sizes[0]..sizes[1]}, each dropped 5 cm above its
+    own sphere: R ~ U[0.2, 0.6], centre offset ~ U[-0.1, 0.1]^2 in the plane, mu ~ U[0, 0.8];
+    dt=1/100, 15 iterations, 300 steps. Draws per instance from one mt19937_64(seed), in
+    the order N, mu, R, offset x, offset y."""
+    rng = MT19937_64(seed)
+    specs = []
+    for k in range(count):
+        span = sizes[1] - sizes[0]
+        N = sizes[0] + min(span, int(rng.uniform01() * (span + 1)))
+        mu = 0.8 * rng.uniform01()
+        R = 0.2 + 0.4 * rng.uniform01()
+        cx = -0.1 + 0.2 * rng.uniform01()
+        cy = -0.1 + 0.2 * rng.uniform01()
+        rest, tris = square_grid(N, 1.0, R + 0.05)
+        specs.append(SceneSpec("batch%d_%dx%d" % (k, N, N), rest, tris, None,
+                               [ObstacleSpec("sphere", (cx, cy, 0.0), radius=R)],
+                               np.array([[mu]]), time_step=0.01, iterations=15, steps=300))
+    return specs

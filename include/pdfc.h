@@ -60,8 +60,17 @@ enum pdfc_precision {
 
 enum pdfc_warm_start {
     PDFC_WARM_PLAIN = 0,     /* the reference: r^0 is the cone projection of the contact force at v^0 */
-    PDFC_WARM_LEARNED = 2    /* r^0 predicted by the contact MLP (pdfc_set_mlp), fallback to plain */
+    PDFC_WARM_PREVIOUS = 1,  /* r^0 = the previous step's reaction of the same (obstacle, vertex) contact,
+                                plain where there was none */
+    PDFC_WARM_LEARNED = 2    /* r^0 predicted by the contact MLP (pdfc_set_mlp); the step restarts with the
+                                plain warm start when the friction residual stagnates */
 };
+
+/* Contact MLP (learned warm start; no counterpart in the reference). */
+#define PDFC_MLP_FEATURES 12
+#define PDFC_MLP_OUTPUTS 3
+#define PDFC_MLP_PARAMS 10448     /* FP32 master parameters, layout in climatenet_amd/csrc/mlp.hpp */
+#define PDFC_MLP_NORM 27          /* feature mean[12], feature std[12], output scale[3] */
 
 typedef struct pdfc_obstacle {
     int32_t type;            /* enum pdfc_obstacle_type */
@@ -98,6 +107,17 @@ typedef struct pdfc_params {
     int32_t refinement_sweeps;       /* FP32_REFINE64 only: corrections after the first FP32 solve */
     int32_t warm_start;              /* enum pdfc_warm_start */
     int32_t record_iterations;       /* nonzero: keep the per-iteration friction residual (costs a sync-free reduction per iteration) */
+    double residual_tolerance;       /* > 0: iterate until the mean friction (Alart-Curnier) residual of an
+                                        iteration is below it, at most max_iterations; steps without contacts
+                                        run local_global_iterations. 0: the reference's fixed count */
+    int32_t max_iterations;          /* cap of the adaptive mode */
+    int32_t use_graph;               /* nonzero: where a step needs no host round trip (no self-collision,
+                                        fixed iteration count, plain warm start, no kernel timing) it is
+                                        captured once as a HIP graph and replayed */
+    int32_t warm_hold;               /* iterations that use the warm-start reaction instead of the cone
+                                        projection (<= 1: iteration 0 only); in the adaptive mode the hold
+                                        ends early once the friction residual stops decreasing */
+    int32_t reserved;
 } pdfc_params;
 
 typedef struct pdfc_contact {
@@ -160,6 +180,12 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
     double detection_us;             /* host time per step of the self-collision round trip and pairing */
     double iterations_us;
     int64_t steps;
+    int64_t iterations;              /* local-global iterations executed, all steps of the call */
+    int64_t iterations_to_tolerance; /* adaptive mode: sum over steps with contacts of the first iteration
+                                        count whose residual is below the tolerance (max_iterations when never) */
+    int64_t contact_steps;           /* steps of the call with at least one contact */
+    int64_t converged_steps;         /* of which reached the tolerance */
+    int64_t fallbacks;               /* learned warm starts abandoned for the plain one */
 } pdfc_phase_stats;
 
 typedef struct pdfc_kernel_times {  /* per-launch device time (HIP events), averaged over an instrumented run */
@@ -174,6 +200,14 @@ typedef struct pdfc_kernel_times {  /* per-launch device time (HIP events), aver
 
 /* Build the simulation: topology, masses, constraints, K, host FP64 factorisation, device upload. */
 int pdfc_create(const pdfc_scene* scene, const pdfc_params* params, pdfc_sim** out);
+
+/* Batched instances (no counterpart in the reference, which runs one scene per process): count
+ * independent scenes advanced together, each step one stream-ordered sequence of kernels over all
+ * of them, K factored as one block-diagonal matrix (one elimination tree per scene). params apply
+ * to every scene; self_collision must be 0. Vertices of scene k follow those of scenes 0..k-1 in
+ * every per-vertex array of the other entry points, obstacles likewise; a vertex meets only the
+ * obstacles of its own scene, with that scene's friction table. */
+int pdfc_create_batch(const pdfc_scene* scenes, size_t count, const pdfc_params* params, pdfc_sim** out);
 
 /* Advance nsteps time steps (SimulationBase::step() nsteps times). State stays on the device. */
 int pdfc_step(pdfc_sim* sim, int nsteps);
@@ -216,6 +250,45 @@ int pdfc_solve(pdfc_sim* sim, const double* b, double* x);
 
 void pdfc_destroy(pdfc_sim* sim);
 const char* pdfc_last_error(void);
+
+/* ---- learned warm start. No counterpart in the reference: a per-contact impulse guess for
+ * iteration 0 of the local-global loop (the reference computes r^0 as the cone projection of the
+ * contact force at v^0, SimulationFrictionEstimation.cpp:350-423).
+ *
+ * Features of an obstacle contact (vertex i, basis B = [n t b], mass m_i), evaluated at the start
+ * of the step: [0..2] relative velocity B^T(v^0_i - u_obstacle); [3..5] contact normal (world);
+ * [6..8] previous-step reaction of the same (obstacle, vertex) pair per unit mass, in B; [9] mu;
+ * [10] gap (signed distance of q_n,i to the obstacle surface); [11] normal component of the
+ * contact force at v^0 per unit mass (the plain warm start's own estimate). The network predicts
+ * the converged reaction per unit mass in B; the step uses m_i times it, projected on the cone. */
+typedef struct pdfc_mlp pdfc_mlp;
+
+/* 12 -> 64 -> 64 -> 64 -> 3, SiLU, Glorot-uniform weights from seed, identity normalisation. */
+int pdfc_mlp_create(pdfc_mlp** out, uint64_t seed);
+void pdfc_mlp_destroy(pdfc_mlp* mlp);
+/* params: PDFC_MLP_PARAMS floats (NULL keeps them); norm: PDFC_MLP_NORM floats (NULL keeps them).
+ * Resets the optimiser state. */
+int pdfc_mlp_set_params(pdfc_mlp* mlp, const float* params, const float* norm);
+int pdfc_mlp_get_params(pdfc_mlp* mlp, float* params, float* norm);
+/* y[rows x 3] = scale * MLP((x - mean) / std) on the device (BF16 MFMA, FP32 accumulation). */
+int pdfc_mlp_forward(pdfc_mlp* mlp, const float* x, size_t rows, float* y);
+/* Gradient of L = mean over rows and outputs of (MLP(x) - t / scale)^2 w.r.t. the parameters. */
+int pdfc_mlp_gradient(pdfc_mlp* mlp, const float* x, const float* t, size_t rows, float* grad, float* loss);
+/* Adam (beta 0.9 / 0.999) on shuffled mini-batches of batch rows (multiple of 16) for epochs epochs;
+ * loss_per_epoch (optional, epochs floats) gets each epoch's mean mini-batch loss. */
+int pdfc_mlp_train(pdfc_mlp* mlp, const float* x, const float* t, size_t rows, int epochs, int batch, float lr,
+                   uint64_t seed, float* loss_per_epoch);
+/* Warm start of the next step only, from given reactions: a contact of that step whose (vertex,
+ * obstacle) matches one of contacts[0..n) starts from its r_local (expressed in the basis of its
+ * normal), projected on the cone, for the first hold iterations; other contacts start plain.
+ * Used to measure what an exact warm start (the converged reactions of the same step) gains. */
+int pdfc_set_warm_reactions(pdfc_sim* sim, const pdfc_contact* contacts, size_t n, int hold);
+/* Attach a network to a simulation (not owned; must outlive its use; NULL detaches). */
+int pdfc_set_mlp(pdfc_sim* sim, pdfc_mlp* mlp);
+/* Obstacle contacts of the last step (pdfc_get_contacts order): features[n x 12] and the final
+ * reaction per unit mass in the contact basis, targets[n x 3]. Available when warm_start is not
+ * PLAIN or residual_tolerance > 0. */
+int pdfc_get_warm_data(pdfc_sim* sim, float* features, float* targets, size_t cap, size_t* n);
 
 #ifdef __cplusplus
 }

@@ -13,8 +13,7 @@
  *     against one FP64 factor computed once.
  * Function-level citations give the reference file:line each part restates.
  *
- * Deterministic policies where the reference is racy or unspecified
- * (documented in DESIGN.md):
+ * Deterministic policies where the reference is racy or unspecified:
  *   - candidate triangles of a vertex are visited in ascending triangle index
  *     (the reference visits them in CGAL box_intersection_d callback order);
  *   - a contact that takes off stores r = 0 for the friction residual (the
@@ -215,6 +214,10 @@ struct orc_sim {
     double mesh_area_density = 0., mesh_bend = 0., mesh_stretch = 0.;
     pdfc_params prm{};
     double tol_eff = 0.;  // sqrt(tol*tol): PhysicScene.cpp:183-202 passes sqrt(tol2)
+    // ---- warm start of the next step (no reference counterpart): obstacle reactions in
+    // contact order, used instead of the cone projection for iterations < warm_hold
+    std::vector<V3> warm_r;
+    int warm_hold = 0;
     // ---- topology ----
     std::vector<std::array<int, 2>> edges;
     std::vector<char> border;
@@ -846,7 +849,26 @@ void step(orc_sim& S) {
                 fv = sub(fv, scale(S.mass[c.v], c.speed));
                 V3 fl = to_local(c.B, fv);
                 V3 r;
-                c.active = cone_project(fl, c.mu, r) ? 1 : 0;
+                const bool warm = it < S.warm_hold && ci < (int)S.warm_r.size() &&
+                                  (S.warm_r[ci][0] != 0. || S.warm_r[ci][1] != 0. || S.warm_r[ci][2] != 0.);
+                if (warm) {
+                    // warm start: the given reaction, projected on the cone (take-off if r_N <= 0);
+                    // a zero reaction (none given) starts plain
+                    r = S.warm_r[ci];
+                    if (!(r[0] > 0.)) {
+                        r = {0, 0, 0};
+                        c.active = 0;
+                    } else {
+                        double rt = std::sqrt(r[1] * r[1] + r[2] * r[2]);
+                        if (rt > c.mu * r[0]) {
+                            r[1] *= c.mu * r[0] / rt;
+                            r[2] *= c.mu * r[0] / rt;
+                        }
+                        c.active = 1;
+                    }
+                } else {
+                    c.active = cone_project(fl, c.mu, r) ? 1 : 0;
+                }
                 c.r = r;  // take-off: r = 0 (documented policy)
                 rg[ci] = c.active ? to_global(c.B, r) : V3{0, 0, 0};
             }
@@ -928,6 +950,8 @@ void step(orc_sim& S) {
     S.q = qk;
     S.v = vk;
     S.time += dt;
+    S.warm_r.clear();
+    S.warm_hold = 0;
 }
 
 void fill_contact(const orc_sim& S, const ObsContact& c, pdfc_contact& o) {
@@ -998,6 +1022,18 @@ int orc_create(const pdfc_scene* sc, const pdfc_params* prm, orc_sim** out) {
 }
 
 void orc_destroy(orc_sim* S) { delete S; }
+
+/* Warm start of the next step only: reactions (3 per obstacle contact, contact basis, in the
+ * order of that step's contact list) used instead of the cone projection in iterations < hold;
+ * a zero reaction leaves its contact plain. */
+void orc_set_warm(orc_sim* S, const double* r_local, size_t n, int hold) {
+    S->warm_r.resize(n);
+    for (size_t i = 0; i < n; ++i) S->warm_r[i] = {r_local[3 * i], r_local[3 * i + 1], r_local[3 * i + 2]};
+    S->warm_hold = hold;
+}
+
+/* Local-global iterations of the following steps (the parameter "LocalGlobal"). */
+void orc_set_iterations(orc_sim* S, int K) { S->prm.local_global_iterations = K; }
 
 int orc_step(orc_sim* S, int nsteps) {
     try {
@@ -1144,6 +1180,121 @@ double orc_alart_curnier(const double* u, const double* r, double mu, double* ac
 void orc_contact_basis(const double* n, double* B9) {
     Basis B = contact_basis(V3{n[0], n[1], n[2]});
     for (int k = 0; k < 3; ++k) { B9[k] = B.n[k]; B9[3 + k] = B.t[k]; B9[6 + k] = B.b[k]; }
+}
+
+/* ---- contact MLP of the learned warm start (no reference counterpart):
+ * 12 -> 64 -> 64 -> 64 -> 3, SiLU, in FP64. With bf16 != 0 the values the device
+ * rounds to BF16 before its MFMA products are rounded the same way here (normalised
+ * inputs, weights, hidden activations, backward operands), so the remaining
+ * difference to the device is its FP32 accumulation and exponential; with bf16 == 0
+ * it is the exact FP64 network, whose gradient tests/test_oracle.py pins by central
+ * finite differences. Parameter layout as climatenet_amd/csrc/mlp.hpp. */
+static float bf16_round(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        u = (u | 0x400000u) & 0xffff0000u;
+    } else {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    }
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+static const int kF = 12, kO = 3;
+static int mk(int l) { return l == 0 ? 16 : 64; }
+static int mn(int l) { return l == 3 ? 16 : 64; }
+static int offw(int l) { return l == 0 ? 0 : (l == 1 ? 1088 : (l == 2 ? 5248 : 9408)); }
+static int offb(int l) { return offw(l) + mk(l) * mn(l); }
+
+struct MlpFwd {
+    std::vector<double> x;           // 16 normalised inputs
+    std::vector<double> z[3], h[3];  // 64 each: pre-activations and activations
+    std::vector<double> y;           // 16
+};
+
+static double rnd(bool bf, double v) { return bf ? (double)bf16_round((float)v) : v; }
+
+static void mlp_fwd(const float* P, const float* norm, const float* xr, bool bf, MlpFwd& F) {
+    F.x.assign(16, 0.);
+    for (int c = 0; c < kF; ++c) F.x[c] = rnd(bf, ((double)xr[c] - norm[c]) / norm[kF + c]);
+    const std::vector<double>* in = &F.x;
+    for (int l = 0; l < 3; ++l) {
+        F.z[l].assign(64, 0.);
+        F.h[l].assign(64, 0.);
+        for (int n = 0; n < 64; ++n) {
+            double s = 0.;
+            for (int k = 0; k < mk(l); ++k) s += (*in)[k] * rnd(bf, P[offw(l) + k * mn(l) + n]);
+            s += (double)P[offb(l) + n];
+            F.z[l][n] = s;
+            F.h[l][n] = rnd(bf, s / (1. + std::exp(-s)));
+        }
+        in = &F.h[l];
+    }
+    F.y.assign(16, 0.);
+    for (int n = 0; n < 16; ++n) {
+        double s = 0.;
+        for (int k = 0; k < 64; ++k) s += F.h[2][k] * rnd(bf, P[offw(3) + k * 16 + n]);
+        F.y[n] = s + (double)P[offb(3) + n];
+    }
+}
+
+void orc_mlp_forward(const float* P, const float* norm, const float* x, size_t rows, int bf16, double* y) {
+    MlpFwd F;
+    for (size_t r = 0; r < rows; ++r) {
+        mlp_fwd(P, norm, x + r * kF, bf16 != 0, F);
+        for (int c = 0; c < kO; ++c) y[r * kO + c] = F.y[c] * norm[2 * kF + c];
+    }
+}
+
+/* L = mean over rows and the 3 outputs of (y - t/scale)^2, y the unscaled output;
+ * grad in the flat parameter layout (padding entries stay 0). Returns L. */
+double orc_mlp_gradient(const float* P, const float* norm, const float* x, const float* t, size_t rows, int bf16,
+                        double* grad) {
+    const bool bf = bf16 != 0;
+    const int NP = 10448;
+    for (int p = 0; p < NP; ++p) grad[p] = 0.;
+    const double inv = 1. / (double)(rows * kO);
+    double loss = 0.;
+    MlpFwd F;
+    std::vector<double> G(16), dz(64);
+    for (size_t r = 0; r < rows; ++r) {
+        mlp_fwd(P, norm, x + r * kF, bf, F);
+        G.assign(16, 0.);
+        for (int c = 0; c < kO; ++c) {
+            double diff = F.y[c] - (double)t[r * kO + c] / norm[2 * kF + c];
+            loss += diff * diff;
+            double d = 2. * diff * inv;
+            grad[offb(3) + c] += d;
+            G[c] = rnd(bf, d);
+        }
+        for (int l = 3; l >= 0; --l) {
+            const std::vector<double>& hp = l == 0 ? F.x : F.h[l - 1];
+            const int K = mk(l), N = mn(l);
+            for (int i = 0; i < K; ++i) {
+                if (l == 0 && i >= kF) continue;
+                for (int n = 0; n < N; ++n) {
+                    if (l == 3 && n >= kO) continue;
+                    grad[offw(l) + i * N + n] += hp[i] * G[n];
+                }
+            }
+            if (l == 0) break;
+            for (int i = 0; i < 64; ++i) {
+                double s = 0.;
+                for (int n = 0; n < N; ++n) s += G[n] * rnd(bf, P[offw(l) + i * N + n]);
+                double zz = F.z[l - 1][i];
+                double sg = 1. / (1. + std::exp(-zz));
+                dz[i] = s * (sg * (1. + zz * (1. - sg)));
+            }
+            G.assign(64, 0.);
+            for (int i = 0; i < 64; ++i) {
+                grad[offb(l - 1) + i] += dz[i];
+                G[i] = rnd(bf, dz[i]);
+            }
+        }
+    }
+    return loss * inv;
 }
 
 int orc_num_threads(void) {

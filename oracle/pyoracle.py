@@ -31,6 +31,10 @@ def lib():
         L.orc_destroy.argtypes = [P]
         L.orc_destroy.restype = None
         L.orc_step.argtypes = [P, C.c_int]
+        L.orc_set_warm.argtypes = [P, C.POINTER(C.c_double), C.c_size_t, C.c_int]
+        L.orc_set_warm.restype = None
+        L.orc_set_iterations.argtypes = [P, C.c_int]
+        L.orc_set_iterations.restype = None
         L.orc_detect.argtypes = [P, C.c_int]
         L.orc_get_state.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_set_state.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -56,6 +60,11 @@ def lib():
         L.orc_alart_curnier.restype = C.c_double
         L.orc_contact_basis.argtypes = [C.POINTER(C.c_double)] * 2
         L.orc_contact_basis.restype = None
+        F = C.POINTER(C.c_float)
+        L.orc_mlp_forward.argtypes = [F, F, F, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.orc_mlp_forward.restype = None
+        L.orc_mlp_gradient.argtypes = [F, F, F, F, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.orc_mlp_gradient.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -96,6 +105,14 @@ class Oracle:
     def detect(self, brute=False):
         if lib().orc_detect(self.h, int(brute)) != 0:
             raise RuntimeError(lib().orc_last_error().decode())
+
+    def set_warm(self, r_local, hold=1):
+        """Warm start of the next step: reactions [C x 3] in that step's contact order."""
+        r = np.ascontiguousarray(r_local, dtype=np.float64).reshape(-1, 3)
+        lib().orc_set_warm(self.h, _d(r), r.shape[0], int(hold))
+
+    def set_iterations(self, K):
+        lib().orc_set_iterations(self.h, int(K))
 
     def state(self):
         q = np.zeros(3 * self.n)
@@ -210,3 +227,28 @@ def contact_basis(n):
     B = np.zeros(9)
     lib().orc_contact_basis(_d(n), _d(B))
     return B.reshape(3, 3)  # rows: n, t, b
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(params, norm, x, bf16=True):
+    """FP64 contact MLP (with the device's BF16 rounding points when bf16): y [rows x 3]."""
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Z = np.ascontiguousarray(norm, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+    y = np.zeros((x.shape[0], 3))
+    lib().orc_mlp_forward(_f(P), _f(Z), _f(x), x.shape[0], int(bf16), _d(y))
+    return y
+
+
+def mlp_gradient(params, norm, x, t, bf16=True):
+    """(gradient [10448], loss) of the mean squared error in normalised output units."""
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Z = np.ascontiguousarray(norm, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+    t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+    g = np.zeros(10448)
+    loss = lib().orc_mlp_gradient(_f(P), _f(Z), _f(x), _f(t), x.shape[0], int(bf16), _d(g))
+    return g, loss
