@@ -35,6 +35,46 @@ def cpu_baseline(spec, budget_s=20.0):
             "sample": "steps 4..%d of the same drape, %d OpenMP threads, factorisation excluded" % (3 + k, threads)}
 
 
+def bench_batched(args):
+    """64 independent cloths (N ~ U{32..128}, one sphere each) advanced as one batch, against
+    the same 64 scenes run one after another; instance-steps/s of each."""
+    from climatenet_amd import native, scenes
+    specs = scenes.batched_instances(64, seed=0)
+    t0 = time.perf_counter()
+    sim = native.BatchSimulation(specs, record_iterations=False, use_graph=not args.no_graph)
+    create_s = time.perf_counter() - t0
+    sim.step(args.warmup)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    batched_s = time.perf_counter() - t0
+    info = sim.info()
+    seq_s = 0.0
+    for sp in specs:
+        one = native.Simulation(sp, record_iterations=False, use_graph=not args.no_graph)
+        one.step(args.warmup)
+        t1 = time.perf_counter()
+        one.step(args.steps)
+        seq_s += time.perf_counter() - t1
+        one.close()
+    value = len(specs) * args.steps / batched_s
+    out = {
+        "metric": "instance-steps/s, 64 batched cloths",
+        "value": round(value, 3), "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * batched_s / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 cloths of N x N quads, N ~ U{32..128} (seed 0), 1 m side, one sphere each "
+                               "(R ~ U[0.2, 0.6], mu ~ U[0, 0.8]), dt=1/100, 15 local-global iterations",
+                   "total_vertices": info["n_vertices"], "levels": info["n_levels"],
+                   "levels_swept": info["n_levels_swept"], "top_block_rows": info["top_m"],
+                   "graph_replay": not args.no_graph, "create_seconds": round(create_s, 2),
+                   "autotune_ms": round(info["autotune_ms"], 1)},
+        "sequential": {"value": round(len(specs) * args.steps / seq_s, 3), "unit": "instance-steps/s",
+                       "note": "the same 64 scenes, one simulation after another on the GPU"},
+        "speedup_vs_sequential": round(seq_s / batched_s, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,10 +85,27 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
+    ap.add_argument("--workload", choices=["drape", "batched"], default="drape",
+                    help="drape: the 40k-vertex drape (default); batched: 64 independent cloths")
+    ap.add_argument("--no-graph", action="store_true", help="batched: launch kernels eagerly")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8, "copies": 4}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     from climatenet_amd import abi, native, scenes
 
@@ -71,10 +128,12 @@ def main():
     sim.set_kernel_timing(False)
     info = sim.info()
     K = spec.iterations
+    # one launch per level, or one dependency-driven launch per sweep
+    per_sweep = 1 if info["flow"] else max(1, info["n_levels_swept"])
     fam = {
         "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
-        "k_forward": (kt["forward_us"], info["bytes_forward"] / max(1, info["n_levels_swept"]), kt["forward_launches"]),
-        "k_backward": (kt["backward_us"], info["bytes_backward"] / max(1, info["n_levels_swept"]), kt["backward_launches"]),
+        "k_forward": (kt["forward_us"], info["bytes_forward"] / per_sweep, kt["forward_launches"]),
+        "k_backward": (kt["backward_us"], info["bytes_backward"] / per_sweep, kt["backward_launches"]),
         "k_top": (kt["top_us"], info["bytes_top"], kt["top_launches"]),
     }
     dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
@@ -105,6 +164,11 @@ def main():
                    "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"],
                    "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
+                   "schedule": ({"sweeps": "dependency-driven", "waves": [info["flow_waves_forward"],
+                                 info["flow_waves_backward"]], "blocks": info["flow_blocks"]}
+                                if info["flow"] else {"sweeps": "level by level"}),
+                   "autotune_pair_ms": {"levels": round(info["levels_pair_ms"], 4),
+                                        "chosen": round(info["best_pair_ms"], 4)},
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},

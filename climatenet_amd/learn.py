@@ -1,15 +1,16 @@
 """Learned warm start: contact data from the simulator, training on the GPU with the
 library's own MLP kernels, and the comparison against the plain warm start.
 
-    python -m climatenet_amd.learn generate --out data.npz [--samples 1000000]
+    python -m climatenet_amd.learn generate --out data.npz
     python -m climatenet_amd.learn train --data data.npz --out contact_mlp.npz
-    python -m climatenet_amd.learn evaluate --model contact_mlp.npz [--steps 40]
+    python -m climatenet_amd.learn evaluate --model contact_mlp.npz
 
-Data: runs of the 20x20 sphere drape (jitter seeds 0, 1, ...) and of the three stacked
-150x150 sheets, every step iterated until the mean friction (Alart-Curnier) residual
-is below 1e-6 (at most 1000 iterations). A sample is one obstacle contact: the 12
-features at the start of its step and the converged reaction per unit mass in the
-contact basis (pdfc_get_warm_data). Only converged steps are kept.
+Data: runs of the 20x20 sphere drape (jitter seeds 0..4, 200 steps, FP64) and of the
+three stacked 150x150 sheets (FP32 substitution + FP64 refinement, from the first
+contacts), every step iterated until the mean friction (Alart-Curnier) residual is
+below 1e-6 or the iteration cap is reached. A sample is one obstacle contact: the 12
+features at the start of its step and the final reaction per unit mass in the contact
+basis (pdfc_get_warm_data), with the residual its step reached.
 """
 import argparse
 import json
@@ -22,50 +23,52 @@ import numpy as np
 from . import abi, native, scenes
 
 TOL = 1e-6
-CAP = 1000
+CAP = 8000
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL = os.path.join(HERE, "data", "contact_mlp.npz")
 
 
-def generate(samples=1_000_000, tol=TOL, cap=CAP, log=print):
-    """Converged contact samples from the 20x20 drape (seeds 0..) and the stacked sheets."""
-    X, Y = [], []
-    total = 0
+def run_for_data(spec, steps, precision=abi.FP64, start_steps=0, cap=CAP, tol=TOL, log=print):
+    """(features, targets, residual reached) of every contact step of `steps` adaptive steps."""
+    sim = native.Simulation(spec, precision=precision)
+    if start_steps:
+        sim.step(start_steps)
+    q, v = sim.state()
+    sim.close()
+    sim = native.Simulation(spec, precision=precision, residual_tolerance=tol, max_iterations=cap)
+    sim.set_state(q, v)
+    X, Y, R = [], [], []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        sim.step(1)
+        if not sim.phase_stats()["contact_steps"]:
+            continue
+        x, y = sim.warm_data()
+        m, _ = sim.iteration_stats()
+        X.append(x)
+        Y.append(y)
+        R.append(np.full(len(x), m[-1], dtype=np.float32))
+    n = sum(len(x) for x in X)
+    log("  %s: %d samples, %.1f s" % (spec.name, n, time.perf_counter() - t0))
+    if not X:
+        return np.zeros((0, 12), np.float32), np.zeros((0, 3), np.float32), np.zeros(0, np.float32)
+    return np.concatenate(X), np.concatenate(Y), np.concatenate(R)
 
-    def run(spec, steps, precision):
-        nonlocal total
-        sim = native.Simulation(spec, precision=precision, residual_tolerance=tol, max_iterations=cap)
-        t0 = time.perf_counter()
-        kept = 0
-        for s in range(steps):
-            sim.step(1)
-            ps = sim.phase_stats()
-            if ps["converged_steps"] != 1:
-                continue
-            x, y = sim.warm_data()
-            if len(x):
-                X.append(x)
-                Y.append(y)
-                kept += len(x)
-                total += len(x)
-            if total >= samples:
-                break
-        log("  %s: %d samples, %.1f s" % (spec.name, kept, time.perf_counter() - t0))
 
-    # the stacked sheets first: most of the samples, FP32 substitution + FP64 refinement
-    run(scenes.stacked_layers(), 400, abi.FP32_REFINE64)
-    seed = 0
-    while total < samples and seed < 200:
-        run(scenes.sphere_drape_small(jitter_seed=seed), 200, abi.FP64)
-        seed += 1
-    X = np.concatenate(X)[:samples]
-    Y = np.concatenate(Y)[:samples]
-    return X, Y
+def generate(small_seeds=5, stacked_steps=25, stacked_cap=4000, log=print):
+    parts = [run_for_data(scenes.sphere_drape_small(jitter_seed=s), 200, log=log) for s in range(small_seeds)]
+    parts.append(run_for_data(scenes.stacked_layers(), stacked_steps, abi.FP32_REFINE64, start_steps=40,
+                              cap=stacked_cap, log=log))
+    return tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
 
 
 def normalisation(X, Y):
-    """Feature mean and std (std floored), per-output scale = std of the target."""
+    """Feature mean and std, per-output scale = std of the target. The std is floored at a
+    quarter of the mean's magnitude, so a feature that is constant in the data (mu, when all
+    runs share one friction coefficient) maps other values to moderate inputs."""
     norm = np.zeros(abi.MLP_NORM, dtype=np.float32)
     norm[:12] = X.mean(axis=0)
-    norm[12:24] = np.maximum(X.std(axis=0), 1e-6)
+    norm[12:24] = np.maximum(X.std(axis=0), np.maximum(0.25 * np.abs(X.mean(axis=0)), 1e-6))
     norm[24:27] = np.maximum(Y.std(axis=0), 1e-9)
     return norm
 
@@ -140,32 +143,38 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     g = sub.add_parser("generate")
     g.add_argument("--out", required=True)
-    g.add_argument("--samples", type=int, default=1_000_000)
+    g.add_argument("--small-seeds", type=int, default=5)
+    g.add_argument("--stacked-steps", type=int, default=25)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
-    t.add_argument("--out", required=True)
-    t.add_argument("--epochs", type=int, default=30)
-    t.add_argument("--batch", type=int, default=512)
+    t.add_argument("--out", default=MODEL)
+    t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
     e = sub.add_parser("evaluate")
-    e.add_argument("--model", required=True)
-    e.add_argument("--steps", type=int, default=40)
-    e.add_argument("--warmup", type=int, default=150)
+    e.add_argument("--model", default=MODEL)
+    e.add_argument("--steps", type=int, default=10)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "generate":
-        X, Y = generate(a.samples, log=log)
-        np.savez(a.out, X=X, Y=Y)
-        log("%d samples -> %s" % (len(X), a.out))
+        X, Y, R = generate(a.small_seeds, a.stacked_steps, log=log)
+        np.savez(a.out, X=X, Y=Y, R=R)
+        log("%d samples -> %s (residual reached: median %.2e, max %.2e)" % (len(X), a.out, np.median(R), R.max()))
     elif a.cmd == "train":
         d = np.load(a.data, allow_pickle=False)
         mlp, losses, dt = train(d["X"], d["Y"], a.epochs, a.batch, a.lr, log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         save(a.out, mlp)
-        log(json.dumps({"epochs": a.epochs, "samples": int(len(d["X"])), "seconds": dt,
-                        "loss_first": float(losses[0]), "loss_last": float(losses[-1])}))
+        X, Y = d["X"], d["Y"]
+        scale = np.linalg.norm(Y, axis=1).mean()
+        err = np.linalg.norm(mlp.forward(X) - Y, axis=1) / (np.linalg.norm(Y, axis=1) + 0.1 * scale)
+        log(json.dumps({"epochs": a.epochs, "samples": int(len(X)), "seconds": round(dt, 2),
+                        "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                        "relative_error_median": float(np.median(err)),
+                        "relative_error_p90": float(np.percentile(err, 90))}))
     else:
         mlp = load(a.model)
-        res = evaluate(scenes.sphere_drape_40k(), mlp, a.steps, a.warmup, log=log)
+        res = evaluate(scenes.sphere_drape_small(jitter_seed=7), mlp, a.steps, 60, log=log)
         log(json.dumps({k: {kk: vv for kk, vv in v.items() if kk not in ("q", "v")} for k, v in res.items()}))
 
 

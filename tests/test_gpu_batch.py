@@ -3,7 +3,8 @@ the HIP-graph replay of a step.
 
 Batched parity is per scene against the oracle run on that scene alone, from
 identical states, to the FP64 bar (1e-9 relative, identical contact sets); the
-batch also agrees with each scene run alone on the GPU. A replayed graph runs the
+batch also agrees with each scene run alone on the GPU (1e-10: only the summation
+order of the autotuned substitution differs). A replayed graph runs the
 same kernels with the same arguments as the eager step, so its states are
 bit-identical to the eager ones; replays are interleaved with eager work on the
 same stream and checked one by one.
@@ -69,8 +70,10 @@ def test_batch_equals_scenes_run_alone(gpu):
         one = gpu.Simulation(sp)
         one.step(30)
         q1, v1 = one.state()
-        assert rel_err(q[3 * a:3 * b], q1) < 1e-12
-        assert rel_err(v[3 * a:3 * b], v1) < 1e-12
+        # same supernodes and operators; the substitution's per-level wave split (autotuned
+        # separately for the batch and for each scene) changes the summation order only
+        assert rel_err(q[3 * a:3 * b], q1) < 1e-10
+        assert rel_err(v[3 * a:3 * b], v1) < 1e-10
 
 
 def test_graph_replay_is_bit_identical_to_eager(gpu):
