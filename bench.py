@@ -18,6 +18,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel, info):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for
+    a device copy of known size), when it was measured on this workload, kernel and level
+    structure; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "drape40k_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("kernel") != kernel or d.get("levels_swept") != info["n_levels_swept"] or \
+            d.get("n_vertices") != info["n_vertices"] or d.get("nnz_F") != info["nnz_F"]:
+        return None
+    return d.get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(spec, budget_s=20.0):
     """The CPU oracle (reference algorithm and OpenMP structure) on a bounded sample."""
     from oracle import pyoracle as po
@@ -33,6 +49,36 @@ def cpu_baseline(spec, budget_s=20.0):
     dt = time.perf_counter() - t0
     return {"value": k / dt, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": "steps 4..%d of the same drape, %d OpenMP threads, factorisation excluded" % (3 + k, threads)}
+
+
+def warm_start_comparison(spec, sim, prec, steps=2, cap=2000, tol=1e-6):
+    """Iterations to a 1e-6 mean friction residual (at most `cap`) from the bench's current
+    state, plain warm start against the committed contact network, `steps` steps each."""
+    from climatenet_amd import abi, learn, native
+    if not os.path.exists(learn.MODEL):
+        return None
+    q, v = sim.state()
+    mlp = learn.load(learn.MODEL)
+    res = {"tolerance": tol, "max_iterations": cap, "steps": steps}
+    for name, ws in (("plain", abi.WARM_PLAIN), ("learned", abi.WARM_LEARNED)):
+        s2 = native.Simulation(spec, precision=prec, residual_tolerance=tol, max_iterations=cap, warm_start=ws,
+                               mlp=mlp if ws == abi.WARM_LEARNED else None)
+        s2.set_state(q, v)
+        it, conv, fb, resid = 0, 0, 0, []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s2.step(1)
+            ps = s2.phase_stats()
+            it += ps["iterations_to_tolerance"]
+            conv += ps["converged_steps"]
+            fb += ps["fallbacks"]
+            resid.append(float(s2.iteration_stats()[0][-1]))
+        res[name] = {"iterations_to_tolerance": it / steps, "converged_steps": conv, "fallbacks": fb,
+                     "final_residual": resid, "seconds_per_step": round((time.perf_counter() - t0) / steps, 3)}
+        s2.close()
+    res["iteration_ratio_plain_over_learned"] = round(res["plain"]["iterations_to_tolerance"] /
+                                                      max(1e-9, res["learned"]["iterations_to_tolerance"]), 4)
+    return res
 
 
 def bench_batched(args):
@@ -82,6 +128,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", choices=["fp64", "fp32r"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the plain vs learned warm-start comparison (iterations to a 1e-6 residual)")
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
@@ -140,7 +188,7 @@ def main():
     us, bytes_per_launch, _ = fam[dom]
     achieved = bytes_per_launch / (us * 1e-6) / 1e9 if us > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, info), "kernel": dom,
                 "avg_launch_us": round(us, 3), "algorithmic_bytes_per_launch": bytes_per_launch}
 
     out = {
@@ -169,11 +217,14 @@ def main():
                                 if info["flow"] else {"sweeps": "level by level"}),
                    "autotune_pair_ms": {"levels": round(info["levels_pair_ms"], 4),
                                         "chosen": round(info["best_pair_ms"], 4)},
+                   "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
         "self_collision_host_us_per_step": round(phases["detection_us"], 1),
     }
+    if not args.no_warm_start:
+        out["warm_start"] = warm_start_comparison(spec, sim, prec)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec)
     print(json.dumps(out), flush=True)

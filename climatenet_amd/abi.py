@@ -144,6 +144,8 @@ class Info(C.Structure):
         ("flow_blocks", C.c_int32),
         ("levels_pair_ms", C.c_double),
         ("best_pair_ms", C.c_double),
+        ("leaf_size", C.c_int32),
+        ("top_chunks", C.c_int32),
     ]
 
 

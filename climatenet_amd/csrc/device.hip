@@ -477,6 +477,9 @@ __global__ void k_warm_features(DevModel M, const double* __restrict__ qn, const
 template <typename T>
 __device__ __forceinline__ double as_d(T x) { return (double)x; }
 
+// Operator entries per thread loaded before a task's gather barrier.
+constexpr int kPrefetch = 4;
+
 // Forward substitution, one task = (supernode s, 64 rows of F_s), W waves:
 //   b~_s = b_s - sum_children u_c|own ;  [y_s ; u_s] = F_s b~_s (+ children's u_c|below)
 // The reduction over the ns columns is split across the W waves (wave w takes
@@ -493,6 +496,16 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     double* bz = lds + 2 * ns;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int* pp = Fa.pull_ptr + d.pull_base;
+    // The first kPrefetch operator entries of this thread's column slice do not depend on
+    // the gather: issue them now so their latency overlaps the gather's.
+    const int my_row = r0 + lane;
+    const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
+    double pre[kPrefetch];
+#pragma unroll
+    for (int q = 0; q < kPrefetch; ++q) {
+        const int j = wave + q * W;
+        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * f]) : 0.;
+    }
     // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
@@ -507,7 +520,6 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
         bz[t] = z;
     }
     // this task's below rows: children's entries to pass on
-    const int my_row = r0 + lane;
     double ux = 0., uy = 0., uz = 0.;
     if (wave == 0 && my_row >= ns && my_row < f) {
         for (int q = pp[my_row]; q < pp[my_row + 1]; ++q) {
@@ -520,9 +532,17 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
-        const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
+#pragma unroll
+        for (int q = 0; q < kPrefetch; ++q) {
+            const int j = wave + q * W;
+            if (j < ns) {
+                ax = fma(pre[q], bx[j], ax);
+                ay = fma(pre[q], by[j], ay);
+                az = fma(pre[q], bz[j], az);
+            }
+        }
 #pragma unroll 8
-        for (int j = wave; j < ns; j += W) {
+        for (int j = wave + kPrefetch * W; j < ns; j += W) {
             double fv = as_d(Fs[(size_t)j * f]);
             ax = fma(fv, bx[j], ax);
             ay = fma(fv, by[j], ay);
@@ -578,6 +598,14 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
     double* gy = lds + f;
     double* gz = lds + 2 * f;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = j0 + lane;
+    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
+    double pre[kPrefetch];  // operator entries independent of the gather, issued first
+#pragma unroll
+    for (int q = 0; q < kPrefetch; ++q) {
+        const int i = wave + q * W;
+        pre[q] = (j < ns && i < f) ? as_d(Ft[(size_t)i * ns]) : 0.;
+    }
     for (int t = tid; t < ns; t += 64 * W) {
         gx[t] = Y[c0 + t];
         gy[t] = Y[n + c0 + t];
@@ -591,12 +619,19 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
         gz[ns + t] = -X[2 * n + r];
     }
     __syncthreads();
-    const int j = j0 + lane;
     double ax = 0., ay = 0., az = 0.;
     if (j < ns) {
-        const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
+#pragma unroll
+        for (int q = 0; q < kPrefetch; ++q) {
+            const int i = wave + q * W;
+            if (i < f) {
+                ax = fma(pre[q], gx[i], ax);
+                ay = fma(pre[q], gy[i], ay);
+                az = fma(pre[q], gz[i], az);
+            }
+        }
 #pragma unroll 8
-        for (int i = wave; i < f; i += W) {
+        for (int i = wave + kPrefetch * W; i < f; i += W) {
             double fv = as_d(Ft[(size_t)i * ns]);
             ax = fma(fv, gx[i], ax);
             ay = fma(fv, gy[i], ay);

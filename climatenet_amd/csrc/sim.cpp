@@ -54,6 +54,7 @@ void pdfc::set_last_error(const char* msg) { g_last_error = msg; }
 // start is abandoned for the plain one (the step restarts) when the residual has not
 // dropped by kStagnationFactor over kStagnationWindow iterations.
 constexpr int kCheckChunk = 8;
+constexpr int kMaxTopChunks = 16;  // reduction split of the dense top-block matvec
 // Dependency-driven substitution sweeps (one launch per sweep): switched off, see autotune_waves().
 constexpr bool kFlowEnabled = false;
 constexpr int kReleaseWindow = 50;
@@ -71,6 +72,7 @@ struct pdfc_sim {
     std::vector<int> obs_ptr;           // per scene: its obstacles
     std::vector<int> scene_v0;          // per scene: first vertex
     int n_scenes = 1, max_local_obs = 0;
+    int leaf = 64;                      // leaf supernode size of the dissection
     size_t cap_contacts = 1;
     std::vector<double> friction;
     int n_mat = 0, mesh_material = 0;
@@ -300,7 +302,12 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     cap_contacts = std::max<size_t>(1, (size_t)n * std::max(1, max_local_obs));
     const double dt = p.time_step;
     try {
-        int leaf = 32;  // largest leaf supernode of the dissection
+        // Largest leaf supernode of the dissection. Larger leaves mean fewer (latency-bound)
+        // substitution levels but denser leaf operators; measured on MI355X: 96 is fastest
+        // for the 40k-vertex drape, 64 for batches of 1k-17k-vertex scenes.
+        int largest = 0;
+        for (size_t k = 0; k < count; ++k) largest = std::max(largest, (int)scenes[k].n_vertices);
+        leaf = largest >= 25000 ? 96 : 64;
         if (const char* e = std::getenv("PDFC_LEAF")) leaf = std::max(8, std::atoi(e));
         pdfc::factorize(model, dt * dt, fac, leaf);
     } catch (const std::runtime_error& e) {
@@ -747,6 +754,21 @@ void pdfc_sim::autotune_waves() {
             }
         }
     }
+    // Dense top block: the split of the matvec's reduction across workgroups (more
+    // workgroups stream S^-1 faster, fewer partial sums finish faster).
+    if (use_top) {
+        float best = 1e30f;
+        int best_kc = dtop.kchunks;
+        for (int kc = 1; kc <= kMaxTopChunks; kc *= 2) {
+            dtop.kchunks = kc;
+            float ms = time_it([&] { pdfc::launch_top(dtop, fp32, dfwd.U, V[1], V[1], n, stream); });
+            if (ms < best) {
+                best = ms;
+                best_kc = kc;
+            }
+        }
+        dtop.kchunks = best_kc;
+    }
     // Schedule: the level-by-level sweeps with the waves above, or one dependency-driven
     // launch per sweep (waves per task and grid size timed over forward + backward pairs,
     // which reset each other's counters).
@@ -803,8 +825,8 @@ void pdfc_sim::build_top() {
     dtop.cidx = upload(top.cidx);
     dtop.bt = dalloc<double>(3 * (size_t)m);
     const int rowblocks = (m + 63) / 64;
-    dtop.kchunks = std::max(1, std::min(16, 512 / std::max(1, rowblocks)));
-    dtop.part = dalloc<double>((size_t)dtop.kchunks * 3 * m);
+    dtop.kchunks = std::max(1, std::min(kMaxTopChunks, 512 / std::max(1, rowblocks)));
+    dtop.part = dalloc<double>((size_t)kMaxTopChunks * 3 * m);  // the autotuner may change kchunks
     double* S64 = dalloc<double>((size_t)m * m);
     for (int j = 0; j < m; j += 3) {
         pdfc::launch_unit3(V[1], n, dtop.rows, m, j, stream);
@@ -1665,6 +1687,8 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         for (size_t l = 0; l < sim->fwd_waves.size() && l < 32; ++l) I.waves_forward[l] = sim->fwd_waves[l];
         for (size_t l = 0; l < sim->bwd_waves.size() && l < 32; ++l) I.waves_backward[l] = sim->bwd_waves[l];
         I.autotune_ms = sim->autotune_ms;
+        I.leaf_size = sim->leaf;
+        I.top_chunks = sim->use_top ? sim->dtop.kchunks : 0;
         I.flow = sim->use_flow ? 1 : 0;
         I.flow_waves_forward = sim->flow_wf;
         I.flow_waves_backward = sim->flow_wb;

@@ -179,6 +179,8 @@ typedef struct pdfc_info {
     int32_t flow_blocks;
     double levels_pair_ms;           /* autotuner: one forward + backward sweep level by level */
     double best_pair_ms;             /* and with the chosen schedule */
+    int32_t leaf_size;               /* largest leaf supernode of the dissection */
+    int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned) */
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
