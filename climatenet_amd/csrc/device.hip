@@ -248,12 +248,43 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
     }
 }
 
+// Stencil widths handled with all of a vertex's loads in flight at once (indices first,
+// then the gathers); wider stencils take the loop below.
+constexpr int kSpringCap = 8;
+constexpr int kAtaCap = 24;
+
 // (A^T p)_i at q = q_n + dt v: sum over the incident springs of w^2 L0 (q_i - q_j)/|q_i - q_j|.
 __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __restrict__ qn,
                                          const double* __restrict__ vcur, double dt, int i, D3 q) {
     const int n = M.n;
     D3 s = {0., 0., 0.};
-    int deg = M.sp_cnt[i];
+    const int deg = M.sp_cnt[i];
+    if (M.sp_slots <= kSpringCap) {
+        int jj[kSpringCap];
+        double kap[kSpringCap];
+#pragma unroll
+        for (int k = 0; k < kSpringCap; ++k)
+            if (k < deg) {
+                jj[k] = M.sp_idx[(size_t)k * n + i];
+                kap[k] = M.sp_kappa[(size_t)k * n + i];
+            }
+        D3 qj[kSpringCap];
+#pragma unroll
+        for (int k = 0; k < kSpringCap; ++k)
+            if (k < deg) {
+                const int j = jj[k];
+                qj[k] = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
+            }
+#pragma unroll
+        for (int k = 0; k < kSpringCap; ++k)
+            if (k < deg) {
+                D3 e = normalized3(D3{q.x - qj[k].x, q.y - qj[k].y, q.z - qj[k].z});
+                s.x += kap[k] * e.x;
+                s.y += kap[k] * e.y;
+                s.z += kap[k] * e.z;
+            }
+        return s;
+    }
     for (int k = 0; k < deg; ++k) {
         int j = M.sp_idx[(size_t)k * n + i];
         double kap = M.sp_kappa[(size_t)k * n + i];
@@ -270,7 +301,26 @@ __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __rest
 __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restrict__ vcur, int i) {
     const int n = M.n;
     D3 av = {0., 0., 0.};
-    int cnt = M.ata_cnt[i];
+    const int cnt = M.ata_cnt[i];
+    if (M.ata_slots <= kAtaCap) {
+        int jj[kAtaCap];
+        double w[kAtaCap];
+#pragma unroll
+        for (int k = 0; k < kAtaCap; ++k)
+            if (k < cnt) {
+                jj[k] = M.ata_idx[(size_t)k * n + i];
+                w[k] = M.ata_val[(size_t)k * n + i];
+            }
+#pragma unroll
+        for (int k = 0; k < kAtaCap; ++k)
+            if (k < cnt) {
+                const int j = jj[k];
+                av.x += w[k] * vcur[j];
+                av.y += w[k] * vcur[n + j];
+                av.z += w[k] * vcur[2 * n + j];
+            }
+        return av;
+    }
     for (int k = 0; k < cnt; ++k) {
         int j = M.ata_idx[(size_t)k * n + i];
         double w = M.ata_val[(size_t)k * n + i];

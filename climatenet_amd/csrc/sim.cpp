@@ -110,6 +110,7 @@ struct pdfc_sim {
     int flow_wf = 4, flow_wb = 4, flow_blocks = 1024;
     float levels_ms = 0.f, flow_ms = 0.f;   // autotuner: level-scheduled vs best schedule, one solve pair
     void upload_flow(int top_level);
+    void apply_pinned_tuning(const char* spec);
     void check_flow_error();
     double autotune_ms = 0.;
     void autotune_waves();
@@ -536,8 +537,12 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         use_top = true;
     }
     {
+        const char* pin = std::getenv("PDFC_TUNE");
         const char* e = std::getenv("PDFC_AUTOTUNE");
-        if (!e || std::atoi(e) != 0) autotune_waves();
+        if (pin && *pin)
+            apply_pinned_tuning(pin);
+        else if (!e || std::atoi(e) != 0)
+            autotune_waves();
     }
     nnzF_lower = 0;
     for (size_t s2 = 0; s2 < fac.ns.size(); ++s2)
@@ -701,6 +706,46 @@ void pdfc_sim::check_flow_error() {
     }
 }
 
+// Tuning given instead of measured, "f=<waves per forward level>;b=<backward>;t=<top chunks>"
+// (comma-separated; e.g. to profile the configuration an unprofiled run chose).
+void pdfc_sim::apply_pinned_tuning(const char* spec) {
+    std::string str(spec);
+    auto parse_list = [&](char key, std::vector<int>& out) {
+        const std::string tag = std::string(1, key) + "=";
+        size_t p = str.find(tag);
+        if (p == std::string::npos) return false;
+        size_t q = str.find(';', p);
+        std::string body = str.substr(p + 2, q == std::string::npos ? std::string::npos : q - p - 2);
+        out.clear();
+        size_t a = 0;
+        while (a < body.size()) {
+            size_t c = body.find(',', a);
+            out.push_back(std::atoi(body.substr(a, c == std::string::npos ? std::string::npos : c - a).c_str()));
+            if (c == std::string::npos) break;
+            a = c + 1;
+        }
+        return true;
+    };
+    std::vector<int> f, b, t;
+    auto valid_w = [](int w) { return w == 1 || w == 2 || w == 4 || w == 8 || w == 16; };
+    if (parse_list('f', f)) {
+        if (f.size() != fwd_waves.size()) throw std::invalid_argument("PDFC_TUNE: forward level count differs");
+        for (int w : f)
+            if (!valid_w(w)) throw std::invalid_argument("PDFC_TUNE: waves must be 1, 2, 4, 8 or 16");
+        fwd_waves = f;
+    }
+    if (parse_list('b', b)) {
+        if (b.size() != bwd_waves.size()) throw std::invalid_argument("PDFC_TUNE: backward level count differs");
+        for (int w : b)
+            if (!valid_w(w)) throw std::invalid_argument("PDFC_TUNE: waves must be 1, 2, 4, 8 or 16");
+        bwd_waves = b;
+    }
+    if (parse_list('t', t) && use_top) {
+        if (t.size() != 1 || t[0] < 1 || t[0] > kMaxTopChunks) throw std::invalid_argument("PDFC_TUNE: top chunks 1..16");
+        dtop.kchunks = t[0];
+    }
+}
+
 // Chooses the waves per task of every substitution level by timing each level alone
 // with 1, 2, 4, 8 and 16 waves on the device (median of 5 launches). The buffers hold
 // valid data from the top-block build or zeros; only the timing matters here.
@@ -712,21 +757,24 @@ void pdfc_sim::autotune_waves() {
     hipEvent_t a, b;
     HIP_CHECK(hipEventCreate(&a));
     HIP_CHECK(hipEventCreate(&b));
+    // fastest of 7 timed launches after a warm one (the least noisy estimate of a config)
     auto time_it = [&](auto&& launch) {
-        std::vector<float> t;
+        float best = 1e30f;
         launch();  // warm
-        for (int r = 0; r < 5; ++r) {
+        for (int r = 0; r < 7; ++r) {
             HIP_CHECK(hipEventRecord(a, stream));
             launch();
             HIP_CHECK(hipEventRecord(b, stream));
             HIP_CHECK(hipEventSynchronize(b));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-            t.push_back(ms);
+            best = std::min(best, ms);
         }
-        std::sort(t.begin(), t.end());
-        return t[2];
+        return best;
     };
+    // a candidate replaces the current choice only when clearly faster, so repeated tunings
+    // of the same scene settle on the same configuration
+    constexpr float kSwitch = 0.97f;
     const int cand[5] = {1, 2, 4, 8, 16};
     for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l) {
         float best = 1e30f;
@@ -754,20 +802,43 @@ void pdfc_sim::autotune_waves() {
             }
         }
     }
-    // Dense top block: the split of the matvec's reduction across workgroups (more
-    // workgroups stream S^-1 faster, fewer partial sums finish faster).
+    // Levels and the top block timed alone run with their operators cache-resident; inside a
+    // solve they are not. So the choices are refined on whole solves: the top block's
+    // reduction split (more workgroups stream S^-1 faster, fewer partial sums finish
+    // faster), then one pass over the levels trying each wave count with the others fixed.
+    auto solve_ms = [&] { return time_it([&] { solve_inplace(V[1]); }); };
     if (use_top) {
-        float best = 1e30f;
-        int best_kc = dtop.kchunks;
-        for (int kc = 1; kc <= kMaxTopChunks; kc *= 2) {
+        const int heuristic = dtop.kchunks;
+        float best = solve_ms();
+        int best_kc = heuristic;
+        for (int kc : {4, 8, 16}) {
+            if (kc == heuristic || kc > kMaxTopChunks) continue;
             dtop.kchunks = kc;
-            float ms = time_it([&] { pdfc::launch_top(dtop, fp32, dfwd.U, V[1], V[1], n, stream); });
-            if (ms < best) {
+            float ms = solve_ms();
+            if (ms < kSwitch * best) {
                 best = ms;
                 best_kc = kc;
             }
         }
         dtop.kchunks = best_kc;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
+        for (size_t l = 0; l < waves.size(); ++l) {
+            const int keep = waves[l];
+            float best = solve_ms();
+            int best_w = keep;
+            for (int W : cand) {
+                if (W == keep) continue;
+                waves[l] = W;
+                float ms = solve_ms();
+                if (ms < kSwitch * best) {
+                    best = ms;
+                    best_w = W;
+                }
+            }
+            waves[l] = best_w;
+        }
     }
     // Schedule: the level-by-level sweeps with the waves above, or one dependency-driven
     // launch per sweep (waves per task and grid size timed over forward + backward pairs,
@@ -872,11 +943,13 @@ void pdfc_sim::solve_inplace(double* X) {
                 pdfc::launch_forward_flow(dfwd, ffwd, fp32, inout, Y, fac.max_ns, flow_wf, flow_blocks, stream);
             });
         } else {
-            for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
-                timed(1, [&] {
+            // one event pair around the whole sweep: per-launch events would add their own
+            // overhead to every few-microsecond level kernel
+            timed(1, [&] {
+                for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
                     pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
                                                inout, Y, fac.max_ns, fwd_waves[l], stream);
-                });
+            });
         }
         if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream); });
         if (use_flow) {
@@ -884,11 +957,11 @@ void pdfc_sim::solve_inplace(double* X) {
                 pdfc::launch_backward_flow(dbwd, fbwd, fp32, Y, inout, fac.max_front, flow_wb, flow_blocks, stream);
             });
         } else {
-            for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
-                timed(2, [&] {
+            timed(2, [&] {
+                for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
                     pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
                                                 Y, inout, fac.max_front, bwd_waves[l], stream);
-                });
+            });
         }
         if (ktiming) ++ksolves;
     };
@@ -1725,16 +1798,19 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         if (!sim || !out) throw std::invalid_argument("null argument");
         sim->collect_kernel_times();
         pdfc_kernel_times t{};
+        // sweeps are timed whole (families 1 and 2 count sweeps); a sweep is one launch per
+        // swept level, or one launch with the dependency-driven schedule
+        const int64_t per_sweep = sim->use_flow ? 1 : (int64_t)(sim->fwd.level_ptr.size() - 1);
         t.local_launches = sim->kcount[0];
-        t.forward_launches = sim->kcount[1];
-        t.backward_launches = sim->kcount[2];
+        t.forward_launches = sim->kcount[1] * per_sweep;
+        t.backward_launches = sim->kcount[2] * per_sweep;
         t.local_us = sim->kcount[0] ? sim->ksum[0] / sim->kcount[0] : 0.;
-        t.forward_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
-        t.backward_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
+        t.forward_sweep_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
+        t.backward_sweep_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
+        t.forward_us = per_sweep ? t.forward_sweep_us / per_sweep : 0.;
+        t.backward_us = per_sweep ? t.backward_sweep_us / per_sweep : 0.;
         t.top_us = sim->kcount[3] ? sim->ksum[3] / sim->kcount[3] : 0.;
         t.top_launches = sim->kcount[3];
-        t.forward_sweep_us = sim->ksolves ? sim->ksum[1] / sim->ksolves : 0.;
-        t.backward_sweep_us = sim->ksolves ? sim->ksum[2] / sim->ksolves : 0.;
         *out = t;
     });
 }

@@ -196,12 +196,12 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
     int64_t fallbacks;               /* learned warm starts abandoned for the plain one */
 } pdfc_phase_stats;
 
-typedef struct pdfc_kernel_times {  /* per-launch device time (HIP events), averaged over an instrumented run */
-    double local_us;                 /* local step + RHS + obstacle friction kernel */
-    double forward_us;               /* one forward-substitution level */
-    double backward_us;              /* one backward-substitution level */
+typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over an instrumented run */
+    double local_us;                 /* local step + RHS + obstacle friction kernel, per launch */
+    double forward_us;               /* one forward-substitution level launch: sweep time / launches per sweep */
+    double backward_us;              /* one backward-substitution level launch: sweep time / launches per sweep */
     double top_us;                   /* dense top block (gather + matvec + scatter) */
-    double forward_sweep_us;         /* all forward levels of one substitution (sum of launches) */
+    double forward_sweep_us;         /* one forward sweep, one event pair around all its launches */
     double backward_sweep_us;
     int64_t local_launches, forward_launches, backward_launches, top_launches;
 } pdfc_kernel_times;
