@@ -15,6 +15,7 @@ PDFC_ERR_CAPACITY = -4
 
 OBSTACLE_SPHERE = 0
 OBSTACLE_PLANE = 1
+OBSTACLE_MESH = 2
 
 FP64 = 0
 FP32_REFINE64 = 1
@@ -35,6 +36,13 @@ class Obstacle(C.Structure):
         ("point", C.c_double * 3),
         ("normal", C.c_double * 3),
         ("radius", C.c_double),
+        ("tolerance", C.c_double),
+        ("n_mesh_vertices", C.c_size_t),
+        ("n_mesh_triangles", C.c_size_t),
+        ("mesh_triangles", C.POINTER(C.c_int32)),
+        ("n_frames", C.c_size_t),
+        ("mesh_frames", C.POINTER(C.c_double)),
+        ("frame_times", C.POINTER(C.c_double)),
     ]
 
 
@@ -85,6 +93,7 @@ class Contact(C.Structure):
         ("point", C.c_double * 3),
         ("mu", C.c_double),
         ("r_local", C.c_double * 3),
+        ("speed", C.c_double * 3),
     ]
 
 

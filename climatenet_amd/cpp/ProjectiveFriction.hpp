@@ -74,6 +74,14 @@ public:
     explicit PhysicScene(FrictionCoefficientTable friction);
     void addMesh(std::shared_ptr<PhysicMesh> mesh);
     void addObstacle(const pdfc_obstacle& obstacle);
+    // Triangle-mesh obstacle (StaticMesh: one frame; MovingMesh: key frames and their
+    // times). The scene owns the arrays; the stored pdfc_obstacle points into them.
+    struct MeshObstacleData {
+        std::vector<int32_t> triangles;  // 3T, frame 0's faces
+        std::vector<double> frames;      // n_frames x 3 n_mesh_vertices
+        std::vector<double> times;       // n_frames (empty for a static mesh)
+    };
+    void addMeshObstacle(pdfc_obstacle obstacle, std::shared_ptr<const MeshObstacleData> data);
     void finalize();
     std::shared_ptr<PhysicMesh> getMesh(std::size_t index) { return m_meshes.at(index); }
     std::shared_ptr<const PhysicMesh> getMesh(std::size_t index) const { return m_meshes.at(index); }
@@ -90,6 +98,7 @@ private:
     FrictionCoefficientTable m_friction;
     std::vector<std::shared_ptr<PhysicMesh>> m_meshes;
     std::vector<pdfc_obstacle> m_obstacles;
+    std::vector<std::shared_ptr<const MeshObstacleData>> m_mesh_data;  // kept alive for m_obstacles
 };
 
 // -------------------------------------------------------------- simulation

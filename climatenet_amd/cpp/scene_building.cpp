@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <limits>
 #include <sstream>
@@ -130,6 +131,16 @@ void PhysicMesh::writeObj(std::ofstream& of) const {
 PhysicScene::PhysicScene(FrictionCoefficientTable friction) : m_friction(std::move(friction)) {}
 void PhysicScene::addMesh(std::shared_ptr<PhysicMesh> mesh) { m_meshes.push_back(std::move(mesh)); }
 void PhysicScene::addObstacle(const pdfc_obstacle& obstacle) { m_obstacles.push_back(obstacle); }
+void PhysicScene::addMeshObstacle(pdfc_obstacle ob, std::shared_ptr<const MeshObstacleData> data) {
+    ob.mesh_triangles = data->triangles.data();
+    ob.n_mesh_triangles = data->triangles.size() / 3;
+    ob.n_frames = data->times.empty() ? 1 : data->times.size();
+    ob.n_mesh_vertices = data->frames.size() / (3 * ob.n_frames);
+    ob.mesh_frames = data->frames.data();
+    ob.frame_times = data->times.empty() ? nullptr : data->times.data();
+    m_mesh_data.push_back(std::move(data));
+    m_obstacles.push_back(ob);
+}
 void PhysicScene::finalize() {
     if (m_meshes.size() != 1)
         throw std::invalid_argument("JSON reader - Error : exactly one dynamic mesh is supported");
@@ -240,9 +251,58 @@ PhysicScene getSceneFromJSONDocument(const json::Value& doc) {
                 ob.normal[k] = nrm[k];
             }
             ob.material = (int32_t)require_uint(o, "Material Identifier");
-        } else if (type == "Axis Aligned Box" || type == "Mesh" || type == "Moving Mesh" || type == "Rotating Sphere") {
+        } else if (type == "Mesh") {
+            // src/scene_building.cpp:222-246 (getStaticMeshFromObject)
+            const std::string path = require_string(o, "Path");
+            std::vector<std::array<double, 3>> pos;
+            std::vector<std::array<int, 3>> tris;
+            if (!read_obj(path, pos, tris)) throw std::invalid_argument("Could not load obj file: " + path);
+            auto data = std::make_shared<PhysicScene::MeshObstacleData>();
+            for (const auto& t : tris) data->triangles.insert(data->triangles.end(), t.begin(), t.end());
+            for (const auto& x : pos) data->frames.insert(data->frames.end(), x.begin(), x.end());
+            ob.type = PDFC_OBSTACLE_MESH;
+            ob.tolerance = require_double(o, "Threshold");
+            ob.material = (int32_t)require_uint(o, "Material Identifier");
+            scene.addMeshObstacle(ob, data);
+            continue;
+        } else if (type == "Moving Mesh") {
+            // src/scene_building.cpp:248-277 (getMovingMeshFromObject): frame k is
+            // <Obj Prefix><k zero-padded to Suffix Size>.obj at time k * Time Between Frame;
+            // the faces are frame 0's (MovingMesh keeps the first frames' topology).
+            const std::string prefix = require_string(o, "Obj Prefix");
+            const std::size_t suffix = require_uint(o, "Suffix Size");
+            const std::size_t nf = require_uint(o, "Number Frame");
+            const double between = require_double(o, "Time Between Frame");
+            ob.type = PDFC_OBSTACLE_MESH;
+            ob.tolerance = require_double(o, "Collision tolerance");
+            ob.material = (int32_t)require_uint(o, "Material Identifier");
+            if (nf < 2) throw std::invalid_argument("JSON reader - Error : a moving mesh needs at least two frames");
+            auto data = std::make_shared<PhysicScene::MeshObstacleData>();
+            std::size_t nv = 0;
+            for (std::size_t k = 0; k < nf; ++k) {
+                std::ostringstream name;
+                name << prefix;
+                if (suffix > 0) name << std::setfill('0') << std::setw((int)suffix);
+                name << k << ".obj";
+                std::vector<std::array<double, 3>> pos;
+                std::vector<std::array<int, 3>> tris;
+                if (!read_obj(name.str(), pos, tris))
+                    throw std::invalid_argument("Could not load obj file: " + name.str());
+                if (k == 0) {
+                    nv = pos.size();
+                    for (const auto& t : tris) data->triangles.insert(data->triangles.end(), t.begin(), t.end());
+                } else if (pos.size() != nv) {
+                    throw std::invalid_argument("JSON reader - Error : moving mesh frame " + name.str() +
+                                                " has a different number of vertices than frame 0");
+                }
+                for (const auto& x : pos) data->frames.insert(data->frames.end(), x.begin(), x.end());
+                data->times.push_back((double)k * between);
+            }
+            scene.addMeshObstacle(ob, data);
+            continue;
+        } else if (type == "Axis Aligned Box" || type == "Rotating Sphere") {
             throw std::invalid_argument("JSON reader - Error : obstacle type \"" + type +
-                                        "\" is not supported by this build (static spheres and planes are)");
+                                        "\" is not supported by this build (spheres, planes and triangle meshes are)");
         } else {
             throw std::invalid_argument("JSON reader - Error : Invalid type for static object type");
         }

@@ -137,6 +137,17 @@ __device__ __forceinline__ void obstacle_range(const DevObstacles& ob, int i, in
     }
 }
 
+__device__ __forceinline__ int mesh_slot(const DevObstacles& ob, int o) {
+    int m = 0;
+    while (m + 1 < ob.nmesh && ob.mesh[m].obstacle != o) ++m;
+    return m;
+}
+__device__ __forceinline__ D3 ld_aos(const double* p, int v) { return {p[3 * v], p[3 * v + 1], p[3 * v + 2]}; }
+__device__ __forceinline__ D3 tri_point3(D3 a, D3 b, D3 c, const double* w) {
+    return {(a.x * w[0] + b.x * w[1]) + c.x * w[2], (a.y * w[0] + b.y * w[1]) + c.y * w[2],
+            (a.z * w[0] + b.z * w[1]) + c.z * w[2]};
+}
+
 // Obstacle detection at q_n: StaticSphere/Sphere::isInside (Sphere.cpp:93-113),
 // StaticPlane (StaticPlane.cpp:57-73). One bit per obstacle.
 __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObstacles ob, int* cflags, int* ccnt,
@@ -155,8 +166,10 @@ __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObs
             D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
             if (ob.type[o] == 0)
                 hit = sqrt(dot3(r, r)) < ob.radius[o] + 1e-2;
-            else
+            else if (ob.type[o] == 1)
                 hit = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]}) < 1.5e-2;
+            else
+                hit = ob.mesh[mesh_slot(ob, o)].hit_tri[i] >= 0;
             if (hit) flags |= 1 << (o - o0);
         }
         cnt = __popc(flags);
@@ -223,18 +236,33 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
     obstacle_range(ob, i, o0, o1);
     for (int o = o0; o < o1; ++o) {
         if (!(flags & (1 << (o - o0)))) continue;
-        D3 nrm, pt;
+        D3 nrm, pt, spd = {0., 0., 0.};
         const D3 c = {ob.p[3 * o], ob.p[3 * o + 1], ob.p[3 * o + 2]};
         if (ob.type[o] == 0) {
             D3 r = normalized3(D3{x.x - c.x, x.y - c.y, x.z - c.z});
             double R = ob.radius[o];
             pt = {c.x + R * r.x, c.y + R * r.y, c.z + R * r.z};
             nrm = r;
-        } else {
+        } else if (ob.type[o] == 1) {
             nrm = {ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]};
             D3 r = {x.x - c.x, x.y - c.y, x.z - c.z};
             double d = dot3(r, nrm);
             pt = {x.x - nrm.x * d, x.y - nrm.y * d, x.z - nrm.z * d};
+        } else {
+            // checkMeshCollision (geometry_util.cpp:32-92): the closest triangle's point,
+            // its normal (Mesh::getTriangleNormal) and MovingMesh::getPointSpeed
+            const DevMeshObs& Mo = ob.mesh[mesh_slot(ob, o)];
+            const int t = Mo.hit_tri[i];
+            const double* w = &Mo.hit_bary[3 * i];
+            const int* f = &Mo.tris[3 * t];
+            D3 a = ld_aos(Mo.cur, f[0]), b = ld_aos(Mo.cur, f[1]), cc = ld_aos(Mo.cur, f[2]);
+            pt = tri_point3(a, b, cc, w);
+            nrm = normalized3(cross3(D3{b.x - a.x, b.y - a.y, b.z - a.z}, D3{cc.x - a.x, cc.y - a.y, cc.z - a.z}));
+            if (Mo.moving) {
+                D3 pp = tri_point3(ld_aos(Mo.prev, f[0]), ld_aos(Mo.prev, f[1]), ld_aos(Mo.prev, f[2]), w);
+                D3 np = tri_point3(ld_aos(Mo.next, f[0]), ld_aos(Mo.next, f[1]), ld_aos(Mo.next, f[2]), w);
+                spd = {Mo.inv_dt * (np.x - pp.x), Mo.inv_dt * (np.y - pp.y), Mo.inv_dt * (np.z - pp.z)};
+            }
         }
         C.vertex[slot] = i;
         C.obstacle[slot] = o;
@@ -244,6 +272,7 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
         C.mu[slot] = ob.mu[o];
         contact_basis(nrm, &C.basis[9 * slot]);
         C.r[3 * slot] = 0.; C.r[3 * slot + 1] = 0.; C.r[3 * slot + 2] = 0.;
+        C.speed[3 * slot] = spd.x; C.speed[3 * slot + 1] = spd.y; C.speed[3 * slot + 2] = spd.z;
         ++slot;
     }
 }
@@ -372,12 +401,17 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
                 double* r = &C.r[3 * cid];
                 D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
                 double mu = C.mu[cid];
+                const D3 sp = {C.speed[3 * cid], C.speed[3 * cid + 1], C.speed[3 * cid + 2]};
                 if (a.record && iteration > 0) {
-                    double an = ac_norm(D3{dot3(bn, v), dot3(bt, v), dot3(bb, v)}, r, mu);
+                    const D3 u = {v.x - sp.x, v.y - sp.y, v.z - sp.z};
+                    double an = ac_norm(D3{dot3(bn, u), dot3(bt, u), dot3(bb, u)}, r, mu);
                     ac_s += an;
                     ac_m = fmax(ac_m, an);
                 }
-                D3 fl = {dot3(bn, Fs), dot3(bt, Fs), dot3(bb, Fs)};
+                // f = B^T (F - m u_obstacle)  (SimulationFrictionEstimation.cpp:360-372)
+                const double mi = M.mass[i];
+                const D3 Fo = {Fs.x - mi * sp.x, Fs.y - mi * sp.y, Fs.z - mi * sp.z};
+                D3 fl = {dot3(bn, Fo), dot3(bt, Fo), dot3(bb, Fo)};
                 double r0, r1, r2;
                 bool warm = false;
                 if (a.warm_mode != 0) {  // the host clears warm_mode once the hold ends
@@ -438,9 +472,11 @@ __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __re
         if (a.record) {
             int nc = ccnt[i], c0 = cstart[i];
             for (int k = 0; k < nc; ++k) {
-                const double* B = &C.basis[9 * (c0 + k)];
+                const int cid = c0 + k;
+                const double* B = &C.basis[9 * cid];
                 D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
-                double an = ac_norm(D3{dot3(bn, v), dot3(bt, v), dot3(bb, v)}, &C.r[3 * (c0 + k)], C.mu[c0 + k]);
+                const D3 u = {v.x - C.speed[3 * cid], v.y - C.speed[3 * cid + 1], v.z - C.speed[3 * cid + 2]};
+                double an = ac_norm(D3{dot3(bn, u), dot3(bt, u), dot3(bb, u)}, &C.r[3 * cid], C.mu[cid]);
                 ac_s += an;
                 ac_m = fmax(ac_m, an);
             }
@@ -497,10 +533,14 @@ __global__ void k_warm_features(DevModel M, const double* __restrict__ qn, const
     double gap;
     {
         D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
-        if (ob.type[o] == 0)
+        if (ob.type[o] == 0) {
             gap = sqrt(dot3(r, r)) - ob.radius[o];
-        else
+        } else if (ob.type[o] == 1) {
             gap = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]});
+        } else {  // signed distance to the contact point along the normal
+            const D3 cp = {C.point[3 * cid], C.point[3 * cid + 1], C.point[3 * cid + 2]};
+            gap = dot3(D3{x.x - cp.x, x.y - cp.y, x.z - cp.z}, bn);
+        }
     }
     D3 g = {0., 0., 0.};
     if (a.prev_g) {
@@ -509,10 +549,13 @@ __global__ void k_warm_features(DevModel M, const double* __restrict__ qn, const
         const float* pg = a.prev_g + 3 * ((size_t)(o - o0) * n + i);
         g = {(double)pg[0], (double)pg[1], (double)pg[2]};
     }
+    const D3 sp = {C.speed[3 * cid], C.speed[3 * cid + 1], C.speed[3 * cid + 2]};
+    const D3 u = {v.x - sp.x, v.y - sp.y, v.z - sp.z};
+    F = {F.x - m * sp.x, F.y - m * sp.y, F.z - m * sp.z};
     float* f = feat + (size_t)cid * kWarmFeatures;
-    f[0] = (float)dot3(bn, v);
-    f[1] = (float)dot3(bt, v);
-    f[2] = (float)dot3(bb, v);
+    f[0] = (float)dot3(bn, u);
+    f[1] = (float)dot3(bt, u);
+    f[2] = (float)dot3(bb, u);
     f[3] = (float)bn.x;
     f[4] = (float)bn.y;
     f[5] = (float)bn.z;

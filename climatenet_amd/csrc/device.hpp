@@ -17,6 +17,28 @@ namespace pdfc {
 constexpr int kMaxObstacles = 16;  // per scene (one contact bit each)
 constexpr int kSolveChunk = 64;  // rows per substitution task (one wave)
 
+// Triangle-mesh obstacle (StaticMesh / MovingMesh): current positions, the key frames
+// around the current time (point speeds), the triangle grid and the per-vertex result
+// of the closest-triangle query of the step.
+constexpr int kMaxMeshObstacles = 4;
+struct DevMeshObs {
+    int obstacle;           // index among the obstacles
+    int nv, T;
+    int moving;             // 1: the clock is between two key frames (point speeds)
+    const int* tris;        // 3T
+    const double* cur;      // 3 nv, xyz interleaved
+    const double* prev;     // key frames around the current time (3 nv each)
+    const double* next;
+    double inv_dt;          // 1 / (t_next - t_prev)
+    double tol, tol_sq, t11;
+    double h;               // grid cell size
+    int P;                  // hash buckets (power of two)
+    const int* bucket_off;  // P + 1
+    const int* bucket_tris;
+    int* hit_tri;           // per cloth vertex: closest triangle within the tolerance, or -1
+    double* hit_bary;       // 3 per cloth vertex
+};
+
 // Obstacles in device memory. With batched scenes a vertex meets only the
 // obstacles of its own scene.
 struct DevObstacles {
@@ -28,7 +50,12 @@ struct DevObstacles {
     const double* mu;      // friction coefficient against the scene's cloth
     const int* vinst;      // per vertex: its scene; null when there is one scene
     const int* obs_ptr;    // per scene k: obstacles [obs_ptr[k], obs_ptr[k+1])
+    int nmesh;
+    DevMeshObs mesh[kMaxMeshObstacles];
 };
+// Closest triangle of mesh obstacle m for every cloth vertex (positions qn, planar
+// supernodal layout): hit_tri / hit_bary.
+void launch_mesh_query(const DevObstacles& ob, int m, const double* qn, int n, hipStream_t st);
 
 // Obstacle contact record (structure of arrays, capacity n * n_obstacles).
 struct DevContacts {
@@ -40,6 +67,7 @@ struct DevContacts {
     double* mu;
     double* basis;   // 9 * cap: n, t, b
     double* r;       // 3 * cap, local frame
+    double* speed;   // 3 * cap: obstacle velocity at the contact point
 };
 
 struct DevModel {

@@ -579,7 +579,60 @@ __global__ void k_self_release(DevSelf S, int* vslot) {
 
 inline unsigned nblk(long long c) { return (unsigned)((c + kB - 1) / kB); }
 
+// Closest triangle of a mesh obstacle for every cloth vertex (checkMeshAllCollision +
+// checkMeshCollision, geometry_util.cpp:32-92,229-262): the triangles registered in the
+// vertex's grid cell pass the closed box test against x +- 1.1 tol and the exact test
+// |cp - x|^2 < tol^2 (early exit against tol); the closest of their points
+// getTriangle(t) * bary wins, the lower triangle index on ties.
+__global__ void k_mesh_query(DevMeshObs Mo, const double* __restrict__ qn, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const P3 x = at_perm(qn, n, i);
+    const long long ix = cell_of(x.x, Mo.h), iy = cell_of(x.y, Mo.h), iz = cell_of(x.z, Mo.h);
+    const unsigned bkt = cell_hash(ix, iy, iz, Mo.P);
+    double best = 1.0 / 0.0;
+    int bt = -1;
+    double bw[3] = {0., 0., 0.};
+    for (int k = Mo.bucket_off[bkt]; k < Mo.bucket_off[bkt + 1]; ++k) {
+        const int t = Mo.bucket_tris[k];
+        const int* f = &Mo.tris[3 * t];
+        const P3 a = {Mo.cur[3 * f[0]], Mo.cur[3 * f[0] + 1], Mo.cur[3 * f[0] + 2]};
+        const P3 b = {Mo.cur[3 * f[1]], Mo.cur[3 * f[1] + 1], Mo.cur[3 * f[1] + 2]};
+        const P3 c = {Mo.cur[3 * f[2]], Mo.cur[3 * f[2] + 1], Mo.cur[3 * f[2] + 2]};
+        const P3 lo = {fmin(fmin(a.x, b.x), c.x), fmin(fmin(a.y, b.y), c.y), fmin(fmin(a.z, b.z), c.z)};
+        const P3 hi = {fmax(fmax(a.x, b.x), c.x), fmax(fmax(a.y, b.y), c.y), fmax(fmax(a.z, b.z), c.z)};
+        if (!(lo.x <= x.x + Mo.t11 && x.x - Mo.t11 <= hi.x && lo.y <= x.y + Mo.t11 && x.y - Mo.t11 <= hi.y &&
+              lo.z <= x.z + Mo.t11 && x.z - Mo.t11 <= hi.z))
+            continue;
+        double w[3];
+        P3 cp;
+        closest_point(x, a, b, c, Mo.tol, w, cp);
+        const P3 dcp = psub(cp, x);
+        if (!(pdot(dcp, dcp) < Mo.tol_sq)) continue;
+        const P3 pt = {(a.x * w[0] + b.x * w[1]) + c.x * w[2], (a.y * w[0] + b.y * w[1]) + c.y * w[2],
+                       (a.z * w[0] + b.z * w[1]) + c.z * w[2]};
+        const P3 dp = psub(pt, x);
+        const double d = pdot(dp, dp);
+        if (d < best || (d == best && t < bt)) {
+            best = d;
+            bt = t;
+            bw[0] = w[0];
+            bw[1] = w[1];
+            bw[2] = w[2];
+        }
+    }
+    const bool hit = bt >= 0 && best < Mo.tol_sq;
+    Mo.hit_tri[i] = hit ? bt : -1;
+    Mo.hit_bary[3 * i] = bw[0];
+    Mo.hit_bary[3 * i + 1] = bw[1];
+    Mo.hit_bary[3 * i + 2] = bw[2];
+}
+
 }  // namespace
+
+void launch_mesh_query(const DevObstacles& ob, int m, const double* qn, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_mesh_query, dim3((n + kB - 1) / kB), dim3(kB), 0, st, ob.mesh[m], qn, n);
+}
 
 void launch_fill_int(int* p, int count, int value, hipStream_t st) {
     if (count > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(count)), dim3(kB), 0, st, p, count, value);

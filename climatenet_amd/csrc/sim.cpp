@@ -68,6 +68,29 @@ struct pdfc_sim {
     pdfc::SweepPlan fwd, bwd;
     pdfc_params prm{};
     std::vector<pdfc_obstacle> obstacles;
+    // triangle-mesh obstacles: host copy, MovingMesh clock (MovingMesh.cpp:43-110), device data
+    struct HostMesh {
+        int obstacle = 0, nv = 0, T = 0;
+        std::vector<int> tris;
+        std::vector<double> frames;     // F x 3 nv
+        std::vector<double> times;
+        double tol = 0.;                // sqrt(tol * tol)
+        double time = 0.;
+        int next = 1;
+        std::vector<double> cur;        // 3 nv
+        bool dirty = true;
+        double* d_frames = nullptr;
+        double* d_cur = nullptr;
+        int* d_off = nullptr;
+        int* d_tris_sorted = nullptr;
+        size_t cap_sorted = 0;
+        int frames_count() const { return (int)(frames.size() / (3 * (size_t)nv)); }
+        bool in_movement() const { return frames_count() >= 2 && time >= times[next - 1] && time <= times[next]; }
+    };
+    std::vector<HostMesh> meshes;
+    bool moving_mesh = false;
+    void mesh_refresh();
+    void mesh_advance(double dt);
     std::vector<double> obs_mu;         // per obstacle, against its scene's cloth
     std::vector<int> obs_ptr;           // per scene: its obstacles
     std::vector<int> scene_v0;          // per scene: first vertex
@@ -146,7 +169,8 @@ struct pdfc_sim {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_capturable() const {
-        return prm.use_graph && !self_on && !(prm.residual_tolerance > 0.) && !warm_data && !ktiming && !given_pending;
+        return prm.use_graph && !self_on && !(prm.residual_tolerance > 0.) && !warm_data && !ktiming && !given_pending &&
+               !moving_mesh;
     }
     void graph_step();
 
@@ -279,6 +303,34 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
                 if (!(z > 0.)) throw std::invalid_argument("plane normal is zero");
                 double s = std::sqrt(z);
                 for (int c = 0; c < 3; ++c) ob.normal[c] /= s;
+            } else if (ob.type == PDFC_OBSTACLE_MESH) {
+                if (count > 1) throw std::invalid_argument("mesh obstacles are not available for batched scenes");
+                if (meshes.size() >= (size_t)pdfc::kMaxMeshObstacles)
+                    throw std::invalid_argument("too many mesh obstacles (max 4)");
+                if (ob.n_mesh_vertices == 0 || ob.n_mesh_triangles == 0 || !ob.mesh_triangles || !ob.mesh_frames ||
+                    ob.n_frames < 1 || (ob.n_frames >= 2 && !ob.frame_times))
+                    throw std::invalid_argument("mesh obstacle without vertices, triangles or frames");
+                if (!(ob.tolerance >= 0.)) throw std::invalid_argument("mesh obstacle tolerance must be non-negative");
+                HostMesh hm;
+                hm.obstacle = (int)obstacles.size();
+                hm.nv = (int)ob.n_mesh_vertices;
+                hm.T = (int)ob.n_mesh_triangles;
+                hm.tris.assign(ob.mesh_triangles, ob.mesh_triangles + 3 * ob.n_mesh_triangles);
+                for (int x : hm.tris)
+                    if (x < 0 || x >= hm.nv) throw std::invalid_argument("mesh obstacle triangle index out of range");
+                hm.frames.assign(ob.mesh_frames, ob.mesh_frames + ob.n_frames * 3 * ob.n_mesh_vertices);
+                if (ob.n_frames >= 2) {
+                    hm.times.assign(ob.frame_times, ob.frame_times + ob.n_frames);
+                    for (size_t f = 1; f < ob.n_frames; ++f)
+                        if (!(hm.times[f] > hm.times[f - 1])) throw std::invalid_argument("frame times must increase");
+                    moving_mesh = true;
+                }
+                hm.tol = std::sqrt(ob.tolerance * ob.tolerance);
+                hm.cur.assign(hm.frames.begin(), hm.frames.begin() + 3 * (size_t)hm.nv);
+                meshes.push_back(std::move(hm));
+                ob.mesh_triangles = nullptr;  // copied above; the caller's arrays are not kept
+                ob.mesh_frames = nullptr;
+                ob.frame_times = nullptr;
             } else if (ob.type != PDFC_OBSTACLE_SPHERE) {
                 throw std::invalid_argument("unknown obstacle type");
             }
@@ -430,6 +482,28 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         dob.mu = upload(obs_mu);
         dob.vinst = nullptr;
         dob.obs_ptr = upload(obs_ptr);
+        dob.nmesh = (int)meshes.size();
+        for (size_t m = 0; m < meshes.size(); ++m) {
+            HostMesh& hm = meshes[m];
+            pdfc::DevMeshObs& D = dob.mesh[m];
+            D.obstacle = hm.obstacle;
+            D.nv = hm.nv;
+            D.T = hm.T;
+            D.tris = upload(hm.tris);
+            hm.d_frames = upload(hm.frames);
+            hm.d_cur = dalloc<double>(3 * (size_t)hm.nv);
+            D.cur = hm.d_cur;
+            D.tol = hm.tol;
+            D.tol_sq = hm.tol * hm.tol;
+            D.t11 = 1.1 * hm.tol;
+            D.hit_tri = dalloc<int>(n);
+            D.hit_bary = dalloc<double>(3 * (size_t)n);
+            int P = 1;
+            while (P < 16 * hm.T) P <<= 1;
+            D.P = P;
+            hm.d_off = dalloc<int>(P + 1);
+            D.bucket_off = hm.d_off;
+        }
         if (n_scenes > 1) {
             std::vector<int> vinst(n);
             for (int k = 0; k < n_scenes; ++k)
@@ -457,6 +531,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     dc.mu = dalloc<double>(cap);
     dc.basis = dalloc<double>(9 * cap);
     dc.r = dalloc<double>(3 * cap);
+    dc.speed = dalloc<double>(3 * cap);
     cflags = dalloc<int>(n);
     ccnt = dalloc<int>(n);
     cstart = dalloc<int>(n);
@@ -547,6 +622,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     nnzF_lower = 0;
     for (size_t s2 = 0; s2 < fac.ns.size(); ++s2)
         if (!use_top || fac.level[s2] < top.level) nnzF_lower += fac.foff[s2 + 1] - fac.foff[s2];
+    mesh_refresh();  // static meshes are ready before any step (and before a graph capture)
 }
 
 // Task lists of the levels below top_level (all levels when there is no top block).
@@ -980,7 +1056,101 @@ void pdfc_sim::solve_inplace(double* X) {
 }
 
 void pdfc_sim::detect() {
+    if (!meshes.empty()) {
+        mesh_refresh();
+        for (int m = 0; m < (int)meshes.size(); ++m) pdfc::launch_mesh_query(dob, m, qn, n, stream);
+    }
     pdfc::launch_detect(dm, qn, dob, cflags, ccnt, cstart, blocksum, ncontacts, dc, stream);
+}
+
+// Uploads the current positions of the mesh obstacles that moved and rebuilds their
+// triangle grids: a triangle is registered in every cell its box, grown by 1.1 tol,
+// overlaps; with cells at least as wide as any grown box, a vertex only needs its own
+// cell. Also sets the key frames around the clock for the point speeds.
+void pdfc_sim::mesh_refresh() {
+    for (size_t m = 0; m < meshes.size(); ++m) {
+        HostMesh& hm = meshes[m];
+        pdfc::DevMeshObs& D = dob.mesh[m];
+        const int F = hm.frames_count();
+        D.moving = hm.in_movement() ? 1 : 0;
+        if (F >= 2) {
+            D.prev = hm.d_frames + (size_t)(hm.next - 1) * 3 * hm.nv;
+            D.next = hm.d_frames + (size_t)hm.next * 3 * hm.nv;
+            D.inv_dt = 1. / (hm.times[hm.next] - hm.times[hm.next - 1]);
+        } else {
+            D.prev = D.next = hm.d_frames;
+            D.inv_dt = 0.;
+        }
+        if (!hm.dirty) continue;
+        hm.dirty = false;
+        const double grow = D.t11 * (1. + 1e-9);
+        std::vector<double> lo(3 * (size_t)hm.T), hi(3 * (size_t)hm.T);
+        double ext = 0.;
+        for (int t = 0; t < hm.T; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const double a = hm.cur[3 * hm.tris[3 * t] + k], b = hm.cur[3 * hm.tris[3 * t + 1] + k],
+                             c = hm.cur[3 * hm.tris[3 * t + 2] + k];
+                lo[3 * t + k] = std::min(std::min(a, b), c) - grow;
+                hi[3 * t + k] = std::max(std::max(a, b), c) + grow;
+                ext = std::max(ext, hi[3 * t + k] - lo[3 * t + k]);
+            }
+        const double h = std::max(ext * (1. + 1e-9), 1e-12);
+        D.h = h;
+        auto cell = [&](double x) { return (long long)std::floor(x / h); };
+        auto hash = [&](long long ix, long long iy, long long iz) {
+            unsigned v = ((unsigned)ix * 73856093u) ^ ((unsigned)iy * 19349663u) ^ ((unsigned)iz * 83492791u);
+            return (int)(v & (unsigned)(D.P - 1));
+        };
+        std::vector<std::vector<int>> buckets(D.P);
+        for (int t = 0; t < hm.T; ++t) {
+            std::vector<int> seen;
+            for (long long a = cell(lo[3 * t]); a <= cell(hi[3 * t]); ++a)
+                for (long long b = cell(lo[3 * t + 1]); b <= cell(hi[3 * t + 1]); ++b)
+                    for (long long c = cell(lo[3 * t + 2]); c <= cell(hi[3 * t + 2]); ++c) {
+                        const int bk = hash(a, b, c);
+                        if (std::find(seen.begin(), seen.end(), bk) == seen.end()) {
+                            seen.push_back(bk);
+                            buckets[bk].push_back(t);
+                        }
+                    }
+        }
+        std::vector<int> off(D.P + 1, 0), sorted;
+        for (int bk = 0; bk < D.P; ++bk) {
+            off[bk] = (int)sorted.size();
+            sorted.insert(sorted.end(), buckets[bk].begin(), buckets[bk].end());
+        }
+        off[D.P] = (int)sorted.size();
+        if (sorted.size() > hm.cap_sorted) {
+            if (hm.d_tris_sorted) free_ptr(hm.d_tris_sorted);
+            hm.cap_sorted = sorted.size() + sorted.size() / 2 + 16;
+            hm.d_tris_sorted = dalloc<int>(hm.cap_sorted);
+        }
+        D.bucket_tris = hm.d_tris_sorted;
+        HIP_CHECK(hipMemcpyAsync(hm.d_cur, hm.cur.data(), hm.cur.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(hm.d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        if (!sorted.empty())
+            HIP_CHECK(hipMemcpyAsync(hm.d_tris_sorted, sorted.data(), sorted.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, stream));
+        // the host vectors die here; the copies above must be done before they do
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+}
+
+// MovingMesh::update (MovingMesh.cpp:43-90) at the end of a step.
+void pdfc_sim::mesh_advance(double dt) {
+    for (HostMesh& hm : meshes) {
+        const int F = hm.frames_count();
+        if (F < 2) continue;
+        hm.time += dt;
+        if (hm.times[hm.next] < hm.time && hm.next + 1 < F) ++hm.next;
+        if (!hm.in_movement()) continue;
+        const double t0 = hm.times[hm.next - 1], t1 = hm.times[hm.next];
+        const double a = (hm.time - t0) / (t1 - t0);
+        const double* P = &hm.frames[(size_t)(hm.next - 1) * 3 * hm.nv];
+        const double* N = &hm.frames[(size_t)hm.next * 3 * hm.nv];
+        for (size_t k = 0; k < hm.cur.size(); ++k) hm.cur[k] = P[k] * (1. - a) + N[k] * a;
+        hm.dirty = true;
+    }
 }
 
 // Device detection, host pairing and level ordering, upload of the step's self
@@ -1351,6 +1521,7 @@ void pdfc_sim::run_step() {
         if (host_ac[K - 1] / total < prm.residual_tolerance) to_tol = base + K;
     }
     HIP_CHECK(hipGetLastError());
+    mesh_advance(prm.time_step);
     stats.iterations += base + K;
     if (total > 0) {
         stats.contact_steps += 1;
@@ -1527,6 +1698,8 @@ int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n) {
         get(pt.data(), sim->dc.point, 3 * nc * sizeof(double));
         get(mu.data(), sim->dc.mu, nc * sizeof(double));
         get(r.data(), sim->dc.r, 3 * nc * sizeof(double));
+        std::vector<double> spd(3 * (size_t)nc);
+        get(spd.data(), sim->dc.speed, 3 * (size_t)nc * sizeof(double));
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         std::vector<int> order(nc);
         for (int i = 0; i < nc; ++i) order[i] = i;
@@ -1544,6 +1717,7 @@ int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n) {
                 c.normal[d] = nrm[3 * i + d];
                 c.point[d] = pt[3 * i + d];
                 c.r_local[d] = r[3 * i + d];
+                c.speed[d] = spd[3 * i + d];
             }
             c.mu = mu[i];
         }

@@ -90,11 +90,16 @@ def jitter_xy(pos: np.ndarray, amplitude: float, seed: int = 0) -> np.ndarray:
 
 @dataclass
 class ObstacleSpec:
-    kind: str  # "sphere" | "plane"
-    point: tuple
+    kind: str  # "sphere" | "plane" | "mesh"
+    point: tuple = (0.0, 0.0, 0.0)
     normal: tuple = (0.0, 0.0, 1.0)
     radius: float = 0.0
     material: int = 0
+    # triangle mesh: frames [F, nv, 3] (F = 1: static), triangles [T, 3], frame times [F]
+    tolerance: float = 0.0
+    frames: Optional[np.ndarray] = None
+    triangles: Optional[np.ndarray] = None
+    times: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -135,14 +140,30 @@ class SceneSpec:
         tris = np.ascontiguousarray(self.triangles, dtype=np.int32)
         fr = np.ascontiguousarray(self.friction, dtype=np.float64)
         obs = (abi.Obstacle * max(1, len(self.obstacles)))()
+        keep = []
+        kinds = {"sphere": abi.OBSTACLE_SPHERE, "plane": abi.OBSTACLE_PLANE, "mesh": abi.OBSTACLE_MESH}
         for k, o in enumerate(self.obstacles):
-            obs[k].type = abi.OBSTACLE_SPHERE if o.kind == "sphere" else abi.OBSTACLE_PLANE
+            obs[k].type = kinds[o.kind]
             obs[k].material = o.material
             for c in range(3):
                 obs[k].point[c] = o.point[c]
                 obs[k].normal[c] = o.normal[c]
             obs[k].radius = o.radius
-        self._keep = [rest, pos, tris, fr, obs]
+            if o.kind == "mesh":
+                frames = np.ascontiguousarray(o.frames, dtype=np.float64).reshape(-1, np.asarray(o.frames).shape[-2], 3)
+                mt = np.ascontiguousarray(o.triangles, dtype=np.int32)
+                obs[k].tolerance = o.tolerance
+                obs[k].n_mesh_vertices = frames.shape[1]
+                obs[k].n_mesh_triangles = mt.size // 3
+                obs[k].mesh_triangles = abi.iptr(mt)
+                obs[k].n_frames = frames.shape[0]
+                obs[k].mesh_frames = abi.dptr(frames)
+                tm = None
+                if frames.shape[0] >= 2:
+                    tm = np.ascontiguousarray(o.times, dtype=np.float64)
+                    obs[k].frame_times = abi.dptr(tm)
+                keep += [frames, mt, tm]
+        self._keep = [rest, pos, tris, fr, obs] + keep
         s = abi.Scene()
         s.n_vertices = self.n_vertices
         s.rest_positions = abi.dptr(rest)
@@ -260,3 +281,55 @@ def batched_instances(count: int = 64, seed: int = 0, sizes=(32, 128)):
                                [ObstacleSpec("sphere", (cx, cy, 0.0), radius=R)],
                                np.array([[mu]]), time_step=0.01, iterations=15, steps=300))
     return specs
+
+
+def icosphere(radius: float, center=(0.0, 0.0, 0.0), subdivisions: int = 3):
+    """Subdivided icosahedron projected on the sphere; triangles wound counter-clockwise seen
+    from outside (outward normals). Returns (vertices [nv, 3], triangles [T, 3])."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        G = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            G += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = G
+    P = np.array(verts) * radius + np.asarray(center, dtype=np.float64)
+    return P, np.array(F, dtype=np.int32)
+
+
+def mesh_sphere_drape(moving: bool = False, jitter_seed: int = 0) -> SceneSpec:
+    """The 20x20 drape (1 m, mu=0.3, dt=1/200, 20 iterations) dropped from 2 cm above a
+    triangle-mesh sphere of radius 0.25 (icosphere, 1,280 faces), collision tolerance 1.5e-2
+    (a mesh obstacle only sees vertices within the tolerance of its surface, so a vertex must
+    not cross it in one step). Moving: 21 key frames 0.035 s apart of the sphere turning about
+    z by 0.1 rad per frame while rising 1 mm per frame (a MovingMesh with point velocities)."""
+    rest, tris = square_grid(20, 1.0, 0.27)
+    pos = jitter_xy(rest, 1e-4, jitter_seed)
+    P, F = icosphere(0.25, (0.0, 0.0, 0.0), 3)
+    if moving:
+        frames, times = [], []
+        for k in range(21):
+            a = 0.1 * k
+            R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+            frames.append(P @ R.T + np.array([0.0, 0.0, 1e-3 * k]))
+            times.append(0.035 * k)
+        ob = ObstacleSpec("mesh", tolerance=1.5e-2, frames=np.array(frames), triangles=F, times=np.array(times))
+    else:
+        ob = ObstacleSpec("mesh", tolerance=1.5e-2, frames=P[None], triangles=F)
+    return SceneSpec("mesh_drape_20x20" + ("_moving" if moving else ""), rest, tris, pos, [ob],
+                     np.array([[0.3]]), time_step=0.005, iterations=20, steps=200)

@@ -50,7 +50,10 @@ typedef struct pdfc_sim pdfc_sim;
 
 enum pdfc_obstacle_type {
     PDFC_OBSTACLE_SPHERE = 0, /* StaticSphere: src/physics/StaticSphere.cpp:35-47 */
-    PDFC_OBSTACLE_PLANE = 1   /* StaticPlane:  src/physics/StaticPlane.cpp:57-73 */
+    PDFC_OBSTACLE_PLANE = 1,  /* StaticPlane:  src/physics/StaticPlane.cpp:57-73 */
+    PDFC_OBSTACLE_MESH = 2    /* StaticMesh (one frame, src/physics/StaticMesh.cpp:24-56) or MovingMesh
+                                 (key frames, src/physics/MovingMesh.cpp:28-253); detection
+                                 checkMeshCollision, src/geometry/geometry_util.cpp:32-92 */
 };
 
 enum pdfc_precision {
@@ -78,6 +81,14 @@ typedef struct pdfc_obstacle {
     double point[3];         /* sphere centre, or a point of the plane ("Position" / "Point") */
     double normal[3];        /* plane normal ("Normal"), normalised by pdfc_create; unused for spheres */
     double radius;           /* sphere radius ("Radius") */
+    /* triangle-mesh obstacles only (the data are copied by pdfc_create): */
+    double tolerance;        /* "Threshold" (static) / "Collision tolerance" (moving) */
+    size_t n_mesh_vertices;
+    size_t n_mesh_triangles;
+    const int32_t* mesh_triangles;   /* 3 per triangle */
+    size_t n_frames;                 /* 1: static mesh; >= 2: key frames of a moving mesh */
+    const double* mesh_frames;       /* n_frames x 3 n_mesh_vertices positions */
+    const double* frame_times;       /* n_frames, increasing ("Time Between Frame" multiples); NULL if static */
 } pdfc_obstacle;
 
 typedef struct pdfc_scene {
@@ -128,6 +139,7 @@ typedef struct pdfc_contact {
     double point[3];                 /* CollisionInfo::contact_point */
     double mu;                       /* CollisionInfo::friction_coefficient */
     double r_local[3];               /* reaction in the contact basis [n, t, b] (m_collisions_local_forces) */
+    double speed[3];                 /* CollisionInfo::speed: obstacle velocity at the contact point */
 } pdfc_contact;
 
 typedef struct pdfc_self_contact {

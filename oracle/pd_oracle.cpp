@@ -201,6 +201,21 @@ struct SelfContact {
 };
 struct Candidate { int v; int t; V3 bary; };
 
+// Triangle-mesh obstacle: StaticMesh (one frame) or MovingMesh (key frames,
+// MovingMesh.cpp:28-253) with its clock and current (interpolated) positions.
+struct MeshObstacle {
+    std::vector<std::array<int, 3>> tris;
+    std::vector<std::vector<V3>> frames;
+    std::vector<double> times;
+    double tol = 0.;          // sqrt(tol * tol), as StaticMesh/MovingMesh pass it on
+    double time = 0.;         // m_current_time
+    int next = 1;             // m_next_main_frame_filename_index
+    std::vector<V3> cur;      // m_current_mesh
+    bool moving() const { return frames.size() >= 2; }
+    // MovingMesh::isInMovement (:249-253)
+    bool in_movement() const { return moving() && time >= times[next - 1] && time <= times[next]; }
+};
+
 }  // namespace
 
 struct orc_sim {
@@ -210,6 +225,7 @@ struct orc_sim {
     std::vector<V3> rest, q, v;
     std::vector<double> mass, invmass;
     std::vector<pdfc_obstacle> obstacles;
+    std::vector<MeshObstacle> mesh_obs;      // per obstacle (empty unless a triangle mesh)
     int n_mat = 0; std::vector<double> friction; int mesh_material = 0;
     double mesh_area_density = 0., mesh_bend = 0., mesh_stretch = 0.;
     pdfc_params prm{};
@@ -551,6 +567,8 @@ void solve1(const orc_sim& S, const double* b, double* out, std::vector<double>&
 }
 
 // --------------------------------------------------------------- detection
+void detect_mesh_obstacle(orc_sim& S, int o, double mu);
+
 // PhysicScene::getCollisionsInfoExceptFor (PhysicScene.cpp:115-181): obstacles in
 // order, vertices ascending. Sphere: Sphere.cpp:93-113. Plane: StaticPlane.cpp:57-73.
 void detect_obstacles(orc_sim& S) {
@@ -558,6 +576,10 @@ void detect_obstacles(orc_sim& S) {
     for (int o = 0; o < (int)S.obstacles.size(); ++o) {
         const pdfc_obstacle& ob = S.obstacles[o];
         double mu = S.friction[(size_t)S.mesh_material * S.n_mat + ob.material];
+        if (ob.type == PDFC_OBSTACLE_MESH) {
+            detect_mesh_obstacle(S, o, mu);
+            continue;
+        }
         for (int i = 0; i < S.n; ++i) {
             const V3& x = S.q[i];
             ObsContact c{};
@@ -644,6 +666,76 @@ void broad_phase(const orc_sim& S, std::vector<std::pair<int, int>>& pairs, bool
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         for (int t : cand) pairs.push_back({v, t});
     }
+}
+
+// checkMeshCollision (geometry_util.cpp:32-92) on a mesh obstacle: the candidates of
+// checkMeshAllCollision (vertex box x +- 1.1 tol against the triangle boxes, exact test
+// with the early exit against tol, kept if |cp - x|^2 < tol^2), then per vertex the
+// candidate whose point getTriangle(t) * bary is closest (first minimum in ascending
+// triangle order), a contact if that distance^2 < tol^2. Normal: Mesh::getTriangleNormal
+// (Mesh.cpp:121-126). Speed: MovingMesh::getPointSpeed (MovingMesh.cpp:225-243).
+void detect_mesh_obstacle(orc_sim& S, int o, double mu) {
+    const MeshObstacle& M = S.mesh_obs[o];
+    const double tol = M.tol, t11 = 1.1 * tol, tol_sq = tol * tol;
+    const int T = (int)M.tris.size();
+    std::vector<V3> tlo(T), thi(T);
+    for (int t = 0; t < T; ++t) {
+        const auto& f = M.tris[t];
+        for (int k = 0; k < 3; ++k) {
+            tlo[t][k] = std::min(std::min(M.cur[f[0]][k], M.cur[f[1]][k]), M.cur[f[2]][k]);
+            thi[t][k] = std::max(std::max(M.cur[f[0]][k], M.cur[f[1]][k]), M.cur[f[2]][k]);
+        }
+    }
+    for (int i = 0; i < S.n; ++i) {
+        const V3& x = S.q[i];
+        double best = std::numeric_limits<double>::infinity();
+        int bt = -1;
+        V3 bbary{0, 0, 0}, bpt{0, 0, 0};
+        for (int t = 0; t < T; ++t) {
+            bool ov = true;
+            for (int k = 0; k < 3 && ov; ++k) ov = tlo[t][k] <= x[k] + t11 && x[k] - t11 <= thi[t][k];
+            if (!ov) continue;
+            const auto& f = M.tris[t];
+            V3 cp, bary;
+            closest_point(x, M.cur[f[0]], M.cur[f[1]], M.cur[f[2]], tol, cp, bary);
+            if (!(sqn(sub(cp, x)) < tol_sq)) continue;
+            V3 pt = tri_point(M.cur[f[0]], M.cur[f[1]], M.cur[f[2]], bary);
+            double d = sqn(sub(pt, x));
+            if (d < best) { best = d; bt = t; bbary = bary; bpt = pt; }
+        }
+        if (!(bt >= 0 && best < tol_sq)) continue;
+        const auto& f = M.tris[bt];
+        ObsContact c{};
+        c.v = i; c.obstacle = o; c.mu = mu;
+        c.point = bpt;
+        c.normal = normalized(cross(sub(M.cur[f[1]], M.cur[f[0]]), sub(M.cur[f[2]], M.cur[f[0]])));
+        c.speed = {0, 0, 0};
+        if (M.in_movement()) {
+            const auto& P = M.frames[M.next - 1];
+            const auto& N = M.frames[M.next];
+            V3 pp = tri_point(P[f[0]], P[f[1]], P[f[2]], bbary);
+            V3 np = tri_point(N[f[0]], N[f[1]], N[f[2]], bbary);
+            c.speed = scale(1. / (M.times[M.next] - M.times[M.next - 1]), sub(np, pp));
+        }
+        c.B = contact_basis(c.normal);
+        c.r = {0, 0, 0}; c.active = 0;
+        S.contacts.push_back(c);
+    }
+}
+
+// MovingMesh::update (MovingMesh.cpp:43-90): advance the clock, move to the next pair of
+// key frames (one per update), interpolate the current mesh when in movement.
+void update_mesh_obstacle(MeshObstacle& M, double dt) {
+    if (!M.moving()) return;
+    M.time += dt;
+    if (M.times[M.next] < M.time && M.next + 1 < (int)M.frames.size()) ++M.next;
+    if (!M.in_movement()) return;
+    const double t0 = M.times[M.next - 1], t1 = M.times[M.next];
+    const double a = (M.time - t0) / (t1 - t0);
+    const auto& P = M.frames[M.next - 1];
+    const auto& N = M.frames[M.next];
+    for (size_t v = 0; v < M.cur.size(); ++v)
+        for (int k = 0; k < 3; ++k) M.cur[v][k] = P[v][k] * (1. - a) + N[v][k] * a;
 }
 
 // checkMeshAllCollision (geometry_util.cpp:229-262) + checkMeshSelfCollisions
@@ -950,13 +1042,16 @@ void step(orc_sim& S) {
     S.q = qk;
     S.v = vk;
     S.time += dt;
+    for (auto& M : S.mesh_obs) update_mesh_obstacle(M, dt);
     S.warm_r.clear();
     S.warm_hold = 0;
 }
 
 void fill_contact(const orc_sim& S, const ObsContact& c, pdfc_contact& o) {
     o.vertex = c.v; o.obstacle = c.obstacle; o.active = c.active;
-    for (int k = 0; k < 3; ++k) { o.normal[k] = c.normal[k]; o.point[k] = c.point[k]; o.r_local[k] = c.r[k]; }
+    for (int k = 0; k < 3; ++k) {
+        o.normal[k] = c.normal[k]; o.point[k] = c.point[k]; o.r_local[k] = c.r[k]; o.speed[k] = c.speed[k];
+    }
     o.mu = c.mu;
     (void)S;
 }
@@ -998,11 +1093,31 @@ int orc_create(const pdfc_scene* sc, const pdfc_params* prm, orc_sim** out) {
         S->mesh_area_density = sc->area_density;
         S->mesh_bend = sc->bending;
         S->mesh_stretch = sc->stretching;
+        S->mesh_obs.resize(sc->n_obstacles);
         for (size_t o = 0; o < sc->n_obstacles; ++o) {
             pdfc_obstacle ob = sc->obstacles[o];
             if (ob.type == PDFC_OBSTACLE_PLANE) {
                 V3 nn = normalized(V3{ob.normal[0], ob.normal[1], ob.normal[2]});
                 for (int k = 0; k < 3; ++k) ob.normal[k] = nn[k];
+            }
+            if (ob.type == PDFC_OBSTACLE_MESH) {
+                MeshObstacle& M = S->mesh_obs[o];
+                if (ob.n_frames < 1 || !ob.mesh_frames || !ob.mesh_triangles || ob.n_mesh_triangles == 0)
+                    throw std::invalid_argument("mesh obstacle without data");
+                const size_t nv = ob.n_mesh_vertices;
+                M.tris.resize(ob.n_mesh_triangles);
+                for (size_t t = 0; t < ob.n_mesh_triangles; ++t)
+                    for (int k = 0; k < 3; ++k) M.tris[t][k] = ob.mesh_triangles[3 * t + k];
+                M.frames.assign(ob.n_frames, std::vector<V3>(nv));
+                for (size_t f = 0; f < ob.n_frames; ++f)
+                    for (size_t v = 0; v < nv; ++v)
+                        for (int k = 0; k < 3; ++k) M.frames[f][v][k] = ob.mesh_frames[(f * nv + v) * 3 + k];
+                if (ob.n_frames >= 2) M.times.assign(ob.frame_times, ob.frame_times + ob.n_frames);
+                M.tol = std::sqrt(ob.tolerance * ob.tolerance);
+                M.cur = M.frames[0];
+                ob.mesh_triangles = nullptr;  // the copies above are the oracle's
+                ob.mesh_frames = nullptr;
+                ob.frame_times = nullptr;
             }
             S->obstacles.push_back(ob);
         }

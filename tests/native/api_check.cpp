@@ -1,7 +1,9 @@
 // Test program for the C++ host interface: reads a scene JSON and prints what the
 // builders produced; with a second argument "sim" it also builds the simulation and
-// runs the given number of steps, printing the final positions of the log vertices.
+// runs the given number of steps, printing the final position of vertex 0 (and, with a
+// fourth argument, writing every final coordinate to that file).
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <string>
 
@@ -24,6 +26,18 @@ int main(int argc, char** argv) {
         std::cout << "iterations " << p.local_global_iterations << "\n";
         std::cout << "self " << p.self_collision << " " << p.self_collision_tolerance << "\n";
         std::cout << "damping " << p.air_damping << "\n";
+        for (std::size_t i = 0; i < scene.getObstacles().size(); ++i) {
+            const pdfc_obstacle& o = scene.getObstacles()[i];
+            std::cout << "obstacle" << i << " " << o.type << " " << o.material << " " << o.tolerance << " "
+                      << o.n_mesh_vertices << " " << o.n_mesh_triangles << " " << o.n_frames;
+            if (o.type == PDFC_OBSTACLE_MESH) {
+                const double* last = o.mesh_frames + (o.n_frames - 1) * 3 * o.n_mesh_vertices;
+                std::cout << " " << o.mesh_triangles[3 * o.n_mesh_triangles - 1] << " "
+                          << (o.frame_times ? o.frame_times[o.n_frames - 1] : -1.0) << " " << last[0] << " "
+                          << last[1] << " " << last[2];
+            }
+            std::cout << "\n";
+        }
         auto q = m->getVertexPosition(1);
         std::cout << "v1 " << q[0] << " " << q[1] << " " << q[2] << "\n";
         if (argc >= 4 && std::string(argv[2]) == "sim") {
@@ -35,6 +49,11 @@ int main(int argc, char** argv) {
             }
             auto x = scene.getMesh(0)->getVertexPosition(0);
             std::cout << "final0 " << x[0] << " " << x[1] << " " << x[2] << "\n";
+            if (argc >= 5) {  // all final positions, one coordinate per line
+                std::ofstream of(argv[4]);
+                of.precision(17);
+                for (double c : scene.getGeneralizedPositions()) of << c << "\n";
+            }
         }
     } catch (const std::invalid_argument& e) {
         std::cout << "invalid_argument: " << e.what() << "\n";

@@ -15,6 +15,12 @@ def write_obj(path, positions, triangles):
             f.write("f %d %d %d\n" % tuple(t + 1))
 
 
+def read_obj_positions(path):
+    """Vertex positions of an OBJ file as the scene reader stores them (float32)."""
+    rows = [l.split()[1:4] for l in open(path) if l.startswith("v ")]
+    return np.array(rows, dtype=np.float32).astype(np.float64)
+
+
 def write_scene(dirpath, spec, frames=2, steps_per_frame=5, log_vertices=(0, 1)):
     """A JSON scene equivalent to a scenes.SceneSpec, with its OBJ files."""
     rest_obj = os.path.join(dirpath, "rest.obj")
@@ -30,9 +36,23 @@ def write_scene(dirpath, spec, frames=2, steps_per_frame=5, log_vertices=(0, 1))
         if o.kind == "sphere":
             obstacles.append({"Type": "Sphere", "Position": [float(x) for x in o.point], "Radius": float(o.radius),
                               "Material Identifier": int(o.material)})
-        else:
+        elif o.kind == "plane":
             obstacles.append({"Type": "Plane", "Point": [float(x) for x in o.point],
                               "Normal": [float(x) for x in o.normal], "Material Identifier": int(o.material)})
+        elif o.times is None:
+            path = os.path.join(dirpath, "obstacle%d.obj" % len(obstacles))
+            write_obj(path, o.frames[0], o.triangles)
+            obstacles.append({"Type": "Mesh", "Path": path, "Threshold": float(o.tolerance),
+                              "Material Identifier": int(o.material)})
+        else:
+            prefix = os.path.join(dirpath, "obstacle%d_" % len(obstacles))
+            between = float(o.times[1] - o.times[0])
+            assert np.allclose(o.times, between * np.arange(len(o.times)), rtol=0, atol=1e-15)
+            for k, fr in enumerate(o.frames):
+                write_obj(prefix + "%03d.obj" % k, fr, o.triangles)
+            obstacles.append({"Type": "Moving Mesh", "Obj Prefix": prefix, "Suffix Size": 3,
+                              "Number Frame": len(o.frames), "Time Between Frame": between,
+                              "Collision tolerance": float(o.tolerance), "Material Identifier": int(o.material)})
     doc = {
         "Parameters": {"Frame number": frames, "LocalGlobal": int(spec.iterations), "Timestep": float(spec.time_step),
                        "Type": "friction", "Steps per frame": steps_per_frame,
