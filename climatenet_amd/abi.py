@@ -16,6 +16,8 @@ PDFC_ERR_CAPACITY = -4
 OBSTACLE_SPHERE = 0
 OBSTACLE_PLANE = 1
 OBSTACLE_MESH = 2
+OBSTACLE_BOX = 3
+OBSTACLE_ROTATING_SPHERE = 4
 
 FP64 = 0
 FP32_REFINE64 = 1
@@ -43,6 +45,10 @@ class Obstacle(C.Structure):
         ("n_frames", C.c_size_t),
         ("mesh_frames", C.POINTER(C.c_double)),
         ("frame_times", C.POINTER(C.c_double)),
+        ("size", C.c_double * 3),
+        ("rotation_axis", C.c_double * 3),
+        ("angular_speed", C.c_double),
+        ("rotation_start", C.c_double),
     ]
 
 
@@ -61,6 +67,10 @@ class Scene(C.Structure):
         ("obstacles", C.POINTER(Obstacle)),
         ("n_materials", C.c_size_t),
         ("friction", C.POINTER(C.c_double)),
+        ("n_attachments", C.c_size_t),
+        ("attachment_vertices", C.POINTER(C.c_int32)),
+        ("attachment_positions", C.POINTER(C.c_double)),
+        ("attachment_weights", C.POINTER(C.c_double)),
     ]
 
 

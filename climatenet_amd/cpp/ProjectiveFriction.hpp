@@ -68,6 +68,17 @@ private:
     std::size_t m_material;
 };
 
+// AttachmentConstraint (include/physics/AttachmentConstraint.hpp:33-70): one row of
+// weight `weight` on vertex `index`, projected on weight * `position`. Added with
+// PhysicScene::addConstraint after the scene is read; the JSON format has no key for it.
+struct AttachmentConstraint {
+    AttachmentConstraint(double weight, std::size_t index, const std::array<double, 3>& position)
+        : weight(weight), index(index), position(position) {}
+    double weight;
+    std::size_t index;
+    std::array<double, 3> position;
+};
+
 class PhysicScene {
 public:
     using FrictionCoefficientTable = std::vector<std::vector<double>>;
@@ -82,6 +93,9 @@ public:
         std::vector<double> times;       // n_frames (empty for a static mesh)
     };
     void addMeshObstacle(pdfc_obstacle obstacle, std::shared_ptr<const MeshObstacleData> data);
+    // PhysicScene::addConstraint (src/physics/PhysicScene.cpp:82-86); call before getSimulation.
+    void addConstraint(const std::shared_ptr<AttachmentConstraint>& constraint);
+    const std::vector<std::shared_ptr<AttachmentConstraint>>& getUserConstraints() const { return m_user_constraints; }
     void finalize();
     std::shared_ptr<PhysicMesh> getMesh(std::size_t index) { return m_meshes.at(index); }
     std::shared_ptr<const PhysicMesh> getMesh(std::size_t index) const { return m_meshes.at(index); }
@@ -99,6 +113,7 @@ private:
     std::vector<std::shared_ptr<PhysicMesh>> m_meshes;
     std::vector<pdfc_obstacle> m_obstacles;
     std::vector<std::shared_ptr<const MeshObstacleData>> m_mesh_data;  // kept alive for m_obstacles
+    std::vector<std::shared_ptr<AttachmentConstraint>> m_user_constraints;
 };
 
 // -------------------------------------------------------------- simulation

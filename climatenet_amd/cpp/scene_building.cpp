@@ -131,6 +131,12 @@ void PhysicMesh::writeObj(std::ofstream& of) const {
 PhysicScene::PhysicScene(FrictionCoefficientTable friction) : m_friction(std::move(friction)) {}
 void PhysicScene::addMesh(std::shared_ptr<PhysicMesh> mesh) { m_meshes.push_back(std::move(mesh)); }
 void PhysicScene::addObstacle(const pdfc_obstacle& obstacle) { m_obstacles.push_back(obstacle); }
+void PhysicScene::addConstraint(const std::shared_ptr<AttachmentConstraint>& constraint) {
+    if (!constraint) throw std::invalid_argument("null constraint");
+    if (constraint->index >= getNumberOfVertices()) throw std::invalid_argument("attachment vertex index out of range");
+    if (!(constraint->weight > 0.)) throw std::invalid_argument("attachment weight must be positive");
+    m_user_constraints.push_back(constraint);
+}
 void PhysicScene::addMeshObstacle(pdfc_obstacle ob, std::shared_ptr<const MeshObstacleData> data) {
     ob.mesh_triangles = data->triangles.data();
     ob.n_mesh_triangles = data->triangles.size() / 3;
@@ -300,9 +306,29 @@ PhysicScene getSceneFromJSONDocument(const json::Value& doc) {
             }
             scene.addMeshObstacle(ob, data);
             continue;
-        } else if (type == "Axis Aligned Box" || type == "Rotating Sphere") {
-            throw std::invalid_argument("JSON reader - Error : obstacle type \"" + type +
-                                        "\" is not supported by this build (spheres, planes and triangle meshes are)");
+        } else if (type == "Axis Aligned Box") {
+            // src/scene_building.cpp:204-211 (getStaticAxisAlignedBoxFromObject)
+            ob.type = PDFC_OBSTACLE_BOX;
+            auto p = vector3(require_array(o, "Position"));
+            ob.material = (int32_t)require_uint(o, "Material Identifier");
+            auto sz = vector3(require_array(o, "Size"));
+            for (int k = 0; k < 3; ++k) {
+                ob.point[k] = p[k];
+                ob.size[k] = sz[k];
+            }
+        } else if (type == "Rotating Sphere") {
+            // src/scene_building.cpp:279-293 (getRotatingSphereFromObject)
+            ob.type = PDFC_OBSTACLE_ROTATING_SPHERE;
+            ob.radius = require_double(o, "Radius");
+            auto c = vector3(require_array(o, "Position"));
+            auto ax = vector3(require_array(o, "Rotation Axe"));
+            for (int k = 0; k < 3; ++k) {
+                ob.point[k] = c[k];
+                ob.rotation_axis[k] = ax[k];
+            }
+            ob.angular_speed = require_double(o, "Radial Speed");
+            ob.rotation_start = require_double(o, "Rotation Start Time");
+            ob.material = (int32_t)require_uint(o, "Material Identifier");
         } else {
             throw std::invalid_argument("JSON reader - Error : Invalid type for static object type");
         }

@@ -41,6 +41,17 @@ SimulationFrictionEstimationHIP::SimulationFrictionEstimationHIP(PhysicScene& sc
     sc.obstacles = obs.data();
     sc.n_materials = table.size();
     sc.friction = fr.data();
+    std::vector<int32_t> att_v;
+    std::vector<double> att_p, att_w;
+    for (const auto& c : scene.getUserConstraints()) {
+        att_v.push_back((int32_t)c->index);
+        att_p.insert(att_p.end(), c->position.begin(), c->position.end());
+        att_w.push_back(c->weight);
+    }
+    sc.n_attachments = att_v.size();
+    sc.attachment_vertices = att_v.data();
+    sc.attachment_positions = att_p.data();
+    sc.attachment_weights = att_w.data();
     pdfc_params prm{};
     prm.time_step = p.time_step;
     prm.local_global_iterations = (int32_t)p.local_global_iterations;

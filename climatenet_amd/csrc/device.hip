@@ -126,6 +126,17 @@ __global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const do
     st3(cvec, n, i, D3{m * t.x - a.dt * s.x, m * t.y - a.dt * s.y, m * t.z - a.dt * s.z});
 }
 
+// Attachment rows: their A^T p is constant (w^2 * position), so it joins the step's
+// constant part c += dt (A^T p)_attachments (AttachmentConstraint.cpp:46-52).
+__global__ void k_attach(DevModel M, double* __restrict__ cvec, double dt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M.n_att) return;
+    const int n = M.n, i = M.att_vertex[k];
+    cvec[i] += dt * M.att_term[3 * k];
+    cvec[n + i] += dt * M.att_term[3 * k + 1];
+    cvec[2 * n + i] += dt * M.att_term[3 * k + 2];
+}
+
 // Obstacles vertex i meets: all of them, or those of its scene when scenes are batched.
 __device__ __forceinline__ void obstacle_range(const DevObstacles& ob, int i, int& o0, int& o1) {
     o0 = 0;
@@ -148,8 +159,38 @@ __device__ __forceinline__ D3 tri_point3(D3 a, D3 b, D3 c, const double* w) {
             (a.z * w[0] + b.z * w[1]) + c.z * w[2]};
 }
 
-// Obstacle detection at q_n: StaticSphere/Sphere::isInside (Sphere.cpp:93-113),
-// StaticPlane (StaticPlane.cpp:57-73). One bit per obstacle.
+// StaticAxisAlignedBox::checkCollision (StaticAxisAlignedBox.cpp:41-60): inside unless
+// some |relative_k| > size_k / 2 (relative = x - corner - size / 2).
+__device__ __forceinline__ bool box_inside(const double* corner, const double* size, D3 x, D3& rel) {
+    rel = {(x.x - corner[0]) - 0.5 * size[0], (x.y - corner[1]) - 0.5 * size[1], (x.z - corner[2]) - 0.5 * size[2]};
+    return !(fabs(rel.x) > size[0] / 2 || fabs(rel.y) > size[1] / 2 || fabs(rel.z) > size[2] / 2);
+}
+// Nearest side: first minimum of |relative_axis - sign size_axis / 2| in the Side enum
+// order TOP (+y), BOTTOM (-y), LEFT (-x), RIGHT (+x), FRONT (+z), BACK (-z); the contact
+// is relative with that axis moved onto the side, back in world coordinates.
+__device__ __forceinline__ void box_contact(const double* corner, const double* size, D3 rel, D3& pt, D3& nrm) {
+    const int axis[6] = {1, 1, 0, 0, 2, 2};
+    const double sign[6] = {1., -1., -1., 1., 1., -1.};
+    double r[3] = {rel.x, rel.y, rel.z};
+    int best = 0;
+    double bd = 0.;
+    for (int sd = 0; sd < 6; ++sd) {
+        double d = fabs(r[axis[sd]] - sign[sd] * 0.5 * size[axis[sd]]);
+        if (sd == 0 || d < bd) {
+            bd = d;
+            best = sd;
+        }
+    }
+    r[axis[best]] = sign[best] * 0.5 * size[axis[best]];
+    pt = {(r[0] + corner[0]) + 0.5 * size[0], (r[1] + corner[1]) + 0.5 * size[1], (r[2] + corner[2]) + 0.5 * size[2]};
+    double nn[3] = {0., 0., 0.};
+    nn[axis[best]] = sign[best];
+    nrm = {nn[0], nn[1], nn[2]};
+}
+
+// Obstacle detection at q_n: StaticSphere/RotatingSphere via Sphere::isInside
+// (Sphere.cpp:93-113), StaticPlane (StaticPlane.cpp:57-73), boxes, mesh hits from
+// k_mesh_query. One bit per obstacle.
 __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObstacles ob, int* cflags, int* ccnt,
                                int* blocksum) {
     __shared__ int red[kBlock / 64];
@@ -163,11 +204,14 @@ __global__ void k_detect_count(DevModel M, const double* __restrict__ qn, DevObs
         obstacle_range(ob, i, o0, o1);
         for (int o = o0; o < o1; ++o) {
             bool hit;
+            const int ty = ob.type[o];
             D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
-            if (ob.type[o] == 0)
+            if (ty == PDFC_OBSTACLE_SPHERE || ty == PDFC_OBSTACLE_ROTATING_SPHERE)
                 hit = sqrt(dot3(r, r)) < ob.radius[o] + 1e-2;
-            else if (ob.type[o] == 1)
+            else if (ty == PDFC_OBSTACLE_PLANE)
                 hit = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]}) < 1.5e-2;
+            else if (ty == PDFC_OBSTACLE_BOX)
+                hit = box_inside(&ob.p[3 * o], &ob.nrm[3 * o], x, r);
             else
                 hit = ob.mesh[mesh_slot(ob, o)].hit_tri[i] >= 0;
             if (hit) flags |= 1 << (o - o0);
@@ -238,12 +282,23 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
         if (!(flags & (1 << (o - o0)))) continue;
         D3 nrm, pt, spd = {0., 0., 0.};
         const D3 c = {ob.p[3 * o], ob.p[3 * o + 1], ob.p[3 * o + 2]};
-        if (ob.type[o] == 0) {
+        const int ty = ob.type[o];
+        if (ty == PDFC_OBSTACLE_SPHERE || ty == PDFC_OBSTACLE_ROTATING_SPHERE) {
             D3 r = normalized3(D3{x.x - c.x, x.y - c.y, x.z - c.z});
             double R = ob.radius[o];
             pt = {c.x + R * r.x, c.y + R * r.y, c.z + R * r.z};
             nrm = r;
-        } else if (ob.type[o] == 1) {
+            // RotatingSphere::checkCollision (RotatingSphere.cpp:71-93): omega x (contact - centre)
+            // once the time before the rotation starts is negative
+            if (ty == PDFC_OBSTACLE_ROTATING_SPHERE && ob.clock[o] < 0.) {
+                const D3 w = {ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]};
+                spd = cross3(w, D3{pt.x - c.x, pt.y - c.y, pt.z - c.z});
+            }
+        } else if (ty == PDFC_OBSTACLE_BOX) {
+            D3 rel;
+            box_inside(&ob.p[3 * o], &ob.nrm[3 * o], x, rel);
+            box_contact(&ob.p[3 * o], &ob.nrm[3 * o], rel, pt, nrm);
+        } else if (ty == PDFC_OBSTACLE_PLANE) {
             nrm = {ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]};
             D3 r = {x.x - c.x, x.y - c.y, x.z - c.z};
             double d = dot3(r, nrm);
@@ -467,6 +522,9 @@ __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __re
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
     double ac_s = 0., ac_m = 0.;
+    // RotatingSphere::update (RotatingSphere.cpp:39-43), once per step; detection of this
+    // step has read the clock already
+    if (i < ob.count && ob.type[i] == PDFC_OBSTACLE_ROTATING_SPHERE) ob.clock[i] -= a.dt;
     if (i < n) {
         D3 v = ld3(vlast, n, i);
         if (a.record) {
@@ -533,9 +591,9 @@ __global__ void k_warm_features(DevModel M, const double* __restrict__ qn, const
     double gap;
     {
         D3 r = {x.x - ob.p[3 * o], x.y - ob.p[3 * o + 1], x.z - ob.p[3 * o + 2]};
-        if (ob.type[o] == 0) {
+        if (ob.type[o] == PDFC_OBSTACLE_SPHERE || ob.type[o] == PDFC_OBSTACLE_ROTATING_SPHERE) {
             gap = sqrt(dot3(r, r)) - ob.radius[o];
-        } else if (ob.type[o] == 1) {
+        } else if (ob.type[o] == PDFC_OBSTACLE_PLANE) {
             gap = dot3(r, D3{ob.nrm[3 * o], ob.nrm[3 * o + 1], ob.nrm[3 * o + 2]});
         } else {  // signed distance to the contact point along the normal
             const D3 cp = {C.point[3 * cid], C.point[3 * cid + 1], C.point[3 * cid + 2]};
@@ -972,6 +1030,8 @@ void launch_axpy(double* x, const double* dx, size_t count, hipStream_t st) {
 void launch_step_begin(const DevModel& M, const double* qn, const double* vstate, double* v0, double* cvec,
                        const StepArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_step_begin, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vstate, v0, cvec, a);
+    if (M.n_att > 0)
+        hipLaunchKernelGGL(k_attach, dim3(blocks_for(M.n_att)), dim3(kBlock), 0, st, M, cvec, a.dt);
 }
 void launch_detect(const DevModel& M, const double* qn, const DevObstacles& ob, int* cflags, int* ccnt, int* cstart,
                    int* blocksum, int* n_contacts, DevContacts C, hipStream_t st) {
@@ -989,8 +1049,8 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const DevObstacles& ob, const StepArgs& a, int iteration,
                    hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vlast, vstate, ccnt, cstart, C, ob,
-                       a, iteration);
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(std::max(M.n, ob.count))), dim3(kBlock), 0, st, M, qn, vlast, vstate,
+                       ccnt, cstart, C, ob, a, iteration);
 }
 void launch_warm_features(const DevModel& M, const double* qn, const double* v0, const double* cvec,
                           const int* ncontacts, int cap, DevContacts C, const DevObstacles& ob, const StepArgs& a,

@@ -43,9 +43,12 @@ struct DevMeshObs {
 // obstacles of its own scene.
 struct DevObstacles {
     int count;
-    const int* type;
-    const double* p;       // 3 per obstacle: sphere centre or plane point
-    const double* nrm;     // 3 per obstacle: plane normal
+    const int* type;       // enum pdfc_obstacle_type
+    const double* p;       // 3 per obstacle: sphere centre, plane point, box corner (smallest coordinates)
+    const double* nrm;     // 3 per obstacle: plane normal; box edge lengths; rotating sphere's
+                           // angular velocity (speed times the unit axis)
+    double* clock;         // per obstacle: a rotating sphere's time before it turns (the step's
+                           // last kernel subtracts dt)
     const double* radius;
     const double* mu;      // friction coefficient against the scene's cloth
     const int* vinst;      // per vertex: its scene; null when there is one scene
@@ -81,6 +84,10 @@ struct DevModel {
     const int* ata_cnt;
     const int* ata_idx;      // ata_slots * n
     const double* ata_val;
+    // attached vertices: (A^T p) of their attachment rows, sum of w^2 * position (constant)
+    int n_att;
+    const int* att_vertex;   // n_att distinct vertices
+    const double* att_term;  // 3 * n_att, xyz interleaved
 };
 
 // One substitution task (supernode s, rows or columns r0 .. r0+63), everything the

@@ -33,6 +33,11 @@ struct Model {
     std::vector<int> bend_ptr;      // B+1, into bend_idx/bend_coef
     std::vector<int> bend_idx;      // [center, ring...]
     std::vector<double> bend_coef;  // weighted row entries sqrt(k_b) * beltrami coefficient
+    // attachment rows (after the bending rows): AttachmentConstraint,
+    // src/physics/AttachmentConstraint.cpp:23-55 (row weight w, projection w * position)
+    std::vector<int> att_vertex;
+    std::vector<double> att_w;
+    std::vector<double> att_pos;    // 3 per attachment
     std::vector<char> border;
     std::vector<std::vector<int>> neighbors;  // one ring, ascending
     // A^T A, symmetric, CSR with ascending columns

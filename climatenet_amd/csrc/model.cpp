@@ -154,6 +154,19 @@ void build_model(const pdfc_scene& sc, Model& M) {
         M.bend_coef[base] = wb * sum;
         M.bend_ptr.push_back((int)M.bend_idx.size());
     }
+    M.att_vertex.clear();
+    M.att_w.clear();
+    M.att_pos.clear();
+    if (sc.n_attachments > 0 && (!sc.attachment_vertices || !sc.attachment_positions || !sc.attachment_weights))
+        throw std::invalid_argument("attachment arrays missing");
+    for (size_t a = 0; a < sc.n_attachments; ++a) {
+        const int v = sc.attachment_vertices[a];
+        if (v < 0 || v >= n) throw std::invalid_argument("attachment vertex out of range");
+        if (!(sc.attachment_weights[a] > 0.)) throw std::invalid_argument("attachment weight must be positive");
+        M.att_vertex.push_back(v);
+        M.att_w.push_back(sc.attachment_weights[a]);
+        M.att_pos.insert(M.att_pos.end(), sc.attachment_positions + 3 * a, sc.attachment_positions + 3 * a + 3);
+    }
     // A^T A: accumulate a_r^T a_r row by row into per-vertex sorted rows.
     std::vector<std::vector<std::pair<int, double>>> rowacc(n);
     auto add_outer = [&](const int* idx, const double* val, int len) {
@@ -167,6 +180,7 @@ void build_model(const pdfc_scene& sc, Model& M) {
     }
     for (size_t b = 0; b + 1 < M.bend_ptr.size(); ++b)
         add_outer(&M.bend_idx[M.bend_ptr[b]], &M.bend_coef[M.bend_ptr[b]], M.bend_ptr[b + 1] - M.bend_ptr[b]);
+    for (size_t a = 0; a < M.att_vertex.size(); ++a) add_outer(&M.att_vertex[a], &M.att_w[a], 1);
     M.ata_ptr.assign(n + 1, 0);
     M.ata_idx.clear();
     M.ata_val.clear();
@@ -212,6 +226,9 @@ void merge_models(const std::vector<Model>& parts, Model& M) {
         for (size_t b = 1; b < P.bend_ptr.size(); ++b) M.bend_ptr.push_back(P.bend_ptr[b] + b0);
         for (int x : P.bend_idx) M.bend_idx.push_back(x + v0);
         M.bend_coef.insert(M.bend_coef.end(), P.bend_coef.begin(), P.bend_coef.end());
+        for (int x : P.att_vertex) M.att_vertex.push_back(x + v0);
+        M.att_w.insert(M.att_w.end(), P.att_w.begin(), P.att_w.end());
+        M.att_pos.insert(M.att_pos.end(), P.att_pos.begin(), P.att_pos.end());
         M.border.insert(M.border.end(), P.border.begin(), P.border.end());
         for (const auto& nb : P.neighbors) {
             std::vector<int> r(nb);

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -331,6 +333,16 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
                 ob.mesh_triangles = nullptr;  // copied above; the caller's arrays are not kept
                 ob.mesh_frames = nullptr;
                 ob.frame_times = nullptr;
+            } else if (ob.type == PDFC_OBSTACLE_BOX) {
+                for (int c = 0; c < 3; ++c)
+                    if (!(ob.size[c] >= 0.)) throw std::invalid_argument("box size must be non-negative");
+            } else if (ob.type == PDFC_OBSTACLE_ROTATING_SPHERE) {
+                // m_rotation_axe = radial_speed * axis.normalized() (RotatingSphere.cpp:28-37)
+                double z = (ob.rotation_axis[0] * ob.rotation_axis[0] + ob.rotation_axis[1] * ob.rotation_axis[1]) +
+                           ob.rotation_axis[2] * ob.rotation_axis[2];
+                if (!(z > 0.)) throw std::invalid_argument("rotation axis is zero");
+                double s = std::sqrt(z);
+                for (int c = 0; c < 3; ++c) ob.rotation_axis[c] = ob.angular_speed * (ob.rotation_axis[c] / s);
             } else if (ob.type != PDFC_OBSTACLE_SPHERE) {
                 throw std::invalid_argument("unknown obstacle type");
             }
@@ -432,6 +444,24 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     dm.ata_cnt = upload(ata_cnt);
     dm.ata_idx = upload(ata_idx);
     dm.ata_val = upload(ata_val);
+    {
+        // attachment rows' A^T p per attached vertex, in row order
+        std::map<int, std::array<double, 3>> term;
+        for (size_t a = 0; a < model.att_vertex.size(); ++a) {
+            auto& t = term.emplace(iperm[model.att_vertex[a]], std::array<double, 3>{0., 0., 0.}).first->second;
+            const double w = model.att_w[a];
+            for (int c = 0; c < 3; ++c) t[c] += w * (w * model.att_pos[3 * a + c]);
+        }
+        std::vector<int> av;
+        std::vector<double> at;
+        for (const auto& kv : term) {
+            av.push_back(kv.first);
+            at.insert(at.end(), kv.second.begin(), kv.second.end());
+        }
+        dm.n_att = (int)av.size();
+        dm.att_vertex = upload(av);
+        dm.att_term = upload(at);
+    }
 
     // ---- factor
     const int S = (int)fac.ns.size();
@@ -465,20 +495,26 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     // ---- obstacles
     {
         std::vector<int> type(nobs);
-        std::vector<double> pt(3 * (size_t)nobs), nr(3 * (size_t)nobs), rad(nobs);
+        std::vector<double> pt(3 * (size_t)nobs), nr(3 * (size_t)nobs), rad(nobs), clock(nobs, 0.);
         for (int o = 0; o < nobs; ++o) {
-            type[o] = obstacles[o].type;
+            const pdfc_obstacle& ob = obstacles[o];
+            type[o] = ob.type;
+            const double* third = ob.type == PDFC_OBSTACLE_BOX               ? ob.size
+                                  : ob.type == PDFC_OBSTACLE_ROTATING_SPHERE ? ob.rotation_axis  // omega
+                                                                             : ob.normal;
             for (int k = 0; k < 3; ++k) {
-                pt[3 * o + k] = obstacles[o].point[k];
-                nr[3 * o + k] = obstacles[o].normal[k];
+                pt[3 * o + k] = ob.point[k];
+                nr[3 * o + k] = third[k];
             }
-            rad[o] = obstacles[o].radius;
+            rad[o] = ob.radius;
+            if (ob.type == PDFC_OBSTACLE_ROTATING_SPHERE) clock[o] = ob.rotation_start;
         }
         dob.count = nobs;
         dob.type = upload(type);
         dob.p = upload(pt);
         dob.nrm = upload(nr);
         dob.radius = upload(rad);
+        dob.clock = upload(clock);
         dob.mu = upload(obs_mu);
         dob.vinst = nullptr;
         dob.obs_ptr = upload(obs_ptr);

@@ -90,7 +90,7 @@ def jitter_xy(pos: np.ndarray, amplitude: float, seed: int = 0) -> np.ndarray:
 
 @dataclass
 class ObstacleSpec:
-    kind: str  # "sphere" | "plane" | "mesh"
+    kind: str  # "sphere" | "plane" | "mesh" | "box" | "rotating_sphere"
     point: tuple = (0.0, 0.0, 0.0)
     normal: tuple = (0.0, 0.0, 1.0)
     radius: float = 0.0
@@ -100,6 +100,12 @@ class ObstacleSpec:
     frames: Optional[np.ndarray] = None
     triangles: Optional[np.ndarray] = None
     times: Optional[np.ndarray] = None
+    # box: point is the corner with the smallest coordinates, size the edge lengths
+    size: tuple = (0.0, 0.0, 0.0)
+    # rotating sphere: axis, angular speed (rad/s), time before the surface turns (s)
+    axis: tuple = (0.0, 0.0, 1.0)
+    angular_speed: float = 0.0
+    rotation_start: float = 0.0
 
 
 @dataclass
@@ -121,6 +127,8 @@ class SceneSpec:
     self_tolerance: float = 0.0
     gravity: tuple = (0.0, 0.0, -9.81)
     air_damping: float = 0.0
+    # attachment constraints: vertex indices, target positions [k, 3], row weights
+    attachments: Optional[tuple] = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -141,7 +149,8 @@ class SceneSpec:
         fr = np.ascontiguousarray(self.friction, dtype=np.float64)
         obs = (abi.Obstacle * max(1, len(self.obstacles)))()
         keep = []
-        kinds = {"sphere": abi.OBSTACLE_SPHERE, "plane": abi.OBSTACLE_PLANE, "mesh": abi.OBSTACLE_MESH}
+        kinds = {"sphere": abi.OBSTACLE_SPHERE, "plane": abi.OBSTACLE_PLANE, "mesh": abi.OBSTACLE_MESH,
+                 "box": abi.OBSTACLE_BOX, "rotating_sphere": abi.OBSTACLE_ROTATING_SPHERE}
         for k, o in enumerate(self.obstacles):
             obs[k].type = kinds[o.kind]
             obs[k].material = o.material
@@ -149,6 +158,11 @@ class SceneSpec:
                 obs[k].point[c] = o.point[c]
                 obs[k].normal[c] = o.normal[c]
             obs[k].radius = o.radius
+            for c in range(3):
+                obs[k].size[c] = o.size[c]
+                obs[k].rotation_axis[c] = o.axis[c]
+            obs[k].angular_speed = o.angular_speed
+            obs[k].rotation_start = o.rotation_start
             if o.kind == "mesh":
                 frames = np.ascontiguousarray(o.frames, dtype=np.float64).reshape(-1, np.asarray(o.frames).shape[-2], 3)
                 mt = np.ascontiguousarray(o.triangles, dtype=np.int32)
@@ -163,8 +177,17 @@ class SceneSpec:
                     tm = np.ascontiguousarray(o.times, dtype=np.float64)
                     obs[k].frame_times = abi.dptr(tm)
                 keep += [frames, mt, tm]
-        self._keep = [rest, pos, tris, fr, obs] + keep
         s = abi.Scene()
+        if self.attachments is not None:
+            av = np.ascontiguousarray(self.attachments[0], dtype=np.int32)
+            ap = np.ascontiguousarray(self.attachments[1], dtype=np.float64).reshape(-1)
+            aw = np.ascontiguousarray(self.attachments[2], dtype=np.float64)
+            s.n_attachments = av.size
+            s.attachment_vertices = abi.iptr(av)
+            s.attachment_positions = abi.dptr(ap)
+            s.attachment_weights = abi.dptr(aw)
+            keep += [av, ap, aw]
+        self._keep = [rest, pos, tris, fr, obs] + keep
         s.n_vertices = self.n_vertices
         s.rest_positions = abi.dptr(rest)
         s.positions = abi.dptr(pos)
@@ -333,3 +356,39 @@ def mesh_sphere_drape(moving: bool = False, jitter_seed: int = 0) -> SceneSpec:
         ob = ObstacleSpec("mesh", tolerance=1.5e-2, frames=P[None], triangles=F)
     return SceneSpec("mesh_drape_20x20" + ("_moving" if moving else ""), rest, tris, pos, [ob],
                      np.array([[0.3]]), time_step=0.005, iterations=20, steps=200)
+
+
+def box_drape(jitter_seed: int = 0) -> SceneSpec:
+    """The 20x20 drape (1 m, mu=0.3, dt=1/100, 10 iterations) dropped from z=0.3 on an
+    axis-aligned box 0.5 x 0.5 x 0.75 whose top face is at z=0.25 (a box sees only the
+    vertices inside it; the nearest face gives the contact)."""
+    rest, tris = square_grid(20, 1.0, 0.3)
+    pos = jitter_xy(rest, 1e-4, jitter_seed)
+    return SceneSpec("box_drape_20x20", rest, tris, pos,
+                     [ObstacleSpec("box", (-0.25, -0.25, -0.5), size=(0.5, 0.5, 0.75))],
+                     np.array([[0.3]]), time_step=0.01, iterations=10, steps=200)
+
+
+def rotating_sphere_drape(jitter_seed: int = 0, start: float = 0.1) -> SceneSpec:
+    """The 20x20 drape on a sphere R=0.25 that starts turning about z at 3 rad/s after
+    `start` seconds (a RotatingSphere: fixed shape, moving surface), mu=0.6."""
+    rest, tris = square_grid(20, 1.0, 0.3)
+    pos = jitter_xy(rest, 1e-4, jitter_seed)
+    return SceneSpec("rotating_sphere_drape_20x20", rest, tris, pos,
+                     [ObstacleSpec("rotating_sphere", (0.0, 0.0, 0.0), radius=0.25, axis=(0.0, 0.0, 2.0),
+                                   angular_speed=3.0, rotation_start=start)],
+                     np.array([[0.6]]), time_step=0.01, iterations=10, steps=200)
+
+
+def hanging_cloth(weight: float = 100.0) -> SceneSpec:
+    """The 20x20 cloth held by two attachment constraints at the corners of one edge (their
+    rest positions, row weight `weight`); it swings down onto a sphere R=0.2 below the
+    free half."""
+    N = 20
+    rest, tris = square_grid(N, 1.0, 0.3)
+    P = rest.reshape(-1, 3)
+    corners = np.array([0, N], dtype=np.int32)
+    return SceneSpec("hanging_20x20", rest, tris, None,
+                     [ObstacleSpec("sphere", (0.0, 0.25, -0.1), radius=0.2)],
+                     np.array([[0.3]]), time_step=0.01, iterations=10, steps=200,
+                     attachments=(corners, P[corners].copy(), np.full(2, weight)))

@@ -51,9 +51,12 @@ typedef struct pdfc_sim pdfc_sim;
 enum pdfc_obstacle_type {
     PDFC_OBSTACLE_SPHERE = 0, /* StaticSphere: src/physics/StaticSphere.cpp:35-47 */
     PDFC_OBSTACLE_PLANE = 1,  /* StaticPlane:  src/physics/StaticPlane.cpp:57-73 */
-    PDFC_OBSTACLE_MESH = 2    /* StaticMesh (one frame, src/physics/StaticMesh.cpp:24-56) or MovingMesh
+    PDFC_OBSTACLE_MESH = 2,   /* StaticMesh (one frame, src/physics/StaticMesh.cpp:24-56) or MovingMesh
                                  (key frames, src/physics/MovingMesh.cpp:28-253); detection
                                  checkMeshCollision, src/geometry/geometry_util.cpp:32-92 */
+    PDFC_OBSTACLE_BOX = 3,    /* StaticAxisAlignedBox: src/physics/StaticAxisAlignedBox.cpp:41-60 */
+    PDFC_OBSTACLE_ROTATING_SPHERE = 4  /* RotatingSphere: src/physics/RotatingSphere.cpp:28-93 (a fixed
+                                 sphere whose surface turns: contact speed omega x (contact - centre)) */
 };
 
 enum pdfc_precision {
@@ -89,6 +92,13 @@ typedef struct pdfc_obstacle {
     size_t n_frames;                 /* 1: static mesh; >= 2: key frames of a moving mesh */
     const double* mesh_frames;       /* n_frames x 3 n_mesh_vertices positions */
     const double* frame_times;       /* n_frames, increasing ("Time Between Frame" multiples); NULL if static */
+    /* axis-aligned boxes only (point is "Position", the corner with the smallest coordinates): */
+    double size[3];                  /* "Size", edge lengths along x, y, z */
+    /* rotating spheres only (point and radius as for a sphere): */
+    double rotation_axis[3];         /* "Rotation Axe" (any length but zero) */
+    double angular_speed;            /* "Radial Speed", rad/s */
+    double rotation_start;           /* "Rotation Start Time", s: the surface turns from the first step that
+                                        begins once this much time has been stepped */
 } pdfc_obstacle;
 
 typedef struct pdfc_scene {
@@ -105,6 +115,13 @@ typedef struct pdfc_scene {
     const pdfc_obstacle* obstacles;
     size_t n_materials;
     const double* friction;          /* n_materials x n_materials, row-major ("Friction coefficients") */
+    /* AttachmentConstraint rows after the mesh's constraints (PhysicScene::addConstraint,
+       src/physics/PhysicScene.cpp:82-86; src/physics/AttachmentConstraint.cpp:23-55): row weight w
+       on the vertex, projection w * position. No JSON key in the reference. */
+    size_t n_attachments;
+    const int32_t* attachment_vertices;  /* n_attachments vertex indices */
+    const double* attachment_positions;  /* 3 n_attachments */
+    const double* attachment_weights;    /* n_attachments, > 0 */
 } pdfc_scene;
 
 typedef struct pdfc_params {

@@ -25,7 +25,10 @@
  * regular grid, mass conservation, rest lengths, scipy spsolve on the
  * oracle's own K, free fall, the inclined-plane closed form, resting contact,
  * cone/Alart-Curnier invariants, brute-force candidate lists, brute-force
- * closest points.
+ * closest points. Mesh obstacles: tests/test_oracle_mesh.py. Boxes, rotating
+ * spheres and attachments: tests/test_oracle_obstacles.py (nearest-face brute
+ * force, omega x r, sticking on a turning surface, the implicit-Euler closed form
+ * of an attachment).
  */
 #include "pdfc.h"
 
@@ -226,6 +229,14 @@ struct orc_sim {
     std::vector<double> mass, invmass;
     std::vector<pdfc_obstacle> obstacles;
     std::vector<MeshObstacle> mesh_obs;      // per obstacle (empty unless a triangle mesh)
+    // rotating spheres: m_rotation_axe (radial speed times the unit axis) and
+    // m_time_beforme_rotation_start, decremented by dt at the end of every step
+    std::vector<V3> rot_omega;
+    std::vector<double> rot_clock;
+    // attachment rows (after the mesh's springs and bending rows): vertex, weight, position
+    std::vector<int> att_v;
+    std::vector<double> att_w;
+    std::vector<V3> att_pos;
     int n_mat = 0; std::vector<double> friction; int mesh_material = 0;
     double mesh_area_density = 0., mesh_bend = 0., mesh_stretch = 0.;
     pdfc_params prm{};
@@ -375,8 +386,8 @@ void build_constraints(orc_sim& S) {
 // (SimulationSpeed::computeLhs, SimulationSpeed.cpp:230-236).
 void assemble(orc_sim& S) {
     const int n = S.n;
-    const int E = (int)S.edges.size(), B = (int)S.bend_center.size();
-    S.m_rows = E + B;
+    const int E = (int)S.edges.size(), B = (int)S.bend_center.size(), NA = (int)S.att_v.size();
+    S.m_rows = E + B + NA;
     CSR& R = S.A_rows;
     R.n = S.m_rows;
     R.ptr.assign(S.m_rows + 1, 0);
@@ -392,6 +403,12 @@ void assemble(orc_sim& S) {
             R.val.push_back(S.bend_w[b] * S.bend_coef[b][k]);
         }
         R.ptr[E + b + 1] = (int64_t)R.idx.size();
+    }
+    // AttachmentConstraint::addConstraintMatrix (AttachmentConstraint.cpp:31-43): one entry w
+    for (int a = 0; a < NA; ++a) {
+        R.idx.push_back(S.att_v[a]);
+        R.val.push_back(S.att_w[a]);
+        R.ptr[E + B + a + 1] = (int64_t)R.idx.size();
     }
     // transpose: vertex -> rows (rows ascending)
     CSR& C = S.A_cols;
@@ -569,6 +586,32 @@ void solve1(const orc_sim& S, const double* b, double* out, std::vector<double>&
 // --------------------------------------------------------------- detection
 void detect_mesh_obstacle(orc_sim& S, int o, double mu);
 
+// StaticAxisAlignedBox::checkCollision (StaticAxisAlignedBox.cpp:41-60): inside unless some
+// |relative_k| > size_k / 2, where relative = x - position - size / 2; the nearest side is
+// the first minimum of |relative_axis - sign size_axis / 2| over TOP (+y), BOTTOM (-y),
+// LEFT (-x), RIGHT (+x), FRONT (+z), BACK (-z) (Side enum order, std::min_element with <);
+// the contact is relative with that axis set to the side, back in world coordinates.
+bool box_contact(const pdfc_obstacle& ob, const V3& x, V3& point, V3& normal) {
+    V3 rel;
+    for (int k = 0; k < 3; ++k) rel[k] = (x[k] - ob.point[k]) - 0.5 * ob.size[k];
+    for (int k = 0; k < 3; ++k)
+        if (std::abs(rel[k]) > ob.size[k] / 2) return false;
+    static const int axis[6] = {1, 1, 0, 0, 2, 2};
+    static const double sign[6] = {1., -1., -1., 1., 1., -1.};
+    int best = 0;
+    double bd = 0.;
+    for (int sd = 0; sd < 6; ++sd) {
+        double d = std::abs(rel[axis[sd]] - sign[sd] * 0.5 * ob.size[axis[sd]]);
+        if (sd == 0 || d < bd) { bd = d; best = sd; }
+    }
+    V3 res = rel;
+    res[axis[best]] = sign[best] * 0.5 * ob.size[axis[best]];
+    for (int k = 0; k < 3; ++k) point[k] = (res[k] + ob.point[k]) + 0.5 * ob.size[k];
+    normal = {0., 0., 0.};
+    normal[axis[best]] = sign[best];
+    return true;
+}
+
 // PhysicScene::getCollisionsInfoExceptFor (PhysicScene.cpp:115-181): obstacles in
 // order, vertices ascending. Sphere: Sphere.cpp:93-113. Plane: StaticPlane.cpp:57-73.
 void detect_obstacles(orc_sim& S) {
@@ -583,13 +626,19 @@ void detect_obstacles(orc_sim& S) {
         for (int i = 0; i < S.n; ++i) {
             const V3& x = S.q[i];
             ObsContact c{};
-            if (ob.type == PDFC_OBSTACLE_SPHERE) {
+            c.speed = {0, 0, 0};
+            if (ob.type == PDFC_OBSTACLE_SPHERE || ob.type == PDFC_OBSTACLE_ROTATING_SPHERE) {
                 V3 ctr{ob.point[0], ob.point[1], ob.point[2]};
                 V3 r = sub(x, ctr);
                 if (!(norm(r) < ob.radius + 1e-2)) continue;
                 r = normalized(r);
                 c.point = add(ctr, scale(ob.radius, r));
                 c.normal = r;
+                // RotatingSphere::checkCollision (RotatingSphere.cpp:71-93)
+                if (ob.type == PDFC_OBSTACLE_ROTATING_SPHERE && S.rot_clock[o] < 0)
+                    c.speed = cross(S.rot_omega[o], sub(c.point, ctr));
+            } else if (ob.type == PDFC_OBSTACLE_BOX) {
+                if (!box_contact(ob, x, c.point, c.normal)) continue;
             } else {
                 V3 p{ob.point[0], ob.point[1], ob.point[2]}, nn{ob.normal[0], ob.normal[1], ob.normal[2]};
                 double d = dot(sub(x, p), nn);
@@ -597,7 +646,7 @@ void detect_obstacles(orc_sim& S) {
                 c.point = sub(x, scale(d, nn));
                 c.normal = nn;
             }
-            c.v = i; c.obstacle = o; c.speed = {0, 0, 0}; c.mu = mu;
+            c.v = i; c.obstacle = o; c.mu = mu;
             c.B = contact_basis(c.normal);
             c.r = {0, 0, 0}; c.active = 0;
             S.contacts.push_back(c);
@@ -886,7 +935,11 @@ void step(orc_sim& S) {
         // computeProjections (SimulationBase.cpp:314-323): omp over constraints
 #pragma omp parallel for schedule(static)
         for (int r = 0; r < m; ++r) {
-            if (r < E) {
+            if (r >= E + (int)S.bend_center.size()) {
+                // AttachmentConstraint::addConstraintProjection (:46-52): w * position
+                const int a = r - E - (int)S.bend_center.size();
+                for (int k = 0; k < 3; ++k) proj[3 * (size_t)r + k] = S.att_w[a] * S.att_pos[a][k];
+            } else if (r < E) {
                 V3 e = normalized(sub(qk[S.edges[r][0]], qk[S.edges[r][1]]));
                 double s = S.spring_w[r] * S.spring_L0[r];
                 for (int k = 0; k < 3; ++k) proj[3 * (size_t)r + k] = s * e[k];
@@ -1043,6 +1096,8 @@ void step(orc_sim& S) {
     S.v = vk;
     S.time += dt;
     for (auto& M : S.mesh_obs) update_mesh_obstacle(M, dt);
+    for (size_t o = 0; o < S.obstacles.size(); ++o)  // RotatingSphere::update (RotatingSphere.cpp:39-43)
+        if (S.obstacles[o].type == PDFC_OBSTACLE_ROTATING_SPHERE) S.rot_clock[o] -= dt;
     S.warm_r.clear();
     S.warm_hold = 0;
 }
@@ -1120,6 +1175,25 @@ int orc_create(const pdfc_scene* sc, const pdfc_params* prm, orc_sim** out) {
                 ob.frame_times = nullptr;
             }
             S->obstacles.push_back(ob);
+        }
+        S->rot_omega.assign(S->obstacles.size(), {0, 0, 0});
+        S->rot_clock.assign(S->obstacles.size(), 0.);
+        for (size_t o = 0; o < S->obstacles.size(); ++o) {
+            const pdfc_obstacle& ob = S->obstacles[o];
+            if (ob.type != PDFC_OBSTACLE_ROTATING_SPHERE) continue;
+            V3 ax{ob.rotation_axis[0], ob.rotation_axis[1], ob.rotation_axis[2]};
+            if (!(norm(ax) > 0.)) throw std::invalid_argument("rotation axis is zero");
+            S->rot_omega[o] = scale(ob.angular_speed, normalized(ax));
+            S->rot_clock[o] = ob.rotation_start;
+        }
+        for (size_t a = 0; a < sc->n_attachments; ++a) {
+            int v = sc->attachment_vertices[a];
+            if (v < 0 || v >= S->n) throw std::invalid_argument("attachment vertex out of range");
+            if (!(sc->attachment_weights[a] > 0.)) throw std::invalid_argument("attachment weight must be positive");
+            S->att_v.push_back(v);
+            S->att_w.push_back(sc->attachment_weights[a]);
+            S->att_pos.push_back({sc->attachment_positions[3 * a], sc->attachment_positions[3 * a + 1],
+                                  sc->attachment_positions[3 * a + 2]});
         }
         S->tol_eff = std::sqrt(prm->self_collision_tolerance * prm->self_collision_tolerance);
         build_topology(*S);

@@ -36,6 +36,14 @@ def write_scene(dirpath, spec, frames=2, steps_per_frame=5, log_vertices=(0, 1))
         if o.kind == "sphere":
             obstacles.append({"Type": "Sphere", "Position": [float(x) for x in o.point], "Radius": float(o.radius),
                               "Material Identifier": int(o.material)})
+        elif o.kind == "box":
+            obstacles.append({"Type": "Axis Aligned Box", "Position": [float(x) for x in o.point],
+                              "Size": [float(x) for x in o.size], "Material Identifier": int(o.material)})
+        elif o.kind == "rotating_sphere":
+            obstacles.append({"Type": "Rotating Sphere", "Position": [float(x) for x in o.point],
+                              "Radius": float(o.radius), "Rotation Axe": [float(x) for x in o.axis],
+                              "Radial Speed": float(o.angular_speed), "Rotation Start Time": float(o.rotation_start),
+                              "Material Identifier": int(o.material)})
         elif o.kind == "plane":
             obstacles.append({"Type": "Plane", "Point": [float(x) for x in o.point],
                               "Normal": [float(x) for x in o.normal], "Material Identifier": int(o.material)})

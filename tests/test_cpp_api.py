@@ -73,11 +73,15 @@ def test_missing_key_and_optional_damping(api_check, tmp_path):
     assert rc == 3 and "cannot found key LocalGlobal" in raw
 
 
-def test_unsupported_obstacle_and_bad_friction(api_check, tmp_path):
+def test_unknown_obstacle_and_bad_friction(api_check, tmp_path):
+    path = write_scene(str(tmp_path), scenes.sphere_drape_small())
+    _edit(path, lambda d: d["Obstacles"].append({"Type": "Cylinder", "Material Identifier": 0}))
+    rc, out, raw = run(api_check, path)
+    assert rc == 3 and "Invalid type for static object type" in raw
     path = write_scene(str(tmp_path), scenes.sphere_drape_small())
     _edit(path, lambda d: d["Obstacles"].append({"Type": "Rotating Sphere", "Material Identifier": 0}))
     rc, out, raw = run(api_check, path)
-    assert rc == 3 and "not supported" in raw
+    assert rc == 3 and "cannot found key Radius" in raw
     path = write_scene(str(tmp_path), scenes.sphere_drape_small())
     _edit(path, lambda d: d.__setitem__("Friction coefficients", [[0.1, 0.2]]))
     rc, out, raw = run(api_check, path)
@@ -136,3 +140,16 @@ def test_moving_mesh_missing_frame(api_check, tmp_path):
     _edit(path, lambda d: d["Obstacles"][0].__setitem__("Number Frame", 1))
     rc, out, raw = run(api_check, path)
     assert rc == 3 and "at least two frames" in raw
+
+
+def test_box_and_rotating_sphere_readers(api_check, tmp_path):
+    """"Axis Aligned Box" (src/scene_building.cpp:204-211) and "Rotating Sphere" (:279-293)."""
+    spec = scenes.box_drape()
+    spec.obstacles.append(scenes.rotating_sphere_drape().obstacles[0])
+    path = write_scene(str(tmp_path), spec)
+    rc, out, raw = run(api_check, path)
+    assert rc == 0, raw
+    f = [float(x) for x in out["obstacle0"].split()]
+    assert int(f[0]) == 3 and f[6:12] == [-0.25, -0.25, -0.5, 0.5, 0.5, 0.75]
+    f = [float(x) for x in out["obstacle1"].split()]
+    assert int(f[0]) == 4 and f[6:12] == [0.25, 0.0, 0.0, 2.0, 3.0, 0.1]

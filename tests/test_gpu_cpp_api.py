@@ -25,6 +25,9 @@ def as_read(spec, dirpath):
     """The spec with every OBJ-borne position replaced by what the reader stores."""
     s = copy.deepcopy(spec)
     s.rest = read_obj_positions(os.path.join(dirpath, "rest.obj")).ravel()
+    if s.attachments is not None:  # anchored at the rest positions as read
+        idx, _, w = s.attachments
+        s.attachments = (idx, s.rest.reshape(-1, 3)[idx].copy(), w)
     s.positions = read_obj_positions(os.path.join(dirpath, "current.obj")).ravel()
     for k, o in enumerate(s.obstacles):
         if o.kind != "mesh":
@@ -37,16 +40,26 @@ def as_read(spec, dirpath):
     return s
 
 
-@pytest.mark.parametrize("name", ["sphere_drape_small", "mesh_drape_moving"])
+SCENES = {"sphere_drape_small": scenes.sphere_drape_small, "mesh_drape_moving": lambda: scenes.mesh_sphere_drape(True),
+          "box_drape": scenes.box_drape, "rotating_sphere_drape": scenes.rotating_sphere_drape,
+          "hanging_cloth": scenes.hanging_cloth}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
 def test_json_scene_matches_direct_abi(gpu, api_check, tmp_path, name):  # noqa: F811
-    spec = scenes.sphere_drape_small() if name == "sphere_drape_small" else scenes.mesh_sphere_drape(moving=True)
+    spec = SCENES[name]()
     if spec.positions is None:
         spec.positions = spec.rest.copy()
     d = str(tmp_path)
     path = write_scene(d, spec)
     out_file = os.path.join(d, "final.txt")
     print("json scene", name, STEPS, "steps", flush=True)
-    rc, out, raw = run(api_check, path, "sim", str(STEPS), out_file)
+    extra = []
+    if spec.attachments is not None:  # PhysicScene::addConstraint(AttachmentConstraint) before getSimulation
+        idx, _, w = spec.attachments
+        assert len(set(w)) == 1
+        extra = [",".join(str(int(i)) for i in idx) + ":%r" % float(w[0])]
+    rc, out, raw = run(api_check, path, "sim", str(STEPS), out_file, *extra)
     assert rc == 0, raw
     q_json = np.loadtxt(out_file)
     sim = gpu.Simulation(as_read(spec, d))
