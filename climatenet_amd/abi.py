@@ -165,6 +165,8 @@ class Info(C.Structure):
         ("best_pair_ms", C.c_double),
         ("leaf_size", C.c_int32),
         ("top_chunks", C.c_int32),
+        ("lhs_seconds", C.c_double),
+        ("steps_done", C.c_int64),
     ]
 
 
@@ -179,6 +181,8 @@ class PhaseStats(C.Structure):
         ("contact_steps", C.c_int64),
         ("converged_steps", C.c_int64),
         ("fallbacks", C.c_int64),
+        ("collision_us", C.c_double),
+        ("timed_steps", C.c_int64),
     ]
 
 

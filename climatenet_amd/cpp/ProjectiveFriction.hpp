@@ -163,15 +163,28 @@ public:
     pdfc_sim* handle() const { return m_sim; }
     // Skip the per-step state copy when no caller reads the scene between steps.
     void setMirrorState(bool on) { m_mirror = on; }
+    // Binary checkpoint (pdfc_save_checkpoint / pdfc_load_checkpoint): positions, velocities,
+    // obstacle clocks, warm-start reactions, step count. No counterpart in the reference.
+    // Loading copies q and v into the scene. Errors throw std::runtime_error /
+    // std::invalid_argument (a file that is not a checkpoint of this scene).
+    void saveCheckpoint(const std::string& path) const;
+    void loadCheckpoint(const std::string& path);
+    std::size_t getStepsDone() const;
 
 private:
     PhysicScene& m_scene;
     pdfc_sim* m_sim = nullptr;
     std::size_t m_iterations;
     bool m_mirror = true;
-    std::vector<double> m_step_times, m_collision_numbers, m_self_collision_numbers;
-    double m_factor_us = 0.;
+    bool m_self = false;
+    // the vectors of SimulationSpeed/SimulationBase/SimulationFrictionEstimation that
+    // exportStats writes; the ones this build does not measure stay empty ("(No data)")
+    std::vector<double> m_lhs_times, m_cholesky_times, m_iteration_times, m_collision_detection_times;
+    std::vector<double> m_collision_numbers, m_self_collision_detection_times, m_self_collision_numbers;
+    std::vector<double> m_global_times, m_rhs_times, m_step_times, m_rhs_base_times, m_friction_times;
+    std::vector<double> m_self_friction_times, m_self_collision_ordering_times;
     std::vector<double> m_q, m_v;
+    void mirrorState();
 };
 
 std::unique_ptr<SimulationBase> getSimulation(PhysicScene& scene, const SimulationParameters& params);

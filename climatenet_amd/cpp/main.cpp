@@ -1,6 +1,9 @@
-// pdfc_main <scene.json>: the reference's driver (src/main.cpp:40-124) on the GPU
-// simulation: output directory dataN/ with a copy of the scene, one OBJ frame
-// every "Steps per frame" steps, log callbacks after each step, stats.txt.
+// pdfc_main <scene.json> [--checkpoint-every K] [--resume FILE]: the reference's driver
+// (src/main.cpp:40-124) on the GPU simulation: output directory dataN/ with a copy of
+// the scene, one OBJ frame every "Steps per frame" steps, log callbacks after each step,
+// stats.txt. The options have no counterpart in the reference: --checkpoint-every writes
+// dataN/checkpoint.bin every K steps and after the last one; --resume loads a checkpoint
+// of the same scene and continues from its step (frames keep their numbers).
 #include <sys/stat.h>
 
 #include <cstdlib>
@@ -31,6 +34,19 @@ int main(int argc, char* argv[]) {
         return EXIT_FAILURE;
     }
     const std::string scene_path(argv[1]);
+    std::size_t checkpoint_every = 0;
+    std::string resume;
+    for (int a = 2; a < argc; ++a) {
+        const std::string opt = argv[a];
+        if (opt == "--checkpoint-every" && a + 1 < argc) {
+            checkpoint_every = (std::size_t)std::strtoull(argv[++a], nullptr, 10);
+        } else if (opt == "--resume" && a + 1 < argc) {
+            resume = argv[++a];
+        } else {
+            std::cerr << "Unknown option " << opt << std::endl;
+            return EXIT_FAILURE;
+        }
+    }
     std::cout << "# Initializing." << std::endl;
     std::string out_dir;
     for (unsigned k = 0;; ++k) {
@@ -52,10 +68,17 @@ int main(int argc, char* argv[]) {
         auto simulation = getSimulation(scene, params);
         StepCallBacks log_callback = getLogCallBacksFromJSONDocument(doc);
         const size_t max_steps = params.max_output_frame * params.sim_steps_per_frame;
+        auto* hip_sim = dynamic_cast<SimulationFrictionEstimationHIP*>(simulation.get());
+        size_t first = 0;
+        if (!resume.empty()) {
+            hip_sim->loadCheckpoint(resume);
+            first = hip_sim->getStepsDone();
+            std::cout << "# Resumed from " << resume << " at step " << first << std::endl;
+        }
         std::cout << "# Initialization done. Starting simulation, " << max_steps << " timesteps" << std::endl;
         std::list<int> milestones;
         for (int i = 5; i <= 100; i += 5) milestones.push_back(i);
-        for (size_t i = 0; i < max_steps; ++i) {
+        for (size_t i = first; i < max_steps; ++i) {
             std::cout << "Time Step: " << i << std::endl;
             simulation->step();
             log_callback(i, scene);
@@ -66,6 +89,8 @@ int main(int argc, char* argv[]) {
                 std::ofstream of(name.str());
                 mesh->writeObj(of);
             }
+            if (checkpoint_every > 0 && ((i + 1) % checkpoint_every == 0 || i + 1 == max_steps))
+                hip_sim->saveCheckpoint(out_dir + "/checkpoint.bin");
             if (!milestones.empty() && i >= milestones.front() * (max_steps - 1) / 100.) {
                 std::cerr << milestones.front() << "% done" << std::endl;
                 milestones.pop_front();
@@ -75,7 +100,7 @@ int main(int argc, char* argv[]) {
         std::ofstream of(out_dir + "/stats.txt");
         simulation->exportStats(of);
         std::cout << "# Writing done. " << std::endl;
-    } catch (const std::invalid_argument& e) {
+    } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;
         return EXIT_FAILURE;
     }

@@ -151,6 +151,40 @@ struct pdfc_sim {
     int64_t ksolves = 0;
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
+    double lhs_seconds = 0.;
+    // Per-step timing: three events per eager step (start, first iteration, end), summed
+    // into stats every kStepEventBatch steps and at the end of pdfc_step.
+    static constexpr size_t kStepEventBatch = 256;
+    std::vector<hipEvent_t> step_ev;
+    size_t step_ev_used = 0;
+    double sum_collision_ms = 0., sum_iterations_ms = 0.;
+    bool capturing = false;
+    void step_event(int k) {
+        if (capturing) return;
+        if (k == 0) {
+            step_ev_used -= step_ev_used % 3;  // drop the events of a step that threw
+            if (step_ev_used + 3 > 3 * kStepEventBatch) flush_step_events();
+        }
+        if (step_ev_used >= step_ev.size()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            step_ev.push_back(e);
+        }
+        HIP_CHECK(hipEventRecord(step_ev[step_ev_used++], stream));
+    }
+    void flush_step_events() {
+        if (step_ev_used == 0) return;
+        HIP_CHECK(hipEventSynchronize(step_ev[step_ev_used - 1]));
+        for (size_t b = 0; b + 3 <= step_ev_used; b += 3) {  // complete triples only
+            float m1 = 0.f, m2 = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&m1, step_ev[b], step_ev[b + 1]));
+            HIP_CHECK(hipEventElapsedTime(&m2, step_ev[b + 1], step_ev[b + 2]));
+            sum_collision_ms += m1;
+            sum_iterations_ms += m2;
+            stats.timed_steps += 1;
+        }
+        step_ev_used = 0;
+    }
 
     // ---- adaptive iteration count and warm starts
     int ac_cap = 0;                 // entries of ac_sum / ac_max
@@ -234,6 +268,7 @@ struct pdfc_sim {
         if (host_nc) (void)hipHostFree(host_nc);
         for (auto& e : ev_ac)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : step_ev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -265,6 +300,8 @@ struct pdfc_sim {
     void collect_kernel_times();
     void to_device_vec(const double* host3n, double* dev);
     void from_device_vec(const double* dev, double* host3n);
+    std::vector<unsigned char> checkpoint_bytes();
+    void restore_checkpoint(const unsigned char* p, size_t size);
     void solve_inplace(double* X);
     void run_step();
     void detect();
@@ -354,6 +391,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     }
     nobs = (int)obstacles.size();
     n_scenes = (int)count;
+    const auto t_lhs = std::chrono::steady_clock::now();
     if (count == 1) {
         pdfc::build_model(sc, model);
     } else {
@@ -361,6 +399,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         for (size_t k = 0; k < count; ++k) pdfc::build_model(scenes[k], parts[k]);
         pdfc::merge_models(parts, model);
     }
+    lhs_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_lhs).count();
     n = model.n;
     scene_v0.assign(1, 0);
     for (size_t k = 0; k < count; ++k) scene_v0.push_back(scene_v0.back() + (int)scenes[k].n_vertices);
@@ -1045,6 +1084,151 @@ void pdfc_sim::from_device_vec(const double* d, double* h) {
         for (int c = 0; c < 3; ++c) h[3 * (size_t)fac.perm[k] + c] = planar[(size_t)c * n + k];
 }
 
+// ------------------------------------------------------------------ checkpoint
+namespace {
+const unsigned char kCkptTag[8] = {'P', 'D', 'F', 'C', 'C', 'K', 'P', 'T'};
+constexpr uint32_t kCkptVersion = 1;
+
+uint64_t fnv1a64(const unsigned char* p, size_t n) {
+    uint64_t h = 14695981039346656037ULL;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+struct ByteWriter {
+    std::vector<unsigned char> b;
+    template <typename T>
+    void put(const T& v) { put_array(&v, 1); }
+    template <typename T>
+    void put_array(const T* v, size_t count) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(v);
+        b.insert(b.end(), p, p + count * sizeof(T));
+    }
+};
+
+struct ByteReader {
+    const unsigned char* p;
+    size_t n, at = 0;
+    template <typename T>
+    T get() {
+        T v;
+        get_array(&v, 1);
+        return v;
+    }
+    template <typename T>
+    void get_array(T* out, size_t count) {
+        if (count > (n - at) / sizeof(T)) throw std::invalid_argument("checkpoint truncated");
+        std::memcpy(out, p + at, count * sizeof(T));
+        at += count * sizeof(T);
+    }
+};
+}  // namespace
+
+std::vector<unsigned char> pdfc_sim::checkpoint_bytes() {
+    ByteWriter w;
+    w.put_array(kCkptTag, 8);
+    w.put<uint32_t>(kCkptVersion);
+    w.put<uint32_t>(0);
+    const size_t slots = prev_g ? (size_t)std::max(1, max_local_obs) : 0;
+    w.put<int64_t>(n);
+    w.put<int64_t>(nobs);
+    w.put<int64_t>((int64_t)meshes.size());
+    w.put<int64_t>((int64_t)slots);
+    w.put<int64_t>(steps_done);
+    std::vector<double> h(3 * (size_t)n);
+    from_device_vec(qn, h.data());
+    w.put_array(h.data(), h.size());
+    from_device_vec(vstate, h.data());
+    w.put_array(h.data(), h.size());
+    std::vector<double> clock(nobs);
+    if (nobs > 0) {
+        HIP_CHECK(hipMemcpyAsync(clock.data(), dob.clock, nobs * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    w.put_array(clock.data(), clock.size());
+    for (const HostMesh& hm : meshes) {
+        w.put<double>(hm.time);
+        w.put<int64_t>(hm.next);
+        w.put<int64_t>(hm.nv);
+        w.put_array(hm.cur.data(), hm.cur.size());
+    }
+    if (slots > 0) {  // device numbering -> OBJ vertex order
+        std::vector<float> dev(3 * (size_t)n * slots), out(dev.size());
+        HIP_CHECK(hipMemcpyAsync(dev.data(), prev_g, dev.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (size_t sl = 0; sl < slots; ++sl)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < 3; ++d)
+                    out[3 * (sl * n + fac.perm[k]) + d] = dev[3 * (sl * n + k) + d];
+        w.put_array(out.data(), out.size());
+    }
+    w.put<uint64_t>(fnv1a64(w.b.data(), w.b.size()));
+    return w.b;
+}
+
+void pdfc_sim::restore_checkpoint(const unsigned char* p, size_t size) {
+    if (size < 8 + 8 + 5 * 8 + 8) throw std::invalid_argument("checkpoint truncated");
+    uint64_t sum;
+    std::memcpy(&sum, p + size - 8, 8);
+    ByteReader r{p, size - 8};
+    unsigned char tag[8];
+    r.get_array(tag, 8);
+    if (std::memcmp(tag, kCkptTag, 8) != 0) throw std::invalid_argument("not a checkpoint");
+    if (r.get<uint32_t>() != kCkptVersion) throw std::invalid_argument("unsupported checkpoint version");
+    r.get<uint32_t>();
+    if (fnv1a64(p, size - 8) != sum) throw std::invalid_argument("checkpoint checksum mismatch");
+    const int64_t cn = r.get<int64_t>(), cobs = r.get<int64_t>(), cmesh = r.get<int64_t>();
+    const int64_t cslots = r.get<int64_t>(), csteps = r.get<int64_t>();
+    if (cn != n || cobs != nobs || cmesh != (int64_t)meshes.size())
+        throw std::invalid_argument("checkpoint of another scene (vertex, obstacle or mesh count)");
+    const size_t slots = prev_g ? (size_t)std::max(1, max_local_obs) : 0;
+    if (cslots != 0 && slots != 0 && cslots != (int64_t)slots)
+        throw std::invalid_argument("checkpoint of another scene (obstacle slots)");
+    std::vector<double> q(3 * (size_t)n), v(3 * (size_t)n), clock(nobs);
+    r.get_array(q.data(), q.size());
+    r.get_array(v.data(), v.size());
+    r.get_array(clock.data(), clock.size());
+    std::vector<double> mtime(meshes.size());
+    std::vector<int64_t> mnext(meshes.size());
+    std::vector<std::vector<double>> mcur(meshes.size());
+    for (size_t m = 0; m < meshes.size(); ++m) {
+        mtime[m] = r.get<double>();
+        mnext[m] = r.get<int64_t>();
+        if (r.get<int64_t>() != meshes[m].nv) throw std::invalid_argument("checkpoint of another scene (mesh size)");
+        if (mnext[m] < 1 || mnext[m] >= std::max(2, meshes[m].frames_count()))
+            throw std::invalid_argument("checkpoint mesh frame out of range");
+        mcur[m].resize(meshes[m].cur.size());
+        r.get_array(mcur[m].data(), mcur[m].size());
+    }
+    std::vector<float> g((size_t)cslots * 3 * n);
+    r.get_array(g.data(), g.size());
+    if (r.at != r.n) throw std::invalid_argument("checkpoint has trailing bytes");
+    // validated: apply
+    to_device_vec(q.data(), qn);
+    to_device_vec(v.data(), vstate);
+    if (nobs > 0) HIP_CHECK(hipMemcpyAsync(dob.clock, clock.data(), nobs * sizeof(double), hipMemcpyHostToDevice, stream));
+    for (size_t m = 0; m < meshes.size(); ++m) {
+        meshes[m].time = mtime[m];
+        meshes[m].next = (int)mnext[m];
+        meshes[m].cur = std::move(mcur[m]);
+        meshes[m].dirty = true;
+    }
+    if (slots > 0) {
+        std::vector<float> dev(3 * (size_t)n * slots, 0.f);
+        if (cslots > 0)
+            for (size_t sl = 0; sl < slots; ++sl)
+                for (int k = 0; k < n; ++k)
+                    for (int d = 0; d < 3; ++d) dev[3 * (sl * n + k) + d] = g[3 * (sl * n + fac.perm[k]) + d];
+        HIP_CHECK(hipMemcpyAsync(prev_g, dev.data(), dev.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    given_pending = false;
+    steps_done = csteps;
+}
+
 // K x = b in place (X holds b on entry): forward levels leaves -> roots, backward
 // levels roots -> leaves; FP32 operators add FP64 residual corrections.
 void pdfc_sim::solve_inplace(double* X) {
@@ -1434,6 +1618,7 @@ void pdfc_sim::run_step() {
     }
     if (mode == PDFC_WARM_LEARNED && !mlp)
         throw std::invalid_argument("learned warm start without a network (pdfc_set_mlp)");
+    step_event(0);
     if (a.record) {
         pdfc::launch_fill(ac_sum, ac_cap, 0., stream);
         pdfc::launch_fill_u64(ac_max, ac_cap, stream);
@@ -1477,6 +1662,7 @@ void pdfc_sim::run_step() {
     int to_tol = -1;      // iterations to reach the tolerance (counting abandoned ones)
     bool fell_back = false;
     int it = 0;
+    step_event(1);
     for (;;) {
         int hold_end = mode != PDFC_WARM_PLAIN ? hold : 0;  // iterations < hold_end use the warm reaction
         int copied = 0;                   // residuals [0, copied) requested from the device
@@ -1556,6 +1742,7 @@ void pdfc_sim::run_step() {
         HIP_CHECK(hipStreamSynchronize(stream));
         if (host_ac[K - 1] / total < prm.residual_tolerance) to_tol = base + K;
     }
+    step_event(2);
     HIP_CHECK(hipGetLastError());
     mesh_advance(prm.time_step);
     stats.iterations += base + K;
@@ -1572,9 +1759,12 @@ void pdfc_sim::run_step() {
 void pdfc_sim::graph_step() {
     if (!graph_exec) {
         HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        capturing = true;
         try {
             run_step();
+            capturing = false;
         } catch (...) {
+            capturing = false;
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -1664,6 +1854,9 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         sim->self_host_us = 0.;
         sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
         sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
+        sim->stats.timed_steps = 0;
+        sim->step_ev_used = 0;
+        sim->sum_collision_ms = sim->sum_iterations_ms = 0.;
         const bool graph = sim->graph_capturable();
         for (int s = 0; s < nsteps; ++s) {
             if (graph)
@@ -1678,6 +1871,10 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         float ms_total = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
         sim->stats.step_us = 1e3 * ms_total / nsteps;
+        sim->flush_step_events();
+        const double ts = (double)std::max<int64_t>(1, sim->stats.timed_steps);
+        sim->stats.collision_us = 1e3 * sim->sum_collision_ms / ts;
+        sim->stats.iterations_us = 1e3 * sim->sum_iterations_ms / ts;
         sim->stats.detection_us = sim->self_host_us / nsteps;
         sim->stats.steps = nsteps;
         sim->steps_done += nsteps;
@@ -1701,6 +1898,23 @@ int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
             HIP_CHECK(hipMemsetAsync(sim->prev_g, 0, 3 * (size_t)sim->n * std::max(1, sim->max_local_obs) * sizeof(float),
                                      sim->stream));
         HIP_CHECK(hipStreamSynchronize(sim->stream));
+    });
+}
+
+int pdfc_save_checkpoint(pdfc_sim* sim, void* buf, size_t cap, size_t* n) {
+    return guarded([&] {
+        if (!sim || !n) throw std::invalid_argument("null argument");
+        std::vector<unsigned char> b = sim->checkpoint_bytes();
+        *n = b.size();
+        if (!buf || cap < b.size()) throw CapacityError("checkpoint buffer too small");
+        std::memcpy(buf, b.data(), b.size());
+    });
+}
+
+int pdfc_load_checkpoint(pdfc_sim* sim, const void* buf, size_t n) {
+    return guarded([&] {
+        if (!sim || !buf) throw std::invalid_argument("null argument");
+        sim->restore_checkpoint(static_cast<const unsigned char*>(buf), n);
     });
 }
 
@@ -1946,6 +2160,8 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.n_self_levels = sim->n_self_levels;
         I.max_front = sim->fac.max_front;
         I.factor_seconds = sim->fac.seconds;
+        I.lhs_seconds = sim->lhs_seconds;
+        I.steps_done = sim->steps_done;
         // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
         //   local kernel: read v, q_n, c and write b (4 x 24 B per vertex); per spring end an
         //   index and kappa (4 + 8 B); per contact the basis (72 B), mu (8 B), r read+write

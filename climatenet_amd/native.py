@@ -18,7 +18,7 @@ EXPORTS = [
     "pdfc_get_self_contacts", "pdfc_get_self_candidates", "pdfc_detect", "pdfc_get_iteration_stats",
     "pdfc_get_phase_stats", "pdfc_get_info", "pdfc_solve", "pdfc_destroy", "pdfc_last_error",
     "pdfc_set_kernel_timing", "pdfc_get_kernel_times", "pdfc_set_mlp", "pdfc_get_warm_data",
-    "pdfc_set_warm_reactions",
+    "pdfc_set_warm_reactions", "pdfc_save_checkpoint", "pdfc_load_checkpoint",
     "pdfc_mlp_create", "pdfc_mlp_destroy", "pdfc_mlp_set_params", "pdfc_mlp_get_params",
     "pdfc_mlp_forward", "pdfc_mlp_gradient", "pdfc_mlp_train",
 ]
@@ -60,6 +60,8 @@ def lib():
         L.pdfc_set_mlp.argtypes = [P, P]
         L.pdfc_set_warm_reactions.argtypes = [P, C.POINTER(abi.Contact), C.c_size_t, C.c_int]
         L.pdfc_get_warm_data.argtypes = [P, F, F, C.c_size_t, SZ]
+        L.pdfc_save_checkpoint.argtypes = [P, C.c_void_p, C.c_size_t, SZ]
+        L.pdfc_load_checkpoint.argtypes = [P, C.c_void_p, C.c_size_t]
         L.pdfc_mlp_create.argtypes = [C.POINTER(P), C.c_uint64]
         L.pdfc_mlp_destroy.argtypes = [P]
         L.pdfc_mlp_destroy.restype = None
@@ -147,6 +149,19 @@ class Simulation:
         v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
         _check(lib().pdfc_set_state(self.h, abi.dptr(q), abi.dptr(v)))
 
+    def save_checkpoint(self) -> bytes:
+        n = C.c_size_t()
+        code = lib().pdfc_save_checkpoint(self.h, None, 0, C.byref(n))
+        if code != abi.PDFC_ERR_CAPACITY:
+            _check(code)
+        buf = C.create_string_buffer(n.value)
+        _check(lib().pdfc_save_checkpoint(self.h, buf, n.value, C.byref(n)))
+        return buf.raw[: n.value]
+
+    def load_checkpoint(self, data: bytes):
+        buf = C.create_string_buffer(bytes(data), len(data))
+        _check(lib().pdfc_load_checkpoint(self.h, buf, len(data)))
+
     def _list(self, fn, T):
         n = C.c_size_t()
         code = fn(self.h, None, 0, C.byref(n))
@@ -183,6 +198,17 @@ class Simulation:
             v = getattr(I, f)
             out[f] = list(v) if hasattr(v, "__len__") else v
         return out
+
+    def tuning(self) -> str:
+        """The chosen substitution schedule as a PDFC_TUNE value: another simulation of the same
+        scene created with it set sums in the same order (bit-identical results)."""
+        I = self.info()
+        nl = I["n_levels_swept"]
+        if nl > len(I["waves_forward"]):
+            raise ValueError("more levels than pdfc_info reports")
+        f = ",".join(str(w) for w in I["waves_forward"][:nl])
+        b = ",".join(str(w) for w in I["waves_backward"][:nl])
+        return "f=%s;b=%s;t=%d" % (f, b, max(1, I["top_chunks"]))
 
     def phase_stats(self):
         P = abi.PhaseStats()

@@ -210,12 +210,14 @@ typedef struct pdfc_info {
     double best_pair_ms;             /* and with the chosen schedule */
     int32_t leaf_size;               /* largest leaf supernode of the dissection */
     int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned) */
+    double lhs_seconds;              /* host time to build topology, masses, constraints and A^T A (K) */
+    int64_t steps_done;              /* steps taken since creation (restored by pdfc_load_checkpoint) */
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
     double step_us;
     double detection_us;             /* host time per step of the self-collision round trip and pairing */
-    double iterations_us;
+    double iterations_us;            /* device time per step of its local-global iterations and epilogue */
     int64_t steps;
     int64_t iterations;              /* local-global iterations executed, all steps of the call */
     int64_t iterations_to_tolerance; /* adaptive mode: sum over steps with contacts of the first iteration
@@ -223,6 +225,10 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
     int64_t contact_steps;           /* steps of the call with at least one contact */
     int64_t converged_steps;         /* of which reached the tolerance */
     int64_t fallbacks;               /* learned warm starts abandoned for the plain one */
+    double collision_us;             /* device time per step from its start to its first iteration (prologue,
+                                        obstacle and self detection, pairing); steps replayed from a HIP
+                                        graph are not timed (then 0, and iterations_us 0) */
+    int64_t timed_steps;             /* steps of the call that collision_us and iterations_us average over */
 } pdfc_phase_stats;
 
 typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over an instrumented run */
@@ -254,6 +260,21 @@ int pdfc_get_state(pdfc_sim* sim, double* q, double* v);
 
 /* Overwrite the state (3n each; either may be NULL). Used for identical-state parity and checkpoints. */
 int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v);
+
+/* Binary checkpoint of the simulation state (no counterpart in the reference, whose
+ * "Current state obj filename" restores positions only, as float, with zero velocity):
+ * positions and velocities (FP64, OBJ vertex order), the obstacle clocks (moving meshes:
+ * time, key frame and current positions; rotating spheres: time before the rotation), the
+ * previous-step reactions kept for the PREVIOUS/LEARNED warm starts and the step count,
+ * behind a format tag and version and followed by a 64-bit FNV-1a checksum. A simulation
+ * built from the same scenes and parameters continues from it as the saved one would: bit
+ * for bit with the same substitution schedule (the same simulation, or PDFC_TUNE pinned), to
+ * rounding otherwise (the autotuner may choose another summation order).
+ * save: with buf NULL or cap too small, returns PDFC_ERR_CAPACITY and *n = the size needed.
+ * load: PDFC_ERR_INVALID, with the state untouched, if the buffer is not a checkpoint of a
+ * simulation of this shape (tag, version, vertex/obstacle/mesh counts, size, checksum). */
+int pdfc_save_checkpoint(pdfc_sim* sim, void* buf, size_t cap, size_t* n);
+int pdfc_load_checkpoint(pdfc_sim* sim, const void* buf, size_t n);
 
 /* Obstacle contacts of the last step, in the reference's order (obstacle, then vertex). */
 int pdfc_get_contacts(pdfc_sim* sim, pdfc_contact* out, size_t cap, size_t* n);

@@ -12,6 +12,14 @@
 
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
+    if (std::string(argv[1]) == "--stats" && argc >= 3) {  // computeAndWriteStats on fixed vectors
+        std::ofstream of(argv[2]);
+        computeAndWriteStats("Empty", of, {});
+        computeAndWriteStats("One", of, {2.5});
+        computeAndWriteStats("Three", of, {3., 1., 2.});
+        computeAndWriteStats("Negative", of, {-1., -2.});
+        return 0;
+    }
     try {
         json::Value doc = getJSONDocumentFromFilename(argv[1]);
         PhysicScene scene = getSceneFromJSONDocument(doc);

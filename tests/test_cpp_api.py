@@ -153,3 +153,18 @@ def test_box_and_rotating_sphere_readers(api_check, tmp_path):
     assert int(f[0]) == 3 and f[6:12] == [-0.25, -0.25, -0.5, 0.5, 0.5, 0.75]
     f = [float(x) for x in out["obstacle1"].split()]
     assert int(f[0]) == 4 and f[6:12] == [0.25, 0.0, 0.0, 2.0, 3.0, 0.1]
+
+
+def test_stats_writer_format(api_check, tmp_path):
+    """computeAndWriteStats (template/physics/io.tpp:23-63), including its start value
+    std::numeric_limits<double>::min() (the smallest positive double) for the maximum,
+    which an all-negative series reports."""
+    out = tmp_path / "stats.txt"
+    rc, _, raw = run(api_check, "--stats", str(out))
+    assert rc == 0, raw
+    sep = "\n-------------------\n\n"
+    want = ("Empty\n\t(No data)\n" + sep + "One\n\t Value : 2.5\n" + sep +
+            "Three\n\t Number      : 3\n\t Min value   : 1\n\t Mean value  : 2\n\t Max value   : 3\n" + sep +
+            "Negative\n\t Number      : 2\n\t Min value   : -2\n\t Mean value  : -1.5\n"
+            "\t Max value   : 2.22507e-308\n" + sep)
+    assert out.read_text() == want

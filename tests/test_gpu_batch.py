@@ -76,9 +76,10 @@ def test_batch_equals_scenes_run_alone(gpu):
         assert rel_err(v[3 * a:3 * b], v1) < 1e-10
 
 
-def test_graph_replay_is_bit_identical_to_eager(gpu):
+def test_graph_replay_is_bit_identical_to_eager(gpu, monkeypatch):
     spec = scenes.sphere_drape_small()
     eager = gpu.Simulation(spec, record_iterations=False)
+    monkeypatch.setenv("PDFC_TUNE", eager.tuning())  # the same summation order
     graph = gpu.Simulation(spec, record_iterations=False, use_graph=True)
     rng = np.random.default_rng(0)
     for s in range(30):
@@ -92,9 +93,10 @@ def test_graph_replay_is_bit_identical_to_eager(gpu):
     assert graph.phase_stats()["iterations"] == spec.iterations
 
 
-def test_graph_replay_of_a_batch(gpu):
+def test_graph_replay_of_a_batch(gpu, monkeypatch):
     specs = small_batch()
     eager = gpu.BatchSimulation(specs, record_iterations=False)
+    monkeypatch.setenv("PDFC_TUNE", eager.tuning())  # the same summation order
     graph = gpu.BatchSimulation(specs, record_iterations=False, use_graph=True)
     for s in range(3):
         eager.step(7)

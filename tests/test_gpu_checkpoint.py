@@ -1,0 +1,136 @@
+"""GPU: binary checkpoints (pdfc_save_checkpoint / pdfc_load_checkpoint) and the driver's
+output files (OBJ frames, stats.txt sections, --checkpoint-every / --resume).
+
+The checkpoint scene exercises every piece of state the step carries between steps:
+positions and velocities, a moving mesh's clock and key frame, a rotating sphere's start
+clock, and the previous-step reactions of the PREVIOUS warm start."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from climatenet_amd import abi, native, scenes
+from tests.scene_files import write_scene
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MAIN = os.path.join(ROOT, "climatenet_amd", "lib", "pdfc_main")
+
+
+def stateful_scene():
+    spec = scenes.mesh_sphere_drape(moving=True)
+    rs = scenes.rotating_sphere_drape(start=0.08).obstacles[0]
+    rs.point = (0.45, 0.0, -0.05)
+    rs.radius = 0.12
+    spec.obstacles.append(rs)
+    return spec
+
+
+def test_checkpoint_continues_bit_for_bit(gpu, monkeypatch):
+    spec = stateful_scene()
+    sim = gpu.Simulation(spec, warm_start=abi.WARM_PREVIOUS)
+    print("stateful scene: 25 steps, checkpoint, 25 more", flush=True)
+    sim.step(25)
+    ck = sim.save_checkpoint()
+    sim.step(25)
+    qa, va = sim.state()
+    ca = sim.contacts()
+    assert sim.info()["steps_done"] == 50
+    sim.load_checkpoint(ck)
+    assert sim.info()["steps_done"] == 25
+    sim.step(25)
+    qb, vb = sim.state()
+    cb = sim.contacts()
+    assert np.array_equal(qa, qb) and np.array_equal(va, vb)
+    assert [(c.obstacle, c.vertex) for c in ca] == [(c.obstacle, c.vertex) for c in cb]
+    assert np.array_equal(np.array([list(c.speed) for c in ca]), np.array([list(c.speed) for c in cb]))
+    speeds = {c.obstacle: np.abs(list(c.speed)).max() for c in ca}
+    assert max(speeds.values()) > 0.05  # the moving obstacles are moving at the restored clocks
+
+    # a second simulation of the same scene with the same schedule: bit for bit as well
+    monkeypatch.setenv("PDFC_TUNE", sim.tuning())
+    other = gpu.Simulation(spec, warm_start=abi.WARM_PREVIOUS)
+    other.load_checkpoint(ck)
+    other.step(25)
+    qc, vc = other.state()
+    assert np.array_equal(qc, qa) and np.array_equal(vc, va)
+    assert other.info()["steps_done"] == 50
+
+
+def test_checkpoint_rejects_foreign_or_damaged_data(gpu):
+    spec = stateful_scene()
+    sim = gpu.Simulation(spec)
+    sim.step(3)
+    ck = bytearray(sim.save_checkpoint())
+    q0, v0 = sim.state()
+    bad = [bytes(ck[:-9]), bytes(ck[:40])]
+    flipped = bytearray(ck)
+    flipped[100] ^= 1
+    bad.append(bytes(flipped))
+    wrong_version = bytearray(ck)
+    wrong_version[8] = 99
+    bad.append(bytes(wrong_version))
+    small = gpu.Simulation(scenes.sphere_drape_small())
+    bad_other = small.save_checkpoint()
+    for data in bad + [bad_other]:
+        with pytest.raises(native.PdfcError) as e:
+            sim.load_checkpoint(data)
+        assert e.value.code == abi.PDFC_ERR_INVALID
+    q1, v1 = sim.state()
+    assert np.array_equal(q0, q1) and np.array_equal(v0, v1)  # untouched
+
+
+def run_main(cwd, *args):
+    r = subprocess.run([MAIN] + list(args), cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def obj_positions(path):
+    return np.array([[float(x) for x in l.split()[1:4]] for l in open(path) if l.startswith("v ")])
+
+
+def test_driver_outputs_and_resume(gpu, tmp_path):
+    spec = scenes.rotating_sphere_drape(start=0.02)
+    spec.positions = spec.rest.copy()
+    scene = write_scene(str(tmp_path), spec, frames=4, steps_per_frame=2)
+    doc = json.load(open(scene))
+    doc["Parameters"]["Frame number"] = 2
+    half = os.path.join(str(tmp_path), "half.json")
+    json.dump(doc, open(half, "w"))
+    full_dir, part_dir = tmp_path / "full", tmp_path / "part"
+    full_dir.mkdir()
+    part_dir.mkdir()
+    print("driver: 8 steps straight, then 4 + resume 4", flush=True)
+    run_main(str(full_dir), scene)
+    run_main(str(part_dir), half, "--checkpoint-every", "4")
+    ck = part_dir / "data0" / "checkpoint.bin"
+    assert ck.exists()
+    r = run_main(str(part_dir), scene, "--resume", str(ck))
+    assert "Resumed from" in r.stdout and "at step 4" in r.stdout
+    full = full_dir / "data0"
+    frames = sorted(os.listdir(full / "output"))
+    assert frames == ["out_%06d.obj" % k for k in range(4)]
+    # the resumed run writes frames 2 and 3 into its own data1/ (frames keep their numbers)
+    resumed = part_dir / "data1" / "output"
+    assert sorted(os.listdir(resumed)) == ["out_000002.obj", "out_000003.obj"]
+    for name in ("out_000002.obj", "out_000003.obj"):
+        a, b = obj_positions(full / "output" / name), obj_positions(resumed / name)
+        # another process may tune another summation order: equal to the OBJ's 6 digits
+        assert a.shape == (441, 3) and np.abs(a - b).max() <= 2e-6
+    # OBJ faces: 1-based, the scene's triangles
+    faces = [list(map(int, l.split()[1:4])) for l in open(full / "output" / frames[0]) if l.startswith("f ")]
+    assert np.array_equal(np.array(faces) - 1, spec.triangles.reshape(-1, 3))
+    # stats.txt: the reference's sections in its order (no self collision)
+    names = [l.strip() for l in open(full / "stats.txt") if l.strip() and not l.startswith("\t") and
+             not l.startswith("---")]
+    assert names == ["LHS computation time (µs)", "Cholesky computation time (µs)", "Iteration time (µs)",
+                     "Collision detection time (µs)", "Number of self collisions", "Solver time (µs)",
+                     "Full RHS computation time (µs)", "Step time (µs)", "RHS base computation time (µs)",
+                     "RHS friction overhead (µs)"]
+    text = open(full / "stats.txt").read()
+    assert "Iteration time (µs)\n\t Number      : 80\n" in text  # 8 steps x 10 iterations
+    assert "Step time (µs)\n\t Number      : 8\n" in text
