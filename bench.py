@@ -121,6 +121,48 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mlp(args):
+    """The learned part's network alone: training (forward + MSE gradient + backward + Adam,
+    BF16 MFMA with FP32 accumulation) and inference on 262,144 synthetic contact rows. Rates
+    include the host-device copies of each call; the kernels' own times and MFMA counters come
+    from rocprofv3 on this workload."""
+    import numpy as np
+    from climatenet_amd import abi, native
+    rows = 1 << 18
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((rows, abi.MLP_FEATURES)).astype(np.float32)
+    t = np.tanh(x[:, :3] * x[:, 3:6] + 0.5 * x[:, 6:9]).astype(np.float32)
+    mlp = native.ContactMLP(seed=0)
+    norm = np.concatenate([np.zeros(12), np.ones(12), np.ones(3)]).astype(np.float32)
+    p, _ = mlp.params()
+    mlp.set_params(p, norm)
+    mlp.forward(x[:4096])
+    mlp.train(x[:4096], t[:4096], 1, batch=512)
+    epochs = max(1, args.steps // 50)
+    t0 = time.perf_counter()
+    losses = mlp.train(x, t, epochs, batch=512, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        mlp.forward(x)
+    fwd_s = (time.perf_counter() - t0) / reps
+    flop_fwd = 2 * (12 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # 18,304 per row
+    out = {
+        "metric": "MLP training rows/s (contact warm-start network)",
+        "value": round(rows * epochs / train_s, 1), "unit": "rows/s", "n_gpus": 1, "steps": epochs,
+        "warmup": 1, "ms_per_step": round(1e3 * train_s / epochs, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "12-64-64-64-3 SiLU MLP, %d rows, batch 512, Adam" % rows, "epochs": epochs},
+        "inference_rows_per_s": round(rows / fwd_s, 1),
+        "flop_per_row": {"forward": flop_fwd, "training": 3 * flop_fwd},
+        "end_to_end_tflops": {"training": round(3 * flop_fwd * rows * epochs / train_s / 1e12, 4),
+                              "inference": round(flop_fwd * rows / fwd_s / 1e12, 4)},
+        "final_loss": float(losses[-1]),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,8 +175,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
-    ap.add_argument("--workload", choices=["drape", "batched"], default="drape",
-                    help="drape: the 40k-vertex drape (default); batched: 64 independent cloths")
+    ap.add_argument("--workload", choices=["drape", "batched", "mlp"], default="drape",
+                    help="drape: the 40k-vertex drape (default); batched: 64 independent cloths; "
+                         "mlp: the warm-start network's training and inference alone")
     ap.add_argument("--no-graph", action="store_true", help="batched: launch kernels eagerly")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -153,6 +196,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "mlp":
+        bench_mlp(args)
         return
 
     from climatenet_amd import abi, native, scenes

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -51,17 +52,15 @@ __device__ __forceinline__ f4 mfma_tile(const unsigned short* A, int lda, const 
     const int l = threadIdx.x & 63;
     const int r = l & 15, kq = (l >> 4) * 4;
     for (int k0 = 0; k0 < K; k0 += 16) {
-        s4 a, b;
-        const unsigned short* ap = A + r * lda + k0 + kq;
-        const unsigned short* bp = Bt + (size_t)(n0 + r) * K + k0 + kq;
-        a[0] = (short)ap[0]; a[1] = (short)ap[1]; a[2] = (short)ap[2]; a[3] = (short)ap[3];
-        b[0] = (short)bp[0]; b[1] = (short)bp[1]; b[2] = (short)bp[2]; b[3] = (short)bp[3];
+        // 4 consecutive bf16 (8-byte aligned: lda, K and k0 + kq are multiples of 4)
+        const s4 a = *reinterpret_cast<const s4*>(A + r * lda + k0 + kq);
+        const s4 b = *reinterpret_cast<const s4*>(Bt + (size_t)(n0 + r) * K + k0 + kq);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     return acc;
 }
 
-struct Tile {
+struct alignas(16) Tile {
     unsigned short X[16 * 16];      // normalised input, bf16
     unsigned short H[3][16 * 64];   // silu(Z_l), bf16
     float Z[3][16 * 64];            // pre-activations
@@ -109,18 +108,92 @@ __device__ void load_inputs(Tile& T, const MlpDev& P, const float* __restrict__ 
     __syncthreads();
 }
 
-// Inference: y = out_scale * MLP((x - mean) / std); one wave per 16 rows.
+// Inference: y = out_scale * MLP((x - mean) / std). One wave per workgroup walks 16-row
+// tiles with a grid stride; it loads every weight operand fragment (40 x 4 bf16, the B
+// operands of all layers) and its biases into registers once and reuses them for each
+// tile. Activations pass through LDS between layers (D layout -> A layout). The
+// arithmetic (k-block order, bias, SiLU, rounding) is that of tile_forward.
+constexpr int kMlpForwardGrid = 2048;
+
 __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev P, const float* __restrict__ x, int nrows,
                                                     float* __restrict__ y) {
-    __shared__ Tile T;
-    const int row0 = blockIdx.x * 16;
-    const int nr = min(16, nrows - row0);
-    load_inputs(T, P, x, row0, nr);
-    tile_forward(T, P);
+    __shared__ alignas(16) unsigned short X[16 * 16];
+    __shared__ alignas(16) unsigned short H[2][16 * 64];
     const int l = threadIdx.x;
-    if (l < 16 * kMlpOut) {
-        int r = l / kMlpOut, c = l % kMlpOut;
-        if (r < nr) y[(size_t)(row0 + r) * kMlpOut + c] = T.Y[r * 16 + c] * P.y_scale[c];
+    const int r = l & 15, kq = (l >> 4) * 4, col = l & 15, rq = (l >> 4) * 4;
+    s4 w0[4], w1[4][4], w2[4][4], w3[4];
+    float b0[4], b1[4], b2[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        w0[nt] = *reinterpret_cast<const s4*>(P.wt_bf[0] + (size_t)(nt * 16 + r) * kMlpIn + kq);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            w1[nt][ks] = *reinterpret_cast<const s4*>(P.wt_bf[1] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 16 + kq);
+            w2[nt][ks] = *reinterpret_cast<const s4*>(P.wt_bf[2] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 16 + kq);
+        }
+        w3[nt] = *reinterpret_cast<const s4*>(P.wt_bf[3] + (size_t)r * kMlpHidden + nt * 16 + kq);
+        b0[nt] = P.b[0][nt * 16 + col];
+        b1[nt] = P.b[1][nt * 16 + col];
+        b2[nt] = P.b[2][nt * 16 + col];
+    }
+    const float b3 = P.b[3][col];
+    const float ys = col < kMlpOut ? P.y_scale[col] : 0.f;
+    const int ntiles = (nrows + 15) / 16;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int row0 = tile * 16;
+        const int nr = min(16, nrows - row0);
+        for (int e = l; e < 16 * 16; e += 64) {
+            const int rr = e >> 4, c = e & 15;
+            float v = 0.f;
+            if (rr < nr && c < kMlpFeatures) v = (x[(size_t)(row0 + rr) * kMlpFeatures + c] - P.x_mean[c]) / P.x_std[c];
+            X[e] = to_bf16(v);
+        }
+        __syncthreads();
+        {  // layer 1
+            const s4 a = *reinterpret_cast<const s4*>(X + r * 16 + kq);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w0[nt], acc, 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) H[0][(rq + j) * 64 + nt * 16 + col] = to_bf16(silu(acc[j] + b0[nt]));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int layer = 1; layer < 3; ++layer) {  // layers 2 and 3
+            const unsigned short* in = H[layer - 1];
+            unsigned short* out = H[layer & 1 ? 1 : 0];
+            s4 a[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const s4*>(in + r * 64 + ks * 16 + kq);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], layer == 1 ? w1[nt][ks] : w2[nt][ks], acc,
+                                                                    0, 0, 0);
+                const float bb = layer == 1 ? b1[nt] : b2[nt];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) out[(rq + j) * 64 + nt * 16 + col] = to_bf16(silu(acc[j] + bb));
+            }
+            __syncthreads();
+        }
+        {  // output layer from H[0] (layer 3's activations)
+            s4 a[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const s4*>(H[0] + r * 64 + ks * 16 + kq);
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], w3[ks], acc, 0, 0, 0);
+            if (col < kMlpOut) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (rq + j < nr) y[(size_t)(row0 + rq + j) * kMlpOut + col] = (acc[j] + b3) * ys;
+            }
+        }
+        __syncthreads();  // X and H are rewritten by the next tile
     }
 }
 
@@ -263,7 +336,9 @@ __global__ void k_mlp_refresh(const float* __restrict__ w, MlpDev P) {
 }  // namespace
 
 void launch_mlp_forward(const MlpDev& P, const float* x, int rows, float* y, hipStream_t st) {
-    if (rows > 0) hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((rows + 15) / 16)), dim3(64), 0, st, P, x, rows, y);
+    if (rows > 0)
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min((rows + 15) / 16, kMlpForwardGrid)), dim3(64), 0, st,
+                           P, x, rows, y);
 }
 
 }  // namespace pdfc
@@ -432,8 +507,7 @@ int pdfc_mlp_forward(pdfc_mlp* M, const float* x, size_t rows, float* y) {
         ensure_io(M, rows);
         chk(hipMemcpyAsync(M->xbuf, x, rows * pdfc::kMlpFeatures * sizeof(float), hipMemcpyHostToDevice, M->stream),
             "copy");
-        hipLaunchKernelGGL(pdfc::k_mlp_forward, dim3((unsigned)((rows + 15) / 16)), dim3(64), 0, M->stream, M->dev,
-                           M->xbuf, (int)rows, M->tbuf);
+        pdfc::launch_mlp_forward(M->dev, M->xbuf, (int)rows, M->tbuf, M->stream);
         chk(hipMemcpyAsync(y, M->tbuf, rows * pdfc::kMlpOut * sizeof(float), hipMemcpyDeviceToHost, M->stream), "copy");
         chk(hipStreamSynchronize(M->stream), "sync");
     });
