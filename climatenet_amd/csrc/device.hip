@@ -539,10 +539,15 @@ __global__ void k_finish(DevModel M, double* __restrict__ qn, const double* __re
                 ac_m = fmax(ac_m, an);
             }
         }
-        qn[i] += a.dt * v.x;
-        qn[n + i] += a.dt * v.y;
-        qn[2 * n + i] += a.dt * v.z;
+        const double qx = qn[i] + a.dt * v.x, qy = qn[n + i] + a.dt * v.y, qz = qn[2 * n + i] + a.dt * v.z;
+        qn[i] = qx;
+        qn[n + i] = qy;
+        qn[2 * n + i] = qz;
         st3(vstate, n, i, v);
+        // failure detection (no counterpart in the reference, which steps on with NaNs)
+        if (a.nonfinite && !(isfinite(qx) && isfinite(qy) && isfinite(qz) && isfinite(v.x) && isfinite(v.y) &&
+                             isfinite(v.z)))
+            *a.nonfinite = 1;
         if (a.prev_g) {
             int o0, o1;
             obstacle_range(ob, i, o0, o1);

@@ -137,6 +137,7 @@ struct StepArgs {
     int warm_stride;
     const float* warm_y;
     float* prev_g;     // end of step: world reaction per (scene obstacle slot, vertex), 3 * n each; null: not kept
+    int* nonfinite;    // set to 1 by the step's last kernel when a new position or velocity is not finite
 };
 
 // Self-collision detection buffers (scene numbering for vertices and triangles).

@@ -191,6 +191,8 @@ struct pdfc_sim {
     int last_records = 0;           // iterations recorded in ac_sum by the last step
     double* host_ac = nullptr;      // pinned: per-iteration residual sums read back in the adaptive mode
     int* host_nc = nullptr;         // pinned: obstacle contact count of the step
+    int* d_nonfinite = nullptr;     // set by k_finish when the new state is not finite
+    int* host_nonfinite = nullptr;  // pinned copy, read after each pdfc_step
     hipEvent_t ev_ac[2] = {nullptr, nullptr};
     bool given_pending = false;     // pdfc_set_warm_reactions: warm start of the next step
     int given_hold = 1;
@@ -266,6 +268,7 @@ struct pdfc_sim {
         if (graph) (void)hipGraphDestroy(graph);
         if (host_ac) (void)hipHostFree(host_ac);
         if (host_nc) (void)hipHostFree(host_nc);
+        if (host_nonfinite) (void)hipHostFree(host_nonfinite);
         for (auto& e : ev_ac)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : step_ev) (void)hipEventDestroy(e);
@@ -617,6 +620,9 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     ac_max = dalloc<unsigned long long>(ac_cap);
     HIP_CHECK(hipHostMalloc((void**)&host_ac, (size_t)ac_cap * sizeof(double), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void**)&host_nc, sizeof(int), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&host_nonfinite, sizeof(int), hipHostMallocDefault));
+    *host_nonfinite = 0;
+    d_nonfinite = upload(std::vector<int>(1, 0));
     for (auto& e : ev_ac) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     warm_data = p.warm_start != PDFC_WARM_PLAIN || p.residual_tolerance > 0.;
     feat = dalloc<float>((size_t)pdfc::kWarmFeatures * cap);
@@ -1606,6 +1612,7 @@ void pdfc_sim::run_step() {
     a.record = (prm.record_iterations || adaptive) ? 1 : 0;
     a.ac_sum = ac_sum;
     a.ac_max = ac_max;
+    a.nonfinite = d_nonfinite;
     const bool given = given_pending;
     given_pending = false;
     const bool features = warm_data || given;
@@ -1865,6 +1872,8 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
                 sim->run_step();
             if (sim->ktiming) sim->collect_kernel_times();
         }
+        HIP_CHECK(hipMemcpyAsync(sim->host_nonfinite, sim->d_nonfinite, sizeof(int), hipMemcpyDeviceToHost,
+                                 sim->stream));
         HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
         HIP_CHECK(hipEventSynchronize(sim->ev[2]));
         sim->check_flow_error();
@@ -1878,6 +1887,14 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         sim->stats.detection_us = sim->self_host_us / nsteps;
         sim->stats.steps = nsteps;
         sim->steps_done += nsteps;
+        if (*sim->host_nonfinite) {
+            // cleared for the next call; the state stays as computed
+            HIP_CHECK(hipMemsetAsync(sim->d_nonfinite, 0, sizeof(int), sim->stream));
+            HIP_CHECK(hipStreamSynchronize(sim->stream));
+            *sim->host_nonfinite = 0;
+            throw NumericError("non-finite positions or velocities within steps " +
+                               std::to_string(sim->steps_done - nsteps + 1) + ".." + std::to_string(sim->steps_done));
+        }
     });
 }
 
@@ -1892,6 +1909,13 @@ int pdfc_get_state(pdfc_sim* sim, double* q, double* v) {
 int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
     return guarded([&] {
         if (!sim) throw std::invalid_argument("null sim");
+        for (const double* a : {q, v}) {
+            if (!a) continue;
+            for (size_t k = 0; k < 3 * (size_t)sim->n; ++k)
+                if (!std::isfinite(a[k]))
+                    throw std::invalid_argument(std::string(a == q ? "positions" : "velocities") +
+                                                ": non-finite value at vertex " + std::to_string(k / 3));
+        }
         if (q) sim->to_device_vec(q, sim->qn);
         if (v) sim->to_device_vec(v, sim->vstate);
         if (sim->prev_g)  // a new state starts a new trajectory: no previous reactions

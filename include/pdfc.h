@@ -252,13 +252,16 @@ int pdfc_create(const pdfc_scene* scene, const pdfc_params* params, pdfc_sim** o
  * obstacles of its own scene, with that scene's friction table. */
 int pdfc_create_batch(const pdfc_scene* scenes, size_t count, const pdfc_params* params, pdfc_sim** out);
 
-/* Advance nsteps time steps (SimulationBase::step() nsteps times). State stays on the device. */
+/* Advance nsteps time steps (SimulationBase::step() nsteps times). State stays on the device.
+ * Returns PDFC_ERR_NUMERIC when a position or velocity became non-finite during the call (the
+ * state is left as computed; the reference steps on silently). */
 int pdfc_step(pdfc_sim* sim, int nsteps);
 
 /* Copy the generalized positions / velocities (3n each) to host memory; either may be NULL. */
 int pdfc_get_state(pdfc_sim* sim, double* q, double* v);
 
-/* Overwrite the state (3n each; either may be NULL). Used for identical-state parity and checkpoints. */
+/* Overwrite the state (3n each; either may be NULL). Used for identical-state parity and checkpoints.
+ * PDFC_ERR_INVALID, with the state untouched, if a value is not finite. */
 int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v);
 
 /* Binary checkpoint of the simulation state (no counterpart in the reference, whose

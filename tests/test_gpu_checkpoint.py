@@ -134,3 +134,27 @@ def test_driver_outputs_and_resume(gpu, tmp_path):
     text = open(full / "stats.txt").read()
     assert "Iteration time (µs)\n\t Number      : 80\n" in text  # 8 steps x 10 iterations
     assert "Step time (µs)\n\t Number      : 8\n" in text
+
+
+def test_failure_detection(gpu):
+    """Non-finite input is rejected by pdfc_set_state; a state that overflows during a step
+    (finite positions of +-1e306: A^T A q_n exceeds the double range) makes pdfc_step return
+    PDFC_ERR_NUMERIC, and the next call starts with the flag cleared."""
+    spec = scenes.sphere_drape_small()
+    sim = gpu.Simulation(spec)
+    q0, v0 = sim.state()
+    bad = q0.copy()
+    bad[3 * 7 + 1] = np.nan
+    with pytest.raises(native.PdfcError) as e:
+        sim.set_state(bad, None)
+    assert e.value.code == abi.PDFC_ERR_INVALID and "vertex 7" in str(e.value)
+    assert np.array_equal(sim.state()[0], q0)
+    sim.step(2)  # healthy steps pass
+    huge = np.where(np.arange(q0.size) % 2 == 0, 1e306, -1e306)
+    sim.set_state(huge, np.zeros_like(v0))
+    with pytest.raises(native.PdfcError) as e:
+        sim.step(3)
+    assert e.value.code == abi.PDFC_ERR_NUMERIC and "non-finite" in str(e.value) and "3..5" in str(e.value)
+    sim.set_state(q0, np.zeros_like(v0))
+    sim.step(2)  # flag cleared: a recovered state steps without error
+    assert np.isfinite(sim.state()[0]).all()
