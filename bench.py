@@ -12,6 +12,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -47,8 +49,16 @@ def cpu_baseline(spec, budget_s=20.0):
     t0 = time.perf_counter()
     orc.step(k)
     dt = time.perf_counter() - t0
+    # one thread: the next 2 steps of the same trajectory
+    po.set_num_threads(1)
+    t0 = time.perf_counter()
+    orc.step(2)
+    dt1 = time.perf_counter() - t0
+    po.set_num_threads(threads)
     return {"value": k / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "steps 4..%d of the same drape, %d OpenMP threads, factorisation excluded" % (3 + k, threads)}
+            "sample": "steps 4..%d of the same drape, %d OpenMP threads, factorisation excluded" % (3 + k, threads),
+            "one_thread": {"value": 2 / dt1, "unit": "steps/s",
+                           "sample": "steps %d..%d, 1 thread" % (4 + k, 5 + k)}}
 
 
 def warm_start_comparison(spec, sim, prec, steps=2, cap=2000, tol=1e-6):
@@ -126,7 +136,6 @@ def bench_mlp(args):
     BF16 MFMA with FP32 accumulation) and inference on 262,144 synthetic contact rows. Rates
     include the host-device copies of each call; the kernels' own times and MFMA counters come
     from rocprofv3 on this workload."""
-    import numpy as np
     from climatenet_amd import abi, native
     rows = 1 << 18
     rng = np.random.default_rng(0)
@@ -163,6 +172,40 @@ def bench_mlp(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_stacked(args):
+    """The self-collision-heavy configuration: three 150 x 150-quad sheets 5 mm apart (68,403
+    vertices) on a sphere, mu 0.3, dt 1/200, 30 iterations, self tolerance 4e-3, in FP32
+    substitution with FP64 refinement. After the timed steps, one more step from the same
+    state in FP32+refinement and in FP64 gives the per-step error of the precision mode."""
+    from climatenet_amd import abi, native, scenes
+    spec = scenes.stacked_layers()
+    sim = native.Simulation(spec, precision=abi.FP32_REFINE64, record_iterations=False)
+    sim.step(args.warmup)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    info = sim.info()
+    q, v = sim.state()
+    ref = native.Simulation(spec, precision=abi.FP64, record_iterations=False)
+    ref.set_state(q, v)
+    sim.step(1)
+    ref.step(1)
+    (q32, v32), (q64, v64) = sim.state(), ref.state()
+    err = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    out = {
+        "metric": "steps/s, 3 stacked 150x150 sheets (self-collision heavy)",
+        "value": round(args.steps / elapsed, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64-refine", "data": "synthetic",
+        "config": {"workload": "3 x 150^2 quads (68,403 vertices), sphere R=0.4, mu=0.3, dt=1/200, 30 iterations, "
+                               "self-collision tol 4e-3, FP32 substitution + 2 FP64 refinement sweeps",
+                   "n_vertices": info["n_vertices"], "levels": info["n_levels"], "top_block_rows": info["top_m"],
+                   "self_contacts_last_step": info["n_self_contacts"], "contacts_last_step": info["n_contacts"]},
+        "error_vs_fp64_one_step": {"q": err(q32, q64), "v": err(v32, v64)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,9 +218,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
-    ap.add_argument("--workload", choices=["drape", "batched", "mlp"], default="drape",
-                    help="drape: the 40k-vertex drape (default); batched: 64 independent cloths; "
-                         "mlp: the warm-start network's training and inference alone")
+    ap.add_argument("--workload", choices=["drape", "batched", "stacked", "mlp"], default="drape",
+                    help="drape: the 40k-vertex drape (default); batched: 64 independent cloths; stacked: three "
+                         "stacked sheets, FP32 + FP64 refinement; mlp: the warm-start network alone")
     ap.add_argument("--no-graph", action="store_true", help="batched: launch kernels eagerly")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -199,6 +242,9 @@ def main():
         return
     if args.workload == "mlp":
         bench_mlp(args)
+        return
+    if args.workload == "stacked":
+        bench_stacked(args)
         return
 
     from climatenet_amd import abi, native, scenes
@@ -266,6 +312,11 @@ def main():
                    "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,
+        # whole step: the algorithmic bytes of its iterations at the measured rate
+        "step_roofline": {"bytes_per_step": info["bytes_per_iteration"] * K,
+                          "achieved": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9, 3),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9 / HBM_PEAK_GBS, 6)},
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
         "self_collision_host_us_per_step": round(phases["detection_us"], 1),
     }

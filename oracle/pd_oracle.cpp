@@ -1494,4 +1494,12 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int k) {
+#ifdef _OPENMP
+    omp_set_num_threads(k > 0 ? k : 1);
+#else
+    (void)k;
+#endif
+}
+
 }  // extern "C"

@@ -50,6 +50,9 @@ def lib():
         L.orc_get_bending_row.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int]
         L.orc_factor_seconds.argtypes = [P]
         L.orc_factor_seconds.restype = C.c_double
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_set_num_threads.restype = None
+        L.orc_num_threads.restype = C.c_int
         L.orc_last_error.restype = C.c_char_p
         L.orc_mean_value_coefficients.argtypes = [C.POINTER(C.c_double)] * 2 + [C.c_int, C.POINTER(C.c_double)]
         L.orc_mean_value_coefficients.restype = None
@@ -252,3 +255,12 @@ def mlp_gradient(params, norm, x, t, bf16=True):
     g = np.zeros(10448)
     loss = lib().orc_mlp_gradient(_f(P), _f(Z), _f(x), _f(t), x.shape[0], int(bf16), _d(g))
     return g, loss
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
+
+
+def set_num_threads(k):
+    """OpenMP threads of the oracle's parallel regions (all threads by default)."""
+    lib().orc_set_num_threads(int(k))

@@ -114,3 +114,27 @@ def test_stacked_layers_identical_state_parity(gpu):
     print("worst", worst, "steps with self contacts", self_steps)
     assert self_steps > 10
     assert max(worst.values()) < 1e-9, worst
+
+
+def test_stacked_full_size_candidates_bit_exact(gpu):
+    """The self-collision-heavy configuration at full size (3 x 150^2 quads, 68,403 vertices,
+    FP32 substitution with FP64 refinement): after 60 steps the sheets press on each other;
+    from that state the candidate list (vertex, triangle, barycentric coordinates), the node-node
+    pairs and their computation levels equal the oracle's exactly."""
+    spec = scenes.stacked_layers()
+    sim = gpu.Simulation(spec, precision=1)
+    print("stacked 3x150^2: 60 FP32+refinement steps", flush=True)
+    sim.step(60)
+    q, v = sim.state()
+    orc = po.Oracle(spec, iterations=1)
+    orc.set_state(q, v)
+    sim.detect()
+    orc.detect()
+    a, b = sim.candidates(), orc.candidates()
+    print("candidates", len(a), len(b), "self contacts", len(orc.self_contacts()))
+    assert len(a) == len(b) and len(b) > 10000
+    assert np.array_equal(np.array([(c.vertex, c.triangle) for c in a]), np.array([(c.vertex, c.triangle) for c in b]))
+    assert np.array_equal(np.array([list(c.barycentric) for c in a]), np.array([list(c.barycentric) for c in b]))
+    sa = [(s.vertex, s.partner, s.level) for s in sim.self_contacts()]
+    sb = [(s.vertex, s.partner, s.level) for s in orc.self_contacts()]
+    assert len(sb) > 1000 and sa == sb
