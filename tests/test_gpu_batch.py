@@ -104,3 +104,38 @@ def test_graph_replay_of_a_batch(gpu, monkeypatch):
         qe, _ = eager.state()
         qg, _ = graph.state()
         assert np.array_equal(qe, qg), s
+
+
+def test_batch_of_boxes_rotating_spheres_and_attachments(gpu):
+    """A batch of a box drape, a rotating-sphere drape (turning from step 6) and a cloth hanging
+    from two attachment constraints: per scene, per step, from identical states, the oracle's
+    states and contacts (with obstacle speeds) to the FP64 bar."""
+    specs = [scenes.box_drape(), scenes.rotating_sphere_drape(start=0.05), scenes.hanging_cloth()]
+    sim = gpu.BatchSimulation(specs)
+    orcs = [po.Oracle(sp) for sp in specs]
+    seen_speed = False
+    n_contact = 0
+    for s in range(60):
+        Q, Vv = zip(*[o.state() for o in orcs])
+        sim.set_state(np.concatenate(Q), np.concatenate(Vv))
+        sim.step(1)
+        for o in orcs:
+            o.step(1)
+        q, v = sim.state()
+        cs = sim.contacts()
+        for k, (o, (a, b)) in enumerate(zip(orcs, sim.ranges)):
+            qo, vo = o.state()
+            assert rel_err(q[3 * a:3 * b], qo) < REL, (s, k)
+            assert rel_err(v[3 * a:3 * b], vo) < REL, (s, k)
+            mine = [c for c in cs if c.obstacle == k]
+            ref = o.contacts()
+            assert [c.vertex - a for c in mine] == [c.vertex for c in ref], (s, k)
+            if ref:
+                n_contact += 1
+                sp_m = np.array([list(c.speed) for c in mine])
+                sp_r = np.array([list(c.speed) for c in ref])
+                assert np.abs(sp_m - sp_r).max() < 1e-14, (s, k)
+                seen_speed = seen_speed or np.abs(sp_r).max() > 0
+                assert rel_err(np.array([list(c.r_local) for c in mine]),
+                               np.array([list(c.r_local) for c in ref])) < REL, (s, k)
+    assert n_contact > 60 and seen_speed
