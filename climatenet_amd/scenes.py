@@ -392,3 +392,24 @@ def hanging_cloth(weight: float = 100.0) -> SceneSpec:
                      [ObstacleSpec("sphere", (0.0, 0.25, -0.1), radius=0.2)],
                      np.array([[0.3]]), time_step=0.01, iterations=10, steps=200,
                      attachments=(corners, P[corners].copy(), np.full(2, weight)))
+
+
+def analytic_incline_mesh(mu: float = 0.1) -> SceneSpec:
+    """The setting of the reference's closed-form sliding example (replicability/scenes/Analytic,
+    replicability/analytic_example_data_processing.py): a 2 m x 2 m cloth of 4 vertices (area
+    density 1, stretching = bending = 2e-4) held parallel to a 45-degree support and released
+    0.2 m above it; the support is a triangle-mesh obstacle (a 4 m x 20 m rectangle, two
+    triangles, collision threshold 1e-3) through the origin, descending towards +y. mu=0.1,
+    dt=1e-3, 20 iterations, 800 steps."""
+    th = np.pi / 4
+    down = np.array([0.0, np.cos(th), -np.sin(th)])   # along the support, descending
+    across = np.array([1.0, 0.0, 0.0])
+    S = np.array([c * across + d * down for c, d in ((2.0, -10.0), (2.0, 10.0), (-2.0, 10.0), (-2.0, -10.0))])
+    support_tris = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    lift = np.array([0.0, 0.0, 0.2])                  # 0.2 m above the support, vertically
+    C = np.array([c * across + d * down + lift for c, d in ((1.0, -1.0), (1.0, 1.0), (-1.0, -1.0), (-1.0, 1.0))])
+    tris = np.array([0, 1, 3, 0, 3, 2], dtype=np.int32)
+    return SceneSpec("analytic_incline", f32(C.ravel()), tris, None,
+                     [ObstacleSpec("mesh", tolerance=1e-3, frames=S[None], triangles=support_tris)],
+                     np.array([[mu]]), area_density=1.0, bending=2e-4, stretching=2e-4,
+                     time_step=1e-3, iterations=20, steps=800)

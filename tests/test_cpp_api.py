@@ -168,3 +168,30 @@ def test_stats_writer_format(api_check, tmp_path):
             "Negative\n\t Number      : 2\n\t Min value   : -2\n\t Mean value  : -1.5\n"
             "\t Max value   : 2.22507e-308\n" + sep)
     assert out.read_text() == want
+
+
+REFERENCE = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "scenes")), reason="reference scenes not present")
+def test_reference_scene_files_parse(api_check):
+    """Every scene file the reference ships (scenes/, replicability/scenes/) reads with the C++
+    interface, from a directory one level below the reference root as its relative OBJ paths
+    assume; the two whose meshes the reference does not ship fail the way the reference's reader
+    does ("Could not load obj file")."""
+    import glob
+    files = sorted(glob.glob(os.path.join(REFERENCE, "scenes", "*.json")) +
+                   glob.glob(os.path.join(REFERENCE, "replicability", "scenes", "*", "*.json")))
+    assert len(files) >= 20
+    ok = 0
+    for f in files:
+        r = subprocess.run([api_check, f], capture_output=True, text=True, cwd=os.path.join(REFERENCE, "scenes"))
+        out = dict(l.split(" ", 1) for l in r.stdout.splitlines() if " " in l)
+        if r.returncode == 3 and "Could not load obj file" in r.stdout:
+            missing = r.stdout.split("Could not load obj file: ")[1].split()[0]
+            assert not os.path.exists(os.path.join(REFERENCE, "scenes", missing)), f
+            continue
+        assert r.returncode == 0, (f, r.stdout[-500:])
+        assert int(out["vertices"]) > 0 and int(out["obstacles"]) >= 1
+        ok += 1
+    assert ok >= 18

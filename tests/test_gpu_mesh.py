@@ -55,3 +55,24 @@ def test_mesh_obstacle_identical_state_parity(gpu, moving):
     assert worst["point"] < 1e-15 and worst["speed"] < 1e-14
     if moving:
         assert max_speed > 0.1
+
+
+def test_analytic_incline_on_a_mesh_support(gpu):
+    """The reference's closed-form sliding example (45-degree support as a two-triangle mesh
+    obstacle, threshold 1e-3, mu 0.1): on the GPU the cloth's speed equals the closed form at
+    every one of 800 steps, before and after the impact."""
+    from tests.test_oracle_mesh import analytic_speed
+    spec = scenes.analytic_incline_mesh()
+    sim = gpu.Simulation(spec)
+    down = np.array([0.0, np.cos(np.pi / 4), -np.sin(np.pi / 4)])
+    worst = 0.0
+    for s in range(800):
+        sim.step(1)
+        _, v = sim.state()
+        V = v.reshape(-1, 3).mean(axis=0)
+        t = (s + 1) * spec.time_step
+        sp = V @ down if sim.contacts() else -V[2]
+        worst = max(worst, abs(sp - analytic_speed(t)) / analytic_speed(t))
+    print("worst relative deviation from the closed form", worst)
+    assert len(sim.contacts()) == 4
+    assert worst < 1e-10, worst

@@ -132,3 +132,31 @@ def test_moving_mesh_translation_speed_and_clock():
         # the vertex sits 3 mm above the moved mesh's first vertex position: its contact
         # point is the moved mesh point right below
         assert np.allclose(list(c0[0].point), P[0] + t * u, atol=1e-12), (k, list(c0[0].point), P[0] + t * u)
+
+
+def analytic_speed(t, g=9.81, th=np.pi / 4, mu=0.1, h=0.2):
+    """replicability/analytic_example_data_processing.py:53-112, restated: free fall until
+    t* = sqrt(2h/g), then speed along the support g t* (cos th - mu sin th) + g (sin th - mu cos th)(t - t*)."""
+    ts = np.sqrt(2 * h / g)
+    if t < ts:
+        return g * t
+    return g * ts * (np.cos(th) - np.sin(th) * mu) + g * (np.sin(th) - np.cos(th) * mu) * (t - ts)
+
+
+def test_analytic_incline_on_a_mesh_support():
+    """The reference's closed-form sliding example with its support as a triangle-mesh
+    obstacle (threshold 1e-3): the cloth's speed (vertical before impact, along the support
+    after) equals the closed form at every step."""
+    spec = scenes.analytic_incline_mesh()
+    orc = po.Oracle(spec)
+    down = np.array([0.0, np.cos(np.pi / 4), -np.sin(np.pi / 4)])
+    worst = 0.0
+    for s in range(800):
+        orc.step(1)
+        _, v = orc.state()
+        V = v.reshape(-1, 3).mean(axis=0)
+        t = (s + 1) * spec.time_step
+        sp = V @ down if orc.contacts() else -V[2]
+        worst = max(worst, abs(sp - analytic_speed(t)) / analytic_speed(t))
+    assert len(orc.contacts()) == 4
+    assert worst < 1e-10, worst
