@@ -62,33 +62,76 @@ def cpu_baseline(spec, budget_s=20.0):
 
 
 def warm_start_comparison(spec, sim, prec, steps=2, cap=2000, tol=1e-6):
-    """Iterations to a 1e-6 mean friction residual (at most `cap`) from the bench's current
-    state, plain warm start against the committed contact network, `steps` steps each."""
-    from climatenet_amd import abi, learn, native
+    """Iterations to a 1e-6 mean friction residual, plain warm start against the committed
+    contact network:
+    * on the bench's 40k drape from its current state, `steps` steps each, at most `cap`
+      iterations per step;
+    * on the 20x20 drape the network was trained on (jitter seed 7, a trajectory outside the
+      training data), steps 61-62 from the same state, at most 8,000 iterations, with the
+      exact converged reactions of each step held for 1,000 iterations as the bound that any
+      reaction warm start could reach."""
+    from climatenet_amd import abi, learn, native, scenes
     if not os.path.exists(learn.MODEL):
         return None
-    q, v = sim.state()
     mlp = learn.load(learn.MODEL)
-    res = {"tolerance": tol, "max_iterations": cap, "steps": steps}
-    for name, ws in (("plain", abi.WARM_PLAIN), ("learned", abi.WARM_LEARNED)):
-        s2 = native.Simulation(spec, precision=prec, residual_tolerance=tol, max_iterations=cap, warm_start=ws,
+
+    def run(sp, q, v, ws, n, limit, exact_from=None):
+        s2 = native.Simulation(sp, precision=prec, residual_tolerance=tol, max_iterations=limit, warm_start=ws,
                                mlp=mlp if ws == abi.WARM_LEARNED else None)
         s2.set_state(q, v)
         it, conv, fb, resid = 0, 0, 0, []
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(n):
+            if exact_from is not None:
+                s2.set_warm_reactions(exact_from[k], 1000)
             s2.step(1)
             ps = s2.phase_stats()
             it += ps["iterations_to_tolerance"]
             conv += ps["converged_steps"]
             fb += ps["fallbacks"]
             resid.append(float(s2.iteration_stats()[0][-1]))
-        res[name] = {"iterations_to_tolerance": it / steps, "converged_steps": conv, "fallbacks": fb,
-                     "final_residual": resid, "seconds_per_step": round((time.perf_counter() - t0) / steps, 3)}
+        out = {"iterations_to_tolerance": it / n, "converged_steps": conv, "fallbacks": fb,
+               "final_residual": resid, "seconds_per_step": round((time.perf_counter() - t0) / n, 3)}
+        return out, s2
+
+    q, v = sim.state()
+    big = {"scene": "40k drape from the bench state", "tolerance": tol, "max_iterations": cap, "steps": steps}
+    for name, ws in (("plain", abi.WARM_PLAIN), ("learned", abi.WARM_LEARNED)):
+        big[name], s2 = run(spec, q, v, ws, steps, cap)
         s2.close()
-    res["iteration_ratio_plain_over_learned"] = round(res["plain"]["iterations_to_tolerance"] /
-                                                      max(1e-9, res["learned"]["iterations_to_tolerance"]), 4)
-    return res
+
+    small = scenes.sphere_drape_small(jitter_seed=7)
+    base = native.Simulation(small)
+    base.step(60)
+    q0, v0 = base.state()
+    base.close()
+    res = {"scene": "20x20 drape, jitter seed 7, steps 61-62", "tolerance": tol, "max_iterations": 8000, "steps": 2}
+    # plain, recording each step's converged reactions for the exact bound
+    s2 = native.Simulation(small, residual_tolerance=tol, max_iterations=8000)
+    s2.set_state(q0, v0)
+    exact, before, it_plain = [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(2):
+        before.append(s2.state())
+        s2.step(1)
+        it_plain += s2.phase_stats()["iterations_to_tolerance"]
+        exact.append(s2.contacts())
+    res["plain"] = {"iterations_to_tolerance": it_plain / 2,
+                    "seconds_per_step": round((time.perf_counter() - t0) / 2, 3)}
+    s2.close()
+    res["learned"], s3 = run(small, q0, v0, abi.WARM_LEARNED, 2, 8000)
+    s3.close()
+    it_exact = 0
+    for k in range(2):  # each step from the plain run's state before it, with that step's reactions
+        r, s4 = run(small, before[k][0], before[k][1], abi.WARM_PLAIN, 1, 8000, exact_from=[exact[k]])
+        it_exact += r["iterations_to_tolerance"]
+        s4.close()
+    res["exact_reactions_held_1000"] = {"iterations_to_tolerance": it_exact / 2}
+    P = res["plain"]["iterations_to_tolerance"]
+    res["iteration_ratio_plain_over_learned"] = round(P / max(1e-9, res["learned"]["iterations_to_tolerance"]), 4)
+    res["iteration_ratio_plain_over_exact"] = round(
+        P / max(1e-9, res["exact_reactions_held_1000"]["iterations_to_tolerance"]), 4)
+    return {"drape_40k": big, "drape_20x20": res}
 
 
 def bench_batched(args):
