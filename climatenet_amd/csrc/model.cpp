@@ -35,6 +35,9 @@ void build_model(const pdfc_scene& sc, Model& M) {
     M.n = n;
     M.T = T;
     M.rest.assign(sc.rest_positions, sc.rest_positions + 3 * (size_t)n);
+    for (size_t k = 0; k < M.rest.size(); ++k)
+        if (!std::isfinite(M.rest[k]) || (sc.positions && !std::isfinite(sc.positions[k])))
+            throw std::invalid_argument("non-finite vertex position");
     M.tris.resize(T);
     for (int t = 0; t < T; ++t) {
         for (int k = 0; k < 3; ++k) {
@@ -162,7 +165,11 @@ void build_model(const pdfc_scene& sc, Model& M) {
     for (size_t a = 0; a < sc.n_attachments; ++a) {
         const int v = sc.attachment_vertices[a];
         if (v < 0 || v >= n) throw std::invalid_argument("attachment vertex out of range");
-        if (!(sc.attachment_weights[a] > 0.)) throw std::invalid_argument("attachment weight must be positive");
+        if (!(sc.attachment_weights[a] > 0.) || !std::isfinite(sc.attachment_weights[a]))
+            throw std::invalid_argument("attachment weight must be positive and finite");
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(sc.attachment_positions[3 * a + c]))
+                throw std::invalid_argument("non-finite attachment position");
         M.att_vertex.push_back(v);
         M.att_w.push_back(sc.attachment_weights[a]);
         M.att_pos.insert(M.att_pos.end(), sc.attachment_positions + 3 * a, sc.attachment_positions + 3 * a + 3);

@@ -314,7 +314,10 @@ struct pdfc_sim {
 void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& p) {
     if (count == 0) throw std::invalid_argument("no scene");
     const pdfc_scene& sc = scenes[0];
-    if (!(p.time_step > 0.)) throw std::invalid_argument("time_step must be positive");
+    if (!(p.time_step > 0.) || !std::isfinite(p.time_step)) throw std::invalid_argument("time_step must be positive");
+    for (double x : {p.gravity[0], p.gravity[1], p.gravity[2], p.air_damping, p.self_collision_tolerance,
+                     p.residual_tolerance})
+        if (!std::isfinite(x)) throw std::invalid_argument("non-finite simulation parameter");
     if (p.local_global_iterations < 1) throw std::invalid_argument("local_global_iterations must be >= 1");
     if (p.precision != PDFC_FP64 && p.precision != PDFC_FP32_REFINE64) throw std::invalid_argument("unknown precision");
     if (p.self_collision && !(p.self_collision_tolerance >= 0.))
@@ -340,6 +343,13 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
             pdfc_obstacle ob = S.obstacles[o];
             if (ob.material < 0 || ob.material >= (int)S.n_materials)
                 throw std::invalid_argument("obstacle material out of range");
+            {
+                const double vals[] = {ob.point[0], ob.point[1], ob.point[2], ob.normal[0], ob.normal[1], ob.normal[2],
+                                       ob.radius, ob.tolerance, ob.size[0], ob.size[1], ob.size[2], ob.rotation_axis[0],
+                                       ob.rotation_axis[1], ob.rotation_axis[2], ob.angular_speed, ob.rotation_start};
+                for (double x : vals)
+                    if (!std::isfinite(x)) throw std::invalid_argument("obstacle with a non-finite parameter");
+            }
             if (ob.type == PDFC_OBSTACLE_PLANE) {
                 double z = (ob.normal[0] * ob.normal[0] + ob.normal[1] * ob.normal[1]) + ob.normal[2] * ob.normal[2];
                 if (!(z > 0.)) throw std::invalid_argument("plane normal is zero");
@@ -361,10 +371,14 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
                 for (int x : hm.tris)
                     if (x < 0 || x >= hm.nv) throw std::invalid_argument("mesh obstacle triangle index out of range");
                 hm.frames.assign(ob.mesh_frames, ob.mesh_frames + ob.n_frames * 3 * ob.n_mesh_vertices);
+                for (double x : hm.frames)
+                    if (!std::isfinite(x)) throw std::invalid_argument("mesh obstacle frame with a non-finite coordinate");
                 if (ob.n_frames >= 2) {
                     hm.times.assign(ob.frame_times, ob.frame_times + ob.n_frames);
                     for (size_t f = 1; f < ob.n_frames; ++f)
                         if (!(hm.times[f] > hm.times[f - 1])) throw std::invalid_argument("frame times must increase");
+                    for (double t : hm.times)
+                        if (!std::isfinite(t)) throw std::invalid_argument("frame times must be finite");
                     moving_mesh = true;
                 }
                 hm.tol = std::sqrt(ob.tolerance * ob.tolerance);

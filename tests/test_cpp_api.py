@@ -26,8 +26,8 @@ def api_check(tmp_path_factory):
     return exe
 
 
-def run(exe, *args):
-    r = subprocess.run([exe] + list(args), capture_output=True, text=True)
+def run(exe, *args, env=None):
+    r = subprocess.run([exe] + list(args), capture_output=True, text=True, env=env)
     return r.returncode, dict(l.split(" ", 1) for l in r.stdout.splitlines() if " " in l), r.stdout
 
 

@@ -158,3 +158,37 @@ def test_failure_detection(gpu):
     sim.set_state(q0, np.zeros_like(v0))
     sim.step(2)  # flag cleared: a recovered state steps without error
     assert np.isfinite(sim.state()[0]).all()
+
+
+def test_non_finite_scene_input_is_rejected(gpu):
+    """pdfc_create rejects non-finite scene data and parameters with PDFC_ERR_INVALID."""
+    import copy
+
+    def rejects(spec, **kw):
+        with pytest.raises(native.PdfcError) as e:
+            gpu.Simulation(spec, **kw)
+        assert e.value.code == abi.PDFC_ERR_INVALID, str(e.value)
+        return str(e.value)
+
+    base = scenes.sphere_drape_small()
+    s = copy.deepcopy(base)
+    s.rest = s.rest.copy()
+    s.rest[5] = np.inf
+    assert "position" in rejects(s)
+    s = copy.deepcopy(base)
+    s.obstacles[0].radius = np.nan
+    assert "obstacle" in rejects(s)
+    s = copy.deepcopy(base)
+    s.time_step = np.inf
+    rejects(s)
+    s = copy.deepcopy(base)
+    s.gravity = (0.0, 0.0, np.nan)
+    rejects(s)
+    s = scenes.mesh_sphere_drape(moving=True)
+    s.obstacles[0].frames = s.obstacles[0].frames.copy()
+    s.obstacles[0].frames[3, 7, 1] = np.nan
+    assert "frame" in rejects(s)
+    s = scenes.hanging_cloth()
+    idx, pos, w = s.attachments
+    s.attachments = (idx, pos, np.array([100.0, np.inf]))
+    assert "attachment" in rejects(s)
