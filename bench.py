@@ -170,6 +170,13 @@ def bench_batched(args):
         "sequential": {"value": round(len(specs) * args.steps / seq_s, 3), "unit": "instance-steps/s",
                        "note": "the same 64 scenes, one simulation after another on the GPU"},
         "speedup_vs_sequential": round(seq_s / batched_s, 3),
+        # whole batch step: algorithmic bytes of its 15 iterations at the measured rate
+        "step_roofline": {"bytes_per_step": info["bytes_per_iteration"] * specs[0].iterations,
+                          "achieved": round(info["bytes_per_iteration"] * specs[0].iterations * args.steps
+                                            / batched_s / 1e9, 3),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(info["bytes_per_iteration"] * specs[0].iterations * args.steps
+                                        / batched_s / 1e9 / HBM_PEAK_GBS, 6)},
     }
     print(json.dumps(out), flush=True)
 
