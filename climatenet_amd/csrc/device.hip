@@ -745,9 +745,36 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
 
 // Backward substitution, one task = (supernode s, 64 columns), W waves:
 //   x_s = F_s^T [ y_s ; -x_R ]      (reduction over the f = ns + nb rows split across waves)
+// Final value of top row t: the matvec's partial products summed in chunk order, exactly as
+// k_top_finish sums them.
+// All the loads are issued before the first add (at most kTopChunksMax chunks).
+constexpr int kTopChunksMax = 16;
+__device__ __forceinline__ void top_sum(const TopFold& fo, int t, double& x, double& y, double& z) {
+    double px[kTopChunksMax], py[kTopChunksMax], pz[kTopChunksMax];
+#pragma unroll
+    for (int c = 0; c < kTopChunksMax; ++c)
+        if (c < fo.kchunks) {
+            const double* P = fo.part + (size_t)c * 3 * fo.m;
+            px[c] = P[t];
+            py[c] = P[fo.m + t];
+            pz[c] = P[2 * fo.m + t];
+        }
+    x = 0.;
+    y = 0.;
+    z = 0.;
+#pragma unroll
+    for (int c = 0; c < kTopChunksMax; ++c)
+        if (c < fo.kchunks) {
+            x += px[c];
+            y += py[c];
+            z += pz[c];
+        }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDesc& d, const double* __restrict__ Y,
-                                              double* __restrict__ X, double* lds, double (*part)[3][64]) {
+                                              double* __restrict__ X, double* lds, double (*part)[3][64],
+                                              const TopFold& fo) {
     const int ns = d.ns, nb = d.nb, f = d.f, c0 = d.c0, j0 = d.r0;
     const int n = Fa.n;
     double* gx = lds;
@@ -770,9 +797,18 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
     const int* R = Fa.rows + d.roff;
     for (int t = tid; t < nb; t += 64 * W) {
         int r = R[t];
-        gx[ns + t] = -X[r];
-        gy[ns + t] = -X[n + r];
-        gz[ns + t] = -X[2 * n + r];
+        const int tr = fo.part ? fo.top_of[r] : -1;
+        if (tr >= 0) {
+            double x, y, z;
+            top_sum(fo, tr, x, y, z);
+            gx[ns + t] = -x;
+            gy[ns + t] = -y;
+            gz[ns + t] = -z;
+        } else {
+            gx[ns + t] = -X[r];
+            gy[ns + t] = -X[n + r];
+            gz[ns + t] = -X[2 * n + r];
+        }
     }
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
@@ -819,10 +855,22 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
 
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const double* __restrict__ Y,
-                                                     double* __restrict__ X) {
+                                                     double* __restrict__ X, TopFold fo) {
     extern __shared__ double lds[];
     __shared__ double part[W > 1 ? W : 1][3][64];
-    backward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Y, X, lds, part);
+    if (fo.part && (int)blockIdx.x >= fo.ntasks) {  // folded top-block finish: write the top rows
+        const int t = ((int)blockIdx.x - fo.ntasks) * (int)blockDim.x + (int)threadIdx.x;
+        if (t < fo.m) {
+            double x, y, z;
+            top_sum(fo, t, x, y, z);
+            const int r = fo.rows[t];
+            X[r] = x;
+            X[Fa.n + r] = y;
+            X[2 * Fa.n + r] = z;
+        }
+        return;
+    }
+    backward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Y, X, lds, part, fo);
 }
 
 // ------------------------------------------------ dependency-driven sweeps
@@ -901,7 +949,7 @@ __global__ void __launch_bounds__(64 * W) k_backward_flow(DevFactor Fa, DevFlow 
         if (t >= G.ntasks) break;
         const TaskDesc d = Fa.desc[t];
         flow_wait(G, d.s);
-        backward_task<T, W>(Fa, d, Y, X, lds, part);
+        backward_task<T, W>(Fa, d, Y, X, lds, part, TopFold{});
         flow_publish(G, d.s);
     }
 }
@@ -1085,13 +1133,13 @@ static void forward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const De
 }
 template <typename T>
 static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* Y,
-                              double* X) {
+                              double* X, const TopFold& fo) {
     switch (W) {
-        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X); break;
-        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X); break;
-        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X); break;
-        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X); break;
-        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X); break;
+        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X, fo); break;
+        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X, fo); break;
+        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X, fo); break;
+        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X, fo); break;
+        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X, fo); break;
     }
 }
 
@@ -1148,15 +1196,22 @@ void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, con
         forward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, B, Y);
 }
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int W, hipStream_t st) {
-    if (ntasks <= 0) return;
+                           int max_front, int W, hipStream_t st, const TopFold* fold) {
+    TopFold fo{};
+    if (fold && fold->part && fold->m > 0) {
+        fo = *fold;
+        fo.ntasks = ntasks;
+        fo.extra = (fo.m + 64 * W - 1) / (64 * W);
+    }
+    if (ntasks + fo.extra <= 0) return;
     size_t lds = (size_t)3 * max_front * sizeof(double);
     if (fp32)
-        backward_dispatch<float>(W, dim3(ntasks), lds, st, F, t0, Y, X);
+        backward_dispatch<float>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
     else
-        backward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, Y, X);
+        backward_dispatch<double>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
 }
-void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st) {
+void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
+                bool finish) {
     if (T.m <= 0) return;
     hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
     dim3 g((T.m + 63) / 64, T.kchunks);
@@ -1164,7 +1219,7 @@ void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, do
         hipLaunchKernelGGL(k_top_matvec<float>, g, dim3(1024), 0, st, T);
     else
         hipLaunchKernelGGL(k_top_matvec<double>, g, dim3(1024), 0, st, T);
-    hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
+    if (finish) hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
 }
 void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st) {
     hipLaunchKernelGGL(k_unit3, dim3(blocks_for(3 * (size_t)n)), dim3(kBlock), 0, st, B, n, rows, m, j);

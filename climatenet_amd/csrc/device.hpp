@@ -263,12 +263,22 @@ void launch_forward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const 
                          int W, int blocks, hipStream_t st);
 void launch_backward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const double* Y, double* X, int max_front,
                           int W, int blocks, hipStream_t st);
+// The top block's ordered finish folded into the first backward level (the one right below the
+// top block, whose supernodes have only top-block ancestors): its tasks read the top rows they
+// gather as the chunk-ordered sum of the matvec's partial products, and `extra` workgroups of
+// the same launch write every top row's final value for the later levels and the result.
+struct TopFold {
+    const double* part;   // kchunks * 3 * m partial products; null: no fold
+    const int* rows;      // m supernodal indices of the top rows
+    const int* top_of;    // n: top index of a supernodal row, -1 elsewhere
+    int m, kchunks, ntasks, extra;
+};
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
 // waves: 1, 2, 4, 8 or 16 waves per task splitting the reduction (ns forward, ns + nb backward).
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y,
                           int max_ns, int waves, hipStream_t st);
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int waves, hipStream_t st);
+                           int max_front, int waves, hipStream_t st, const TopFold* fold = nullptr);
 // Heuristic wave count: chains of about 8 loads, at most 16 waves.
 int default_waves(int kmax);
 // Dense top block of the substitution (see TopPlan).
@@ -281,9 +291,12 @@ struct DevTop {
     const void* Sinv;       // m x m, double or float
     double* bt;             // 3 m: right-hand side of the top block (planar)
     double* part;           // kchunks * 3 * m partial products
+    const int* top_of;      // n: top index of a supernodal row, -1 elsewhere
 };
-// x_T = S^-1 (b_T - lower updates): gather, split matvec, ordered reduction + scatter.
-void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st);
+// x_T = S^-1 (b_T - lower updates): gather, split matvec, ordered reduction + scatter
+// (the last step is left out with finish = false: the first backward level does it, see TopFold).
+void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
+                bool finish = true);
 // Unit right-hand sides e_{rows[j+c]} in plane c (c = 0..2) and extraction of the
 // solution's top rows into columns j..j+2 of S^-1 (build of the top block).
 void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st);

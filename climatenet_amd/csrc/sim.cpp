@@ -57,6 +57,7 @@ void pdfc::set_last_error(const char* msg) { g_last_error = msg; }
 // dropped by kStagnationFactor over kStagnationWindow iterations.
 constexpr int kCheckChunk = 8;
 constexpr int kMaxTopChunks = 16;  // reduction split of the dense top-block matvec
+static_assert(kMaxTopChunks <= 16, "the folded top finish (top_sum) reads at most 16 chunks");
 // Dependency-driven substitution sweeps (one launch per sweep): switched off, see autotune_waves().
 constexpr bool kFlowEnabled = false;
 constexpr int kReleaseWindow = 50;
@@ -143,6 +144,7 @@ struct pdfc_sim {
     void build_top();
     // kernel-timing instrumentation
     bool ktiming = false;
+    bool top_fold = true;           // the top block's finish inside the first backward level
     std::vector<hipEvent_t> kev;        // event pool, two per timed launch
     std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level, 3 top block
     size_t kused = 0;
@@ -707,6 +709,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         use_top = true;
     }
     {
+        if (const char* tf = std::getenv("PDFC_TOP_FOLD")) top_fold = std::atoi(tf) != 0;
         const char* pin = std::getenv("PDFC_TUNE");
         const char* e = std::getenv("PDFC_AUTOTUNE");
         if (pin && *pin)
@@ -1063,6 +1066,11 @@ void pdfc_sim::build_top() {
     const int m = top.m;
     dtop.m = m;
     dtop.rows = upload(top.rows);
+    {
+        std::vector<int> top_of((size_t)n, -1);
+        for (int t = 0; t < m; ++t) top_of[top.rows[t]] = t;
+        dtop.top_of = upload(top_of);
+    }
     dtop.cptr = upload(top.cptr);
     dtop.cidx = upload(top.cidx);
     dtop.bt = dalloc<double>(3 * (size_t)m);
@@ -1267,7 +1275,10 @@ void pdfc_sim::solve_inplace(double* X) {
                                                inout, Y, fac.max_ns, fwd_waves[l], stream);
             });
         }
-        if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream); });
+        // the top block's ordered finish runs inside the first backward level when there is one
+        const bool fold = use_top && top_fold && bwd.level_ptr.size() > 1;
+        if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream, !fold); });
+        pdfc::TopFold tf{dtop.part, dtop.rows, dtop.top_of, dtop.m, dtop.kchunks, 0, 0};
         if (use_flow) {
             timed(2, [&] {
                 pdfc::launch_backward_flow(dbwd, fbwd, fp32, Y, inout, fac.max_front, flow_wb, flow_blocks, stream);
@@ -1276,7 +1287,8 @@ void pdfc_sim::solve_inplace(double* X) {
             timed(2, [&] {
                 for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
                     pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
-                                                Y, inout, fac.max_front, bwd_waves[l], stream);
+                                                Y, inout, fac.max_front, bwd_waves[l], stream,
+                                                fold && l == 0 ? &tf : nullptr);
             });
         }
         if (ktiming) ++ksolves;

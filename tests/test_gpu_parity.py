@@ -173,3 +173,23 @@ def test_fp32_refinement_error_40k(gpu, solve_mode):
     err = rel_err(s32.solve(b), s64.solve(b))
     print("fp32 + 2 refinement sweeps, max rel error vs fp64:", err)
     assert err < 1e-10
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_REFINE64])
+def test_folded_top_finish_is_bit_identical(gpu, monkeypatch, precision):
+    """The top block's ordered finish run inside the first backward level sums the same partial
+    products in the same order as the separate finish kernel: with the same substitution
+    schedule, the two paths give identical states."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    monkeypatch.setenv("PDFC_TOP_FOLD", "1")
+    a = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    assert a.info()["top_m"] > 0
+    monkeypatch.setenv("PDFC_TUNE", a.tuning())
+    monkeypatch.setenv("PDFC_TOP_FOLD", "0")
+    b = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    a.step(5)
+    b.step(5)
+    qa, va = a.state()
+    qb, vb = b.state()
+    assert np.array_equal(qa, qb) and np.array_equal(va, vb)
