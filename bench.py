@@ -31,7 +31,8 @@ def measured_traffic(kernel, info):
     with open(path) as f:
         d = json.load(f)
     if d.get("kernel") != kernel or d.get("levels_swept") != info["n_levels_swept"] or \
-            d.get("n_vertices") != info["n_vertices"] or d.get("nnz_F") != info["nnz_F"]:
+            d.get("n_vertices") != info["n_vertices"] or d.get("nnz_F") != info["nnz_F"] or \
+            d.get("top_fold") != info["top_fold"]:
         return None
     return d.get("traffic_bytes_per_launch")
 
@@ -359,7 +360,7 @@ def main():
                                 if info["flow"] else {"sweeps": "level by level"}),
                    "autotune_pair_ms": {"levels": round(info["levels_pair_ms"], 4),
                                         "chosen": round(info["best_pair_ms"], 4)},
-                   "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"],
+                   "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"], "top_fold": info["top_fold"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,
         # whole step: the algorithmic bytes of its iterations at the measured rate

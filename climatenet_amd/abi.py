@@ -167,6 +167,8 @@ class Info(C.Structure):
         ("top_chunks", C.c_int32),
         ("lhs_seconds", C.c_double),
         ("steps_done", C.c_int64),
+        ("top_fold", C.c_int32),
+        ("reserved_info", C.c_int32),
     ]
 
 

@@ -2212,6 +2212,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.factor_seconds = sim->fac.seconds;
         I.lhs_seconds = sim->lhs_seconds;
         I.steps_done = sim->steps_done;
+        I.top_fold = (sim->use_top && sim->top_fold && sim->bwd.level_ptr.size() > 1) ? 1 : 0;
         // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
         //   local kernel: read v, q_n, c and write b (4 x 24 B per vertex); per spring end an
         //   index and kappa (4 + 8 B); per contact the basis (72 B), mu (8 B), r read+write
@@ -2224,10 +2225,13 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         const double ent = sim->fp32 ? 4. : 8.;
         const double nbtot = (double)sim->fac.uoff.back();
         const double fwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 2. * 24. * nbtot + 4. * nbtot;
-        const double bwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 28. * nbtot;
         const double mt = sim->use_top ? (double)sim->top.m : 0.;
-        // top block: S^-1 once, b_T in, x_T out, gathered update entries (index + 24 B)
-        const double topb = ent * mt * mt + 2. * 24. * mt + 28. * (double)sim->top.cidx.size() * (sim->use_top ? 1. : 0.);
+        // with the top block's finish folded into the first backward level, x_T is written there
+        const bool folded = sim->use_top && sim->top_fold && sim->bwd.level_ptr.size() > 1;
+        const double bwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 28. * nbtot + (folded ? 24. * mt : 0.);
+        // top block: S^-1 once, b_T in, x_T out (unless folded), gathered update entries (index + 24 B)
+        const double topb = ent * mt * mt + (folded ? 24. : 48.) * mt +
+                            28. * (double)sim->top.cidx.size() * (sim->use_top ? 1. : 0.);
         const double local = 4. * 24. * n + 2. * E * 12. + nc * (72. + 8. + 48. + 19. * 36.);
         double solve = fwd + bwd + topb;
         if (sim->fp32) solve = solve * (1. + sim->prm.refinement_sweeps) +

@@ -212,6 +212,8 @@ typedef struct pdfc_info {
     int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned) */
     double lhs_seconds;              /* host time to build topology, masses, constraints and A^T A (K) */
     int64_t steps_done;              /* steps taken since creation (restored by pdfc_load_checkpoint) */
+    int32_t top_fold;                /* 1: the top block's ordered finish runs in the first backward level */
+    int32_t reserved_info;
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
