@@ -319,8 +319,8 @@ def main():
     sim.set_kernel_timing(False)
     info = sim.info()
     K = spec.iterations
-    # one launch per level, or one dependency-driven launch per sweep
-    per_sweep = 1 if info["flow"] else max(1, info["n_levels_swept"])
+    # one launch per swept level
+    per_sweep = max(1, info["n_levels_swept"])
     fam = {
         "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
         "k_forward": (kt["forward_us"], info["bytes_forward"] / per_sweep, kt["forward_launches"]),
@@ -355,11 +355,6 @@ def main():
                    "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"],
                    "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
-                   "schedule": ({"sweeps": "dependency-driven", "waves": [info["flow_waves_forward"],
-                                 info["flow_waves_backward"]], "blocks": info["flow_blocks"]}
-                                if info["flow"] else {"sweeps": "level by level"}),
-                   "autotune_pair_ms": {"levels": round(info["levels_pair_ms"], 4),
-                                        "chosen": round(info["best_pair_ms"], 4)},
                    "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"], "top_fold": info["top_fold"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,

@@ -157,12 +157,6 @@ class Info(C.Structure):
         ("waves_forward", C.c_int32 * 32),
         ("waves_backward", C.c_int32 * 32),
         ("autotune_ms", C.c_double),
-        ("flow", C.c_int32),
-        ("flow_waves_forward", C.c_int32),
-        ("flow_waves_backward", C.c_int32),
-        ("flow_blocks", C.c_int32),
-        ("levels_pair_ms", C.c_double),
-        ("best_pair_ms", C.c_double),
         ("leaf_size", C.c_int32),
         ("top_chunks", C.c_int32),
         ("lhs_seconds", C.c_double),

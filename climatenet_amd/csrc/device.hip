@@ -873,87 +873,6 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
     backward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Y, X, lds, part, fo);
 }
 
-// ------------------------------------------------ dependency-driven sweeps
-// One launch runs every task of a sweep. Blocks take tasks from a counter in
-// topological order (forward: leaves first; backward: root side first), wait until
-// the supernodes a task depends on have published all their tasks, run it, and
-// publish it. A task only waits for tasks with smaller indices, all of them taken
-// by running blocks, so the sweep always progresses; a wait that exceeds its bound
-// sets the error flag and gives up (the host reports it) instead of hanging.
-__device__ __forceinline__ void flow_wait(const DevFlow& G, int s) {
-    const int lane = threadIdx.x;
-    if (threadIdx.x < 64) {
-        for (int k = G.dep_ptr[s] + lane; k < G.dep_ptr[s + 1]; k += 64) {
-            const int c = G.dep[k];
-            const int target = G.nchunks[c];
-            int spins = 0;
-            while (__hip_atomic_load(&G.done[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > (1 << 22)) {
-                    atomicExch(G.error, 1);
-                    break;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
-__device__ __forceinline__ void flow_publish(const DevFlow& G, int s) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(&G.done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Zero the other sweep's counters (its launch has finished: stream order).
-__device__ __forceinline__ void flow_clear_other(const DevFlow& G) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < G.nclear; i += gridDim.x * blockDim.x) G.clear_done[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *G.clear_queue = 0;
-}
-
-__device__ __forceinline__ int flow_take(const DevFlow& G, int* slot) {
-    if (threadIdx.x == 0) *slot = atomicAdd(G.queue, 1);
-    __syncthreads();
-    const int t = *slot;
-    __syncthreads();
-    return t;
-}
-
-template <typename T, int W>
-__global__ void __launch_bounds__(64 * W) k_forward_flow(DevFactor Fa, DevFlow G, const double* __restrict__ Bv,
-                                                         double* __restrict__ Y) {
-    extern __shared__ double lds[];
-    __shared__ double part[W > 1 ? W : 1][3][64];
-    __shared__ int slot;
-    flow_clear_other(G);
-    for (;;) {
-        const int t = flow_take(G, &slot);
-        if (t >= G.ntasks) break;
-        const TaskDesc d = Fa.desc[t];
-        flow_wait(G, d.s);
-        forward_task<T, W>(Fa, d, Bv, Y, lds, part);
-        flow_publish(G, d.s);
-    }
-}
-
-template <typename T, int W>
-__global__ void __launch_bounds__(64 * W) k_backward_flow(DevFactor Fa, DevFlow G, const double* __restrict__ Y,
-                                                          double* __restrict__ X) {
-    extern __shared__ double lds[];
-    __shared__ double part[W > 1 ? W : 1][3][64];
-    __shared__ int slot;
-    flow_clear_other(G);
-    for (;;) {
-        const int t = flow_take(G, &slot);
-        if (t >= G.ntasks) break;
-        const TaskDesc d = Fa.desc[t];
-        flow_wait(G, d.s);
-        backward_task<T, W>(Fa, d, Y, X, lds, part, TopFold{});
-        flow_publish(G, d.s);
-    }
-}
-
 // ---------------------------------------------------------------- top block
 __global__ void k_top_gather(DevTop T, const double* __restrict__ U, const double* __restrict__ B, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1141,49 +1060,6 @@ static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const D
         case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X, fo); break;
         default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X, fo); break;
     }
-}
-
-template <typename T>
-static void forward_flow_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, const DevFlow& G,
-                                  const double* B, double* Y) {
-    switch (W) {
-        case 1: hipLaunchKernelGGL((k_forward_flow<T, 1>), g, dim3(64), lds, st, F, G, B, Y); break;
-        case 2: hipLaunchKernelGGL((k_forward_flow<T, 2>), g, dim3(128), lds, st, F, G, B, Y); break;
-        case 4: hipLaunchKernelGGL((k_forward_flow<T, 4>), g, dim3(256), lds, st, F, G, B, Y); break;
-        case 8: hipLaunchKernelGGL((k_forward_flow<T, 8>), g, dim3(512), lds, st, F, G, B, Y); break;
-        default: hipLaunchKernelGGL((k_forward_flow<T, 16>), g, dim3(1024), lds, st, F, G, B, Y); break;
-    }
-}
-template <typename T>
-static void backward_flow_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, const DevFlow& G,
-                                   const double* Y, double* X) {
-    switch (W) {
-        case 1: hipLaunchKernelGGL((k_backward_flow<T, 1>), g, dim3(64), lds, st, F, G, Y, X); break;
-        case 2: hipLaunchKernelGGL((k_backward_flow<T, 2>), g, dim3(128), lds, st, F, G, Y, X); break;
-        case 4: hipLaunchKernelGGL((k_backward_flow<T, 4>), g, dim3(256), lds, st, F, G, Y, X); break;
-        case 8: hipLaunchKernelGGL((k_backward_flow<T, 8>), g, dim3(512), lds, st, F, G, Y, X); break;
-        default: hipLaunchKernelGGL((k_backward_flow<T, 16>), g, dim3(1024), lds, st, F, G, Y, X); break;
-    }
-}
-void launch_forward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const double* B, double* Y, int max_ns,
-                         int W, int blocks, hipStream_t st) {
-    if (G.ntasks <= 0) return;
-    size_t lds = (size_t)3 * max_ns * sizeof(double);
-    dim3 g((unsigned)std::max(1, std::min(blocks, G.ntasks)));
-    if (fp32)
-        forward_flow_dispatch<float>(W, g, lds, st, F, G, B, Y);
-    else
-        forward_flow_dispatch<double>(W, g, lds, st, F, G, B, Y);
-}
-void launch_backward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const double* Y, double* X, int max_front,
-                          int W, int blocks, hipStream_t st) {
-    if (G.ntasks <= 0) return;
-    size_t lds = (size_t)3 * max_front * sizeof(double);
-    dim3 g((unsigned)std::max(1, std::min(blocks, G.ntasks)));
-    if (fp32)
-        backward_flow_dispatch<float>(W, g, lds, st, F, G, Y, X);
-    else
-        backward_flow_dispatch<double>(W, g, lds, st, F, G, Y, X);
 }
 
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y, int max_ns,

@@ -246,23 +246,6 @@ constexpr int kWarmFeatures = 12;
 void launch_warm_features(const DevModel& M, const double* qn, const double* v0, const double* cvec,
                           const int* ncontacts, int cap, DevContacts C, const DevObstacles& ob, const StepArgs& a,
                           float* feat, hipStream_t st);
-// Dependency-driven sweep: all tasks of one substitution sweep in one launch.
-struct DevFlow {
-    int ntasks;              // tasks [0, ntasks) of the sweep's descriptors, topological order
-    int* queue;              // next task to take
-    int* done;               // per supernode: published tasks
-    const int* nchunks;      // per supernode: tasks of the supernode in this sweep
-    const int* dep_ptr;      // per supernode: [dep_ptr[s], dep_ptr[s+1]) supernodes it waits for
-    const int* dep;          //   (forward: its children; backward: its parent below the top block)
-    int* clear_queue;        // the other sweep's counters, zeroed by this launch
-    int* clear_done;
-    int nclear;
-    int* error;              // set when a wait exceeds its bound
-};
-void launch_forward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const double* B, double* Y, int max_ns,
-                         int W, int blocks, hipStream_t st);
-void launch_backward_flow(const DevFactor& F, const DevFlow& G, bool fp32, const double* Y, double* X, int max_front,
-                          int W, int blocks, hipStream_t st);
 // The top block's ordered finish folded into the first backward level (the one right below the
 // top block, whose supernodes have only top-block ancestors): its tasks read the top rows they
 // gather as the chunk-ordered sum of the matvec's partial products, and `extra` workgroups of

@@ -59,7 +59,6 @@ constexpr int kCheckChunk = 8;
 constexpr int kMaxTopChunks = 16;  // reduction split of the dense top-block matvec
 static_assert(kMaxTopChunks <= 16, "the folded top finish (top_sum) reads at most 16 chunks");
 // Dependency-driven substitution sweeps (one launch per sweep): switched off, see autotune_waves().
-constexpr bool kFlowEnabled = false;
 constexpr int kReleaseWindow = 50;
 constexpr double kReleaseFactor = 0.9;
 constexpr int kStagnationWindow = 200;
@@ -127,17 +126,7 @@ struct pdfc_sim {
     pdfc::TaskDesc* desc_buf = nullptr;
     std::vector<int> fwd_waves, bwd_waves;  // waves per task, per level (autotuned)
     // dependency-driven sweeps (one launch per sweep), chosen by the autotuner when faster
-    pdfc::DevFlow ffwd{}, fbwd{};
-    std::vector<void*> flow_allocs;
-    int* flow_error = nullptr;
-    int* flow_counters = nullptr;
-    int flow_ncounters = 0;
-    bool use_flow = false;
-    int flow_wf = 4, flow_wb = 4, flow_blocks = 1024;
-    float levels_ms = 0.f, flow_ms = 0.f;   // autotuner: level-scheduled vs best schedule, one solve pair
-    void upload_flow(int top_level);
     void apply_pinned_tuning(const char* spec);
-    void check_flow_error();
     double autotune_ms = 0.;
     void autotune_waves();
     void upload_sweeps(int top_level);
@@ -791,94 +780,8 @@ void pdfc_sim::upload_sweeps(int top_level) {
     desc_buf = upload(df);
     dfwd.desc = desc_buf;
     dbwd.desc = desc_buf + nf;
-    upload_flow(top_level);
 }
 
-// Dependency lists and counters of the dependency-driven sweeps over the levels below
-// top_level: a forward task of s waits for all tasks of s's children, a backward task
-// for all tasks of s's parent when the parent is swept (the top block is solved first).
-void pdfc_sim::upload_flow(int top_level) {
-    const int S = (int)fac.ns.size();
-    std::vector<int> nch_f(S, 0), nch_b(S, 0);
-    for (int s2 : fwd.task_snode) nch_f[s2]++;
-    for (int s2 : bwd.task_snode) nch_b[s2]++;
-    std::vector<int> fptr(1, 0), fdep, bptr(1, 0), bdep;
-    for (int s2 = 0; s2 < S; ++s2) {
-        if (fac.level[s2] < top_level) {
-            for (int64_t q = fac.choff[s2]; q < fac.choff[s2 + 1]; ++q) fdep.push_back(fac.children[q]);
-            const int p = fac.parent[s2];
-            if (p >= 0 && fac.level[p] < top_level) bdep.push_back(p);
-        }
-        fptr.push_back((int)fdep.size());
-        bptr.push_back((int)bdep.size());
-    }
-    // Progress of a sweep needs every dependency of a task to sit at smaller task indices
-    // (tasks are taken in index order): check it here, once.
-    {
-        std::vector<int> first_f(S, INT32_MAX), last_f(S, -1), first_b(S, INT32_MAX), last_b(S, -1);
-        for (int t = 0; t < (int)fwd.task_snode.size(); ++t) {
-            const int s2 = fwd.task_snode[t];
-            first_f[s2] = std::min(first_f[s2], t);
-            last_f[s2] = std::max(last_f[s2], t);
-        }
-        for (int t = 0; t < (int)bwd.task_snode.size(); ++t) {
-            const int s2 = bwd.task_snode[t];
-            first_b[s2] = std::min(first_b[s2], t);
-            last_b[s2] = std::max(last_b[s2], t);
-        }
-        for (int s2 = 0; s2 < S; ++s2) {
-            for (int k = fptr[s2]; k < fptr[s2 + 1]; ++k)
-                if (last_f[fdep[k]] < 0 || last_f[fdep[k]] >= first_f[s2])
-                    throw std::logic_error("forward sweep order violates a dependency");
-            for (int k = bptr[s2]; k < bptr[s2 + 1]; ++k)
-                if (last_b[bdep[k]] < 0 || last_b[bdep[k]] >= first_b[s2])
-                    throw std::logic_error("backward sweep order violates a dependency");
-        }
-    }
-    for (void* p : flow_allocs) free_ptr(p);
-    flow_allocs.clear();
-    auto up = [&](const std::vector<int>& h) {
-        int* d = upload(h);
-        flow_allocs.push_back(d);
-        return d;
-    };
-    std::vector<int> zeros(2 * (size_t)S + 3, 0);
-    int* counters = up(zeros);  // done_f[S], done_b[S], queue_f, queue_b, error
-    ffwd = pdfc::DevFlow{};
-    fbwd = pdfc::DevFlow{};
-    ffwd.ntasks = (int)fwd.task_snode.size();
-    fbwd.ntasks = (int)bwd.task_snode.size();
-    ffwd.done = counters;
-    fbwd.done = counters + S;
-    ffwd.queue = counters + 2 * S;
-    fbwd.queue = counters + 2 * S + 1;
-    ffwd.error = fbwd.error = flow_error = counters + 2 * S + 2;
-    ffwd.nchunks = up(nch_f);
-    fbwd.nchunks = up(nch_b);
-    ffwd.dep_ptr = up(fptr);
-    ffwd.dep = up(fdep);
-    fbwd.dep_ptr = up(bptr);
-    fbwd.dep = up(bdep);
-    ffwd.clear_queue = fbwd.queue;
-    ffwd.clear_done = fbwd.done;
-    fbwd.clear_queue = ffwd.queue;
-    fbwd.clear_done = ffwd.done;
-    ffwd.nclear = fbwd.nclear = S;
-    flow_counters = counters;
-    flow_ncounters = 2 * S + 2;
-}
-
-void pdfc_sim::check_flow_error() {
-    if (!flow_error) return;
-    int e = 0;
-    HIP_CHECK(hipMemcpyAsync(&e, flow_error, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIP_CHECK(hipStreamSynchronize(stream));
-    if (e) {
-        HIP_CHECK(hipMemsetAsync(flow_error, 0, sizeof(int), stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        throw DeviceError("substitution: a dependency wait exceeded its bound");
-    }
-}
 
 // Tuning given instead of measured, "f=<waves per forward level>;b=<backward>;t=<top chunks>"
 // (comma-separated; e.g. to profile the configuration an unprofiled run chose).
@@ -1013,48 +916,6 @@ void pdfc_sim::autotune_waves() {
             }
             waves[l] = best_w;
         }
-    }
-    // Schedule: the level-by-level sweeps with the waves above, or one dependency-driven
-    // launch per sweep (waves per task and grid size timed over forward + backward pairs,
-    // which reset each other's counters).
-    // The dependency-driven schedule stalled in its first GPU run and its cause has not been
-    // found: it is switched off (requesting it is an error) and not offered to the tuner.
-    const char* fe = std::getenv("PDFC_FLOW");
-    const int flow_env = fe ? std::atoi(fe) : 0;
-    if (flow_env != 0)
-        throw std::invalid_argument("PDFC_FLOW: the dependency-driven substitution schedule is switched off");
-    use_flow = false;
-    if (kFlowEnabled && ffwd.ntasks > 0 && fbwd.ntasks > 0) {
-        float best = 1e30f;
-        if (flow_env < 0) {
-            best = time_it([&] {
-                for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
-                    pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
-                                               V[1], Y, fac.max_ns, fwd_waves[l], stream);
-                for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
-                    pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
-                                                Y, V[1], fac.max_front, bwd_waves[l], stream);
-            });
-        }
-        levels_ms = best;
-        const int wc[3] = {2, 4, 8}, bc[3] = {512, 1024, 2048};
-        for (int wf : wc)
-            for (int wb : wc)
-                for (int blocks : bc) {
-                    float ms = time_it([&] {
-                        pdfc::launch_forward_flow(dfwd, ffwd, fp32, V[1], Y, fac.max_ns, wf, blocks, stream);
-                        pdfc::launch_backward_flow(dbwd, fbwd, fp32, Y, V[1], fac.max_front, wb, blocks, stream);
-                    });
-                    if (ms < best) {
-                        best = ms;
-                        use_flow = true;
-                        flow_wf = wf;
-                        flow_wb = wb;
-                        flow_blocks = blocks;
-                    }
-                }
-        flow_ms = best;
-        check_flow_error();
     }
     HIP_CHECK(hipEventDestroy(a));
     HIP_CHECK(hipEventDestroy(b));
@@ -1262,35 +1123,22 @@ void pdfc_sim::restore_checkpoint(const unsigned char* p, size_t size) {
 void pdfc_sim::solve_inplace(double* X) {
     const size_t n3 = 3 * (size_t)n;
     auto sweep = [&](double* inout) {
-        if (use_flow) {
-            timed(1, [&] {
-                pdfc::launch_forward_flow(dfwd, ffwd, fp32, inout, Y, fac.max_ns, flow_wf, flow_blocks, stream);
-            });
-        } else {
-            // one event pair around the whole sweep: per-launch events would add their own
-            // overhead to every few-microsecond level kernel
-            timed(1, [&] {
-                for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
-                    pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
-                                               inout, Y, fac.max_ns, fwd_waves[l], stream);
-            });
-        }
+        // one event pair around the whole sweep: per-launch events would add their own
+        // overhead to every few-microsecond level kernel
+        timed(1, [&] {
+            for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
+                pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
+                                           inout, Y, fac.max_ns, fwd_waves[l], stream);
+        });
         // the top block's ordered finish runs inside the first backward level when there is one
         const bool fold = use_top && top_fold && bwd.level_ptr.size() > 1;
         if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream, !fold); });
         pdfc::TopFold tf{dtop.part, dtop.rows, dtop.top_of, dtop.m, dtop.kchunks, 0, 0};
-        if (use_flow) {
-            timed(2, [&] {
-                pdfc::launch_backward_flow(dbwd, fbwd, fp32, Y, inout, fac.max_front, flow_wb, flow_blocks, stream);
-            });
-        } else {
-            timed(2, [&] {
-                for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
-                    pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
-                                                Y, inout, fac.max_front, bwd_waves[l], stream,
-                                                fold && l == 0 ? &tf : nullptr);
-            });
-        }
+        timed(2, [&] {
+            for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
+                pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
+                                            inout, fac.max_front, bwd_waves[l], stream, fold && l == 0 ? &tf : nullptr);
+        });
         if (ktiming) ++ksolves;
     };
     if (!fp32) {
@@ -1902,7 +1750,6 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
                                  sim->stream));
         HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
         HIP_CHECK(hipEventSynchronize(sim->ev[2]));
-        sim->check_flow_error();
         float ms_total = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
         sim->stats.step_us = 1e3 * ms_total / nsteps;
@@ -2242,12 +2089,6 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.autotune_ms = sim->autotune_ms;
         I.leaf_size = sim->leaf;
         I.top_chunks = sim->use_top ? sim->dtop.kchunks : 0;
-        I.flow = sim->use_flow ? 1 : 0;
-        I.flow_waves_forward = sim->flow_wf;
-        I.flow_waves_backward = sim->flow_wb;
-        I.flow_blocks = sim->flow_blocks;
-        I.levels_pair_ms = sim->levels_ms;
-        I.best_pair_ms = sim->flow_ms;
         I.top_m = (int64_t)mt;
         I.n_levels_swept = (int64_t)(sim->fwd.level_ptr.size() - 1);
         I.bytes_local = local;
@@ -2279,8 +2120,8 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         sim->collect_kernel_times();
         pdfc_kernel_times t{};
         // sweeps are timed whole (families 1 and 2 count sweeps); a sweep is one launch per
-        // swept level, or one launch with the dependency-driven schedule
-        const int64_t per_sweep = sim->use_flow ? 1 : (int64_t)(sim->fwd.level_ptr.size() - 1);
+        // swept level
+        const int64_t per_sweep = (int64_t)(sim->fwd.level_ptr.size() - 1);
         t.local_launches = sim->kcount[0];
         t.forward_launches = sim->kcount[1] * per_sweep;
         t.backward_launches = sim->kcount[2] * per_sweep;
@@ -2302,7 +2143,6 @@ int pdfc_solve(pdfc_sim* sim, const double* b, double* x) {
         sim->solve_inplace(sim->V[1]);
         HIP_CHECK(hipGetLastError());
         sim->from_device_vec(sim->V[1], x);
-        sim->check_flow_error();
     });
 }
 

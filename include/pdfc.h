@@ -202,12 +202,6 @@ typedef struct pdfc_info {
     int32_t waves_forward[32];       /* waves per task chosen for each forward level (autotuned) */
     int32_t waves_backward[32];      /* same for the backward levels (root side first) */
     double autotune_ms;              /* time spent choosing them at creation */
-    int32_t flow;                    /* schedule: 1 one dependency-driven launch per sweep, 0 one launch per level */
-    int32_t flow_waves_forward;      /* dependency-driven sweeps: waves per task and grid size */
-    int32_t flow_waves_backward;
-    int32_t flow_blocks;
-    double levels_pair_ms;           /* autotuner: one forward + backward sweep level by level */
-    double best_pair_ms;             /* and with the chosen schedule */
     int32_t leaf_size;               /* largest leaf supernode of the dissection */
     int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned) */
     double lhs_seconds;              /* host time to build topology, masses, constraints and A^T A (K) */
