@@ -344,25 +344,35 @@ __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __rest
     D3 s = {0., 0., 0.};
     const int deg = M.sp_cnt[i];
     if (M.sp_slots <= kSpringCap) {
+        // Unused slots gather the vertex itself, so every gather is issued without a branch
+        // (a gather inside `if (k < deg)` waits for its own data before the next one starts).
         int jj[kSpringCap];
         double kap[kSpringCap];
 #pragma unroll
-        for (int k = 0; k < kSpringCap; ++k)
+        for (int k = 0; k < kSpringCap; ++k) {
+            jj[k] = i;
+            kap[k] = 0.;
             if (k < deg) {
                 jj[k] = M.sp_idx[(size_t)k * n + i];
                 kap[k] = M.sp_kappa[(size_t)k * n + i];
             }
-        D3 qj[kSpringCap];
+        }
+        double gq[kSpringCap][3], gv[kSpringCap][3];
+#pragma unroll
+        for (int k = 0; k < kSpringCap; ++k) {
+            const int j = jj[k];
+            gq[k][0] = qn[j];
+            gq[k][1] = qn[n + j];
+            gq[k][2] = qn[2 * n + j];
+            gv[k][0] = vcur[j];
+            gv[k][1] = vcur[n + j];
+            gv[k][2] = vcur[2 * n + j];
+        }
 #pragma unroll
         for (int k = 0; k < kSpringCap; ++k)
             if (k < deg) {
-                const int j = jj[k];
-                qj[k] = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
-            }
-#pragma unroll
-        for (int k = 0; k < kSpringCap; ++k)
-            if (k < deg) {
-                D3 e = normalized3(D3{q.x - qj[k].x, q.y - qj[k].y, q.z - qj[k].z});
+                const D3 qj = {gq[k][0] + dt * gv[k][0], gq[k][1] + dt * gv[k][1], gq[k][2] + dt * gv[k][2]};
+                D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
                 s.x += kap[k] * e.x;
                 s.y += kap[k] * e.y;
                 s.z += kap[k] * e.z;
@@ -387,21 +397,32 @@ __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restric
     D3 av = {0., 0., 0.};
     const int cnt = M.ata_cnt[i];
     if (M.ata_slots <= kAtaCap) {
+        // branch-free gathers, as in spring_sum
         int jj[kAtaCap];
         double w[kAtaCap];
 #pragma unroll
-        for (int k = 0; k < kAtaCap; ++k)
+        for (int k = 0; k < kAtaCap; ++k) {
+            jj[k] = i;
+            w[k] = 0.;
             if (k < cnt) {
                 jj[k] = M.ata_idx[(size_t)k * n + i];
                 w[k] = M.ata_val[(size_t)k * n + i];
             }
+        }
+        double g[kAtaCap][3];
+#pragma unroll
+        for (int k = 0; k < kAtaCap; ++k) {
+            const int j = jj[k];
+            g[k][0] = vcur[j];
+            g[k][1] = vcur[n + j];
+            g[k][2] = vcur[2 * n + j];
+        }
 #pragma unroll
         for (int k = 0; k < kAtaCap; ++k)
             if (k < cnt) {
-                const int j = jj[k];
-                av.x += w[k] * vcur[j];
-                av.y += w[k] * vcur[n + j];
-                av.z += w[k] * vcur[2 * n + j];
+                av.x += w[k] * g[k][0];
+                av.y += w[k] * g[k][1];
+                av.z += w[k] * g[k][2];
             }
         return av;
     }
@@ -415,6 +436,113 @@ __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restric
     return av;
 }
 
+// A vertex's stencils held in registers between the loads that fetch them and the sums
+// that use them (spring and A^T A rows up to kSpringCap / kAtaCap entries). Slots beyond
+// a row's length point at the vertex itself, so every gather is issued without a branch.
+struct SpringRow {
+    int deg;
+    int j[kSpringCap];
+    double kap[kSpringCap];
+};
+struct AtaRowRegs {
+    int cnt;
+    int j[kAtaCap];
+    double w[kAtaCap];
+};
+
+__device__ __forceinline__ void spring_row_load(const DevModel& M, int i, SpringRow& R) {
+    const int n = M.n;
+    R.deg = M.sp_cnt[i];
+#pragma unroll
+    for (int k = 0; k < kSpringCap; ++k) {
+        R.j[k] = i;
+        R.kap[k] = 0.;
+        if (k < M.sp_slots) {  // uniform: the loads do not wait for deg
+            R.j[k] = M.sp_idx[(size_t)k * n + i];
+            R.kap[k] = M.sp_kappa[(size_t)k * n + i];
+        }
+    }
+}
+
+// Same sum, same order as spring_sum.
+__device__ __forceinline__ D3 spring_row_sum(const SpringRow& R, const double* __restrict__ qn,
+                                             const double* __restrict__ vcur, int n, int i, double dt, D3 q) {
+    double g[kSpringCap][6];
+#pragma unroll
+    for (int k = 0; k < kSpringCap; ++k) {
+        const int j = k < R.deg ? R.j[k] : i;
+        g[k][0] = qn[j];
+        g[k][1] = qn[n + j];
+        g[k][2] = qn[2 * n + j];
+        g[k][3] = vcur[j];
+        g[k][4] = vcur[n + j];
+        g[k][5] = vcur[2 * n + j];
+    }
+    D3 s = {0., 0., 0.};
+#pragma unroll
+    for (int k = 0; k < kSpringCap; ++k)
+        if (k < R.deg) {
+            const D3 qj = {g[k][0] + dt * g[k][3], g[k][1] + dt * g[k][4], g[k][2] + dt * g[k][5]};
+            const D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
+            s.x += R.kap[k] * e.x;
+            s.y += R.kap[k] * e.y;
+            s.z += R.kap[k] * e.z;
+        }
+    return s;
+}
+
+__device__ __forceinline__ void ata_row_load(const DevModel& M, int i, AtaRowRegs& R) {
+    const int n = M.n;
+    R.cnt = M.ata_cnt[i];
+#pragma unroll
+    for (int k = 0; k < kAtaCap; ++k) {
+        R.j[k] = i;
+        R.w[k] = 0.;
+        if (k < M.ata_slots) {
+            R.j[k] = M.ata_idx[(size_t)k * n + i];
+            R.w[k] = M.ata_val[(size_t)k * n + i];
+        }
+    }
+}
+
+// Same sum, same order as ata_row.
+__device__ __forceinline__ D3 ata_row_sum(const AtaRowRegs& R, const double* __restrict__ vcur, int n, int i) {
+    double g[kAtaCap][3];
+#pragma unroll
+    for (int k = 0; k < kAtaCap; ++k) {
+        const int j = k < R.cnt ? R.j[k] : i;
+        g[k][0] = vcur[j];
+        g[k][1] = vcur[n + j];
+        g[k][2] = vcur[2 * n + j];
+    }
+    D3 av = {0., 0., 0.};
+#pragma unroll
+    for (int k = 0; k < kAtaCap; ++k)
+        if (k < R.cnt) {
+            av.x += R.w[k] * g[k][0];
+            av.y += R.w[k] * g[k][1];
+            av.z += R.w[k] * g[k][2];
+        }
+    return av;
+}
+
+// One obstacle contact's inputs to the cone projection.
+struct ContactIn {
+    double B[9];     // n, t, b
+    double sp[3];    // obstacle speed at the contact point
+    double r[3];     // previous iteration's reaction (residual bookkeeping)
+    double mu;
+};
+__device__ __forceinline__ void contact_load(const DevContacts& C, int cid, bool with_r, ContactIn& c) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.B[k] = C.basis[9 * (size_t)cid + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c.sp[k] = C.speed[3 * (size_t)cid + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c.r[k] = with_r ? C.r[3 * (size_t)cid + k] : 0.;
+    c.mu = C.mu[cid];
+}
+
 // Local step + right-hand side + obstacle friction, one thread per vertex:
 //   q = q_n + dt v;  s_i = sum_springs w^2 L0 (q_i - q_j)/|q_i - q_j|  (A^T p; bending
 //   projections vanish because the reference's rest Laplacian norm is 0);
@@ -422,24 +550,42 @@ __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restric
 //   F = b - dt^2 (A^T A v)_i at contact vertices, r = cone(B^T F), b += B r
 //                                               (SimulationFrictionEstimation.cpp:337-423)
 // The friction residual of the previous iteration (r^{k-1}, v^k) is folded in.
-__global__ void k_local(DevModel M, const double* __restrict__ qn, const double* __restrict__ vcur,
-                        const double* __restrict__ cvec, double* __restrict__ rhs, const int* __restrict__ ccnt,
-                        const int* __restrict__ cstart, DevContacts C, StepArgs a, int iteration, LocalSelf ls) {
+// The loads are issued in three dependent rounds: everything indexed by the vertex (with
+// the stencil indices), then the spring gathers, the A^T A indices and the first contact,
+// then the A^T A gathers.
+__global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __restrict__ qn,
+                                                  const double* __restrict__ vcur, const double* __restrict__ cvec,
+                                                  double* __restrict__ rhs, const int* __restrict__ ccnt,
+                                                  const int* __restrict__ cstart, DevContacts C, StepArgs a,
+                                                  int iteration, LocalSelf ls) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
     double ac_s = 0., ac_m = 0.;
     if (i < n) {
         const double dt = a.dt;
-        D3 v = ld3(vcur, n, i);
-        D3 q = {qn[i] + dt * v.x, qn[n + i] + dt * v.y, qn[2 * n + i] + dt * v.z};
-        D3 s = spring_sum(M, qn, vcur, dt, i, q);
-        int nc = ccnt[i];
+        const bool regs = M.sp_slots <= kSpringCap && M.ata_slots <= kAtaCap;
+        const bool rec = a.record && iteration > 0;
+        const int nc = ccnt[i];
         const int slot = ls.selfmode ? ls.vslot[i] : -1;
-        double damp = (nc > 0 || slot >= 0) ? 0. : a.damping;
-        D3 c = ld3(cvec, n, i);
+        const int c0 = cstart[i];
+        const D3 v = ld3(vcur, n, i);
+        const D3 qi = ld3(qn, n, i);
+        const D3 c = ld3(cvec, n, i);
+        const double mi = M.mass[i];
+        SpringRow SR;
+        if (regs) spring_row_load(M, i, SR);
+        const bool contact = nc > 0 || slot >= 0;
+        AtaRowRegs AR;
+        AR.cnt = 0;
+        ContactIn first;
+        if (contact && regs) ata_row_load(M, i, AR);
+        if (nc > 0) contact_load(C, c0, rec, first);
+        const D3 q = {qi.x + dt * v.x, qi.y + dt * v.y, qi.z + dt * v.z};
+        const D3 s = regs ? spring_row_sum(SR, qn, vcur, n, i, dt, q) : spring_sum(M, qn, vcur, dt, i, q);
+        double damp = contact ? 0. : a.damping;
         D3 b = {c.x + dt * (s.x - damp * v.x), c.y + dt * (s.y - damp * v.y), c.z + dt * (s.z - damp * v.z)};
-        if (nc > 0 || slot >= 0) {
-            D3 av = ata_row(M, vcur, i);
+        if (contact) {
+            const D3 av = regs ? ata_row_sum(AR, vcur, n, i) : ata_row(M, vcur, i);
             const double dt2 = dt * dt;
             D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
             D3 Fs = F;  // what the obstacle contacts see
@@ -449,32 +595,34 @@ __global__ void k_local(DevModel M, const double* __restrict__ qn, const double*
                 ls.Fc[3 * slot + 2] = F.z;
                 Fs = {F.x + ls.SCFc[3 * slot], F.y + ls.SCFc[3 * slot + 1], F.z + ls.SCFc[3 * slot + 2]};
             }
-            int c0 = cstart[i];
             for (int k = 0; k < nc; ++k) {
-                int cid = c0 + k;
-                const double* B = &C.basis[9 * cid];
+                const int cid = c0 + k;
+                ContactIn ci;
+                if (k == 0)
+                    ci = first;
+                else
+                    contact_load(C, cid, rec, ci);
                 double* r = &C.r[3 * cid];
-                D3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
-                double mu = C.mu[cid];
-                const D3 sp = {C.speed[3 * cid], C.speed[3 * cid + 1], C.speed[3 * cid + 2]};
-                if (a.record && iteration > 0) {
+                const D3 bn = {ci.B[0], ci.B[1], ci.B[2]}, bt = {ci.B[3], ci.B[4], ci.B[5]},
+                         bb = {ci.B[6], ci.B[7], ci.B[8]};
+                const double mu = ci.mu;
+                const D3 sp = {ci.sp[0], ci.sp[1], ci.sp[2]};
+                if (rec) {
                     const D3 u = {v.x - sp.x, v.y - sp.y, v.z - sp.z};
-                    double an = ac_norm(D3{dot3(bn, u), dot3(bt, u), dot3(bb, u)}, r, mu);
+                    double an = ac_norm(D3{dot3(bn, u), dot3(bt, u), dot3(bb, u)}, ci.r, mu);
                     ac_s += an;
                     ac_m = fmax(ac_m, an);
                 }
                 // f = B^T (F - m u_obstacle)  (SimulationFrictionEstimation.cpp:360-372)
-                const double mi = M.mass[i];
                 const D3 Fo = {Fs.x - mi * sp.x, Fs.y - mi * sp.y, Fs.z - mi * sp.z};
                 D3 fl = {dot3(bn, Fo), dot3(bt, Fo), dot3(bb, Fo)};
                 double r0, r1, r2;
                 bool warm = false;
                 if (a.warm_mode != 0) {  // the host clears warm_mode once the hold ends
                     const float* y = a.warm_y + (size_t)cid * a.warm_stride;
-                    const double m = M.mass[i];
-                    r0 = m * (double)y[0];
-                    r1 = m * (double)y[1];
-                    r2 = m * (double)y[2];
+                    r0 = mi * (double)y[0];
+                    r1 = mi * (double)y[1];
+                    r2 = mi * (double)y[2];
                     warm = a.warm_mode == 2 || r0 != 0. || r1 != 0. || r2 != 0.;
                 }
                 if (warm ? !(r0 > 0.) : fl.x > 0.) {
@@ -749,7 +897,37 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
 // k_top_finish sums them.
 // All the loads are issued before the first add (at most kTopChunksMax chunks).
 constexpr int kTopChunksMax = 16;
+// Symmetric tiles: the nb source blocks of row t in order, 16 loads in flight at a time.
+__device__ __forceinline__ void top_sum_blocks(const double* part, int nb, int t, double& x, double& y, double& z) {
+    const double* P = part + (size_t)(t >> 6) * nb * 192 + (t & 63);
+    x = 0.;
+    y = 0.;
+    z = 0.;
+    for (int c0 = 0; c0 < nb; c0 += kTopChunksMax) {
+        double px[kTopChunksMax], py[kTopChunksMax], pz[kTopChunksMax];
+#pragma unroll
+        for (int c = 0; c < kTopChunksMax; ++c)
+            if (c0 + c < nb) {
+                const double* Q = P + (size_t)(c0 + c) * 192;
+                px[c] = Q[0];
+                py[c] = Q[64];
+                pz[c] = Q[128];
+            }
+#pragma unroll
+        for (int c = 0; c < kTopChunksMax; ++c)
+            if (c0 + c < nb) {
+                x += px[c];
+                y += py[c];
+                z += pz[c];
+            }
+    }
+}
+
 __device__ __forceinline__ void top_sum(const TopFold& fo, int t, double& x, double& y, double& z) {
+    if (fo.nb > 0) {
+        top_sum_blocks(fo.part, fo.nb, t, x, y, z);
+        return;
+    }
     double px[kTopChunksMax], py[kTopChunksMax], pz[kTopChunksMax];
 #pragma unroll
     for (int c = 0; c < kTopChunksMax; ++c)
@@ -927,15 +1105,105 @@ __global__ void __launch_bounds__(1024) k_top_matvec(DevTop Tp) {
     P[2 * m + row] = az;
 }
 
+// Symmetric tile products, one workgroup (4 waves) per block pair I <= J of S^-1 (A below;
+// A(r, k) = Sinv[k m + r]). Wave w holds columns w, w + 4, ... of the 64 x 64 block in
+// registers (lane = row), so the block is read from memory once:
+//   rows I:  y_I(r) = sum_c A(r0 + r, c0 + c) b_J(c)      (waves summed 0..3, then the block's
+//   rows J:  y_J(c) = sum_r A(r0 + r, c0 + c) b_I(r)       partial goes to (I, J) / (J, I))
+// b entries are gathered here (b_T - lower updates, as k_top_gather).
+template <typename T>
+__global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __restrict__ U,
+                                                  const double* __restrict__ B, int n) {
+    __shared__ double blk[64][65];
+    __shared__ double xb[2][3][64];
+    __shared__ double red[4][3][64];
+    const int I = Tp.tiles[2 * blockIdx.x], J = Tp.tiles[2 * blockIdx.x + 1];
+    const int m = Tp.m, nb = Tp.nb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 64 * I, c0 = 64 * J;
+    const int row = r0 + lane;
+    const T* S = static_cast<const T*>(Tp.Sinv);
+    double a[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int col = c0 + wave + 4 * s;
+        a[s] = (row < m && col < m) ? (double)S[(size_t)col * m + row] : 0.;
+    }
+    if (wave < 2 && (wave == 0 || I != J)) {
+        const int t = (wave == 0 ? r0 : c0) + lane;
+        double bx = 0., by = 0., bz = 0.;
+        if (t < m) {
+            const int rw = Tp.rows[t];
+            bx = B[rw];
+            by = B[n + rw];
+            bz = B[2 * n + rw];
+            for (int q = Tp.cptr[t]; q < Tp.cptr[t + 1]; ++q) {
+                const double* u = U + 3 * (size_t)Tp.cidx[q];
+                bx -= u[0];
+                by -= u[1];
+                bz -= u[2];
+            }
+        }
+        xb[wave][0][lane] = bx;
+        xb[wave][1][lane] = by;
+        xb[wave][2][lane] = bz;
+    }
+    __syncthreads();
+    const int jb = I != J ? 1 : 0;  // b_J
+    double ax = 0., ay = 0., az = 0.;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int c = wave + 4 * s;
+        ax = fma(a[s], xb[jb][0][c], ax);
+        ay = fma(a[s], xb[jb][1][c], ay);
+        az = fma(a[s], xb[jb][2][c], az);
+        if (I != J) blk[lane][c] = a[s];
+    }
+    red[wave][0][lane] = ax;
+    red[wave][1][lane] = ay;
+    red[wave][2][lane] = az;
+    __syncthreads();
+    if (wave == 0) {
+        double* P = Tp.part + ((size_t)I * nb + J) * 192 + lane;
+        for (int k = 0; k < 3; ++k) P[64 * k] = ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
+    }
+    if (I == J) return;
+    __syncthreads();  // red is reused
+    // rows J: lane = column c, wave w sums rows 16 w .. 16 w + 15
+    ax = 0.;
+    ay = 0.;
+    az = 0.;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int r = 16 * wave + s;
+        const double f = blk[r][lane];
+        ax = fma(f, xb[0][0][r], ax);
+        ay = fma(f, xb[0][1][r], ay);
+        az = fma(f, xb[0][2][r], az);
+    }
+    red[wave][0][lane] = ax;
+    red[wave][1][lane] = ay;
+    red[wave][2][lane] = az;
+    __syncthreads();
+    if (wave == 0) {
+        double* P = Tp.part + ((size_t)J * nb + I) * 192 + lane;
+        for (int k = 0; k < 3; ++k) P[64 * k] = ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
+    }
+}
+
 __global__ void k_top_finish(DevTop T, double* __restrict__ X, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T.m) return;
     double ax = 0., ay = 0., az = 0.;
-    for (int c = 0; c < T.kchunks; ++c) {
-        const double* P = T.part + (size_t)c * 3 * T.m;
-        ax += P[t];
-        ay += P[T.m + t];
-        az += P[2 * T.m + t];
+    if (T.nb > 0) {
+        top_sum_blocks(T.part, T.nb, t, ax, ay, az);
+    } else {
+        for (int c = 0; c < T.kchunks; ++c) {
+            const double* P = T.part + (size_t)c * 3 * T.m;
+            ax += P[t];
+            ay += P[T.m + t];
+            az += P[2 * T.m + t];
+        }
     }
     int row = T.rows[t];
     X[row] = ax;
@@ -1089,6 +1357,14 @@ void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, co
 void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
                 bool finish) {
     if (T.m <= 0) return;
+    if (T.nb > 0) {
+        if (fp32)
+            hipLaunchKernelGGL(k_top_tile<float>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
+        else
+            hipLaunchKernelGGL(k_top_tile<double>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
+        if (finish) hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
+        return;
+    }
     hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
     dim3 g((T.m + 63) / 64, T.kchunks);
     if (fp32)

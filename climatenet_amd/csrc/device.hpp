@@ -31,10 +31,12 @@ struct DevMeshObs {
     const double* next;
     double inv_dt;          // 1 / (t_next - t_prev)
     double tol, tol_sq, t11;
-    double h;               // grid cell size
+    const unsigned long long* ext_bits;  // largest grown triangle box extent (bit pattern of a
+                                         // non-negative double); cell size max(ext (1 + 1e-9), 1e-12)
     int P;                  // hash buckets (power of two)
     const int* bucket_off;  // P + 1
     const int* bucket_tris;
+    const double* bucket_pos;  // 9 per bucket slot: the triangle's vertex positions
     int* hit_tri;           // per cloth vertex: closest triangle within the tolerance, or -1
     double* hit_bary;       // 3 per cloth vertex
 };
@@ -59,6 +61,28 @@ struct DevObstacles {
 // Closest triangle of mesh obstacle m for every cloth vertex (positions qn, planar
 // supernodal layout): hit_tri / hit_bary.
 void launch_mesh_query(const DevObstacles& ob, int m, const double* qn, int n, hipStream_t st);
+
+// Device build of a mesh obstacle's triangle grid from its current positions: every
+// triangle is registered in each cell its box, grown by grow, overlaps (at most 2 x 2 x 2
+// cells, since the cell is wider than any grown box).
+struct DevMeshBuild {
+    int T, P;
+    const int* tris;                // 3T
+    const double* cur;              // 3 nv, xyz interleaved
+    double grow;                    // 1.1 tol (1 + 1e-9)
+    double* tbox;                   // 6T: grown lo xyz, hi xyz
+    unsigned long long* ext_bits;   // largest grown extent
+    int* off;                       // P + 1: counts, then their exclusive scan
+    int* fill;                      // P
+    int* tris_sorted;               // 8T
+    double* pos_sorted;             // 9 per slot of tris_sorted
+    int* scan_sums;                 // one per 256 entries of off, plus one
+    int* scan_total;
+};
+void launch_mesh_build(const DevMeshBuild& B, hipStream_t st);
+// cur = prev (1 - a) + next a, elementwise over count doubles (MovingMesh::update's
+// interpolation between two key frames, the host's operation order).
+void launch_mesh_lerp(double* cur, const double* prev, const double* next, double a, int count, hipStream_t st);
 
 // Obstacle contact record (structure of arrays, capacity n * n_obstacles).
 struct DevContacts {
@@ -251,10 +275,11 @@ void launch_warm_features(const DevModel& M, const double* qn, const double* v0,
 // gather as the chunk-ordered sum of the matvec's partial products, and `extra` workgroups of
 // the same launch write every top row's final value for the later levels and the result.
 struct TopFold {
-    const double* part;   // kchunks * 3 * m partial products; null: no fold
+    const double* part;   // partial products (layout of DevTop::part); null: no fold
     const int* rows;      // m supernodal indices of the top rows
     const int* top_of;    // n: top index of a supernodal row, -1 elsewhere
     int m, kchunks, ntasks, extra;
+    int nb;               // DevTop::nb
 };
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
 // waves: 1, 2, 4, 8 or 16 waves per task splitting the reduction (ns forward, ns + nb backward).
@@ -273,11 +298,20 @@ struct DevTop {
     const int* cidx;        // update-vector entries to subtract
     const void* Sinv;       // m x m, double or float
     double* bt;             // 3 m: right-hand side of the top block (planar)
-    double* part;           // kchunks * 3 * m partial products
+    // Partial products. Split matvec: kchunks x 3 x m. Symmetric tiles (nb > 0): for row
+    // block R and source block C, 3 x 64 values at ((R nb + C) 3 + component) 64.
+    double* part;
     const int* top_of;      // n: top index of a supernodal row, -1 elsewhere
+    int nb;                 // > 0: symmetric tile products over nb blocks of 64 rows; 0: split matvec
+    int ntiles;             // nb (nb + 1) / 2 block pairs I <= J
+    const int* tiles;       // 2 per tile: I, J
 };
-// x_T = S^-1 (b_T - lower updates): gather, split matvec, ordered reduction + scatter
-// (the last step is left out with finish = false: the first backward level does it, see TopFold).
+// x_T = S^-1 (b_T - lower updates). Split matvec: gather, partial products over kchunks
+// column ranges, ordered reduction + scatter. Symmetric tiles: one workgroup per block pair
+// I <= J reads the 64 x 64 block S_IJ once, gathers its own b entries and writes both
+// S_IJ b_J (rows I) and S_IJ^T b_I (rows J; S^-1 is symmetric), so only the upper half of
+// S^-1 is read. The reduction + scatter is left out with finish = false: the first backward
+// level does it, see TopFold.
 void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
                 bool finish = true);
 // Unit right-hand sides e_{rows[j+c]} in plane c (c = 0..2) and extraction of the

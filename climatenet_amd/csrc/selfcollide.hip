@@ -584,43 +584,152 @@ inline unsigned nblk(long long c) { return (unsigned)((c + kB - 1) / kB); }
 // vertex's grid cell pass the closed box test against x +- 1.1 tol and the exact test
 // |cp - x|^2 < tol^2 (early exit against tol); the closest of their points
 // getTriangle(t) * bary wins, the lower triangle index on ties.
-__global__ void k_mesh_query(DevMeshObs Mo, const double* __restrict__ qn, int n) {
+__device__ __forceinline__ double mesh_cell(const unsigned long long* ext_bits) {
+    const double h = __longlong_as_double((long long)*ext_bits) * (1. + 1e-9);
+    return h > 1e-12 ? h : 1e-12;
+}
+
+__global__ void k_mesh_lerp(double* __restrict__ cur, const double* __restrict__ prev,
+                            const double* __restrict__ next, double a, int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) cur[k] = prev[k] * (1. - a) + next[k] * a;
+}
+
+__global__ void k_mesh_clear(DevMeshBuild B) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const P3 x = at_perm(qn, n, i);
-    const long long ix = cell_of(x.x, Mo.h), iy = cell_of(x.y, Mo.h), iz = cell_of(x.z, Mo.h);
-    const unsigned bkt = cell_hash(ix, iy, iz, Mo.P);
+    if (i <= B.P) B.off[i] = 0;
+    if (i < B.P) B.fill[i] = 0;
+    if (i == 0) *B.ext_bits = 0ull;
+}
+
+// Grown triangle boxes and their largest extent.
+__global__ void k_mesh_box(DevMeshBuild B) {
+    __shared__ double red[kB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double ext = 0.;
+    if (t < B.T) {
+        const int* f = &B.tris[3 * t];
+        for (int k = 0; k < 3; ++k) {
+            const double a = B.cur[3 * f[0] + k], b = B.cur[3 * f[1] + k], c = B.cur[3 * f[2] + k];
+            const double lo = fmin(fmin(a, b), c) - B.grow, hi = fmax(fmax(a, b), c) + B.grow;
+            B.tbox[6 * t + k] = lo;
+            B.tbox[6 * t + 3 + k] = hi;
+            ext = fmax(ext, hi - lo);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) ext = fmax(ext, __shfl_xor(ext, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ext;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
+        atomicMax(B.ext_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// The distinct buckets of the cells a grown box overlaps (at most 8).
+__device__ __forceinline__ int mesh_buckets(const DevMeshBuild& B, int t, double h, unsigned* bk) {
+    long long c0[3], c1[3];
+    for (int k = 0; k < 3; ++k) {
+        c0[k] = cell_of(B.tbox[6 * t + k], h);
+        c1[k] = cell_of(B.tbox[6 * t + 3 + k], h);
+        if (c1[k] > c0[k] + 1) c1[k] = c0[k] + 1;  // h exceeds every grown extent
+    }
+    int nb = 0;
+    for (long long a = c0[0]; a <= c1[0]; ++a)
+        for (long long b = c0[1]; b <= c1[1]; ++b)
+            for (long long c = c0[2]; c <= c1[2]; ++c) {
+                const unsigned x = cell_hash(a, b, c, B.P);
+                bool seen = false;
+                for (int j = 0; j < nb; ++j) seen |= bk[j] == x;
+                if (!seen) bk[nb++] = x;
+            }
+    return nb;
+}
+
+__global__ void k_mesh_count(DevMeshBuild B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B.T) return;
+    unsigned bk[8];
+    const int nb = mesh_buckets(B, t, mesh_cell(B.ext_bits), bk);
+    for (int j = 0; j < nb; ++j) atomicAdd(&B.off[bk[j]], 1);
+}
+
+// Triangle index and vertex positions into every bucket slot of the triangle. Bucket
+// order is arbitrary: the query keeps the nearest triangle, ties to the lowest index.
+__global__ void k_mesh_scatter(DevMeshBuild B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B.T) return;
+    unsigned bk[8];
+    const int nb = mesh_buckets(B, t, mesh_cell(B.ext_bits), bk);
+    const int* f = &B.tris[3 * t];
+    double x[9];
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) x[3 * v + k] = B.cur[3 * f[v] + k];
+    for (int j = 0; j < nb; ++j) {
+        const int s = B.off[bk[j]] + atomicAdd(&B.fill[bk[j]], 1);
+        B.tris_sorted[s] = t;
+        for (int k = 0; k < 9; ++k) B.pos_sorted[9 * (size_t)s + k] = x[k];
+    }
+}
+
+// kMeshLanes lanes per cloth vertex share its bucket (lane j takes slots j, j + kMeshLanes,
+// ...); the nearest hit, ties to the lowest triangle index, is reduced across the lanes.
+constexpr int kMeshLanes = 8;
+__global__ void k_mesh_query(DevMeshObs Mo, const double* __restrict__ qn, int n) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = g / kMeshLanes, lane = g % kMeshLanes;
+    const bool live = i < n;  // every lane of the wave takes part in the reduction
     double best = 1.0 / 0.0;
     int bt = -1;
     double bw[3] = {0., 0., 0.};
-    for (int k = Mo.bucket_off[bkt]; k < Mo.bucket_off[bkt + 1]; ++k) {
-        const int t = Mo.bucket_tris[k];
-        const int* f = &Mo.tris[3 * t];
-        const P3 a = {Mo.cur[3 * f[0]], Mo.cur[3 * f[0] + 1], Mo.cur[3 * f[0] + 2]};
-        const P3 b = {Mo.cur[3 * f[1]], Mo.cur[3 * f[1] + 1], Mo.cur[3 * f[1] + 2]};
-        const P3 c = {Mo.cur[3 * f[2]], Mo.cur[3 * f[2] + 1], Mo.cur[3 * f[2] + 2]};
-        const P3 lo = {fmin(fmin(a.x, b.x), c.x), fmin(fmin(a.y, b.y), c.y), fmin(fmin(a.z, b.z), c.z)};
-        const P3 hi = {fmax(fmax(a.x, b.x), c.x), fmax(fmax(a.y, b.y), c.y), fmax(fmax(a.z, b.z), c.z)};
-        if (!(lo.x <= x.x + Mo.t11 && x.x - Mo.t11 <= hi.x && lo.y <= x.y + Mo.t11 && x.y - Mo.t11 <= hi.y &&
-              lo.z <= x.z + Mo.t11 && x.z - Mo.t11 <= hi.z))
-            continue;
-        double w[3];
-        P3 cp;
-        closest_point(x, a, b, c, Mo.tol, w, cp);
-        const P3 dcp = psub(cp, x);
-        if (!(pdot(dcp, dcp) < Mo.tol_sq)) continue;
-        const P3 pt = {(a.x * w[0] + b.x * w[1]) + c.x * w[2], (a.y * w[0] + b.y * w[1]) + c.y * w[2],
-                       (a.z * w[0] + b.z * w[1]) + c.z * w[2]};
-        const P3 dp = psub(pt, x);
-        const double d = pdot(dp, dp);
-        if (d < best || (d == best && t < bt)) {
-            best = d;
-            bt = t;
-            bw[0] = w[0];
-            bw[1] = w[1];
-            bw[2] = w[2];
+    if (live) {
+        const P3 x = at_perm(qn, n, i);
+        const double h = mesh_cell(Mo.ext_bits);
+        const long long ix = cell_of(x.x, h), iy = cell_of(x.y, h), iz = cell_of(x.z, h);
+        const unsigned bkt = cell_hash(ix, iy, iz, Mo.P);
+        const int k1 = Mo.bucket_off[bkt + 1];
+        for (int k = Mo.bucket_off[bkt] + lane; k < k1; k += kMeshLanes) {
+            const double* r = &Mo.bucket_pos[9 * (size_t)k];
+            const P3 a = {r[0], r[1], r[2]}, b = {r[3], r[4], r[5]}, c = {r[6], r[7], r[8]};
+            const int t = Mo.bucket_tris[k];
+            const P3 lo = {fmin(fmin(a.x, b.x), c.x), fmin(fmin(a.y, b.y), c.y), fmin(fmin(a.z, b.z), c.z)};
+            const P3 hi = {fmax(fmax(a.x, b.x), c.x), fmax(fmax(a.y, b.y), c.y), fmax(fmax(a.z, b.z), c.z)};
+            if (!(lo.x <= x.x + Mo.t11 && x.x - Mo.t11 <= hi.x && lo.y <= x.y + Mo.t11 && x.y - Mo.t11 <= hi.y &&
+                  lo.z <= x.z + Mo.t11 && x.z - Mo.t11 <= hi.z))
+                continue;
+            double w[3];
+            P3 cp;
+            closest_point(x, a, b, c, Mo.tol, w, cp);
+            const P3 dcp = psub(cp, x);
+            if (!(pdot(dcp, dcp) < Mo.tol_sq)) continue;
+            const P3 pt = {(a.x * w[0] + b.x * w[1]) + c.x * w[2], (a.y * w[0] + b.y * w[1]) + c.y * w[2],
+                           (a.z * w[0] + b.z * w[1]) + c.z * w[2]};
+            const P3 dp = psub(pt, x);
+            const double d = pdot(dp, dp);
+            if (d < best || (d == best && t < bt)) {
+                best = d;
+                bt = t;
+                bw[0] = w[0];
+                bw[1] = w[1];
+                bw[2] = w[2];
+            }
         }
     }
+    for (int o = 1; o < kMeshLanes; o <<= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int ot = __shfl_xor(bt, o, 64);
+        const double o0 = __shfl_xor(bw[0], o, 64), o1 = __shfl_xor(bw[1], o, 64), o2 = __shfl_xor(bw[2], o, 64);
+        // a lane without a hit holds (inf, -1) and never wins against a hit
+        if (ot >= 0 && (bt < 0 || ob < best || (ob == best && ot < bt))) {
+            best = ob;
+            bt = ot;
+            bw[0] = o0;
+            bw[1] = o1;
+            bw[2] = o2;
+        }
+    }
+    if (!live || lane != 0) return;
     const bool hit = bt >= 0 && best < Mo.tol_sq;
     Mo.hit_tri[i] = hit ? bt : -1;
     Mo.hit_bary[3 * i] = bw[0];
@@ -631,7 +740,22 @@ __global__ void k_mesh_query(DevMeshObs Mo, const double* __restrict__ qn, int n
 }  // namespace
 
 void launch_mesh_query(const DevObstacles& ob, int m, const double* qn, int n, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_mesh_query, dim3((n + kB - 1) / kB), dim3(kB), 0, st, ob.mesh[m], qn, n);
+    static_assert(kB % kMeshLanes == 0 && 64 % kMeshLanes == 0, "a vertex's lanes share one wave");
+    if (n > 0) hipLaunchKernelGGL(k_mesh_query, dim3(nblk((long long)n * kMeshLanes)), dim3(kB), 0, st, ob.mesh[m], qn, n);
+}
+
+void launch_mesh_lerp(double* cur, const double* prev, const double* next, double a, int count, hipStream_t st) {
+    if (count > 0) hipLaunchKernelGGL(k_mesh_lerp, dim3(nblk(count)), dim3(kB), 0, st, cur, prev, next, a, count);
+}
+
+void launch_mesh_build(const DevMeshBuild& B, hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_clear, dim3(nblk(B.P + 1)), dim3(kB), 0, st, B);
+    if (B.T > 0) {
+        hipLaunchKernelGGL(k_mesh_box, dim3(nblk(B.T)), dim3(kB), 0, st, B);
+        hipLaunchKernelGGL(k_mesh_count, dim3(nblk(B.T)), dim3(kB), 0, st, B);
+    }
+    launch_scan_int(B.off, B.P + 1, B.scan_sums, B.scan_total, nullptr, st);
+    if (B.T > 0) hipLaunchKernelGGL(k_mesh_scatter, dim3(nblk(B.T)), dim3(kB), 0, st, B);
 }
 
 void launch_fill_int(int* p, int count, int value, hipStream_t st) {

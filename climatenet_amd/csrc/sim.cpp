@@ -79,19 +79,22 @@ struct pdfc_sim {
         double tol = 0.;                // sqrt(tol * tol)
         double time = 0.;
         int next = 1;
-        std::vector<double> cur;        // 3 nv
-        bool dirty = true;
+        std::vector<double> cur;        // 3 nv: positions to upload (creation, checkpoint restore)
+        bool upload = true;             // cur must go to the device
+        bool lerp = false;              // the device interpolates key frames lerp_frame, +1 at lerp_a
+        int lerp_frame = 0;
+        double lerp_a = 0.;
+        bool dirty = true;              // the triangle grid must be rebuilt
         double* d_frames = nullptr;
         double* d_cur = nullptr;
-        int* d_off = nullptr;
-        int* d_tris_sorted = nullptr;
-        size_t cap_sorted = 0;
+        pdfc::DevMeshBuild build{};
         int frames_count() const { return (int)(frames.size() / (3 * (size_t)nv)); }
         bool in_movement() const { return frames_count() >= 2 && time >= times[next - 1] && time <= times[next]; }
     };
     std::vector<HostMesh> meshes;
     bool moving_mesh = false;
     void mesh_refresh();
+    void mesh_positions();
     void mesh_advance(double dt);
     std::vector<double> obs_mu;         // per obstacle, against its scene's cloth
     std::vector<int> obs_ptr;           // per scene: its obstacles
@@ -129,11 +132,23 @@ struct pdfc_sim {
     void apply_pinned_tuning(const char* spec);
     double autotune_ms = 0.;
     void autotune_waves();
+    int heuristic_chunks = 1;          // split of the matvec by the workgroup-count heuristic
+    int top_blocks = 0;                // 64-row blocks of the top block
+    bool top_sym = true;               // symmetric tiles are a candidate (PDFC_TOP_SYM=0: never)
+    void set_top_mode(int mode) {      // 0: symmetric tiles; k > 0: split matvec over k ranges
+        if (mode == 0) {
+            dtop.nb = top_blocks;
+        } else {
+            dtop.nb = 0;
+            dtop.kchunks = mode;
+        }
+    }
     void upload_sweeps(int top_level);
     void build_top();
     // kernel-timing instrumentation
     bool ktiming = false;
     bool top_fold = true;           // the top block's finish inside the first backward level
+    bool top_fold_allowed = true;   // PDFC_TOP_FOLD=0: never folded
     std::vector<hipEvent_t> kev;        // event pool, two per timed launch
     std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level, 3 top block
     size_t kused = 0;
@@ -584,8 +599,24 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
             int P = 1;
             while (P < 16 * hm.T) P <<= 1;
             D.P = P;
-            hm.d_off = dalloc<int>(P + 1);
-            D.bucket_off = hm.d_off;
+            pdfc::DevMeshBuild& B = hm.build;
+            B.T = hm.T;
+            B.P = P;
+            B.tris = D.tris;
+            B.cur = hm.d_cur;
+            B.grow = D.t11 * (1. + 1e-9);
+            B.tbox = dalloc<double>(6 * (size_t)std::max(1, hm.T));
+            B.ext_bits = dalloc<unsigned long long>(1);
+            B.off = dalloc<int>((size_t)P + 1);
+            B.fill = dalloc<int>((size_t)P);
+            B.tris_sorted = dalloc<int>(8 * (size_t)std::max(1, hm.T));
+            B.pos_sorted = dalloc<double>(72 * (size_t)std::max(1, hm.T));
+            B.scan_sums = dalloc<int>(((size_t)P + 1 + 255) / 256 + 1);
+            B.scan_total = dalloc<int>(1);
+            D.ext_bits = B.ext_bits;
+            D.bucket_off = B.off;
+            D.bucket_tris = B.tris_sorted;
+            D.bucket_pos = B.pos_sorted;
         }
         if (n_scenes > 1) {
             std::vector<int> vinst(n);
@@ -698,7 +729,10 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         use_top = true;
     }
     {
-        if (const char* tf = std::getenv("PDFC_TOP_FOLD")) top_fold = std::atoi(tf) != 0;
+        if (const char* tf = std::getenv("PDFC_TOP_FOLD")) top_fold_allowed = std::atoi(tf) != 0;
+        top_fold = top_fold_allowed;
+        if (const char* ts = std::getenv("PDFC_TOP_SYM")) top_sym = std::atoi(ts) != 0;
+        if (use_top) set_top_mode(top_sym ? 0 : heuristic_chunks);
         const char* pin = std::getenv("PDFC_TUNE");
         const char* e = std::getenv("PDFC_AUTOTUNE");
         if (pin && *pin)
@@ -783,8 +817,10 @@ void pdfc_sim::upload_sweeps(int top_level) {
 }
 
 
-// Tuning given instead of measured, "f=<waves per forward level>;b=<backward>;t=<top chunks>"
-// (comma-separated; e.g. to profile the configuration an unprofiled run chose).
+// Tuning given instead of measured, "f=<waves per forward level>;b=<backward>;t=<top mode>;
+// o=<fold>" (comma-separated; e.g. to profile the configuration an unprofiled run chose). Top
+// mode 0: symmetric tiles; 1..16: split matvec over that many column ranges. o=1: the top
+// block's finish runs in the first backward level, o=0: in a kernel of its own.
 void pdfc_sim::apply_pinned_tuning(const char* spec) {
     std::string str(spec);
     auto parse_list = [&](char key, std::vector<int>& out) {
@@ -817,9 +853,15 @@ void pdfc_sim::apply_pinned_tuning(const char* spec) {
             if (!valid_w(w)) throw std::invalid_argument("PDFC_TUNE: waves must be 1, 2, 4, 8 or 16");
         bwd_waves = b;
     }
+    std::vector<int> o;
+    if (parse_list('o', o) && use_top) {
+        if (o.size() != 1 || (o[0] != 0 && o[0] != 1)) throw std::invalid_argument("PDFC_TUNE: o=0 or o=1");
+        top_fold = o[0] == 1 && top_fold_allowed;
+    }
     if (parse_list('t', t) && use_top) {
-        if (t.size() != 1 || t[0] < 1 || t[0] > kMaxTopChunks) throw std::invalid_argument("PDFC_TUNE: top chunks 1..16");
-        dtop.kchunks = t[0];
+        if (t.size() != 1 || t[0] < 0 || t[0] > kMaxTopChunks)
+            throw std::invalid_argument("PDFC_TUNE: top chunks 0 (symmetric tiles) or 1..16");
+        set_top_mode(t[0]);
     }
 }
 
@@ -885,19 +927,39 @@ void pdfc_sim::autotune_waves() {
     // faster), then one pass over the levels trying each wave count with the others fixed.
     auto solve_ms = [&] { return time_it([&] { solve_inplace(V[1]); }); };
     if (use_top) {
-        const int heuristic = dtop.kchunks;
+        // mode 0: symmetric tiles (half of S^-1 read, but nb partial sums per row to finish);
+        // k > 0: split matvec over k column ranges. Each with the finish folded into the first
+        // backward level or in a kernel of its own: the fold puts the partial sums on the
+        // critical path of every backward task that reads a top row.
+        struct Cand {
+            int mode;
+            bool fold;
+        };
+        const Cand first{top_sym ? 0 : dtop.kchunks, top_fold_allowed};
+        std::vector<Cand> cands;
+        for (int mode : {0, heuristic_chunks, 4, 8, 16})
+            for (bool fold : {true, false}) {
+                if ((mode == 0 && !top_sym) || mode > kMaxTopChunks || (fold && !top_fold_allowed)) continue;
+                if (mode == first.mode && fold == first.fold) continue;
+                bool dup = false;
+                for (const Cand& c : cands) dup |= c.mode == mode && c.fold == fold;
+                if (!dup) cands.push_back({mode, fold});
+            }
+        set_top_mode(first.mode);
+        top_fold = first.fold;
         float best = solve_ms();
-        int best_kc = heuristic;
-        for (int kc : {4, 8, 16}) {
-            if (kc == heuristic || kc > kMaxTopChunks) continue;
-            dtop.kchunks = kc;
+        Cand chosen = first;
+        for (const Cand& c : cands) {
+            set_top_mode(c.mode);
+            top_fold = c.fold;
             float ms = solve_ms();
             if (ms < kSwitch * best) {
                 best = ms;
-                best_kc = kc;
+                chosen = c;
             }
         }
-        dtop.kchunks = best_kc;
+        set_top_mode(chosen.mode);
+        top_fold = chosen.fold;
     }
     for (int pass = 0; pass < 2; ++pass) {
         std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
@@ -936,8 +998,22 @@ void pdfc_sim::build_top() {
     dtop.cidx = upload(top.cidx);
     dtop.bt = dalloc<double>(3 * (size_t)m);
     const int rowblocks = (m + 63) / 64;
-    dtop.kchunks = std::max(1, std::min(kMaxTopChunks, 512 / std::max(1, rowblocks)));
-    dtop.part = dalloc<double>((size_t)kMaxTopChunks * 3 * m);  // the autotuner may change kchunks
+    heuristic_chunks = std::max(1, std::min(kMaxTopChunks, 512 / std::max(1, rowblocks)));
+    dtop.kchunks = heuristic_chunks;
+    top_blocks = rowblocks;
+    {
+        std::vector<int> tiles;
+        for (int I = 0; I < rowblocks; ++I)
+            for (int J = I; J < rowblocks; ++J) {
+                tiles.push_back(I);
+                tiles.push_back(J);
+            }
+        dtop.tiles = upload(tiles);
+        dtop.ntiles = (int)(tiles.size() / 2);
+    }
+    dtop.nb = 0;  // the unit solves below sweep every level; the mode is chosen afterwards
+    // partial products of either mode (the autotuner switches between them)
+    dtop.part = dalloc<double>(std::max((size_t)kMaxTopChunks * 3 * m, (size_t)rowblocks * rowblocks * 192));
     double* S64 = dalloc<double>((size_t)m * m);
     for (int j = 0; j < m; j += 3) {
         pdfc::launch_unit3(V[1], n, dtop.rows, m, j, stream);
@@ -1038,11 +1114,15 @@ std::vector<unsigned char> pdfc_sim::checkpoint_bytes() {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
     w.put_array(clock.data(), clock.size());
+    mesh_positions();  // a pending interpolation belongs to the saved state
     for (const HostMesh& hm : meshes) {
+        std::vector<double> cur(3 * (size_t)hm.nv);
+        HIP_CHECK(hipMemcpyAsync(cur.data(), hm.d_cur, cur.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
         w.put<double>(hm.time);
         w.put<int64_t>(hm.next);
         w.put<int64_t>(hm.nv);
-        w.put_array(hm.cur.data(), hm.cur.size());
+        w.put_array(cur.data(), cur.size());
     }
     if (slots > 0) {  // device numbering -> OBJ vertex order
         std::vector<float> dev(3 * (size_t)n * slots), out(dev.size());
@@ -1103,6 +1183,8 @@ void pdfc_sim::restore_checkpoint(const unsigned char* p, size_t size) {
         meshes[m].time = mtime[m];
         meshes[m].next = (int)mnext[m];
         meshes[m].cur = std::move(mcur[m]);
+        meshes[m].upload = true;
+        meshes[m].lerp = false;
         meshes[m].dirty = true;
     }
     if (slots > 0) {
@@ -1133,7 +1215,7 @@ void pdfc_sim::solve_inplace(double* X) {
         // the top block's ordered finish runs inside the first backward level when there is one
         const bool fold = use_top && top_fold && bwd.level_ptr.size() > 1;
         if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream, !fold); });
-        pdfc::TopFold tf{dtop.part, dtop.rows, dtop.top_of, dtop.m, dtop.kchunks, 0, 0};
+        pdfc::TopFold tf{dtop.part, dtop.rows, dtop.top_of, dtop.m, dtop.kchunks, 0, 0, dtop.nb};
         timed(2, [&] {
             for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
                 pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
@@ -1163,11 +1245,14 @@ void pdfc_sim::detect() {
     pdfc::launch_detect(dm, qn, dob, cflags, ccnt, cstart, blocksum, ncontacts, dc, stream);
 }
 
-// Uploads the current positions of the mesh obstacles that moved and rebuilds their
-// triangle grids: a triangle is registered in every cell its box, grown by 1.1 tol,
-// overlaps; with cells at least as wide as any grown box, a vertex only needs its own
-// cell. Also sets the key frames around the clock for the point speeds.
+// Brings the mesh obstacles' device positions up to date (an upload after creation or a
+// checkpoint restore, else the key-frame interpolation of the last advance) and rebuilds
+// the triangle grids of those that moved, all on the stream: a triangle is registered in
+// every cell its box, grown by 1.1 tol, overlaps; with cells at least as wide as any grown
+// box, a vertex only needs its own cell. Also sets the key frames around the clock for
+// the point speeds.
 void pdfc_sim::mesh_refresh() {
+    mesh_positions();
     for (size_t m = 0; m < meshes.size(); ++m) {
         HostMesh& hm = meshes[m];
         pdfc::DevMeshObs& D = dob.mesh[m];
@@ -1183,56 +1268,24 @@ void pdfc_sim::mesh_refresh() {
         }
         if (!hm.dirty) continue;
         hm.dirty = false;
-        const double grow = D.t11 * (1. + 1e-9);
-        std::vector<double> lo(3 * (size_t)hm.T), hi(3 * (size_t)hm.T);
-        double ext = 0.;
-        for (int t = 0; t < hm.T; ++t)
-            for (int k = 0; k < 3; ++k) {
-                const double a = hm.cur[3 * hm.tris[3 * t] + k], b = hm.cur[3 * hm.tris[3 * t + 1] + k],
-                             c = hm.cur[3 * hm.tris[3 * t + 2] + k];
-                lo[3 * t + k] = std::min(std::min(a, b), c) - grow;
-                hi[3 * t + k] = std::max(std::max(a, b), c) + grow;
-                ext = std::max(ext, hi[3 * t + k] - lo[3 * t + k]);
-            }
-        const double h = std::max(ext * (1. + 1e-9), 1e-12);
-        D.h = h;
-        auto cell = [&](double x) { return (long long)std::floor(x / h); };
-        auto hash = [&](long long ix, long long iy, long long iz) {
-            unsigned v = ((unsigned)ix * 73856093u) ^ ((unsigned)iy * 19349663u) ^ ((unsigned)iz * 83492791u);
-            return (int)(v & (unsigned)(D.P - 1));
-        };
-        std::vector<std::vector<int>> buckets(D.P);
-        for (int t = 0; t < hm.T; ++t) {
-            std::vector<int> seen;
-            for (long long a = cell(lo[3 * t]); a <= cell(hi[3 * t]); ++a)
-                for (long long b = cell(lo[3 * t + 1]); b <= cell(hi[3 * t + 1]); ++b)
-                    for (long long c = cell(lo[3 * t + 2]); c <= cell(hi[3 * t + 2]); ++c) {
-                        const int bk = hash(a, b, c);
-                        if (std::find(seen.begin(), seen.end(), bk) == seen.end()) {
-                            seen.push_back(bk);
-                            buckets[bk].push_back(t);
-                        }
-                    }
+        pdfc::launch_mesh_build(hm.build, stream);
+    }
+}
+
+// The device positions of every mesh obstacle as the host state says they are.
+void pdfc_sim::mesh_positions() {
+    for (HostMesh& hm : meshes) {
+        if (hm.upload) {
+            HIP_CHECK(hipMemcpyAsync(hm.d_cur, hm.cur.data(), hm.cur.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     stream));
+            HIP_CHECK(hipStreamSynchronize(stream));  // cur may change once this returns
+            hm.upload = false;
+        } else if (hm.lerp) {
+            const size_t f = (size_t)hm.lerp_frame * 3 * hm.nv;
+            pdfc::launch_mesh_lerp(hm.d_cur, hm.d_frames + f, hm.d_frames + f + 3 * hm.nv, hm.lerp_a, 3 * hm.nv,
+                                   stream);
         }
-        std::vector<int> off(D.P + 1, 0), sorted;
-        for (int bk = 0; bk < D.P; ++bk) {
-            off[bk] = (int)sorted.size();
-            sorted.insert(sorted.end(), buckets[bk].begin(), buckets[bk].end());
-        }
-        off[D.P] = (int)sorted.size();
-        if (sorted.size() > hm.cap_sorted) {
-            if (hm.d_tris_sorted) free_ptr(hm.d_tris_sorted);
-            hm.cap_sorted = sorted.size() + sorted.size() / 2 + 16;
-            hm.d_tris_sorted = dalloc<int>(hm.cap_sorted);
-        }
-        D.bucket_tris = hm.d_tris_sorted;
-        HIP_CHECK(hipMemcpyAsync(hm.d_cur, hm.cur.data(), hm.cur.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(hm.d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-        if (!sorted.empty())
-            HIP_CHECK(hipMemcpyAsync(hm.d_tris_sorted, sorted.data(), sorted.size() * sizeof(int),
-                                     hipMemcpyHostToDevice, stream));
-        // the host vectors die here; the copies above must be done before they do
-        HIP_CHECK(hipStreamSynchronize(stream));
+        hm.lerp = false;
     }
 }
 
@@ -1245,10 +1298,9 @@ void pdfc_sim::mesh_advance(double dt) {
         if (hm.times[hm.next] < hm.time && hm.next + 1 < F) ++hm.next;
         if (!hm.in_movement()) continue;
         const double t0 = hm.times[hm.next - 1], t1 = hm.times[hm.next];
-        const double a = (hm.time - t0) / (t1 - t0);
-        const double* P = &hm.frames[(size_t)(hm.next - 1) * 3 * hm.nv];
-        const double* N = &hm.frames[(size_t)hm.next * 3 * hm.nv];
-        for (size_t k = 0; k < hm.cur.size(); ++k) hm.cur[k] = P[k] * (1. - a) + N[k] * a;
+        hm.lerp_a = (hm.time - t0) / (t1 - t0);
+        hm.lerp_frame = hm.next - 1;
+        hm.lerp = true;  // positions and grid follow on the device at the next detection
         hm.dirty = true;
     }
 }
@@ -2076,8 +2128,10 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         // with the top block's finish folded into the first backward level, x_T is written there
         const bool folded = sim->use_top && sim->top_fold && sim->bwd.level_ptr.size() > 1;
         const double bwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 28. * nbtot + (folded ? 24. * mt : 0.);
-        // top block: S^-1 once, b_T in, x_T out (unless folded), gathered update entries (index + 24 B)
-        const double topb = ent * mt * mt + (folded ? 24. : 48.) * mt +
+        // top block: S^-1 once (its upper block triangle with symmetric tiles), b_T in, x_T out
+        // (unless folded), gathered update entries (index + 24 B)
+        const double sbytes = sim->use_top && sim->dtop.nb > 0 ? ent * 4096. * sim->dtop.ntiles : ent * mt * mt;
+        const double topb = sbytes + (folded ? 24. : 48.) * mt +
                             28. * (double)sim->top.cidx.size() * (sim->use_top ? 1. : 0.);
         const double local = 4. * 24. * n + 2. * E * 12. + nc * (72. + 8. + 48. + 19. * 36.);
         double solve = fwd + bwd + topb;
@@ -2088,7 +2142,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         for (size_t l = 0; l < sim->bwd_waves.size() && l < 32; ++l) I.waves_backward[l] = sim->bwd_waves[l];
         I.autotune_ms = sim->autotune_ms;
         I.leaf_size = sim->leaf;
-        I.top_chunks = sim->use_top ? sim->dtop.kchunks : 0;
+        I.top_chunks = sim->use_top && sim->dtop.nb == 0 ? sim->dtop.kchunks : 0;
         I.top_m = (int64_t)mt;
         I.n_levels_swept = (int64_t)(sim->fwd.level_ptr.size() - 1);
         I.bytes_local = local;

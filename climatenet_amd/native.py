@@ -208,7 +208,8 @@ class Simulation:
             raise ValueError("more levels than pdfc_info reports")
         f = ",".join(str(w) for w in I["waves_forward"][:nl])
         b = ",".join(str(w) for w in I["waves_backward"][:nl])
-        return "f=%s;b=%s;t=%d" % (f, b, max(1, I["top_chunks"]))
+        # t=0: symmetric tiles (or no top block); o: the top block's finish folded or not
+        return "f=%s;b=%s;t=%d;o=%d" % (f, b, I["top_chunks"], I["top_fold"])
 
     def phase_stats(self):
         P = abi.PhaseStats()

@@ -413,3 +413,27 @@ def analytic_incline_mesh(mu: float = 0.1) -> SceneSpec:
                      [ObstacleSpec("mesh", tolerance=1e-3, frames=S[None], triangles=support_tris)],
                      np.array([[mu]]), area_density=1.0, bending=2e-4, stretching=2e-4,
                      time_step=1e-3, iterations=20, steps=800)
+
+
+def rotating_mesh_drape() -> SceneSpec:
+    """A scene shaped like the reference's replicability drapes (replicability/scenes/Square1):
+    a 2.4 m square cloth of 76 x 76 quads (5,929 vertices; area density 3, stretching 200,
+    bending 5e-4) at z = 0.093 over a sphere of radius 0.5 given as a key-framed triangle mesh
+    (icosphere, 2,562 vertices, 5,120 faces; 200 frames 0.035 s apart turning 60 degrees about
+    z; collision tolerance 0.015), plus a ground plane at z = -1; mu = 0.1, dt = 1/200,
+    20 iterations."""
+    rest, tris = square_grid(76, 2.4, 0.093)
+    P, F = icosphere(0.5, center=(0.0, 0.0, -0.52), subdivisions=4)
+    c = np.array([0.0, 0.0, -0.52])
+    frames, times = [], []
+    for k in range(200):
+        a = np.deg2rad(60.0) * k / 199.0
+        R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+        frames.append((P - c) @ R.T + c)
+        times.append(0.035 * k)
+    return SceneSpec("rotating_mesh_drape_76x76", rest, tris, None,
+                     [ObstacleSpec("mesh", tolerance=0.015, frames=np.array(frames), triangles=F,
+                                   times=np.array(times)),
+                      ObstacleSpec("plane", (0.0, 0.0, -1.0), (0.0, 0.0, 1.0))],
+                     np.array([[0.1]]), area_density=3.0, bending=5e-4, stretching=200.0,
+                     time_step=0.005, iterations=20, steps=2450)

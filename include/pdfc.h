@@ -203,7 +203,8 @@ typedef struct pdfc_info {
     int32_t waves_backward[32];      /* same for the backward levels (root side first) */
     double autotune_ms;              /* time spent choosing them at creation */
     int32_t leaf_size;               /* largest leaf supernode of the dissection */
-    int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned) */
+    int32_t top_chunks;              /* reduction split of the top-block matvec (autotuned); 0 when the
+                                        symmetric tile products are used, or without a top block */
     double lhs_seconds;              /* host time to build topology, masses, constraints and A^T A (K) */
     int64_t steps_done;              /* steps taken since creation (restored by pdfc_load_checkpoint) */
     int32_t top_fold;                /* 1: the top block's ordered finish runs in the first backward level */

@@ -7,6 +7,8 @@ Tolerances (FP64 mode): positions, velocities and contact reactions within
 1e-9 relative (max-norm, relative to the largest magnitude of the quantity);
 contact sets identical.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -175,21 +177,52 @@ def test_fp32_refinement_error_40k(gpu, solve_mode):
     assert err < 1e-10
 
 
+def pinned(probe, top_mode, fold):
+    """probe's substitution schedule with the top block's mode and fold set."""
+    base = re.sub(r";t=\d+;o=\d", "", probe.tuning())
+    return "%s;t=%d;o=%d" % (base, top_mode, fold)
+
+
+@pytest.mark.parametrize("top_mode", [0, 8])
 @pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_REFINE64])
-def test_folded_top_finish_is_bit_identical(gpu, monkeypatch, precision):
+def test_folded_top_finish_is_bit_identical(gpu, monkeypatch, precision, top_mode):
     """The top block's ordered finish run inside the first backward level sums the same partial
     products in the same order as the separate finish kernel: with the same substitution
-    schedule, the two paths give identical states."""
+    schedule, the two paths give identical states (symmetric tiles, t=0, and the split
+    matvec over 8 column ranges)."""
     spec = scenes.sphere_drape_40k()
     spec.self_collision = False
-    monkeypatch.setenv("PDFC_TOP_FOLD", "1")
-    a = gpu.Simulation(spec, precision=precision, record_iterations=False)
-    assert a.info()["top_m"] > 0
-    monkeypatch.setenv("PDFC_TUNE", a.tuning())
-    monkeypatch.setenv("PDFC_TOP_FOLD", "0")
-    b = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    probe = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    assert probe.info()["top_m"] > 0
+    sims = []
+    for fold in (1, 0):
+        monkeypatch.setenv("PDFC_TUNE", pinned(probe, top_mode, fold))
+        s = gpu.Simulation(spec, precision=precision, record_iterations=False)
+        assert s.info()["top_fold"] == fold and s.info()["top_chunks"] == top_mode
+        sims.append(s)
+    a, b = sims
     a.step(5)
     b.step(5)
     qa, va = a.state()
     qb, vb = b.state()
     assert np.array_equal(qa, qb) and np.array_equal(va, vb)
+
+
+def test_symmetric_tiles_match_split_matvec(gpu, monkeypatch):
+    """The symmetric tile products (upper block triangle of S^-1, each block used for its rows
+    and, transposed, for its columns) solve K x = b as the split matvec does, to rounding:
+    S^-1 is symmetric up to the rounding of its unit solves."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    probe = gpu.Simulation(spec, record_iterations=False)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    xs = {}
+    for mode in (0, 8):
+        monkeypatch.setenv("PDFC_TUNE", pinned(probe, mode, 1))
+        s = gpu.Simulation(spec, record_iterations=False)
+        assert s.info()["top_chunks"] == mode
+        xs[mode] = s.solve(b)
+    err = rel_err(xs[0], xs[8])
+    print("symmetric tiles vs split matvec, max rel difference:", err)
+    assert err < 1e-12
