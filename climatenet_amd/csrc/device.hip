@@ -336,6 +336,7 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
 // then the gathers); wider stencils take the loop below.
 constexpr int kSpringCap = 8;
 constexpr int kAtaCap = 24;
+constexpr int kAtaChunk = 8;  // A^T A gathers in flight at a time
 
 // (A^T p)_i at q = q_n + dt v: sum over the incident springs of w^2 L0 (q_i - q_j)/|q_i - q_j|.
 __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __restrict__ qn,
@@ -357,26 +358,31 @@ __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __rest
                 kap[k] = M.sp_kappa[(size_t)k * n + i];
             }
         }
-        double gq[kSpringCap][3], gv[kSpringCap][3];
+        // half of the stencil's gathers in flight at a time (fewer registers held)
+        constexpr int kHalf = kSpringCap / 2;
 #pragma unroll
-        for (int k = 0; k < kSpringCap; ++k) {
-            const int j = jj[k];
-            gq[k][0] = qn[j];
-            gq[k][1] = qn[n + j];
-            gq[k][2] = qn[2 * n + j];
-            gv[k][0] = vcur[j];
-            gv[k][1] = vcur[n + j];
-            gv[k][2] = vcur[2 * n + j];
-        }
+        for (int k0 = 0; k0 < kSpringCap; k0 += kHalf) {
+            double gq[kHalf][3], gv[kHalf][3];
 #pragma unroll
-        for (int k = 0; k < kSpringCap; ++k)
-            if (k < deg) {
-                const D3 qj = {gq[k][0] + dt * gv[k][0], gq[k][1] + dt * gv[k][1], gq[k][2] + dt * gv[k][2]};
-                D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
-                s.x += kap[k] * e.x;
-                s.y += kap[k] * e.y;
-                s.z += kap[k] * e.z;
+            for (int k = 0; k < kHalf; ++k) {
+                const int j = jj[k0 + k];
+                gq[k][0] = qn[j];
+                gq[k][1] = qn[n + j];
+                gq[k][2] = qn[2 * n + j];
+                gv[k][0] = vcur[j];
+                gv[k][1] = vcur[n + j];
+                gv[k][2] = vcur[2 * n + j];
             }
+#pragma unroll
+            for (int k = 0; k < kHalf; ++k)
+                if (k0 + k < deg) {
+                    const D3 qj = {gq[k][0] + dt * gv[k][0], gq[k][1] + dt * gv[k][1], gq[k][2] + dt * gv[k][2]};
+                    D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
+                    s.x += kap[k0 + k] * e.x;
+                    s.y += kap[k0 + k] * e.y;
+                    s.z += kap[k0 + k] * e.z;
+                }
+        }
         return s;
     }
     for (int k = 0; k < deg; ++k) {
@@ -396,42 +402,32 @@ __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restric
     const int n = M.n;
     D3 av = {0., 0., 0.};
     const int cnt = M.ata_cnt[i];
-    if (M.ata_slots <= kAtaCap) {
-        // branch-free gathers, as in spring_sum
-        int jj[kAtaCap];
-        double w[kAtaCap];
+    // kAtaChunk entries at a time: indices, then branch-free gathers (unused slots gather
+    // the vertex itself), then the sums in entry order
+    for (int k0 = 0; k0 < cnt; k0 += kAtaChunk) {
+        int jj[kAtaChunk];
+        double w[kAtaChunk];
 #pragma unroll
-        for (int k = 0; k < kAtaCap; ++k) {
-            jj[k] = i;
-            w[k] = 0.;
-            if (k < cnt) {
-                jj[k] = M.ata_idx[(size_t)k * n + i];
-                w[k] = M.ata_val[(size_t)k * n + i];
-            }
+        for (int k = 0; k < kAtaChunk; ++k) {
+            const int e = min(k0 + k, cnt - 1);
+            jj[k] = M.ata_idx[(size_t)e * n + i];
+            w[k] = M.ata_val[(size_t)e * n + i];
         }
-        double g[kAtaCap][3];
+        double g[kAtaChunk][3];
 #pragma unroll
-        for (int k = 0; k < kAtaCap; ++k) {
+        for (int k = 0; k < kAtaChunk; ++k) {
             const int j = jj[k];
             g[k][0] = vcur[j];
             g[k][1] = vcur[n + j];
             g[k][2] = vcur[2 * n + j];
         }
 #pragma unroll
-        for (int k = 0; k < kAtaCap; ++k)
-            if (k < cnt) {
+        for (int k = 0; k < kAtaChunk; ++k)
+            if (k0 + k < cnt) {
                 av.x += w[k] * g[k][0];
                 av.y += w[k] * g[k][1];
                 av.z += w[k] * g[k][2];
             }
-        return av;
-    }
-    for (int k = 0; k < cnt; ++k) {
-        int j = M.ata_idx[(size_t)k * n + i];
-        double w = M.ata_val[(size_t)k * n + i];
-        av.x += w * vcur[j];
-        av.y += w * vcur[n + j];
-        av.z += w * vcur[2 * n + j];
     }
     return av;
 }
@@ -505,24 +501,29 @@ __device__ __forceinline__ void ata_row_load(const DevModel& M, int i, AtaRowReg
     }
 }
 
-// Same sum, same order as ata_row.
+// Same sum, same order as ata_row; the gathers go out kAtaChunk entries at a time (fewer
+// registers held than all kAtaCap at once).
 __device__ __forceinline__ D3 ata_row_sum(const AtaRowRegs& R, const double* __restrict__ vcur, int n, int i) {
-    double g[kAtaCap][3];
-#pragma unroll
-    for (int k = 0; k < kAtaCap; ++k) {
-        const int j = k < R.cnt ? R.j[k] : i;
-        g[k][0] = vcur[j];
-        g[k][1] = vcur[n + j];
-        g[k][2] = vcur[2 * n + j];
-    }
     D3 av = {0., 0., 0.};
 #pragma unroll
-    for (int k = 0; k < kAtaCap; ++k)
-        if (k < R.cnt) {
-            av.x += R.w[k] * g[k][0];
-            av.y += R.w[k] * g[k][1];
-            av.z += R.w[k] * g[k][2];
+    for (int k0 = 0; k0 < kAtaCap; k0 += kAtaChunk) {
+        if (k0 >= R.cnt) break;
+        double g[kAtaChunk][3];
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k) {
+            const int j = k0 + k < R.cnt ? R.j[k0 + k] : i;
+            g[k][0] = vcur[j];
+            g[k][1] = vcur[n + j];
+            g[k][2] = vcur[2 * n + j];
         }
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k)
+            if (k0 + k < R.cnt) {
+                av.x += R.w[k0 + k] * g[k][0];
+                av.y += R.w[k0 + k] * g[k][1];
+                av.z += R.w[k0 + k] * g[k][2];
+            }
+    }
     return av;
 }
 
@@ -550,9 +551,11 @@ __device__ __forceinline__ void contact_load(const DevContacts& C, int cid, bool
 //   F = b - dt^2 (A^T A v)_i at contact vertices, r = cone(B^T F), b += B r
 //                                               (SimulationFrictionEstimation.cpp:337-423)
 // The friction residual of the previous iteration (r^{k-1}, v^k) is folded in.
-// The loads are issued in three dependent rounds: everything indexed by the vertex (with
-// the stencil indices), then the spring gathers, the A^T A indices and the first contact,
-// then the A^T A gathers.
+// kLowLatency: the loads are issued in three dependent rounds (everything indexed by the
+// vertex with the stencil indices; the spring gathers, the A^T A indices and the first
+// contact; the A^T A gathers), at the price of registers (one wave per SIMD). For grids of
+// many blocks the other instance keeps more waves resident instead.
+template <bool kLowLatency>
 __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __restrict__ qn,
                                                   const double* __restrict__ vcur, const double* __restrict__ cvec,
                                                   double* __restrict__ rhs, const int* __restrict__ ccnt,
@@ -563,7 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
     double ac_s = 0., ac_m = 0.;
     if (i < n) {
         const double dt = a.dt;
-        const bool regs = M.sp_slots <= kSpringCap && M.ata_slots <= kAtaCap;
+        const bool regs = kLowLatency && M.sp_slots <= kSpringCap && M.ata_slots <= kAtaCap;
         const bool rec = a.record && iteration > 0;
         const int nc = ccnt[i];
         const int slot = ls.selfmode ? ls.vslot[i] : -1;
@@ -784,12 +787,46 @@ __device__ __forceinline__ double as_d(T x) { return (double)x; }
 // Operator entries per thread loaded before a task's gather barrier.
 constexpr int kPrefetch = 4;
 
+// x op= U[idx[q]] for q = q0 .. q1-1 in order (op: -= when kSub, else +=), with the index and
+// entry loads of kPullBatch entries in flight at a time (clamped indices keep every load
+// in bounds and unconditional; entries past q1 are loaded but not used).
+constexpr int kPullBatch = 4;
+template <bool kSub>
+__device__ __forceinline__ void pull_entries(const double* __restrict__ U, const int* __restrict__ idx, int q0, int q1,
+                                             double& x, double& y, double& z) {
+    for (int q = q0; q < q1; q += kPullBatch) {
+        int j[kPullBatch];
+#pragma unroll
+        for (int k = 0; k < kPullBatch; ++k) j[k] = idx[min(q + k, q1 - 1)];
+        double u[kPullBatch][3];
+#pragma unroll
+        for (int k = 0; k < kPullBatch; ++k) {
+            const double* p = U + 3 * (size_t)j[k];
+            u[k][0] = p[0];
+            u[k][1] = p[1];
+            u[k][2] = p[2];
+        }
+#pragma unroll
+        for (int k = 0; k < kPullBatch; ++k)
+            if (q + k < q1) {
+                if (kSub) {
+                    x -= u[k][0];
+                    y -= u[k][1];
+                    z -= u[k][2];
+                } else {
+                    x += u[k][0];
+                    y += u[k][1];
+                    z += u[k][2];
+                }
+            }
+    }
+}
+
 // Forward substitution, one task = (supernode s, 64 rows of F_s), W waves:
 //   b~_s = b_s - sum_children u_c|own ;  [y_s ; u_s] = F_s b~_s (+ children's u_c|below)
 // The reduction over the ns columns is split across the W waves (wave w takes
 // columns w, w+W, ...) so each wave has a short chain of independent loads; the
-// partial sums are combined in LDS in wave order (deterministic). All threads of
-// the block reach the end (the dependency-driven sweep runs tasks back to back).
+// partial sums are combined in LDS in wave order (deterministic).
 template <typename T, int W>
 __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc& d, const double* __restrict__ Bv,
                                              double* __restrict__ Y, double* lds, double (*part)[3][64]) {
@@ -813,26 +850,14 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
-        for (int q = pp[t]; q < pp[t + 1]; ++q) {
-            const double* u = Fa.U + 3 * (size_t)Fa.pull_idx[q];
-            x -= u[0];
-            y -= u[1];
-            z -= u[2];
-        }
+        pull_entries<true>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
         bx[t] = x;
         by[t] = y;
         bz[t] = z;
     }
     // this task's below rows: children's entries to pass on
     double ux = 0., uy = 0., uz = 0.;
-    if (wave == 0 && my_row >= ns && my_row < f) {
-        for (int q = pp[my_row]; q < pp[my_row + 1]; ++q) {
-            const double* u = Fa.U + 3 * (size_t)Fa.pull_idx[q];
-            ux += u[0];
-            uy += u[1];
-            uz += u[2];
-        }
-    }
+    if (wave == 0 && my_row >= ns && my_row < f) pull_entries<false>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
@@ -898,28 +923,43 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
 // All the loads are issued before the first add (at most kTopChunksMax chunks).
 constexpr int kTopChunksMax = 16;
 // Symmetric tiles: the nb source blocks of row t in order, 16 loads in flight at a time.
-__device__ __forceinline__ void top_sum_blocks(const double* part, int nb, int t, double& x, double& y, double& z) {
+// Group g's sum (source blocks 16 g .. 16 g + 15 of row t, in order, starting from 0).
+__device__ __forceinline__ void top_group_sum(const double* part, int nb, int t, int g, double& x, double& y,
+                                              double& z) {
     const double* P = part + (size_t)(t >> 6) * nb * 192 + (t & 63);
+    const int c0 = g * kTopChunksMax;
+    double px[kTopChunksMax], py[kTopChunksMax], pz[kTopChunksMax];
+#pragma unroll
+    for (int c = 0; c < kTopChunksMax; ++c)
+        if (c0 + c < nb) {
+            const double* Q = P + (size_t)(c0 + c) * 192;
+            px[c] = Q[0];
+            py[c] = Q[64];
+            pz[c] = Q[128];
+        }
     x = 0.;
     y = 0.;
     z = 0.;
-    for (int c0 = 0; c0 < nb; c0 += kTopChunksMax) {
-        double px[kTopChunksMax], py[kTopChunksMax], pz[kTopChunksMax];
 #pragma unroll
-        for (int c = 0; c < kTopChunksMax; ++c)
-            if (c0 + c < nb) {
-                const double* Q = P + (size_t)(c0 + c) * 192;
-                px[c] = Q[0];
-                py[c] = Q[64];
-                pz[c] = Q[128];
-            }
-#pragma unroll
-        for (int c = 0; c < kTopChunksMax; ++c)
-            if (c0 + c < nb) {
-                x += px[c];
-                y += py[c];
-                z += pz[c];
-            }
+    for (int c = 0; c < kTopChunksMax; ++c)
+        if (c0 + c < nb) {
+            x += px[c];
+            y += py[c];
+            z += pz[c];
+        }
+}
+
+// Symmetric tiles: row t's nb source blocks in groups of 16, each group summed in block
+// order, then the group sums added in group order (k_top_finish computes the groups on
+// separate lanes and combines them in the same order).
+__device__ __forceinline__ void top_sum_blocks(const double* part, int nb, int t, double& x, double& y, double& z) {
+    top_group_sum(part, nb, t, 0, x, y, z);
+    for (int g = 1; g * kTopChunksMax < nb; ++g) {
+        double gx, gy, gz;
+        top_group_sum(part, nb, t, g, gx, gy, gz);
+        x += gx;
+        y += gy;
+        z += gz;
     }
 }
 
@@ -1057,12 +1097,7 @@ __global__ void k_top_gather(DevTop T, const double* __restrict__ U, const doubl
     if (t >= T.m) return;
     int row = T.rows[t];
     double bx = B[row], by = B[n + row], bz = B[2 * n + row];
-    for (int q = T.cptr[t]; q < T.cptr[t + 1]; ++q) {
-        const double* u = U + 3 * (size_t)T.cidx[q];
-        bx -= u[0];
-        by -= u[1];
-        bz -= u[2];
-    }
+    pull_entries<true>(U, T.cidx, T.cptr[t], T.cptr[t + 1], bx, by, bz);
     T.bt[t] = bx;
     T.bt[T.m + t] = by;
     T.bt[2 * T.m + t] = bz;
@@ -1121,14 +1156,11 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
     const int m = Tp.m, nb = Tp.nb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = 64 * I, c0 = 64 * J;
-    const int row = r0 + lane;
-    const T* S = static_cast<const T*>(Tp.Sinv);
+    // this tile's 64 x 64 block, contiguous in the packed copy (zero outside the matrix)
+    const T* S = static_cast<const T*>(Tp.Stile) + (size_t)blockIdx.x * 4096 + lane;
     double a[16];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int col = c0 + wave + 4 * s;
-        a[s] = (row < m && col < m) ? (double)S[(size_t)col * m + row] : 0.;
-    }
+    for (int s = 0; s < 16; ++s) a[s] = (double)S[64 * (wave + 4 * s)];
     if (wave < 2 && (wave == 0 || I != J)) {
         const int t = (wave == 0 ? r0 : c0) + lane;
         double bx = 0., by = 0., bz = 0.;
@@ -1137,12 +1169,7 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
             bx = B[rw];
             by = B[n + rw];
             bz = B[2 * n + rw];
-            for (int q = Tp.cptr[t]; q < Tp.cptr[t + 1]; ++q) {
-                const double* u = U + 3 * (size_t)Tp.cidx[q];
-                bx -= u[0];
-                by -= u[1];
-                bz -= u[2];
-            }
+            pull_entries<true>(U, Tp.cidx, Tp.cptr[t], Tp.cptr[t + 1], bx, by, bz);
         }
         xb[wave][0][lane] = bx;
         xb[wave][1][lane] = by;
@@ -1191,19 +1218,42 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
     }
 }
 
+// Symmetric tiles: kFinishLanes lanes per top row, lane g sums group g (as top_sum_blocks),
+// lane 0 adds the group sums in group order and scatters.
+constexpr int kFinishLanes = 8;
+__global__ void k_top_finish_blocks(DevTop T, double* __restrict__ X, int n) {
+    const int g = threadIdx.x % kFinishLanes;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / kFinishLanes;
+    const int groups = (T.nb + kTopChunksMax - 1) / kTopChunksMax;
+    double x = 0., y = 0., z = 0.;
+    if (t < T.m && g < groups) top_group_sum(T.part, T.nb, t, g, x, y, z);
+    double sx = x, sy = y, sz = z;  // lane 0: group 0
+    for (int k = 1; k < kFinishLanes; ++k) {
+        const double ox = __shfl(x, (threadIdx.x & 63) - g + k, 64);
+        const double oy = __shfl(y, (threadIdx.x & 63) - g + k, 64);
+        const double oz = __shfl(z, (threadIdx.x & 63) - g + k, 64);
+        if (k < groups) {
+            sx += ox;
+            sy += oy;
+            sz += oz;
+        }
+    }
+    if (t >= T.m || g != 0) return;
+    const int row = T.rows[t];
+    X[row] = sx;
+    X[n + row] = sy;
+    X[2 * n + row] = sz;
+}
+
 __global__ void k_top_finish(DevTop T, double* __restrict__ X, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T.m) return;
     double ax = 0., ay = 0., az = 0.;
-    if (T.nb > 0) {
-        top_sum_blocks(T.part, T.nb, t, ax, ay, az);
-    } else {
-        for (int c = 0; c < T.kchunks; ++c) {
-            const double* P = T.part + (size_t)c * 3 * T.m;
-            ax += P[t];
-            ay += P[T.m + t];
-            az += P[2 * T.m + t];
-        }
+    for (int c = 0; c < T.kchunks; ++c) {
+        const double* P = T.part + (size_t)c * 3 * T.m;
+        ax += P[t];
+        ay += P[T.m + t];
+        az += P[2 * T.m + t];
     }
     int row = T.rows[t];
     X[row] = ax;
@@ -1225,6 +1275,18 @@ __global__ void k_take_cols(const double* X, int n, const int* rows, int m, int 
     for (int c = 0; c < 3; ++c)
         if (j + c < m) S[(size_t)(j + c) * m + t] = X[(size_t)c * n + rows[t]];
 }
+// Upper block triangle of S^-1 in tile order: tile p = (I, J) holds A(64 I + r, 64 J + c) at
+// 4096 p + 64 c + r, zero outside the m x m matrix.
+template <typename T>
+__global__ void k_pack_tiles(const double* __restrict__ S, T* __restrict__ P, int m, const int* __restrict__ tiles,
+                             int ntiles) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)ntiles * 4096) return;
+    const int p = (int)(e >> 12), c = (int)((e >> 6) & 63), r = (int)(e & 63);
+    const int row = 64 * tiles[2 * p] + r, col = 64 * tiles[2 * p + 1] + c;
+    P[e] = (row < m && col < m) ? (T)S[(size_t)col * m + row] : (T)0;
+}
+
 __global__ void k_to_float(const double* src, float* dst, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = (float)src[i];
@@ -1252,6 +1314,8 @@ __global__ void k_residual(DevModel M, double dt2, const double* __restrict__ b,
 }
 
 inline unsigned blocks_for(size_t count, int bs = kBlock) { return (unsigned)((count + bs - 1) / bs); }
+// Up to two blocks per CU (256 CUs) the local kernel's latency matters more than its occupancy.
+constexpr unsigned kLocalLowLatencyBlocks = 512;
 
 }  // namespace
 
@@ -1283,8 +1347,13 @@ void launch_detect(const DevModel& M, const double* qn, const DevObstacles& ob, 
 void launch_local(const DevModel& M, const double* qn, const double* vcur, const double* cvec, double* rhs,
                   const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,
                   const LocalSelf& ls, hipStream_t st) {
-    hipLaunchKernelGGL(k_local, dim3(blocks_for(M.n)), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
-                       iteration, ls);
+    const unsigned blocks = blocks_for(M.n);
+    if (blocks <= kLocalLowLatencyBlocks)
+        hipLaunchKernelGGL(k_local<true>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
+                           iteration, ls);
+    else
+        hipLaunchKernelGGL(k_local<false>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C,
+                           a, iteration, ls);
 }
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const DevObstacles& ob, const StepArgs& a, int iteration,
@@ -1362,7 +1431,11 @@ void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, do
             hipLaunchKernelGGL(k_top_tile<float>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
         else
             hipLaunchKernelGGL(k_top_tile<double>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
-        if (finish) hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
+        // nb <= kFinishLanes * kTopChunksMax (the host offers the tiles for m <= 8192 only)
+        static_assert(64 % kFinishLanes == 0 && kFinishLanes * kTopChunksMax == 128, "finish lanes");
+        if (finish)
+            hipLaunchKernelGGL(k_top_finish_blocks, dim3(blocks_for((size_t)T.m * kFinishLanes)), dim3(kBlock), 0, st,
+                               T, X, n);
         return;
     }
     hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
@@ -1379,6 +1452,16 @@ void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t s
 }
 void launch_take_cols(const double* X, int n, const int* rows, int m, int j, double* S, hipStream_t st) {
     hipLaunchKernelGGL(k_take_cols, dim3(blocks_for(m)), dim3(kBlock), 0, st, X, n, rows, m, j, S);
+}
+void launch_pack_tiles(const double* S, void* P, bool fp32, int m, const int* tiles, int ntiles, hipStream_t st) {
+    const size_t count = (size_t)ntiles * 4096;
+    if (!count) return;
+    if (fp32)
+        hipLaunchKernelGGL(k_pack_tiles<float>, dim3(blocks_for(count)), dim3(kBlock), 0, st, S,
+                           static_cast<float*>(P), m, tiles, ntiles);
+    else
+        hipLaunchKernelGGL(k_pack_tiles<double>, dim3(blocks_for(count)), dim3(kBlock), 0, st, S,
+                           static_cast<double*>(P), m, tiles, ntiles);
 }
 void launch_to_float(const double* src, float* dst, size_t count, hipStream_t st) {
     if (count) hipLaunchKernelGGL(k_to_float, dim3(blocks_for(count)), dim3(kBlock), 0, st, src, dst, count);

@@ -305,6 +305,7 @@ struct DevTop {
     int nb;                 // > 0: symmetric tile products over nb blocks of 64 rows; 0: split matvec
     int ntiles;             // nb (nb + 1) / 2 block pairs I <= J
     const int* tiles;       // 2 per tile: I, J
+    const void* Stile;      // the tiles' blocks of S^-1, 4096 entries each (double or float)
 };
 // x_T = S^-1 (b_T - lower updates). Split matvec: gather, partial products over kchunks
 // column ranges, ordered reduction + scatter. Symmetric tiles: one workgroup per block pair
@@ -319,6 +320,8 @@ void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, do
 void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st);
 void launch_take_cols(const double* X, int n, const int* rows, int m, int j, double* S, hipStream_t st);
 void launch_to_float(const double* src, float* dst, size_t count, hipStream_t st);
+// The blocks of S^-1 (m x m, column-major) at the tile list's block pairs, tile by tile.
+void launch_pack_tiles(const double* S, void* P, bool fp32, int m, const int* tiles, int ntiles, hipStream_t st);
 
 // r = b - K x with K = diag(mass) + dt2 * A^T A
 void launch_residual(const DevModel& M, double dt2, const double* b, const double* x, double* r, hipStream_t st);

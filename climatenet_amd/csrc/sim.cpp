@@ -444,7 +444,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
     {
-        double top_mb = 64.;
+        double top_mb = 160.;  // full S^-1; the symmetric tiles read about half of it
         if (const char* e = std::getenv("PDFC_TOP_MB")) top_mb = std::atof(e);
         pdfc::plan_top(fac, top_mb * 1048576., fp32 ? 4 : 8, top);
     }
@@ -732,6 +732,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         if (const char* tf = std::getenv("PDFC_TOP_FOLD")) top_fold_allowed = std::atoi(tf) != 0;
         top_fold = top_fold_allowed;
         if (const char* ts = std::getenv("PDFC_TOP_SYM")) top_sym = std::atoi(ts) != 0;
+        top_sym = top_sym && top_blocks <= 128;  // the finish sums at most 128 source blocks per row
         if (use_top) set_top_mode(top_sym ? 0 : heuristic_chunks);
         const char* pin = std::getenv("PDFC_TUNE");
         const char* e = std::getenv("PDFC_AUTOTUNE");
@@ -1021,6 +1022,10 @@ void pdfc_sim::build_top() {
         pdfc::launch_take_cols(V[1], n, dtop.rows, m, j, S64, stream);
     }
     HIP_CHECK(hipGetLastError());
+    const size_t tile_entries = (size_t)dtop.ntiles * 4096;
+    void* packed = fp32 ? (void*)dalloc<float>(tile_entries) : (void*)dalloc<double>(tile_entries);
+    pdfc::launch_pack_tiles(S64, packed, fp32, m, dtop.tiles, dtop.ntiles, stream);
+    dtop.Stile = packed;
     if (fp32) {
         float* S32 = dalloc<float>((size_t)m * m);
         pdfc::launch_to_float(S64, S32, (size_t)m * m, stream);
