@@ -936,7 +936,9 @@ void pdfc_sim::autotune_waves() {
             int mode;
             bool fold;
         };
-        const Cand first{top_sym ? 0 : dtop.kchunks, top_fold_allowed};
+        // incumbents: symmetric tiles with the finish in its own lane-parallel kernel (the fold
+        // re-reads nb partial sums per top row in every task), else the split matvec folded
+        const Cand first = top_sym ? Cand{0, false} : Cand{dtop.kchunks, top_fold_allowed};
         std::vector<Cand> cands;
         for (int mode : {0, heuristic_chunks, 4, 8, 16})
             for (bool fold : {true, false}) {
