@@ -345,44 +345,29 @@ __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __rest
     D3 s = {0., 0., 0.};
     const int deg = M.sp_cnt[i];
     if (M.sp_slots <= kSpringCap) {
-        // Unused slots gather the vertex itself, so every gather is issued without a branch
-        // (a gather inside `if (k < deg)` waits for its own data before the next one starts).
         int jj[kSpringCap];
         double kap[kSpringCap];
 #pragma unroll
-        for (int k = 0; k < kSpringCap; ++k) {
-            jj[k] = i;
-            kap[k] = 0.;
+        for (int k = 0; k < kSpringCap; ++k)
             if (k < deg) {
                 jj[k] = M.sp_idx[(size_t)k * n + i];
                 kap[k] = M.sp_kappa[(size_t)k * n + i];
             }
-        }
-        // half of the stencil's gathers in flight at a time (fewer registers held)
-        constexpr int kHalf = kSpringCap / 2;
+        D3 qj[kSpringCap];
 #pragma unroll
-        for (int k0 = 0; k0 < kSpringCap; k0 += kHalf) {
-            double gq[kHalf][3], gv[kHalf][3];
-#pragma unroll
-            for (int k = 0; k < kHalf; ++k) {
-                const int j = jj[k0 + k];
-                gq[k][0] = qn[j];
-                gq[k][1] = qn[n + j];
-                gq[k][2] = qn[2 * n + j];
-                gv[k][0] = vcur[j];
-                gv[k][1] = vcur[n + j];
-                gv[k][2] = vcur[2 * n + j];
+        for (int k = 0; k < kSpringCap; ++k)
+            if (k < deg) {
+                const int j = jj[k];
+                qj[k] = {qn[j] + dt * vcur[j], qn[n + j] + dt * vcur[n + j], qn[2 * n + j] + dt * vcur[2 * n + j]};
             }
 #pragma unroll
-            for (int k = 0; k < kHalf; ++k)
-                if (k0 + k < deg) {
-                    const D3 qj = {gq[k][0] + dt * gv[k][0], gq[k][1] + dt * gv[k][1], gq[k][2] + dt * gv[k][2]};
-                    D3 e = normalized3(D3{q.x - qj.x, q.y - qj.y, q.z - qj.z});
-                    s.x += kap[k0 + k] * e.x;
-                    s.y += kap[k0 + k] * e.y;
-                    s.z += kap[k0 + k] * e.z;
-                }
-        }
+        for (int k = 0; k < kSpringCap; ++k)
+            if (k < deg) {
+                D3 e = normalized3(D3{q.x - qj[k].x, q.y - qj[k].y, q.z - qj[k].z});
+                s.x += kap[k] * e.x;
+                s.y += kap[k] * e.y;
+                s.z += kap[k] * e.z;
+            }
         return s;
     }
     for (int k = 0; k < deg; ++k) {
@@ -402,32 +387,31 @@ __device__ __forceinline__ D3 ata_row(const DevModel& M, const double* __restric
     const int n = M.n;
     D3 av = {0., 0., 0.};
     const int cnt = M.ata_cnt[i];
-    // kAtaChunk entries at a time: indices, then branch-free gathers (unused slots gather
-    // the vertex itself), then the sums in entry order
-    for (int k0 = 0; k0 < cnt; k0 += kAtaChunk) {
-        int jj[kAtaChunk];
-        double w[kAtaChunk];
+    if (M.ata_slots <= kAtaCap) {
+        int jj[kAtaCap];
+        double w[kAtaCap];
 #pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k) {
-            const int e = min(k0 + k, cnt - 1);
-            jj[k] = M.ata_idx[(size_t)e * n + i];
-            w[k] = M.ata_val[(size_t)e * n + i];
-        }
-        double g[kAtaChunk][3];
-#pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k) {
-            const int j = jj[k];
-            g[k][0] = vcur[j];
-            g[k][1] = vcur[n + j];
-            g[k][2] = vcur[2 * n + j];
-        }
-#pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k)
-            if (k0 + k < cnt) {
-                av.x += w[k] * g[k][0];
-                av.y += w[k] * g[k][1];
-                av.z += w[k] * g[k][2];
+        for (int k = 0; k < kAtaCap; ++k)
+            if (k < cnt) {
+                jj[k] = M.ata_idx[(size_t)k * n + i];
+                w[k] = M.ata_val[(size_t)k * n + i];
             }
+#pragma unroll
+        for (int k = 0; k < kAtaCap; ++k)
+            if (k < cnt) {
+                const int j = jj[k];
+                av.x += w[k] * vcur[j];
+                av.y += w[k] * vcur[n + j];
+                av.z += w[k] * vcur[2 * n + j];
+            }
+        return av;
+    }
+    for (int k = 0; k < cnt; ++k) {
+        int j = M.ata_idx[(size_t)k * n + i];
+        double w = M.ata_val[(size_t)k * n + i];
+        av.x += w * vcur[j];
+        av.y += w * vcur[n + j];
+        av.z += w * vcur[2 * n + j];
     }
     return av;
 }
@@ -554,7 +538,8 @@ __device__ __forceinline__ void contact_load(const DevContacts& C, int cid, bool
 // kLowLatency: the loads are issued in three dependent rounds (everything indexed by the
 // vertex with the stencil indices; the spring gathers, the A^T A indices and the first
 // contact; the A^T A gathers), at the price of registers (one wave per SIMD). For grids of
-// many blocks the other instance keeps more waves resident instead.
+// many blocks the other instance (spring_sum / ata_row: a gather per stencil entry, 111
+// VGPRs) keeps four waves per SIMD resident instead.
 template <bool kLowLatency>
 __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __restrict__ qn,
                                                   const double* __restrict__ vcur, const double* __restrict__ cvec,
@@ -582,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
         AR.cnt = 0;
         ContactIn first;
         if (contact && regs) ata_row_load(M, i, AR);
-        if (nc > 0) contact_load(C, c0, rec, first);
+        if (kLowLatency && nc > 0) contact_load(C, c0, rec, first);
         const D3 q = {qi.x + dt * v.x, qi.y + dt * v.y, qi.z + dt * v.z};
         const D3 s = regs ? spring_row_sum(SR, qn, vcur, n, i, dt, q) : spring_sum(M, qn, vcur, dt, i, q);
         double damp = contact ? 0. : a.damping;
@@ -601,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
             for (int k = 0; k < nc; ++k) {
                 const int cid = c0 + k;
                 ContactIn ci;
-                if (k == 0)
+                if (kLowLatency && k == 0)
                     ci = first;
                 else
                     contact_load(C, cid, rec, ci);
@@ -789,9 +774,10 @@ constexpr int kPrefetch = 4;
 
 // x op= U[idx[q]] for q = q0 .. q1-1 in order (op: -= when kSub, else +=), with the index and
 // entry loads of kPullBatch entries in flight at a time (clamped indices keep every load
-// in bounds and unconditional; entries past q1 are loaded but not used).
-constexpr int kPullBatch = 4;
-template <bool kSub>
+// in bounds and unconditional; entries past q1 are loaded but not used). Forward tasks pull
+// 2 at a time (most rows have one or two entries; more loads cost issue slots on large
+// levels), the top block's gathers 4 (their chains sit in every tile's critical path).
+template <bool kSub, int kPullBatch>
 __device__ __forceinline__ void pull_entries(const double* __restrict__ U, const int* __restrict__ idx, int q0, int q1,
                                              double& x, double& y, double& z) {
     for (int q = q0; q < q1; q += kPullBatch) {
@@ -850,14 +836,14 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
-        pull_entries<true>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
+        pull_entries<true, 2>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
         bx[t] = x;
         by[t] = y;
         bz[t] = z;
     }
     // this task's below rows: children's entries to pass on
     double ux = 0., uy = 0., uz = 0.;
-    if (wave == 0 && my_row >= ns && my_row < f) pull_entries<false>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
+    if (wave == 0 && my_row >= ns && my_row < f) pull_entries<false, 2>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
@@ -1097,7 +1083,7 @@ __global__ void k_top_gather(DevTop T, const double* __restrict__ U, const doubl
     if (t >= T.m) return;
     int row = T.rows[t];
     double bx = B[row], by = B[n + row], bz = B[2 * n + row];
-    pull_entries<true>(U, T.cidx, T.cptr[t], T.cptr[t + 1], bx, by, bz);
+    pull_entries<true, 4>(U, T.cidx, T.cptr[t], T.cptr[t + 1], bx, by, bz);
     T.bt[t] = bx;
     T.bt[T.m + t] = by;
     T.bt[2 * T.m + t] = bz;
@@ -1169,7 +1155,7 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
             bx = B[rw];
             by = B[n + rw];
             bz = B[2 * n + rw];
-            pull_entries<true>(U, Tp.cidx, Tp.cptr[t], Tp.cptr[t + 1], bx, by, bz);
+            pull_entries<true, 4>(U, Tp.cidx, Tp.cptr[t], Tp.cptr[t + 1], bx, by, bz);
         }
         xb[wave][0][lane] = bx;
         xb[wave][1][lane] = by;
