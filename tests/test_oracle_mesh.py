@@ -160,3 +160,24 @@ def test_analytic_incline_on_a_mesh_support():
         worst = max(worst, abs(sp - analytic_speed(t)) / analytic_speed(t))
     assert len(orc.contacts()) == 4
     assert worst < 1e-10, worst
+
+
+def test_reference_shaped_scene_frames():
+    """The moving-mesh bench scene: 5,929 cloth vertices; 200 key frames of a 2,562-vertex,
+    5,120-face sphere of radius 0.5 about (0, 0, -0.52), 0.035 s apart, turning rigidly about
+    z by 60 degrees in all (each frame a rotation of frame 0)."""
+    spec = scenes.rotating_mesh_drape()
+    o = spec.obstacles[0]
+    assert spec.n_vertices == 5929 and o.frames.shape == (200, 2562, 3)
+    assert o.triangles.reshape(-1, 3).shape == (5120, 3)
+    assert np.allclose(np.diff(o.times), 0.035) and o.times[0] == 0.0
+    c = np.array([0.0, 0.0, -0.52])
+    r = np.linalg.norm(o.frames - c, axis=2)
+    assert np.abs(r - 0.5).max() < 1e-12
+    # frame k = R_z(60 deg * k / 199) frame 0, checked through the angle of every vertex
+    d0, d1 = o.frames[0] - c, o.frames[199] - c
+    moved = np.hypot(d0[:, 0], d0[:, 1]) > 1e-3
+    ang = np.arctan2(d1[moved, 1], d1[moved, 0]) - np.arctan2(d0[moved, 1], d0[moved, 0])
+    ang = (ang + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(ang - np.deg2rad(60.0)).max() < 1e-12
+    assert np.abs(d1[:, 2] - d0[:, 2]).max() < 1e-15
