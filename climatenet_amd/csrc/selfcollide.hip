@@ -675,7 +675,7 @@ __global__ void k_mesh_scatter(DevMeshBuild B) {
 
 // kMeshLanes lanes per cloth vertex share its bucket (lane j takes slots j, j + kMeshLanes,
 // ...); the nearest hit, ties to the lowest triangle index, is reduced across the lanes.
-constexpr int kMeshLanes = 8;
+constexpr int kMeshLanes = 16;
 __global__ void k_mesh_query(DevMeshObs Mo, const double* __restrict__ qn, int n) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = g / kMeshLanes, lane = g % kMeshLanes;

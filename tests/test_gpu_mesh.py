@@ -83,7 +83,7 @@ def test_reference_shaped_moving_mesh_drape_parity(gpu):
     200-frame, 5,120-face key-framed sphere mesh, tolerance 0.015, plus a plane): from the
     oracle's state after 36 steps, 8 steps with hundreds of mesh contacts agree (contact sets
     identical; positions, velocities and reactions within 1e-9). Exercises the device-built
-    grid with buckets of tens of triangles and the 8-lane query."""
+    grid with buckets of tens of triangles and the 16-lane query."""
     spec = scenes.rotating_mesh_drape()
     sim = gpu.Simulation(spec)
     orc = po.Oracle(spec)
