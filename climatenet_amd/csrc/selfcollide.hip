@@ -297,8 +297,17 @@ __global__ void k_vertex_query_slots(DevSelfDetect D) {
             const long long iz = c0[2] + ci / (span[0] * span[1]);
             unsigned h = cell_hash(ix, iy, iz, D.P);
             int b0 = D.bucket_off[h], b1 = D.bucket_off[h + 1];
+            // the next slot's box is loaded while this one is tested
+            double nbx[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nbx[k] = b0 < b1 ? D.bk_box[6 * (size_t)b0 + k] : 0.;
             for (int q = b0; q < b1; ++q) {
-                const double* bx = &D.bk_box[6 * q];
+                double bx[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) bx[k] = nbx[k];
+                const int qn = min(q + 1, b1 - 1);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) nbx[k] = D.bk_box[6 * (size_t)qn + k];
                 if (cell_of(bx[0], c) != ix || cell_of(bx[1], c) != iy || cell_of(bx[2], c) != iz) continue;
                 if (!(bx[0] <= hi[0] && lo[0] <= bx[3] && bx[1] <= hi[1] && lo[1] <= bx[4] && bx[2] <= hi[2] &&
                       lo[2] <= bx[5]))
