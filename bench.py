@@ -397,7 +397,8 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9 / HBM_PEAK_GBS, 6)},
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
-        "self_collision_host_us_per_step": round(phases["detection_us"], 1),
+        "self_collision_host_us_per_step": {"round_trip": round(phases["detection_us"], 1),
+                                            "pairing": round(phases["pairing_us"], 1)},
     }
     if not args.no_warm_start:
         out["warm_start"] = warm_start_comparison(spec, sim, prec)

@@ -179,6 +179,7 @@ class PhaseStats(C.Structure):
         ("fallbacks", C.c_int64),
         ("collision_us", C.c_double),
         ("timed_steps", C.c_int64),
+        ("pairing_us", C.c_double),
     ]
 
 

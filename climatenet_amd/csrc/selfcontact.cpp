@@ -67,23 +67,38 @@ void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand,
 
 int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPair>& pairs,
                         std::vector<std::vector<int>>& levels) {
+    (void)n;
     levels.clear();
     const int ns = (int)pairs.size();
     if (ns == 0) return 0;
-    // adjacency of the self-contact graph, only for vertices that appear in it
-    std::vector<int> deg(n, 0);
+    // The self-contact graph over the vertices that appear in it, numbered in ascending
+    // vertex order (the reference's scans over all vertices visit them in that order and
+    // skip every other vertex: no adjacency). Per-step work is O(pairs log pairs).
+    std::vector<int> verts;
+    verts.reserve(2 * (size_t)ns);
     for (const auto& p : pairs) {
-        deg[p.v]++;
-        deg[p.partner]++;
+        verts.push_back(p.v);
+        verts.push_back(p.partner);
     }
-    std::vector<int> ptr(n + 1, 0);
-    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + deg[i];
-    std::vector<std::pair<int, int>> adj(ptr[n]);
+    std::sort(verts.begin(), verts.end());
+    verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+    const int nv = (int)verts.size();
+    auto id = [&](int v) { return (int)(std::lower_bound(verts.begin(), verts.end(), v) - verts.begin()); };
+    std::vector<int> pv(ns), pp(ns);
+    std::vector<int> ptr(nv + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        pv[s] = id(pairs[s].v);
+        pp[s] = id(pairs[s].partner);
+        ptr[pv[s] + 1]++;
+        ptr[pp[s] + 1]++;
+    }
+    for (int i = 0; i < nv; ++i) ptr[i + 1] += ptr[i];
+    std::vector<std::pair<int, int>> adj(ptr[nv]);
     {
         std::vector<int> fill(ptr.begin(), ptr.end() - 1);
         for (int s = 0; s < ns; ++s) {
-            adj[fill[pairs[s].v]++] = {pairs[s].partner, s};
-            adj[fill[pairs[s].partner]++] = {pairs[s].v, s};
+            adj[fill[pv[s]]++] = {pp[s], s};
+            adj[fill[pp[s]]++] = {pv[s], s};
         }
     }
     std::vector<char> visited(ns, 0);
@@ -106,16 +121,23 @@ int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPa
             ++lvl;
         }
     };
-    auto all_visited = [&](int v) {
-        for (int q = ptr[v]; q < ptr[v + 1]; ++q)
+    auto all_visited = [&](int u) {
+        for (int q = ptr[u]; q < ptr[u + 1]; ++q)
             if (!visited[adj[q].second]) return false;
         return true;
     };
-    bfs(seeds);
-    for (int v = 0; v < n; ++v)
-        if (deg[v] == 1 && !all_visited(v)) bfs({v});
-    for (int v = 0; v < n; ++v)
-        if (deg[v] > 1 && !all_visited(v)) bfs({v});
+    // seeds outside the graph have no adjacency and add nothing to a level
+    std::vector<int> start;
+    start.reserve(seeds.size());
+    for (int v : seeds) {
+        const int u = id(v);
+        if (u < nv && verts[u] == v) start.push_back(u);
+    }
+    bfs(start);
+    for (int u = 0; u < nv; ++u)
+        if (ptr[u + 1] - ptr[u] == 1 && !all_visited(u)) bfs({u});
+    for (int u = 0; u < nv; ++u)
+        if (ptr[u + 1] - ptr[u] > 1 && !all_visited(u)) bfs({u});
     // drop empty trailing levels produced by the last BFS frontier
     while (!levels.empty() && levels.back().empty()) levels.pop_back();
     for (size_t l = 0; l < levels.size(); ++l)

@@ -46,6 +46,33 @@ struct CapacityError : std::runtime_error {
             throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));                    \
     } while (0)
 
+// Pinned host array for the per-step transfers of the self-collision round trip (no staging
+// copy through pageable memory). grow() keeps the first `keep` entries; the capacity at
+// least doubles, so sizes that drift from step to step do not allocate every step.
+template <typename T>
+struct Pinned {
+    T* p = nullptr;
+    size_t cap = 0;
+    void grow(size_t count, size_t keep = 0) {
+        if (count <= cap) return;
+        count = std::max(count, 2 * cap);
+        T* q = nullptr;
+        HIP_CHECK(hipHostMalloc((void**)&q, count * sizeof(T)));
+        if (p) {
+            if (keep) std::memcpy(q, p, std::min(keep, cap) * sizeof(T));
+            (void)hipHostFree(p);
+        }
+        p = q;
+        cap = count;
+    }
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 }  // namespace
 
 void pdfc::set_last_error(const char* msg) { g_last_error = msg; }
@@ -224,14 +251,25 @@ struct pdfc_sim {
     pdfc::DevSelfDetect sd{};
     unsigned long long* seed_bits = nullptr;
     int* vslot = nullptr;
-    std::vector<int> host_totals = std::vector<int>(3, 0);
-    std::vector<unsigned long long> host_bits;
-    std::vector<pdfc::SelfCandidate> host_filt;
+    Pinned<int> host_small;                        // [0..2] detection totals, [3] slot overflow
+    std::vector<int> slot_of_scratch;              // per vertex, -1 between uploads
+    template <typename F>
+    void for_each_bit(F&& f) const {               // set seed bits in ascending vertex order
+        for (int w = 0; w <= n / 64; ++w)
+            for (unsigned long long b = host_bits.p[w]; b; b &= b - 1) {
+                const int i = 64 * w + __builtin_ctzll(b);
+                if (i < n) f(i);
+            }
+    }
+    Pinned<unsigned long long> host_bits;          // seed bits, n / 64 + 1 words
+    Pinned<pdfc::SelfCandidate> host_filt;         // filtered candidates
     std::vector<pdfc::SelfPair> pairs;             // pairing order
     std::vector<std::vector<int>> levels;          // pair indices per level
     std::vector<int> sorted_pair;                  // device order -> pair index
-    std::vector<int> up_int;                       // staging of the per-step upload
+    std::vector<int> up_int;                       // the per-step upload, assembled
     std::vector<double> up_dbl;
+    Pinned<int> pin_int;                           // and staged for the copy (read by the copy until
+    Pinned<double> pin_dbl;                        // the next step's detection synchronises)
     int* d_self_int = nullptr;
     double* d_self_dbl = nullptr;
     size_t cap_self_int = 0, cap_self_dbl = 0;
@@ -244,6 +282,7 @@ struct pdfc_sim {
     bool keep_full_candidates = false;
     size_t filt_guess = 4096;
     double self_host_us = 0.;   // host time in detection round trips + pairing, accumulated per pdfc_step
+    double pairing_host_us = 0.; // the part after the detection results arrived
     int n_full_candidates = 0;
 
     void free_ptr(void* p) {
@@ -709,7 +748,8 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         sd.vslots = dalloc<pdfc::SelfCandidate>((size_t)n * pdfc::kSlotsPerVertex);
         sd.overflow = dalloc<int>(1);
         seed_bits = dalloc<unsigned long long>(n / 64 + 1);
-        host_bits.resize(n / 64 + 1);
+        host_bits.grow(n / 64 + 1);
+        host_small.grow(4);
         vslot = dalloc<int>(n);
         pdfc::launch_fill_int(vslot, n, -1, stream);
     }
@@ -1319,7 +1359,9 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
     // One synchronisation in the common case: counts, seed bits and a guessed prefix of
     // the filtered list come back together; a second copy only if the guess was short.
     size_t guess = std::min<size_t>((size_t)sd.cap_filt, (size_t)filt_guess);
-    if (host_filt.size() < guess) host_filt.resize(guess);
+    host_filt.grow(guess);
+    int* const host_totals = host_small.p;
+    const size_t bit_words = (size_t)n / 64 + 1;
     bool single_pass = !write_full;
     for (int attempt = 0; attempt < 4; ++attempt) {
         sd.q = qn;
@@ -1328,17 +1370,17 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
             pdfc::launch_self_detect_filtered(sd, ccnt, seed_bits, stream);
         else
             pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
-        HIP_CHECK(hipMemcpyAsync(host_totals.data(), sd.totals, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        int overflow = 0;
+        HIP_CHECK(hipMemcpyAsync(host_totals, sd.totals, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        host_totals[3] = 0;
         if (single_pass)
-            HIP_CHECK(hipMemcpyAsync(&overflow, sd.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipMemcpyAsync(host_bits.data(), seed_bits, host_bits.size() * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(host_totals + 3, sd.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(host_bits.p, seed_bits, bit_words * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 stream));
         if (guess > 0)
-            HIP_CHECK(hipMemcpyAsync(host_filt.data(), sd.filt, guess * sizeof(pdfc::SelfCandidate),
-                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(host_filt.p, sd.filt, guess * sizeof(pdfc::SelfCandidate), hipMemcpyDeviceToHost,
+                                     stream));
         HIP_CHECK(hipStreamSynchronize(stream));
-        if (single_pass && overflow) {  // a vertex has more candidates than slots: two-pass path
+        if (single_pass && host_totals[3]) {  // a vertex has more candidates than slots: two-pass path
             single_pass = false;
             guess = 0;
             continue;
@@ -1368,20 +1410,22 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
     else n_full_candidates = host_totals[0];
     const size_t nf = (size_t)host_totals[1];
     if (nf > guess) {
-        host_filt.resize(nf);
-        HIP_CHECK(hipMemcpyAsync(host_filt.data() + guess, sd.filt + guess, (nf - guess) * sizeof(pdfc::SelfCandidate),
+        host_filt.grow(nf, guess);
+        HIP_CHECK(hipMemcpyAsync(host_filt.p + guess, sd.filt + guess, (nf - guess) * sizeof(pdfc::SelfCandidate),
                                  hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
     filt_guess = nf + nf / 2 + 256;
-    pdfc::pair_self_contacts(model, host_filt.data(), nf, pairs);
+    const auto t_pair0 = std::chrono::steady_clock::now();
+    pdfc::pair_self_contacts(model, host_filt.p, nf, pairs);
     std::vector<int> seeds;
-    for (int i = 0; i < n; ++i)
-        if (host_bits[i >> 6] >> (i & 63) & 1ull) seeds.push_back(fac.perm[i]);
+    for_each_bit([&](int i) { seeds.push_back(fac.perm[i]); });
     n_self_levels = pdfc::order_self_contacts(n, seeds, pairs, levels);
     nsc = (int)pairs.size();
     if (nsc > 0) self_upload();
-    self_host_us += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t_host0).count();
+    const auto t_end = std::chrono::steady_clock::now();
+    self_host_us += std::chrono::duration<double, std::micro>(t_end - t_host0).count();
+    pairing_host_us += std::chrono::duration<double, std::micro>(t_end - t_pair0).count();
 }
 
 // Packs the self contacts sorted by level, the involved-vertex slots and the
@@ -1392,12 +1436,12 @@ void pdfc_sim::self_upload() {
     for (const auto& lv : levels) sorted_pair.insert(sorted_pair.end(), lv.begin(), lv.end());
     // involved vertices: obstacle-contact vertices, then contact and paired vertices
     std::vector<int> slot_vertex;
-    std::vector<int> slot_of(n, -1);
-    for (int i = 0; i < n; ++i)
-        if (host_bits[i >> 6] >> (i & 63) & 1ull) {
-            slot_of[i] = (int)slot_vertex.size();
-            slot_vertex.push_back(i);
-        }
+    std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
+    if ((int)slot_of.size() != n) slot_of.assign(n, -1);
+    for_each_bit([&](int i) {
+        slot_of[i] = (int)slot_vertex.size();
+        slot_vertex.push_back(i);
+    });
     std::vector<int> A(nsc), B(nsc), SA(nsc), SB(nsc);
     std::vector<double> basis(9 * (size_t)nsc), mu(nsc, self_mu);
     for (int j = 0; j < nsc; ++j) {
@@ -1415,6 +1459,7 @@ void pdfc_sim::self_upload() {
         pdfc::host_contact_basis(nr, &basis[9 * (size_t)j]);
     }
     const int nslots = (int)slot_vertex.size();
+    for (int x : slot_vertex) slot_of[x] = -1;
     // per-level groups: each touched slot once, entries in the level's contact order
     std::vector<int> lvl_ptr(L + 1, 0), grp_ptr(L + 1, 0), grp_slot, ent_ptr(1, 0), ent_contact, ent_sign;
     std::vector<int> gidx(nslots, -1);
@@ -1480,8 +1525,14 @@ void pdfc_sim::self_upload() {
         d_self_work = dalloc<double>(9 * cap_self_contacts + 15 * cap_self_slots);
         d_self_active = dalloc<int>(cap_self_contacts);
     }
-    HIP_CHECK(hipMemcpyAsync(d_self_int, up_int.data(), up_int.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_self_dbl, up_dbl.data(), up_dbl.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    // staged in pinned memory: the copies run asynchronously; the staging is next written by
+    // the next step's upload, after that step's detection has synchronised the stream
+    pin_int.grow(up_int.size());
+    pin_dbl.grow(up_dbl.size());
+    std::memcpy(pin_int.p, up_int.data(), up_int.size() * sizeof(int));
+    std::memcpy(pin_dbl.p, up_dbl.data(), up_dbl.size() * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(d_self_int, pin_int.p, up_int.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_self_dbl, pin_dbl.p, up_dbl.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     pdfc::DevSelf S{};
     S.nsc = nsc;
     S.nslots = nslots;
@@ -1510,8 +1561,6 @@ void pdfc_sim::self_upload() {
     S.SCFn = ws + 12 * cap_self_slots;
     S.active = d_self_active;
     ds = S;
-    // the staging vectors must outlive the asynchronous copies
-    HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // One local-global iteration: local step (+ self-contact levels) and the global solve. The
@@ -1792,6 +1841,7 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         if (nsteps == 0) return;
         HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
         sim->self_host_us = 0.;
+        sim->pairing_host_us = 0.;
         sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
         sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
         sim->stats.timed_steps = 0;
@@ -1817,6 +1867,7 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         sim->stats.collision_us = 1e3 * sim->sum_collision_ms / ts;
         sim->stats.iterations_us = 1e3 * sim->sum_iterations_ms / ts;
         sim->stats.detection_us = sim->self_host_us / nsteps;
+        sim->stats.pairing_us = sim->pairing_host_us / nsteps;
         sim->stats.steps = nsteps;
         sim->steps_done += nsteps;
         if (*sim->host_nonfinite) {
@@ -2112,7 +2163,8 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         I.n_contacts = nc;
         I.n_self_contacts = sim->nsc;
-        I.n_self_candidates = sim->n_full_candidates >= 0 ? sim->n_full_candidates : sim->host_totals[1];
+        I.n_self_candidates = sim->n_full_candidates >= 0 ? sim->n_full_candidates
+                                                          : (sim->host_small.p ? sim->host_small.p[1] : 0);
         I.n_self_levels = sim->n_self_levels;
         I.max_front = sim->fac.max_front;
         I.factor_seconds = sim->fac.seconds;

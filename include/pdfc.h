@@ -213,7 +213,8 @@ typedef struct pdfc_info {
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */
     double step_us;
-    double detection_us;             /* host time per step of the self-collision round trip and pairing */
+    double detection_us;             /* host time per step of the self-collision round trip and pairing
+                                        (including the wait for the device's detection and earlier work) */
     double iterations_us;            /* device time per step of its local-global iterations and epilogue */
     int64_t steps;
     int64_t iterations;              /* local-global iterations executed, all steps of the call */
@@ -226,6 +227,9 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
                                         obstacle and self detection, pairing); steps replayed from a HIP
                                         graph are not timed (then 0, and iterations_us 0) */
     int64_t timed_steps;             /* steps of the call that collision_us and iterations_us average over */
+    double pairing_us;               /* host time per step of the self-contact pairing, level ordering and
+                                        upload, after the detection results arrived (part of detection_us,
+                                        which also waits for the device) */
 } pdfc_phase_stats;
 
 typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over an instrumented run */
