@@ -1462,38 +1462,49 @@ void pdfc_sim::self_upload() {
     for (int x : slot_vertex) slot_of[x] = -1;
     // per-level groups: each touched slot once, entries in the level's contact order
     std::vector<int> lvl_ptr(L + 1, 0), grp_ptr(L + 1, 0), grp_slot, ent_ptr(1, 0), ent_contact, ent_sign;
-    std::vector<int> gidx(nslots, -1);
+    // groups in order of first appearance, entries of a group in the level's contact order:
+    // a counting pass, then a placing pass (no per-group containers)
+    std::vector<int> gidx(nslots, -1), gs, gcount, gfill;
+    ent_contact.resize(2 * (size_t)nsc);
+    ent_sign.resize(2 * (size_t)nsc);
     lvl_counts.assign(L, 0);
     grp_counts.assign(L, 0);
-    int j0 = 0;
+    int j0 = 0, e0 = 0;
     for (int l = 0; l < L; ++l) {
         const int cnt = (int)levels[l].size();
         lvl_ptr[l + 1] = lvl_ptr[l] + cnt;
-        std::vector<std::vector<std::pair<int, int>>> groups;
-        std::vector<int> gs;
+        gs.clear();
+        gcount.clear();
         for (int j = j0; j < j0 + cnt; ++j)
             for (int side = 0; side < 2; ++side) {
-                int sl = side == 0 ? SA[j] : SB[j];
+                const int sl = side == 0 ? SA[j] : SB[j];
                 if (gidx[sl] < 0) {
-                    gidx[sl] = (int)groups.size();
-                    groups.emplace_back();
+                    gidx[sl] = (int)gs.size();
                     gs.push_back(sl);
+                    gcount.push_back(0);
                 }
-                groups[gidx[sl]].push_back({j, side == 0 ? 1 : -1});
+                ++gcount[gidx[sl]];
             }
-        for (size_t g = 0; g < groups.size(); ++g) {
+        const int ng = (int)gs.size();
+        gfill.assign(ng, 0);
+        for (int g = 0, acc = e0; g < ng; ++g) {
+            gfill[g] = acc;
+            acc += gcount[g];
             grp_slot.push_back(gs[g]);
-            for (auto& e : groups[g]) {
-                ent_contact.push_back(e.first);
-                ent_sign.push_back(e.second);
-            }
-            ent_ptr.push_back((int)ent_contact.size());
-            gidx[gs[g]] = -1;
+            ent_ptr.push_back(acc);
         }
-        grp_ptr[l + 1] = grp_ptr[l] + (int)groups.size();
+        for (int j = j0; j < j0 + cnt; ++j)
+            for (int side = 0; side < 2; ++side) {
+                const int e = gfill[gidx[side == 0 ? SA[j] : SB[j]]]++;
+                ent_contact[e] = j;
+                ent_sign[e] = side == 0 ? 1 : -1;
+            }
+        for (int sl : gs) gidx[sl] = -1;
+        grp_ptr[l + 1] = grp_ptr[l] + ng;
         lvl_counts[l] = cnt;
-        grp_counts[l] = (int)groups.size();
+        grp_counts[l] = ng;
         j0 += cnt;
+        e0 += 2 * cnt;
     }
     // staging: ints then doubles
     up_int.clear();
