@@ -933,12 +933,37 @@ void pdfc_sim::autotune_waves() {
         return best;
     };
     // a candidate replaces the current choice only when clearly faster, so repeated tunings
-    // of the same scene settle on the same configuration
+    // of the same scene settle on the same configuration; the level pass takes 1.5% (a leaf
+    // level that streams its operators from HBM in the step, but from cache when timed
+    // alone, differs by about that much in a whole solve)
     constexpr float kSwitch = 0.97f;
+    constexpr float kSwitchLevels = 0.985f;
     const int cand[5] = {1, 2, 4, 8, 16};
+    // A level whose operators exceed kStreamBytes streams them from HBM inside a step, where
+    // it needs about two waves per SIMD in flight (2,048 on 256 CUs); timed alone (or in
+    // back-to-back solves) its operators stay in cache and fewer waves look as fast. Such a
+    // level is given at least the waves per task (up to 4) that reach 2,048 waves.
+    constexpr double kStreamBytes = 8. * 1048576.;
+    auto level_bytes = [&](const pdfc::SweepPlan& P, size_t l) {
+        std::vector<int> sn(P.task_snode.begin() + P.level_ptr[l], P.task_snode.begin() + P.level_ptr[l + 1]);
+        std::sort(sn.begin(), sn.end());
+        sn.erase(std::unique(sn.begin(), sn.end()), sn.end());
+        double b = 0.;
+        for (int s2 : sn) b += (double)(fac.foff[s2 + 1] - fac.foff[s2]) * (fp32 ? 4. : 8.);
+        return b;
+    };
+    auto min_waves = [&](const pdfc::SweepPlan& P, size_t l) {
+        if (level_bytes(P, l) <= kStreamBytes) return 1;
+        const long tasks = P.level_ptr[l + 1] - P.level_ptr[l];
+        int w = 1;
+        while (w < 4 && tasks * w < 2048) w *= 2;
+        return w;
+    };
     for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l) {
         float best = 1e30f;
+        const int wmin = min_waves(fwd, l);
         for (int W : cand) {
+            if (W < wmin) continue;
             float ms = time_it([&] {
                 pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l], V[1],
                                            Y, fac.max_ns, W, stream);
@@ -951,7 +976,9 @@ void pdfc_sim::autotune_waves() {
     }
     for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l) {
         float best = 1e30f;
+        const int wmin = min_waves(bwd, l);
         for (int W : cand) {
+            if (W < wmin) continue;
             float ms = time_it([&] {
                 pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
                                             V[1], fac.max_front, W, stream);
@@ -1006,15 +1033,17 @@ void pdfc_sim::autotune_waves() {
     }
     for (int pass = 0; pass < 2; ++pass) {
         std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
+        const pdfc::SweepPlan& plan = pass == 0 ? fwd : bwd;
         for (size_t l = 0; l < waves.size(); ++l) {
             const int keep = waves[l];
+            const int wmin = min_waves(plan, l);
             float best = solve_ms();
             int best_w = keep;
             for (int W : cand) {
-                if (W == keep) continue;
+                if (W == keep || W < wmin) continue;
                 waves[l] = W;
                 float ms = solve_ms();
-                if (ms < kSwitch * best) {
+                if (ms < kSwitchLevels * best) {
                     best = ms;
                     best_w = W;
                 }
