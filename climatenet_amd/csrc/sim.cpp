@@ -902,6 +902,8 @@ void pdfc_sim::apply_pinned_tuning(const char* spec) {
     if (parse_list('t', t) && use_top) {
         if (t.size() != 1 || t[0] < 0 || t[0] > kMaxTopChunks)
             throw std::invalid_argument("PDFC_TUNE: top chunks 0 (symmetric tiles) or 1..16");
+        if (t[0] == 0 && !top_sym)
+            throw std::invalid_argument("PDFC_TUNE: symmetric tiles are off (PDFC_TOP_SYM=0, or more than 128 row blocks)");
         set_top_mode(t[0]);
     }
 }
