@@ -16,6 +16,7 @@ namespace pdfc {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kAtaChunk = 8;  // A^T A gathers in flight at a time
 
 __device__ __forceinline__ double wave_sum(double x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -109,19 +110,41 @@ __global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const do
                              double* __restrict__ v0, double* __restrict__ cvec, StepArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
+    if (a.record)  // the step's residual records start from zero
+        for (int k = i; k < a.ac_cap; k += gridDim.x * blockDim.x) {
+            a.ac_sum[k] = 0.;
+            a.ac_max[k] = 0ull;
+        }
     if (i >= n) return;
     double m = M.mass[i], im = M.invmass[i];
     D3 v = ld3(vstate, n, i);
     D3 t = {v.x + a.dt * (im * (m * a.g[0])), v.y + a.dt * (im * (m * a.g[1])), v.z + a.dt * (im * (m * a.g[2]))};
     st3(v0, n, i, t);
+    // (A^T A q_n)_i, kAtaChunk entries in flight at a time, summed in entry order
     D3 s = {0., 0., 0.};
-    int cnt = M.ata_cnt[i];
-    for (int k = 0; k < cnt; ++k) {
-        int j = M.ata_idx[(size_t)k * n + i];
-        double w = M.ata_val[(size_t)k * n + i];
-        s.x += w * qn[j];
-        s.y += w * qn[n + j];
-        s.z += w * qn[2 * n + j];
+    const int cnt = M.ata_cnt[i];
+    for (int k0 = 0; k0 < cnt; k0 += kAtaChunk) {
+        int jj[kAtaChunk];
+        double w[kAtaChunk], g[kAtaChunk][3];
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k) {
+            const int e = min(k0 + k, cnt - 1);
+            jj[k] = M.ata_idx[(size_t)e * n + i];
+            w[k] = M.ata_val[(size_t)e * n + i];
+        }
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k) {
+            g[k][0] = qn[jj[k]];
+            g[k][1] = qn[n + jj[k]];
+            g[k][2] = qn[2 * n + jj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k)
+            if (k0 + k < cnt) {
+                s.x += w[k] * g[k][0];
+                s.y += w[k] * g[k][1];
+                s.z += w[k] * g[k][2];
+            }
     }
     st3(cvec, n, i, D3{m * t.x - a.dt * s.x, m * t.y - a.dt * s.y, m * t.z - a.dt * s.z});
 }
@@ -336,7 +359,6 @@ __global__ void k_detect_write(DevModel M, const double* __restrict__ qn, DevObs
 // then the gathers); wider stencils take the loop below.
 constexpr int kSpringCap = 8;
 constexpr int kAtaCap = 24;
-constexpr int kAtaChunk = 8;  // A^T A gathers in flight at a time
 
 // (A^T p)_i at q = q_n + dt v: sum over the incident springs of w^2 L0 (q_i - q_j)/|q_i - q_j|.
 __device__ __forceinline__ D3 spring_sum(const DevModel& M, const double* __restrict__ qn,

@@ -155,6 +155,7 @@ struct StepArgs {
     int record;        // accumulate Alart-Curnier norms
     double* ac_sum;    // per iteration
     unsigned long long* ac_max;  // per iteration, bit pattern of a non-negative double
+    int ac_cap;        // entries of ac_sum / ac_max (cleared by k_step_begin when record is set)
     // warm start of obstacle contacts in the iterations launched with warm_mode != 0:
     // r = m_i * warm_y[cid * warm_stride + 0..2] (contact basis, per unit mass), projected on the cone
     int warm_mode;     // 0 plain, 1 previous step (plain where it is zero), 2 learned

@@ -1636,6 +1636,7 @@ void pdfc_sim::run_step() {
     a.record = (prm.record_iterations || adaptive) ? 1 : 0;
     a.ac_sum = ac_sum;
     a.ac_max = ac_max;
+    a.ac_cap = ac_cap;
     a.nonfinite = d_nonfinite;
     const bool given = given_pending;
     given_pending = false;
@@ -1650,11 +1651,7 @@ void pdfc_sim::run_step() {
     if (mode == PDFC_WARM_LEARNED && !mlp)
         throw std::invalid_argument("learned warm start without a network (pdfc_set_mlp)");
     step_event(0);
-    if (a.record) {
-        pdfc::launch_fill(ac_sum, ac_cap, 0., stream);
-        pdfc::launch_fill_u64(ac_max, ac_cap, stream);
-    }
-    pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
+    pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);  // also clears the residual records
     detect();
     const bool need_nc = adaptive || features;
     if (need_nc) HIP_CHECK(hipMemcpyAsync(host_nc, ncontacts, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1756,9 +1753,7 @@ void pdfc_sim::run_step() {
         mode = PDFC_WARM_PLAIN;
         a.warm_mode = 0;
         a.warm_y = nullptr;
-        pdfc::launch_fill(ac_sum, ac_cap, 0., stream);
-        pdfc::launch_fill_u64(ac_max, ac_cap, stream);
-        pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);
+        pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);  // also clears the residual records
         if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
     }
     const int K = it;
