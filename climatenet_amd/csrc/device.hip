@@ -103,6 +103,38 @@ __global__ void k_axpy(double* x, const double* dx, size_t count) {
     if (i < count) x[i] += dx[i];
 }
 
+// (A^T A x)_i with kAtaChunk index and gather loads in flight at a time (clamped indices:
+// every load in bounds and unconditional), summed in entry order.
+__device__ __forceinline__ D3 ata_product(const DevModel& M, const double* __restrict__ x, int i) {
+    const int n = M.n;
+    D3 s = {0., 0., 0.};
+    const int cnt = M.ata_cnt[i];
+    for (int k0 = 0; k0 < cnt; k0 += kAtaChunk) {
+        int jj[kAtaChunk];
+        double w[kAtaChunk], g[kAtaChunk][3];
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k) {
+            const int e = min(k0 + k, cnt - 1);
+            jj[k] = M.ata_idx[(size_t)e * n + i];
+            w[k] = M.ata_val[(size_t)e * n + i];
+        }
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k) {
+            g[k][0] = x[jj[k]];
+            g[k][1] = x[n + jj[k]];
+            g[k][2] = x[2 * n + jj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kAtaChunk; ++k)
+            if (k0 + k < cnt) {
+                s.x += w[k] * g[k][0];
+                s.y += w[k] * g[k][1];
+                s.z += w[k] * g[k][2];
+            }
+    }
+    return s;
+}
+
 // t_n = v_n + dt M^-1 (M g); v^0 = t_n; c = M t_n - dt A^T A q_n
 // (SimulationSpeed::initializeStep, SimulationSpeed.cpp:62-76; constant part of
 // SimulationSpeed::computeRhs, :263-277, hoisted out of the iteration loop).
@@ -120,32 +152,7 @@ __global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const do
     D3 v = ld3(vstate, n, i);
     D3 t = {v.x + a.dt * (im * (m * a.g[0])), v.y + a.dt * (im * (m * a.g[1])), v.z + a.dt * (im * (m * a.g[2]))};
     st3(v0, n, i, t);
-    // (A^T A q_n)_i, kAtaChunk entries in flight at a time, summed in entry order
-    D3 s = {0., 0., 0.};
-    const int cnt = M.ata_cnt[i];
-    for (int k0 = 0; k0 < cnt; k0 += kAtaChunk) {
-        int jj[kAtaChunk];
-        double w[kAtaChunk], g[kAtaChunk][3];
-#pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k) {
-            const int e = min(k0 + k, cnt - 1);
-            jj[k] = M.ata_idx[(size_t)e * n + i];
-            w[k] = M.ata_val[(size_t)e * n + i];
-        }
-#pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k) {
-            g[k][0] = qn[jj[k]];
-            g[k][1] = qn[n + jj[k]];
-            g[k][2] = qn[2 * n + jj[k]];
-        }
-#pragma unroll
-        for (int k = 0; k < kAtaChunk; ++k)
-            if (k0 + k < cnt) {
-                s.x += w[k] * g[k][0];
-                s.y += w[k] * g[k][1];
-                s.z += w[k] * g[k][2];
-            }
-    }
+    const D3 s = ata_product(M, qn, i);  // (A^T A q_n)_i
     st3(cvec, n, i, D3{m * t.x - a.dt * s.x, m * t.y - a.dt * s.y, m * t.z - a.dt * s.z});
 }
 
@@ -1306,15 +1313,7 @@ __global__ void k_residual(DevModel M, double dt2, const double* __restrict__ b,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = M.n;
     if (i >= n) return;
-    D3 s = {0., 0., 0.};
-    int cnt = M.ata_cnt[i];
-    for (int k = 0; k < cnt; ++k) {
-        int j = M.ata_idx[(size_t)k * n + i];
-        double w = M.ata_val[(size_t)k * n + i];
-        s.x += w * x[j];
-        s.y += w * x[n + j];
-        s.z += w * x[2 * n + j];
-    }
+    const D3 s = ata_product(M, x, i);
     double m = M.mass[i];
     r[i] = b[i] - (m * x[i] + dt2 * s.x);
     r[n + i] = b[n + i] - (m * x[n + i] + dt2 * s.y);
