@@ -1233,27 +1233,28 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
     }
 }
 
-// Symmetric tiles: kFinishLanes lanes per top row, lane g sums group g (as top_sum_blocks),
-// lane 0 adds the group sums in group order and scatters.
-constexpr int kFinishLanes = 8;
-__global__ void k_top_finish_blocks(DevTop T, double* __restrict__ X, int n) {
-    const int g = threadIdx.x % kFinishLanes;
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / kFinishLanes;
+// Symmetric tiles: one workgroup per 64-row block, wave g sums group g of the block's rows
+// (as top_sum_blocks; lane = row, so every load instruction reads 64 consecutive partials),
+// wave 0 adds the group sums in group order and scatters.
+constexpr int kFinishLanes = 8;  // groups (waves) per row block
+__global__ void __launch_bounds__(64 * kFinishLanes) k_top_finish_blocks(DevTop T, double* __restrict__ X, int n) {
+    __shared__ double gsum[kFinishLanes][3][64];
+    const int g = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = 64 * blockIdx.x + lane;
     const int groups = (T.nb + kTopChunksMax - 1) / kTopChunksMax;
     double x = 0., y = 0., z = 0.;
     if (t < T.m && g < groups) top_group_sum(T.part, T.nb, t, g, x, y, z);
-    double sx = x, sy = y, sz = z;  // lane 0: group 0
-    for (int k = 1; k < kFinishLanes; ++k) {
-        const double ox = __shfl(x, (threadIdx.x & 63) - g + k, 64);
-        const double oy = __shfl(y, (threadIdx.x & 63) - g + k, 64);
-        const double oz = __shfl(z, (threadIdx.x & 63) - g + k, 64);
-        if (k < groups) {
-            sx += ox;
-            sy += oy;
-            sz += oz;
-        }
+    gsum[g][0][lane] = x;
+    gsum[g][1][lane] = y;
+    gsum[g][2][lane] = z;
+    __syncthreads();
+    if (g != 0 || t >= T.m) return;
+    double sx = gsum[0][0][lane], sy = gsum[0][1][lane], sz = gsum[0][2][lane];
+    for (int k = 1; k < groups; ++k) {
+        sx += gsum[k][0][lane];
+        sy += gsum[k][1][lane];
+        sz += gsum[k][2][lane];
     }
-    if (t >= T.m || g != 0) return;
     const int row = T.rows[t];
     X[row] = sx;
     X[n + row] = sy;
@@ -1441,8 +1442,7 @@ void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, do
         // nb <= kFinishLanes * kTopChunksMax (the host offers the tiles for m <= 8192 only)
         static_assert(64 % kFinishLanes == 0 && kFinishLanes * kTopChunksMax == 128, "finish lanes");
         if (finish)
-            hipLaunchKernelGGL(k_top_finish_blocks, dim3(blocks_for((size_t)T.m * kFinishLanes)), dim3(kBlock), 0, st,
-                               T, X, n);
+            hipLaunchKernelGGL(k_top_finish_blocks, dim3(T.nb), dim3(64 * kFinishLanes), 0, st, T, X, n);
         return;
     }
     hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
