@@ -61,18 +61,13 @@ __device__ void contact_basis(D3 nrm, double* B) {
 }
 
 // Block-wide accumulation of (sum, max) into per-iteration slots.
+// Per wave (no block barrier: a wave that is done need not wait for the block's others).
 __device__ void block_ac_accumulate(double s, double m, double* sum_slot, unsigned long long* max_slot) {
-    __shared__ double red_s[kBlock / 64], red_m[kBlock / 64];
     s = wave_sum(s);
     m = wave_max(m);
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) { red_s[w] = s; red_m[w] = m; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double ts = 0., tm = 0.;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { ts += red_s[k]; tm = fmax(tm, red_m[k]); }
-        if (ts != 0.) atomicAdd(sum_slot, ts);
-        if (tm > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(tm));
+    if ((threadIdx.x & 63) == 0) {
+        if (s != 0.) atomicAdd(sum_slot, s);
+        if (m > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(m));
     }
 }
 
