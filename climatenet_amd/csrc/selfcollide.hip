@@ -446,25 +446,15 @@ __device__ double ac_norm3(P3 u, const double* r, double mu) {
     return sqrt((an * an + a1 * a1) + a2 * a2);
 }
 
+// Per wave (no block barrier), as the obstacle contacts' accumulation.
 __device__ void ac_block(double s, double m, double* sum_slot, unsigned long long* max_slot) {
-    __shared__ double rs[kB / 64], rm[kB / 64];
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o, 64);
         m = fmax(m, __shfl_xor(m, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-        rs[threadIdx.x >> 6] = s;
-        rm[threadIdx.x >> 6] = m;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double ts = 0., tm = 0.;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-            ts += rs[k];
-            tm = fmax(tm, rm[k]);
-        }
-        if (ts != 0.) atomicAdd(sum_slot, ts);
-        if (tm > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(tm));
+        if (s != 0.) atomicAdd(sum_slot, s);
+        if (m > 0.) atomicMax(max_slot, (unsigned long long)__double_as_longlong(m));
     }
 }
 
