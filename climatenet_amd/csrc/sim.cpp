@@ -473,7 +473,9 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         // for the 40k-vertex drape, 64 for batches of 1k-17k-vertex scenes.
         int largest = 0;
         for (size_t k = 0; k < count; ++k) largest = std::max(largest, (int)scenes[k].n_vertices);
-        leaf = largest >= 25000 ? 96 : 64;
+        // 96-vertex leaves for large scenes and for single scenes of a few thousand vertices
+        // (fewer levels); batches of small scenes stay at 64
+        leaf = (largest >= 25000 || (count == 1 && largest >= 4000)) ? 96 : 64;
         if (const char* e = std::getenv("PDFC_LEAF")) leaf = std::max(8, std::atoi(e));
         pdfc::factorize(model, dt * dt, fac, leaf);
     } catch (const std::runtime_error& e) {
