@@ -28,7 +28,10 @@
  * closest points. Mesh obstacles: tests/test_oracle_mesh.py. Boxes, rotating
  * spheres and attachments: tests/test_oracle_obstacles.py (nearest-face brute
  * force, omega x r, sticking on a turning surface, the implicit-Euler closed form
- * of an attachment).
+ * of an attachment). Self contacts: tests/test_oracle_self.py (greedy pairing on a
+ * hand-built candidate list with hand-derived pairs, BFS levels written by hand from
+ * the three seeding passes, the two-body closed form of the mass-weighted level
+ * update, hand-computed contact bases).
  */
 #include "pdfc.h"
 
@@ -787,9 +790,9 @@ void update_mesh_obstacle(MeshObstacle& M, double dt) {
         for (int k = 0; k < 3; ++k) M.cur[v][k] = P[v][k] * (1. - a) + N[v][k] * a;
 }
 
-// checkMeshAllCollision (geometry_util.cpp:229-262) + checkMeshSelfCollisions
-// greedy node-node pairing (geometry_util.cpp:94-206).
-void detect_self(orc_sim& S, bool brute) {
+// checkMeshAllCollision (geometry_util.cpp:229-262): vertex boxes against triangle boxes,
+// exact closest-point test, kept if |cp - x|^2 < tol^2; sorted by (vertex, triangle).
+void self_candidates(orc_sim& S, bool brute) {
     std::vector<std::pair<int, int>> pairs;
     broad_phase(S, pairs, brute);
     const double tol = S.tol_eff, tol_sq = tol * tol;
@@ -800,9 +803,19 @@ void detect_self(orc_sim& S, bool brute) {
         closest_point(S.q[pr.first], S.q[f[0]], S.q[f[1]], S.q[f[2]], tol, cp, bary);
         if (sqn(sub(cp, S.q[pr.first])) < tol_sq) S.candidates.push_back({pr.first, pr.second, bary});
     }
+}
+
+// checkMeshSelfCollisions greedy node-node pairing (geometry_util.cpp:94-206) over
+// S.candidates in list order (vertex ascending, then triangle): skip a candidate whose
+// triangle holds the vertex or has a zero barycentric coordinate, or one of whose vertices
+// the vertex is already paired with; a triangle vertex already paired with a one-ring
+// neighbour of the vertex is excluded; the partner is the remaining triangle vertex of
+// largest barycentric coordinate (none: skip). Normal from the unsnapped closest point.
+void pair_candidates(orc_sim& S) {
     S.selfc.clear();
     const double mu = S.friction[(size_t)S.mesh_material * S.n_mat + S.mesh_material];
     std::vector<std::vector<int>> registered(S.n);
+    // isTriangleVertexIndexIn (geometry_util.cpp:211-227): first triangle slot whose vertex is listed
     auto index_in = [](const std::array<int, 3>& tri, const std::vector<int>& list) {
         for (int k = 0; k < 3; ++k)
             if (std::find(list.begin(), list.end(), tri[k]) != list.end()) return k;
@@ -837,27 +850,38 @@ void detect_self(orc_sim& S, bool brute) {
     }
 }
 
-// computeCollisionComputationOrder and helpers: SimulationFrictionEstimation.cpp:161-293.
-void self_levels(orc_sim& S) {
-    S.levels.clear();
-    const int ns = (int)S.selfc.size();
-    if (ns == 0) return;
-    std::vector<std::vector<std::pair<int, int>>> graph(S.n);
+void detect_self(orc_sim& S, bool brute) {
+    self_candidates(S, brute);
+    pair_candidates(S);
+}
+
+// computeCollisionComputationOrder and helpers (SimulationFrictionEstimation.cpp:161-293):
+// the graph joins each pair's vertex and partner (both directions, contact order); BFS
+// fills levels from index 0 for every seed set: first all obstacle-contact vertices
+// together, then each vertex of degree 1 not yet fully visited, then each remaining
+// vertex of degree > 1, in vertex order. A contact is visited once; every BFS round
+// opens its level even when it adds nothing.
+std::vector<std::vector<int>> contact_levels(int n, const std::vector<std::pair<int, int>>& pairs,
+                                             const std::vector<int>& seeds) {
+    std::vector<std::vector<int>> levels;
+    const int ns = (int)pairs.size();
+    if (ns == 0) return levels;
+    std::vector<std::vector<std::pair<int, int>>> graph(n);
     for (int s = 0; s < ns; ++s) {
-        graph[S.selfc[s].v].push_back({S.selfc[s].partner, s});
-        graph[S.selfc[s].partner].push_back({S.selfc[s].v, s});
+        graph[pairs[s].first].push_back({pairs[s].second, s});
+        graph[pairs[s].second].push_back({pairs[s].first, s});
     }
     std::vector<char> visited(ns, 0);
     auto fill = [&](std::vector<int> last, size_t idx) {
         std::vector<int> visiting;
         while (!last.empty()) {
-            if (S.levels.size() <= idx) S.levels.emplace_back();
+            if (levels.size() <= idx) levels.emplace_back();
             for (int u : last)
                 for (auto& nb : graph[u]) {
                     if (visited[nb.second]) continue;
                     visiting.push_back(nb.first);
                     visited[nb.second] = 1;
-                    S.levels[idx].push_back(nb.second);
+                    levels[idx].push_back(nb.second);
                 }
             last.swap(visiting);
             visiting.clear();
@@ -868,17 +892,24 @@ void self_levels(orc_sim& S) {
         for (auto& nb : graph[v]) if (!visited[nb.second]) return false;
         return true;
     };
-    std::vector<int> seeds;
-    for (auto& c : S.contacts) seeds.push_back(c.v);
     fill(seeds, 0);
-    for (int v = 0; v < S.n; ++v) {
+    for (int v = 0; v < n; ++v) {
         if (vvisited(v)) continue;
         if (graph[v].size() == 1) fill({v}, 0);
     }
-    for (int v = 0; v < S.n; ++v) {
+    for (int v = 0; v < n; ++v) {
         if (vvisited(v)) continue;
         if (graph[v].size() > 1) fill({v}, 0);
     }
+    return levels;
+}
+
+void self_levels(orc_sim& S) {
+    std::vector<std::pair<int, int>> pairs;
+    for (const auto& s : S.selfc) pairs.push_back({s.v, s.partner});
+    std::vector<int> seeds;
+    for (auto& c : S.contacts) seeds.push_back(c.v);
+    S.levels = contact_levels(S.n, pairs, seeds);
     for (int l = 0; l < (int)S.levels.size(); ++l)
         for (int s : S.levels[l]) S.selfc[s].level = l;
 }
@@ -1340,6 +1371,47 @@ int orc_get_bending_row(orc_sim* S, int b, int32_t* idx, double* coef, int cap) 
 double orc_factor_seconds(orc_sim* S) { return S->factor_seconds; }
 
 /* ---- pieces exposed for unit tests ---- */
+/* Greedy node-node pairing and level ordering of a given candidate list: the candidates
+ * (vertex, triangle, barycentric) replace the detection's, are paired at the current
+ * positions and ordered by level (seeded by the current obstacle contacts); read the
+ * result with orc_get_self_contacts. */
+int orc_pair_candidates(orc_sim* S, const pdfc_candidate* c, size_t nc) {
+    S->candidates.clear();
+    for (size_t i = 0; i < nc; ++i) {
+        if (c[i].vertex < 0 || c[i].vertex >= S->n || c[i].triangle < 0 || c[i].triangle >= S->T) {
+            g_err = "candidate index out of range";
+            return -1;
+        }
+        S->candidates.push_back({(int)c[i].vertex, (int)c[i].triangle,
+                                 V3{c[i].barycentric[0], c[i].barycentric[1], c[i].barycentric[2]}});
+    }
+    pair_candidates(*S);
+    self_levels(*S);
+    return 0;
+}
+
+/* Level ordering alone: pairs (v[s], partner[s]) over n vertices, BFS seeds = obstacle-contact
+ * vertices. level[s] gets each pair's level, order[] the pair indices level by level (ns
+ * entries). Returns the number of levels (empty trailing levels included), or -1. */
+int orc_contact_levels(int n, const int64_t* v, const int64_t* partner, size_t ns, const int64_t* seeds,
+                       size_t nseeds, int32_t* level, int32_t* order) {
+    std::vector<std::pair<int, int>> pairs;
+    for (size_t s = 0; s < ns; ++s) {
+        if (v[s] < 0 || v[s] >= n || partner[s] < 0 || partner[s] >= n) { g_err = "pair vertex out of range"; return -1; }
+        pairs.push_back({(int)v[s], (int)partner[s]});
+    }
+    std::vector<int> sd;
+    for (size_t k = 0; k < nseeds; ++k) {
+        if (seeds[k] < 0 || seeds[k] >= n) { g_err = "seed vertex out of range"; return -1; }
+        sd.push_back((int)seeds[k]);
+    }
+    auto L = contact_levels(n, pairs, sd);
+    size_t o = 0;
+    for (int l = 0; l < (int)L.size(); ++l)
+        for (int s : L[l]) { level[s] = l; order[o++] = s; }
+    return (int)L.size();
+}
+
 void orc_mean_value_coefficients(const double* c, const double* ring, int N, double* out) {
     std::vector<V3> r(N);
     for (int i = 0; i < N; ++i) r[i] = {ring[3 * i], ring[3 * i + 1], ring[3 * i + 2]};

@@ -63,6 +63,10 @@ def lib():
         L.orc_alart_curnier.restype = C.c_double
         L.orc_contact_basis.argtypes = [C.POINTER(C.c_double)] * 2
         L.orc_contact_basis.restype = None
+        L.orc_pair_candidates.argtypes = [P, C.POINTER(abi.Candidate), C.c_size_t]
+        I64 = C.POINTER(C.c_int64)
+        I32 = C.POINTER(C.c_int32)
+        L.orc_contact_levels.argtypes = [C.c_int, I64, I64, C.c_size_t, I64, C.c_size_t, I32, I32]
         F = C.POINTER(C.c_float)
         L.orc_mlp_forward.argtypes = [F, F, F, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.orc_mlp_forward.restype = None
@@ -193,6 +197,19 @@ class Oracle:
     def factor_seconds(self):
         return lib().orc_factor_seconds(self.h)
 
+    def pair_candidates(self, cands):
+        """Greedy pairing + level ordering of given candidates [(vertex, triangle, (b0, b1, b2))]
+        at the current positions; returns the self contacts."""
+        arr = (abi.Candidate * max(len(cands), 1))()
+        for i, (v, t, b) in enumerate(cands):
+            arr[i].vertex = v
+            arr[i].triangle = t
+            for k in range(3):
+                arr[i].barycentric[k] = b[k]
+        if lib().orc_pair_candidates(self.h, arr, len(cands)) != 0:
+            raise ValueError(lib().orc_last_error().decode())
+        return self.self_contacts()
+
 
 def mean_value_coefficients(center, ring):
     center = np.ascontiguousarray(center, dtype=np.float64)
@@ -223,6 +240,27 @@ def alart_curnier(u, r, mu):
     ac = np.zeros(3)
     nrm = lib().orc_alart_curnier(_d(u), _d(r), float(mu), _d(ac))
     return nrm, ac
+
+
+def contact_levels(n_vertices, pairs, seeds=()):
+    """Level ordering of self-contact pairs [(vertex, partner)] seeded by obstacle-contact
+    vertices: (levels as lists of pair indices in computation order, level of each pair)."""
+    pr = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    v = np.ascontiguousarray(pr[:, 0])
+    w = np.ascontiguousarray(pr[:, 1])
+    sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64).reshape(-1))
+    lev = np.full(len(pr), -1, dtype=np.int32)
+    order = np.full(len(pr), -1, dtype=np.int32)
+    I64 = C.POINTER(C.c_int64)
+    I32 = C.POINTER(C.c_int32)
+    nl = lib().orc_contact_levels(int(n_vertices), v.ctypes.data_as(I64), w.ctypes.data_as(I64), len(pr),
+                                  sd.ctypes.data_as(I64), sd.size, lev.ctypes.data_as(I32), order.ctypes.data_as(I32))
+    if nl < 0:
+        raise ValueError(lib().orc_last_error().decode())
+    levels = [[] for _ in range(nl)]
+    for s in order:
+        levels[lev[s]].append(int(s))
+    return levels, lev
 
 
 def contact_basis(n):
