@@ -4,13 +4,19 @@
 
 A step is one SimulationBase::step() of the 200x200-quad drape (40,401
 vertices, 80,000 triangles, sphere R=0.5 + ground plane, mu=0.1, dt=1/200,
-20 local-global iterations) with state resident in HBM. Prints one JSON line.
+20 local-global iterations, self collision on) with state resident in HBM.
+The cloth is advanced untimed to step --first-step (110: on the sphere and
+folding onto itself), then --warmup steps, then --steps steps are timed. The
+whole 500-step drape is also timed once. Prints one JSON line.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+# thread placement of the CPU oracle's OpenMP runtime (read when it starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np
 
@@ -37,31 +43,57 @@ def measured_traffic(kernel, info):
     return d.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(spec, budget_s=20.0, start=2):
-    """The CPU oracle (reference algorithm and OpenMP structure) on a bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, budget_s=20.0, start=2, state=None, window=None, runs=3):
+    """The CPU oracle (reference algorithm and OpenMP structure) on a bounded sample.
+    With `state` (q, v at the first timed GPU step `start`), each of `runs` runs starts from
+    that state and times the same steps the GPU timed (at most `window`, cut to the budget);
+    the median run is reported. Without it the oracle steps from the scene's start to `start`."""
     from oracle import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     orc = po.Oracle(spec)
-    orc.step(start)  # warm-up into the first steps of the same trajectory
+    if state is None:
+        orc.step(start)  # warm-up into the first steps of the same trajectory
+    else:
+        orc.set_state(*state)
     t0 = time.perf_counter()
     orc.step(1)
     t1 = time.perf_counter() - t0
-    k = max(2, min(100, int(budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    orc.step(k)
-    dt = time.perf_counter() - t0
-    # one thread: the next 2 steps of the same trajectory
+    nr = runs if state is not None else 1
+    k = max(2, min(window or 100, int(budget_s / nr / max(t1, 1e-6))))
+    rates = []
+    for r in range(nr):
+        if state is not None:
+            orc.set_state(*state)
+        t0 = time.perf_counter()
+        orc.step(k)
+        rates.append(k / (time.perf_counter() - t0))
     po.set_num_threads(1)
+    if state is not None:
+        orc.set_state(*state)
     t0 = time.perf_counter()
-    orc.step(2)
+    orc.step(1)
     dt1 = time.perf_counter() - t0
     po.set_num_threads(threads)
-    s0 = start + 2
-    return {"value": k / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "steps %d..%d of the same scene, %d OpenMP threads, factorisation excluded"
-                      % (s0, s0 + k - 1, threads),
-            "one_thread": {"value": 2 / dt1, "unit": "steps/s",
-                           "sample": "steps %d..%d, 1 thread" % (s0 + k, s0 + k + 1)}}
+    s0 = start if state is not None else start + 1
+    out = {"value": float(np.median(rates)), "unit": "steps/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+           "runs": [round(x, 4) for x in rates],
+           "sample": "steps %d..%d (%s), %d OpenMP threads, median of %d runs, factorisation excluded"
+                     % (s0, s0 + k - 1, "from the GPU's state at step %d" % start if state is not None
+                        else "the oracle's own trajectory", threads, len(rates)),
+           "one_thread": {"value": 1 / dt1, "unit": "steps/s", "sample": "step %d, 1 thread" % s0}}
+    return out
 
 
 def warm_start_comparison(spec, sim, prec, steps=2, cap=2000, tol=1e-6):
@@ -300,6 +332,9 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the plain vs learned warm-start comparison (iterations to a 1e-6 residual)")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--first-step", type=int, default=110,
+                    help="drape: untimed steps before the warm-up (110: the cloth is on the sphere and "
+                         "folds onto itself, the self-contact phase)")
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
     ap.add_argument("--workload", choices=["drape", "batched", "stacked", "meshdrape", "mlp"], default="drape",
@@ -341,13 +376,22 @@ def main():
     if args.no_self_collision:
         spec.self_collision = False
     prec = abi.FP64 if args.precision == "fp64" else abi.FP32_REFINE64
+    # the whole drape once (500 steps from the flat start), then the timed window
+    full = native.Simulation(spec, precision=prec, refinement_sweeps=2, record_iterations=True)
+    full.step(1)
+    t0 = time.perf_counter()
+    full.step(spec.steps - 1)
+    full_s = time.perf_counter() - t0
+    full.close()
     sim = native.Simulation(spec, precision=prec, refinement_sweeps=2, record_iterations=True)
-    sim.step(args.warmup)
+    sim.step(args.first_step + args.warmup)  # untimed advance into the contact phase, then warm-up
+    window_state = sim.state()
     t0 = time.perf_counter()
     sim.step(args.steps)  # returns after the stream is synchronised
     elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed
     phases = sim.phase_stats()
+    first_timed = args.first_step + args.warmup + 1
 
     # kernel times: instrumented continuation of the same run (events around every launch)
     sim.set_kernel_timing(True)
@@ -386,7 +430,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "drape_200x200: 40,401 vertices, sphere R=0.5 + plane, mu=0.1, dt=1/200, "
                                "20 local-global iterations, self-collision %s" % ("on (tol 5e-3)" if spec.self_collision else "off"),
-                   "first_step": args.warmup, "self_contacts_last_step": info["n_self_contacts"],
+                   "first_step": first_timed, "untimed_advance": args.first_step,
+                   "timed_steps": "%d..%d of %d" % (first_timed, first_timed + args.steps - 1, spec.steps),
+                   "window_contacts_per_step": round(phases["contacts_sum"] / args.steps, 1),
+                   "window_self_contacts_per_step": round(phases["self_contacts_sum"] / args.steps, 1),
+                   "window_self_levels_max": phases["self_levels_max"],
+                   "self_contacts_last_step": info["n_self_contacts"],
                    "iterations": K, "n_vertices": info["n_vertices"], "nnz_L": info["nnz_L"],
                    "nnz_F": info["nnz_F"], "levels": info["n_levels"], "levels_swept": info["n_levels_swept"],
                    "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"],
@@ -402,12 +451,17 @@ def main():
                           "frac": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9 / HBM_PEAK_GBS, 6)},
         "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
         "self_collision_host_us_per_step": {"round_trip": round(phases["detection_us"], 1),
-                                            "pairing": round(phases["pairing_us"], 1)},
+                                            "pairing": round(phases["pairing_us"], 1),
+                                            "of_which_levels": round(phases["order_us"], 1),
+                                            "of_which_upload": round(phases["upload_us"], 1)},
+        "full_drape": {"value": round((spec.steps - 1) / full_s, 3), "unit": "steps/s",
+                       "sample": "steps 2..%d of the drape from its flat start (step 1 untimed)" % spec.steps},
     }
     if not args.no_warm_start:
         out["warm_start"] = warm_start_comparison(spec, sim, prec)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(spec)
+        out["cpu_baseline"] = cpu_baseline(spec, budget_s=24.0, start=first_timed, state=window_state,
+                                           window=args.steps)
     print(json.dumps(out), flush=True)
 
 

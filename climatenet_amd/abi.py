@@ -180,6 +180,11 @@ class PhaseStats(C.Structure):
         ("collision_us", C.c_double),
         ("timed_steps", C.c_int64),
         ("pairing_us", C.c_double),
+        ("order_us", C.c_double),
+        ("upload_us", C.c_double),
+        ("contacts_sum", C.c_int64),
+        ("self_contacts_sum", C.c_int64),
+        ("self_levels_max", C.c_int64),
     ]
 
 

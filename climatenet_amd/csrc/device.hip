@@ -597,6 +597,10 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
         double damp = contact ? 0. : a.damping;
         D3 b = {c.x + dt * (s.x - damp * v.x), c.y + dt * (s.y - damp * v.y), c.z + dt * (s.z - damp * v.z)};
         if (contact) {
+            // self-collision mode, vertex without a self contact: only its last active obstacle
+            // reaction reaches the right-hand side (m_contact_forces is assigned, not summed,
+            // SimulationFrictionEstimation.cpp:410-418; rhs += CF + SCF with SCF = 0 there)
+            D3 glast = {0., 0., 0.};
             const D3 av = regs ? ata_row_sum(AR, vcur, n, i) : ata_row(M, vcur, i);
             const double dt2 = dt * dt;
             D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
@@ -662,11 +666,18 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
                     ls.CFn[3 * slot] = g.x;
                     ls.CFn[3 * slot + 1] = g.y;
                     ls.CFn[3 * slot + 2] = g.z;
+                } else if (ls.selfmode) {
+                    glast = g;
                 } else {
                     b.x += g.x;
                     b.y += g.y;
                     b.z += g.z;
                 }
+            }
+            if (ls.selfmode && slot < 0) {
+                b.x += glast.x;
+                b.y += glast.y;
+                b.z += glast.z;
             }
         }
         st3(rhs, n, i, b);

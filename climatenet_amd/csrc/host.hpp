@@ -7,6 +7,7 @@
 #pragma once
 
 #include <array>
+#include <functional>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -127,10 +128,11 @@ void host_contact_basis(const double n[3], double B[9]);
 
 // Greedy node-node pairing over candidates sorted by (vertex, triangle).
 void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out);
-// Computation levels: BFS from the obstacle-contact vertices, then from degree-1 and
-// remaining vertices in ascending order. Fills pair.level and the per-level lists
-// (contact indices in BFS order); returns the number of levels.
-int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPair>& pairs,
+// Computation levels: BFS from the obstacle-contact vertices (is_seed), then from degree-1
+// and remaining vertices in ascending order. Fills pair.level and the per-level lists
+// (contact indices in BFS order); returns the number of levels. Work is O(pairs log pairs):
+// is_seed is asked only for vertices of the pairs.
+int order_self_contacts(const std::function<bool(int)>& is_seed, std::vector<SelfPair>& pairs,
                         std::vector<std::vector<int>>& levels);
 
 }  // namespace pdfc

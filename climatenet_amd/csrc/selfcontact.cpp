@@ -65,9 +65,8 @@ void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand,
     }
 }
 
-int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPair>& pairs,
+int order_self_contacts(const std::function<bool(int)>& is_seed, std::vector<SelfPair>& pairs,
                         std::vector<std::vector<int>>& levels) {
-    (void)n;
     levels.clear();
     const int ns = (int)pairs.size();
     if (ns == 0) return 0;
@@ -126,13 +125,11 @@ int order_self_contacts(int n, const std::vector<int>& seeds, std::vector<SelfPa
             if (!visited[adj[q].second]) return false;
         return true;
     };
-    // seeds outside the graph have no adjacency and add nothing to a level
+    // the first search starts from every obstacle-contact vertex at once; those outside the
+    // graph add nothing, and the order of a frontier changes only the order inside a level
     std::vector<int> start;
-    start.reserve(seeds.size());
-    for (int v : seeds) {
-        const int u = id(v);
-        if (u < nv && verts[u] == v) start.push_back(u);
-    }
+    for (int u = 0; u < nv; ++u)
+        if (is_seed(verts[u])) start.push_back(u);
     bfs(start);
     for (int u = 0; u < nv; ++u)
         if (ptr[u + 1] - ptr[u] == 1 && !all_visited(u)) bfs({u});

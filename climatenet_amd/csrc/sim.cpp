@@ -253,14 +253,6 @@ struct pdfc_sim {
     int* vslot = nullptr;
     Pinned<int> host_small;                        // [0..2] detection totals, [3] slot overflow
     std::vector<int> slot_of_scratch;              // per vertex, -1 between uploads
-    template <typename F>
-    void for_each_bit(F&& f) const {               // set seed bits in ascending vertex order
-        for (int w = 0; w <= n / 64; ++w)
-            for (unsigned long long b = host_bits.p[w]; b; b &= b - 1) {
-                const int i = 64 * w + __builtin_ctzll(b);
-                if (i < n) f(i);
-            }
-    }
     Pinned<unsigned long long> host_bits;          // seed bits, n / 64 + 1 words
     Pinned<pdfc::SelfCandidate> host_filt;         // filtered candidates
     std::vector<pdfc::SelfPair> pairs;             // pairing order
@@ -283,6 +275,9 @@ struct pdfc_sim {
     size_t filt_guess = 4096;
     double self_host_us = 0.;   // host time in detection round trips + pairing, accumulated per pdfc_step
     double pairing_host_us = 0.; // the part after the detection results arrived
+    double order_host_us = 0.;   // of which the level ordering
+    double upload_host_us = 0.;  // of which packing and enqueuing the upload
+    bool nc_known = true;        // every step of the call copied its obstacle-contact count back
     int n_full_candidates = 0;
 
     void free_ptr(void* p) {
@@ -1451,14 +1446,22 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
     filt_guess = nf + nf / 2 + 256;
     const auto t_pair0 = std::chrono::steady_clock::now();
     pdfc::pair_self_contacts(model, host_filt.p, nf, pairs);
-    std::vector<int> seeds;
-    for_each_bit([&](int i) { seeds.push_back(fac.perm[i]); });
-    n_self_levels = pdfc::order_self_contacts(n, seeds, pairs, levels);
+    const auto t_order0 = std::chrono::steady_clock::now();
+    const unsigned long long* bits = host_bits.p;
+    n_self_levels = pdfc::order_self_contacts(
+        [&](int v) {
+            const int i = fac.iperm[v];
+            return ((bits[i >> 6] >> (i & 63)) & 1ull) != 0;
+        },
+        pairs, levels);
     nsc = (int)pairs.size();
+    const auto t_up0 = std::chrono::steady_clock::now();
     if (nsc > 0) self_upload();
     const auto t_end = std::chrono::steady_clock::now();
     self_host_us += std::chrono::duration<double, std::micro>(t_end - t_host0).count();
     pairing_host_us += std::chrono::duration<double, std::micro>(t_end - t_pair0).count();
+    order_host_us += std::chrono::duration<double, std::micro>(t_up0 - t_order0).count();
+    upload_host_us += std::chrono::duration<double, std::micro>(t_end - t_up0).count();
 }
 
 // Packs the self contacts sorted by level, the involved-vertex slots and the
@@ -1467,14 +1470,11 @@ void pdfc_sim::self_upload() {
     const int L = (int)levels.size();
     sorted_pair.clear();
     for (const auto& lv : levels) sorted_pair.insert(sorted_pair.end(), lv.begin(), lv.end());
-    // involved vertices: obstacle-contact vertices, then contact and paired vertices
+    // involved vertices: those of the self contacts, in order of first appearance (the local
+    // kernel keeps the last obstacle reaction of every other contact vertex itself)
     std::vector<int> slot_vertex;
     std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
     if ((int)slot_of.size() != n) slot_of.assign(n, -1);
-    for_each_bit([&](int i) {
-        slot_of[i] = (int)slot_vertex.size();
-        slot_vertex.push_back(i);
-    });
     std::vector<int> A(nsc), B(nsc), SA(nsc), SB(nsc);
     std::vector<double> basis(9 * (size_t)nsc), mu(nsc, self_mu);
     for (int j = 0; j < nsc; ++j) {
@@ -1656,17 +1656,26 @@ void pdfc_sim::run_step() {
     pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);  // also clears the residual records
     detect();
     const bool need_nc = adaptive || features;
-    if (need_nc) HIP_CHECK(hipMemcpyAsync(host_nc, ncontacts, sizeof(int), hipMemcpyDeviceToHost, stream));
+    // the count comes back with the self-collision round trip's synchronisation at no extra cost
+    const bool copy_nc = need_nc || self_on;
+    if (copy_nc) HIP_CHECK(hipMemcpyAsync(host_nc, ncontacts, sizeof(int), hipMemcpyDeviceToHost, stream));
     nsc = 0;
     if (self_on) {
-        self_detect_and_pair(keep_full_candidates);
+        self_detect_and_pair(keep_full_candidates);  // synchronises the stream
         if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
+        stats.self_contacts_sum += nsc;
+        stats.self_levels_max = std::max<int64_t>(stats.self_levels_max, n_self_levels);
     }
     int nc = -1;
-    if (need_nc) {
-        HIP_CHECK(hipStreamSynchronize(stream));
-        nc = *host_nc;
-        last_nc = nc;
+    if (copy_nc) {
+        if (!self_on) HIP_CHECK(hipStreamSynchronize(stream));
+        if (need_nc) {
+            nc = *host_nc;
+            last_nc = nc;
+        }
+        stats.contacts_sum += *host_nc;
+    } else {
+        nc_known = false;
     }
     if (features && nc > 0) {
         pdfc::launch_warm_features(dm, qn, V[0], cvec, ncontacts, (int)std::min<size_t>(cap_contacts, nc), dc, dob, a,
@@ -1881,6 +1890,10 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
         sim->self_host_us = 0.;
         sim->pairing_host_us = 0.;
+        sim->order_host_us = 0.;
+        sim->upload_host_us = 0.;
+        sim->nc_known = true;
+        sim->stats.contacts_sum = sim->stats.self_contacts_sum = sim->stats.self_levels_max = 0;
         sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
         sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
         sim->stats.timed_steps = 0;
@@ -1907,6 +1920,9 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
         sim->stats.iterations_us = 1e3 * sim->sum_iterations_ms / ts;
         sim->stats.detection_us = sim->self_host_us / nsteps;
         sim->stats.pairing_us = sim->pairing_host_us / nsteps;
+        sim->stats.order_us = sim->order_host_us / nsteps;
+        sim->stats.upload_us = sim->upload_host_us / nsteps;
+        if (!sim->nc_known) sim->stats.contacts_sum = -1;
         sim->stats.steps = nsteps;
         sim->steps_done += nsteps;
         if (*sim->host_nonfinite) {

@@ -230,6 +230,12 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
     double pairing_us;               /* host time per step of the self-contact pairing, level ordering and
                                         upload, after the detection results arrived (part of detection_us,
                                         which also waits for the device) */
+    double order_us;                 /* of pairing_us: the level ordering (BFS) */
+    double upload_us;                /* of pairing_us: packing the self contacts and enqueuing their upload */
+    int64_t contacts_sum;            /* obstacle contacts summed over the call's steps (-1 when not known on
+                                        the host: steps without self-collision, adaptive mode or warm data) */
+    int64_t self_contacts_sum;       /* self contacts summed over the call's steps */
+    int64_t self_levels_max;         /* most self-contact levels of one step in the call */
 } pdfc_phase_stats;
 
 typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over an instrumented run */

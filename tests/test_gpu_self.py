@@ -138,3 +138,48 @@ def test_stacked_full_size_candidates_bit_exact(gpu):
     sa = [(s.vertex, s.partner, s.level) for s in sim.self_contacts()]
     sb = [(s.vertex, s.partner, s.level) for s in orc.self_contacts()]
     assert len(sb) > 1000 and sa == sb
+
+
+def test_double_obstacle_contacts_with_self_contacts(gpu):
+    """Two sheets 3 mm apart sliding on a plane over a sphere cap that rises to just below them:
+    central vertices touch both the sphere and the plane while the sheets are in self contact.
+    In that mode only the last active obstacle reaction of a vertex reaches the right-hand side
+    (m_contact_forces is assigned, SimulationFrictionEstimation.cpp:410-418), also for vertices
+    without a self contact. Per-step q, v and reactions within 1e-9 of the oracle."""
+    p0, t0 = scenes.square_grid(12, 0.3, 0.006)
+    p1, t1 = scenes.square_grid(12, 0.3, 0.009, x0=0.17, y0=0.013)  # overlaps part of the first
+    rest = np.concatenate([p0, p1])
+    tris = np.concatenate([t0, t1 + p0.size // 3]).astype(np.int32)
+    spec = scenes.SceneSpec("two_sheets", rest, tris, None,
+                            [scenes.ObstacleSpec("sphere", (0.0, 0.0, -0.296), radius=0.3, material=1),
+                             scenes.ObstacleSpec("plane", (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), material=1)],
+                            np.array([[0.3, 0.5], [0.5, 0.3]]), time_step=0.005, iterations=30,
+                            self_collision=True, self_tolerance=4e-3)
+    sim = gpu.Simulation(spec)
+    orc = po.Oracle(spec)
+    v0 = np.zeros((spec.n_vertices, 3))
+    v0[:169] = [0.3, 0.1, -0.02]    # both sheets slide, so friction decides the velocities
+    v0[169:] = [-0.2, 0.15, -0.05]
+    orc.set_state(rest.copy(), v0.ravel())
+    worst = dict(q=0.0, v=0.0, r=0.0)
+    doubles = selfs = 0
+    for s in range(10):
+        q0, v0 = orc.state()
+        sim.set_state(q0, v0)
+        orc.step(1)
+        sim.step(1)
+        q, v = sim.state()
+        qo, vo = orc.state()
+        worst["q"] = max(worst["q"], rel_err(q, qo))
+        worst["v"] = max(worst["v"], rel_err(v, vo))
+        co, cg = orc.contacts(), sim.contacts()
+        assert [(c.obstacle, c.vertex) for c in co] == [(c.obstacle, c.vertex) for c in cg]
+        worst["r"] = max(worst["r"], rel_err(np.array([list(c.r_local) for c in cg]),
+                                             np.array([list(c.r_local) for c in co])))
+        verts = [c.vertex for c in co]
+        in_self = {x for c in orc.self_contacts() for x in (c.vertex, c.partner)}
+        doubles += sum(1 for x in set(verts) if verts.count(x) == 2 and x not in in_self)
+        selfs += len(orc.self_contacts())
+    print("worst", worst, "double-contact vertices outside self contacts", doubles, "self contacts", selfs)
+    assert doubles > 10 and selfs > 10
+    assert max(worst.values()) < 1e-9, worst
