@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -284,11 +285,11 @@ __global__ void __launch_bounds__(64) k_mlp_grad_tile(MlpDev P, const float* __r
     }
 }
 
-// grad[p] = sum over tiles in order; loss = sum of tile losses.
-// loss_acc[0] += batch loss / loss_norm (mean over an epoch's batches), loss_acc[1] += 1.
+// grad[p] = sum over tiles in order; loss = (sum of tile losses) / loss_div.
+// With loss_acc: loss_acc[0] += sum * loss_norm (mean over an epoch's batches), loss_acc[1] += 1.
 __global__ void k_mlp_reduce(const float* __restrict__ part, int tiles, float* __restrict__ grad,
                              const float* __restrict__ loss_part, float* __restrict__ loss, float loss_norm,
-                             float* __restrict__ loss_acc) {
+                             float* __restrict__ loss_acc, float loss_div) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < kMlpParams) {
         float s = 0.f;
@@ -298,9 +299,11 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, int tiles, float* _
     if (p == 0) {
         float s = 0.f;
         for (int k = 0; k < tiles; ++k) s += loss_part[k];
-        *loss = s;
-        loss_acc[0] += s * loss_norm;
-        loss_acc[1] += 1.f;
+        *loss = s / loss_div;
+        if (loss_acc) {
+            loss_acc[0] += s * loss_norm;
+            loss_acc[1] += 1.f;
+        }
     }
 }
 
@@ -405,7 +408,35 @@ void batch_gradient(pdfc_mlp* M, const float* x, const float* t, int rows) {
     hipLaunchKernelGGL(pdfc::k_mlp_grad_tile, dim3(tiles), dim3(64), 0, M->stream, M->dev, x, t, rows, inv, M->part,
                        M->loss_part);
     hipLaunchKernelGGL(pdfc::k_mlp_reduce, dim3((pdfc::kMlpParams + 255) / 256), dim3(256), 0, M->stream, M->part,
-                       tiles, M->grad, M->loss_part, M->loss, inv, M->loss + 1);
+                       tiles, M->grad, M->loss_part, M->loss, inv, M->loss + 1, 1.f);
+}
+
+// ---- the allocation-free gradient (pdfc_mlp_gradient_noalloc) ----
+// Its per-tile partial gradients and tile losses live in this fixed array, one copy per GPU,
+// held until the process ends: kNoAllocTiles x kMlpParams partial gradients, then the tile
+// losses. In host code `scratch` only names the symbol; its device address on the current GPU
+// comes from hipGetSymbolAddress (noalloc_scratch below) and is what the kernels receive.
+constexpr int kNoAllocTiles = PDFC_MLP_NOALLOC_MAX_ROWS / 16;
+constexpr size_t kScratchFloats = (size_t)kNoAllocTiles * pdfc::kMlpParams + kNoAllocTiles;
+static_assert(PDFC_MLP_NOALLOC_MAX_ROWS % 16 == 0, "whole tiles");
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current GPU, looked up once per device.
+float* noalloc_scratch() {
+    int dev = 0;
+    chk(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        chk(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<float*>(ptr);
+    }
+    return g_scratch_ptr[dev];
 }
 template <typename F>
 int guard(F&& f) {
@@ -464,6 +495,7 @@ int pdfc_mlp_create(pdfc_mlp** out, uint64_t seed) {
         chk(hipMemset(M->v, 0, pdfc::kMlpParams * sizeof(float)), "memset");
         refresh(M);
         chk(hipStreamSynchronize(M->stream), "sync");
+        (void)noalloc_scratch();  // set-up of the allocation-free gradient on this device
         *out = M;
     });
 }
@@ -527,6 +559,26 @@ int pdfc_mlp_gradient(pdfc_mlp* M, const float* x, const float* t, size_t rows, 
         chk(hipMemcpyAsync(&l, M->loss, sizeof(float), hipMemcpyDeviceToHost, M->stream), "copy");
         chk(hipStreamSynchronize(M->stream), "sync");
         if (loss) *loss = l / (float)(rows * pdfc::kMlpOut);
+    });
+}
+
+int pdfc_mlp_gradient_noalloc(pdfc_mlp* M, const float* x, const float* t, size_t rows, float* grad, float* loss) {
+    return guard([&] {
+        if (!M || !x || !t || !grad || !loss || rows == 0) throw std::invalid_argument("bad argument");
+        if (rows > (size_t)PDFC_MLP_NOALLOC_MAX_ROWS)
+            throw std::invalid_argument("pdfc_mlp_gradient_noalloc: " + std::to_string(rows) +
+                                        " rows exceed the maximum of " + std::to_string(PDFC_MLP_NOALLOC_MAX_ROWS));
+        float* part = noalloc_scratch();
+        float* loss_part = part + (size_t)kNoAllocTiles * pdfc::kMlpParams;
+        const int n = (int)rows, tiles = (n + 15) / 16;
+        const float inv = 1.f / (float)(n * pdfc::kMlpOut);
+        // every tile writes all its partials and its loss, so nothing is cleared
+        hipLaunchKernelGGL(pdfc::k_mlp_grad_tile, dim3(tiles), dim3(64), 0, M->stream, M->dev, x, t, n, inv, part,
+                           loss_part);
+        hipLaunchKernelGGL(pdfc::k_mlp_reduce, dim3((pdfc::kMlpParams + 255) / 256), dim3(256), 0, M->stream, part,
+                           tiles, grad, loss_part, loss, inv, (float*)nullptr, (float)(n * pdfc::kMlpOut));
+        chk(hipGetLastError(), "launch");
+        chk(hipStreamSynchronize(M->stream), "sync");
     });
 }
 

@@ -342,6 +342,18 @@ int pdfc_mlp_get_params(pdfc_mlp* mlp, float* params, float* norm);
 int pdfc_mlp_forward(pdfc_mlp* mlp, const float* x, size_t rows, float* y);
 /* Gradient of L = mean over rows and outputs of (MLP(x) - t / scale)^2 w.r.t. the parameters. */
 int pdfc_mlp_gradient(pdfc_mlp* mlp, const float* x, const float* t, size_t rows, float* grad, float* loss);
+/* Allocation-free gradient. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training or serving loops, harnesses
+ * that forbid hipMalloc). The same gradient and mean loss as pdfc_mlp_gradient, by the same
+ * kernels in the same summation order, with every operand in DEVICE memory: x [rows x 12] and
+ * t [rows x 3] in, grad [PDFC_MLP_PARAMS] and loss [1] out. The per-tile partial gradients live in
+ * a fixed array of the library, one per GPU (PDFC_MLP_NOALLOC_MAX_ROWS / 16 tiles, 5.35 MB), held
+ * until the process ends; more rows are rejected with PDFC_ERR_INVALID. Calls no hipMalloc,
+ * hipFree or other allocating routine. Runs on the network's stream and returns when it is done.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (several threads or
+ * networks at once) race. */
+#define PDFC_MLP_NOALLOC_MAX_ROWS 2048
+int pdfc_mlp_gradient_noalloc(pdfc_mlp* mlp, const float* x, const float* t, size_t rows, float* grad, float* loss);
 /* Adam (beta 0.9 / 0.999) on shuffled mini-batches of batch rows (multiple of 16) for epochs epochs;
  * loss_per_epoch (optional, epochs floats) gets each epoch's mean mini-batch loss. */
 int pdfc_mlp_train(pdfc_mlp* mlp, const float* x, const float* t, size_t rows, int epochs, int batch, float lr,
