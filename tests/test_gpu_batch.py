@@ -139,3 +139,39 @@ def test_batch_of_boxes_rotating_spheres_and_attachments(gpu):
                 assert rel_err(np.array([list(c.r_local) for c in mine]),
                                np.array([list(c.r_local) for c in ref])) < REL, (s, k)
     assert n_contact > 60 and seen_speed
+
+
+def test_batch_64_full_size_parity(gpu):
+    """The batched configuration at full size: 64 cloths of 32^2..128^2 quads (517,876 vertices) in
+    one batch, advanced 30 steps (every cloth on its sphere), then 3 steps from identical states,
+    each scene against the oracle run on that scene alone (SimulationSpeed.cpp:171-187 per
+    instance): q, v and reactions within 1e-9, identical contact sets."""
+    specs = scenes.batched_instances(64, seed=0)
+    sim = gpu.BatchSimulation(specs)
+    print("64 cloths: 30 batched steps", flush=True)
+    sim.step(30)
+    orcs = [po.Oracle(sp) for sp in specs]
+    worst = dict(q=0.0, v=0.0, r=0.0)
+    in_contact = 0
+    for s in range(3):
+        q, v = sim.state()
+        for o, (a, b) in zip(orcs, sim.ranges):
+            o.set_state(q[3 * a:3 * b].copy(), v[3 * a:3 * b].copy())
+            o.step(1)
+        sim.step(1)
+        q, v = sim.state()
+        cs = sim.contacts()
+        for k, (o, (a, b)) in enumerate(zip(orcs, sim.ranges)):
+            qo, vo = o.state()
+            worst["q"] = max(worst["q"], rel_err(q[3 * a:3 * b], qo))
+            worst["v"] = max(worst["v"], rel_err(v[3 * a:3 * b], vo))
+            mine = [(c.vertex - a, list(c.r_local)) for c in cs if c.obstacle == k]
+            ref = [(c.vertex, list(c.r_local)) for c in o.contacts()]
+            assert [m[0] for m in mine] == [r[0] for r in ref], (s, k)
+            if ref:
+                in_contact += 1
+                worst["r"] = max(worst["r"], rel_err(np.array([m[1] for m in mine]),
+                                                     np.array([r[1] for r in ref])))
+    print("64 cloths: worst relative errors", worst, "scene-steps in contact", in_contact)
+    assert in_contact == 3 * 64
+    assert max(worst.values()) < REL, worst

@@ -226,3 +226,78 @@ def test_symmetric_tiles_match_split_matvec(gpu, monkeypatch):
     err = rel_err(xs[0], xs[8])
     print("symmetric tiles vs split matvec, max rel difference:", err)
     assert err < 1e-12
+
+
+def test_air_damping_identical_state_parity(gpu):
+    """Air damping on (P5): the coefficient is zeroed at contact vertices (SimulationBase.cpp:
+    347-355). 20x20 drape with damping 2e-3 (dt alpha / m = 0.08 at an interior vertex; the
+    reference's explicit damping term diverges once that exceeds 1), every step from the
+    oracle's state."""
+    spec = scenes.sphere_drape_small()
+    spec.air_damping = 2e-3
+    sim = gpu.Simulation(spec)
+    orc = po.Oracle(spec)
+    worst = dict(q=0.0, v=0.0, r=0.0)
+    n_contact_steps = 0
+    for s in range(120):
+        q0, v0 = orc.state()
+        sim.set_state(q0, v0)
+        orc.step(1)
+        sim.step(1)
+        q, v = sim.state()
+        qo, vo = orc.state()
+        worst["q"] = max(worst["q"], rel_err(q, qo))
+        worst["v"] = max(worst["v"], rel_err(v, vo))
+        ko, ro, _, _ = contact_arrays(orc.contacts())
+        kg, rg, _, _ = contact_arrays(sim.contacts())
+        assert np.array_equal(ko, kg), "contact sets differ at step %d" % s
+        if len(ko):
+            n_contact_steps += 1
+            worst["r"] = max(worst["r"], rel_err(rg, ro))
+    print("air damping 2e-3: worst relative errors", worst, "steps with contact", n_contact_steps)
+    assert n_contact_steps > 50
+    assert max(worst.values()) < REL, worst
+    # the damping acts: the same steps without it differ
+    plain = scenes.sphere_drape_small()
+    o2 = po.Oracle(plain)
+    o2.step(120)
+    assert rel_err(o2.state()[1], orc.state()[1]) > 1e-3
+
+
+def test_drape_40k_full_step_parity_in_self_contact_phase(gpu):
+    """The 40k drape at full size, its whole step: advanced on the GPU to step 118, where the cloth
+    on the sphere folds onto itself (thousands of obstacle contacts, hundreds of self contacts),
+    then 3 steps from identical states against the oracle: q, v, obstacle and self reactions
+    within 1e-9, identical contact sets, node-node pairs and levels (SimulationBase.cpp:103-173)."""
+    spec = scenes.sphere_drape_40k()
+    sim = gpu.Simulation(spec)
+    print("40k drape: 118 GPU steps", flush=True)
+    sim.step(118)
+    orc = po.Oracle(spec)
+    worst = dict(q=0.0, v=0.0, r=0.0, rs=0.0)
+    for s in range(3):
+        q0, v0 = sim.state()
+        orc.set_state(q0, v0)
+        sim.step(1)
+        orc.step(1)
+        q, v = sim.state()
+        qo, vo = orc.state()
+        worst["q"] = max(worst["q"], rel_err(q, qo))
+        worst["v"] = max(worst["v"], rel_err(v, vo))
+        ko, ro, no, ao = contact_arrays(orc.contacts())
+        kg, rg, ng, ag = contact_arrays(sim.contacts())
+        assert np.array_equal(ko, kg), "contact sets differ at step %d" % s
+        assert np.array_equal(ao, ag), "take-off flags differ at step %d" % s
+        worst["r"] = max(worst["r"], rel_err(rg, ro))
+        so, sg = orc.self_contacts(), sim.self_contacts()
+        assert [(c.vertex, c.partner, c.level) for c in so] == [(c.vertex, c.partner, c.level) for c in sg]
+        worst["rs"] = max(worst["rs"], rel_err(np.array([list(c.r_local) for c in sg]),
+                                               np.array([list(c.r_local) for c in so])))
+        mo, _ = orc.iteration_stats()
+        mg, _ = sim.iteration_stats()
+        assert np.allclose(mg, mo, rtol=1e-6, atol=1e-12), (s, mg, mo)
+        print("step", 119 + s, "contacts", len(ko), "self contacts", len(so),
+              "levels", 1 + max(c.level for c in so), flush=True)
+        assert len(ko) > 1000 and len(so) > 100
+    print("worst relative errors", worst)
+    assert max(worst.values()) < REL, worst

@@ -306,3 +306,56 @@ def test_self_pairing_rules():
         seen.add(key)
         assert abs(np.linalg.norm(s.normal) - 1) < 1e-14
     assert any(tuple(t) == tuple(s.face) for s in sc[:1] for t in tris)
+
+
+def _triangle(z, obstacles, mu=0.3, damping=0.0, iterations=10):
+    rest = np.array([[0.0, 0.0, z], [1.0, 0.0, z], [0.0, 1.0, z]]).ravel()
+    return scenes.SceneSpec("triangle", rest, np.array([0, 1, 2], dtype=np.int32), None, obstacles,
+                            np.array([[mu]]), area_density=0.1, time_step=0.01, iterations=iterations,
+                            air_damping=damping)
+
+
+def test_air_damping_free_fall_closed_form():
+    """Damping D = alpha (SimulationSpeed.cpp:238-278, SimulationBase.cpp:347-355) on a rigidly
+    translating triangle: A^T p - A^T A q^k = 0 and A^T A v = 0, so every iterate solves
+    m v^{k+1} = m t - dt alpha v^k with t = v_n + dt g: v^K = t * sum_{j=0..K} (-c)^j, c = dt alpha / m."""
+    alpha, K = 0.05, 10
+    spec = _triangle(5.0, [scenes.ObstacleSpec("sphere", (0.0, 0.0, -10.0), radius=0.1)], damping=alpha,
+                     iterations=K)
+    o = po.Oracle(spec)
+    v0 = np.tile([0.2, -0.1, 0.3], 3)
+    o.set_state(spec.rest.copy(), v0)
+    o.step(1)
+    q1, v1 = o.state()
+    m = o.masses()
+    assert np.allclose(m, 0.1 * 0.5 / 3, rtol=1e-15)
+    c = spec.time_step * alpha / m[0]
+    t = np.array([0.2, -0.1, 0.3]) + spec.time_step * np.array(spec.gravity)
+    want = t * sum((-c) ** j for j in range(K + 1))
+    assert np.abs(v1.reshape(3, 3) - want).max() < 1e-13
+    assert np.abs(q1 - (spec.rest + spec.time_step * v1)).max() < 1e-15
+    # without damping the same step is plain free fall
+    o2 = po.Oracle(_triangle(5.0, [scenes.ObstacleSpec("sphere", (0.0, 0.0, -10.0), radius=0.1)]))
+    o2.set_state(spec.rest.copy(), v0)
+    o2.step(1)
+    assert np.abs(o2.state()[1].reshape(3, 3) - t).max() < 1e-14
+
+
+def test_air_damping_is_off_at_contact_vertices():
+    """The damping coefficient is zeroed at obstacle-contact vertices (SimulationBase.cpp:347-355):
+    a triangle sliding on a frictionless plane keeps its tangential velocity exactly, while the
+    same triangle in the air loses it to damping."""
+    alpha = 0.5
+    plane = [scenes.ObstacleSpec("plane", (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))]
+    v0 = np.tile([0.4, -0.3, 0.0], 3)
+    o = po.Oracle(_triangle(0.005, plane, mu=0.0, damping=alpha))
+    o.set_state(o.spec.rest.copy(), v0)
+    o.step(1)
+    assert len(o.contacts()) == 3
+    v1 = o.state()[1].reshape(3, 3)
+    assert np.abs(v1[:, :2] - [0.4, -0.3]).max() < 1e-14
+    air = po.Oracle(_triangle(0.5, plane, mu=0.0, damping=alpha))
+    air.set_state(air.spec.rest.copy(), v0)
+    air.step(1)
+    assert len(air.contacts()) == 0
+    assert np.abs(air.state()[1].reshape(3, 3)[:, 0]).max() < 0.4 * (1 - 0.5 * 0.01 * alpha / (0.1 * 0.5 / 3))
