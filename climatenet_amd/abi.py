@@ -200,6 +200,8 @@ class KernelTimes(C.Structure):
         ("forward_launches", C.c_int64),
         ("backward_launches", C.c_int64),
         ("top_launches", C.c_int64),
+        ("self_us", C.c_double),
+        ("self_iterations", C.c_int64),
     ]
 
 

@@ -14,8 +14,13 @@
 //   computeAndWriteStats             template/physics/io.tpp:23-63
 //
 // Construction errors throw std::invalid_argument (as the reference's JSON
-// builders do); step() is noexcept and aborts with the library's message on a
-// device failure, the reference's fail-stop behaviour.
+// builders do); the step's hooks abort with the library's message on a device
+// failure, the reference's fail-stop behaviour (its step methods are noexcept).
+//
+// One deliberate departure: the JSON document type is this build's own parser's
+// (json::Value, json.hpp) where the reference takes rapidjson::Document; rapidjson is a
+// third-party header library vendored inside the reference (extern/rapidjson) and is not
+// part of this build. Key names, defaults and error messages are the reference's.
 #pragma once
 
 #include <array>
@@ -127,17 +132,27 @@ struct SimulationParameters {
     double self_collision_tolerance = 0.;
     std::array<double, 3> gravity{{0., 0., 0.}};
     double air_damping = 0.;
-    // Build-specific options (not in the reference's JSON): precision and warm start.
+    // Build-specific options (not in the reference's JSON): precision, warm start, and device
+    // timing of the step's phases for exportStats (events around every launch; measurement only).
     int precision = PDFC_FP64;
     int refinement_sweeps = 2;
     int warm_start = PDFC_WARM_PLAIN;
+    bool phase_timing = false;
 };
 
+// SimulationBase (include/physics/SimulationBase.hpp:39-290). step() is not virtual: it runs
+// the protected hooks in the reference's order (src/physics/SimulationBase.cpp:103-173):
+// initializeStep, collision handling (updateCollision), then for each local-global iteration
+// buildIteration, solve and updateIterationData, then updateScene. The reference detects the
+// collisions on the host between initializeStep and updateCollision (baseUpdateColisions,
+// SimulationBase.cpp:333-413); here detection is part of the device prologue.
 class SimulationBase {
 public:
     using IterationDuration = std::chrono::high_resolution_clock::duration;
+    SimulationBase(PhysicScene& scene, double time_step, std::size_t iteration_number, double air_damping,
+                   bool handle_self_collision, double self_collision_tolerance);
     virtual ~SimulationBase() = default;
-    virtual void step() = 0;
+    void step();
     virtual void exportStats(std::ofstream& of) const noexcept = 0;
     const std::vector<double>& getLastStepErrors() const noexcept { return m_current_step_error; }
     const std::vector<IterationDuration>& getLastStepDurations() const noexcept { return m_current_step_duration; }
@@ -146,19 +161,35 @@ public:
     }
 
 protected:
+    virtual void initializeStep() = 0;
+    virtual void buildIteration() = 0;
+    virtual void updateCollision() = 0;
+    virtual double solve() = 0;
+    virtual void updateScene() = 0;
+    virtual void updateIterationData() noexcept {}
+
+    PhysicScene& m_scene;
+    const double m_time_step;
+    double m_current_time = 0.;
+    const std::size_t m_iteration_number;
+    const double m_damping_coefficient;
+    const bool m_handle_self_collision;
+    const double m_self_collision_tolerance;
     std::vector<double> m_current_step_error;
     std::vector<IterationDuration> m_current_step_duration;
     std::vector<std::size_t> m_current_step_number_of_collisions;
 };
 
-// SimulationFrictionEstimation on the GPU: the whole step runs on the device; the
-// scene's q and v are mirrored back after every step (callbacks and OBJ output
-// read them right after step(), src/main.cpp:91-105).
+// SimulationFrictionEstimation on the GPU. Its hooks drive the device step through the
+// phase entry points: initializeStep -> pdfc_step_begin (per-step constant, obstacle and
+// self-collision detection, pairing, levels); buildIteration -> nothing (the local step is
+// fused into the device iteration); solve -> pdfc_step_iterate(1); updateScene ->
+// pdfc_step_end, and the scene's q and v are mirrored back (callbacks and OBJ output read
+// them right after step(), src/main.cpp:91-105).
 class SimulationFrictionEstimationHIP : public SimulationBase {
 public:
     SimulationFrictionEstimationHIP(PhysicScene& scene, const SimulationParameters& params);
     ~SimulationFrictionEstimationHIP() override;
-    void step() noexcept override;
     void exportStats(std::ofstream& of) const noexcept override;
     pdfc_sim* handle() const { return m_sim; }
     // Skip the per-step state copy when no caller reads the scene between steps.
@@ -171,12 +202,18 @@ public:
     void loadCheckpoint(const std::string& path);
     std::size_t getStepsDone() const;
 
+protected:
+    void initializeStep() override;
+    void buildIteration() override {}
+    void updateCollision() override {}
+    double solve() override;
+    void updateScene() override;
+
 private:
-    PhysicScene& m_scene;
     pdfc_sim* m_sim = nullptr;
-    std::size_t m_iterations;
     bool m_mirror = true;
-    bool m_self = false;
+    bool m_phase_timing = false;
+    std::size_t m_levels_swept = 0;
     // the vectors of SimulationSpeed/SimulationBase/SimulationFrictionEstimation that
     // exportStats writes; the ones this build does not measure stay empty ("(No data)")
     std::vector<double> m_lhs_times, m_cholesky_times, m_iteration_times, m_collision_detection_times;

@@ -1,9 +1,11 @@
-// pdfc_main <scene.json> [--checkpoint-every K] [--resume FILE]: the reference's driver
+// pdfc_main <scene.json> [--checkpoint-every K] [--resume FILE] [--phase-timing]: the reference's driver
 // (src/main.cpp:40-124) on the GPU simulation: output directory dataN/ with a copy of
 // the scene, one OBJ frame every "Steps per frame" steps, log callbacks after each step,
 // stats.txt. The options have no counterpart in the reference: --checkpoint-every writes
 // dataN/checkpoint.bin every K steps and after the last one; --resume loads a checkpoint
-// of the same scene and continues from its step (frames keep their numbers).
+// of the same scene and continues from its step (frames keep their numbers); --phase-timing
+// times the device phases of every step (events around each launch, a synchronisation per
+// step) so stats.txt also gets the solver, RHS and self-collision sections.
 #include <sys/stat.h>
 
 #include <cstdlib>
@@ -36,12 +38,15 @@ int main(int argc, char* argv[]) {
     const std::string scene_path(argv[1]);
     std::size_t checkpoint_every = 0;
     std::string resume;
+    bool phase_timing = false;
     for (int a = 2; a < argc; ++a) {
         const std::string opt = argv[a];
         if (opt == "--checkpoint-every" && a + 1 < argc) {
             checkpoint_every = (std::size_t)std::strtoull(argv[++a], nullptr, 10);
         } else if (opt == "--resume" && a + 1 < argc) {
             resume = argv[++a];
+        } else if (opt == "--phase-timing") {
+            phase_timing = true;
         } else {
             std::cerr << "Unknown option " << opt << std::endl;
             return EXIT_FAILURE;
@@ -64,7 +69,8 @@ int main(int argc, char* argv[]) {
         json::Value doc = getJSONDocumentFromFilename(scene_path);
         PhysicScene scene = getSceneFromJSONDocument(doc);
         auto mesh = scene.getMesh(0);
-        const SimulationParameters params = getSimulationParametersFromJSONDocument(doc);
+        SimulationParameters params = getSimulationParametersFromJSONDocument(doc);
+        params.phase_timing = phase_timing;
         auto simulation = getSimulation(scene, params);
         StepCallBacks log_callback = getLogCallBacksFromJSONDocument(doc);
         const size_t max_steps = params.max_output_frame * params.sim_steps_per_frame;

@@ -1,5 +1,6 @@
 // getSimulation and the GPU-backed SimulationFrictionEstimation
 // (src/simulation_building.cpp:80-98, src/physics/SimulationFrictionEstimation.cpp).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -16,8 +17,38 @@ void fail_stop(const char* where) {
 }
 }  // namespace
 
+SimulationBase::SimulationBase(PhysicScene& scene, double time_step, std::size_t iteration_number,
+                               double air_damping, bool handle_self_collision, double self_collision_tolerance)
+    : m_scene(scene),
+      m_time_step(time_step),
+      m_iteration_number(iteration_number),
+      m_damping_coefficient(air_damping),
+      m_handle_self_collision(handle_self_collision),
+      m_self_collision_tolerance(self_collision_tolerance),
+      m_current_step_error(iteration_number, 0.),
+      m_current_step_duration(iteration_number, IterationDuration::zero()),
+      m_current_step_number_of_collisions(iteration_number, 0) {}
+
+// SimulationBase::step (src/physics/SimulationBase.cpp:103-173), hook by hook.
+void SimulationBase::step() {
+    initializeStep();
+    updateCollision();
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    m_current_step_error.assign(m_iteration_number, 0.);
+    m_current_step_duration.assign(m_iteration_number, IterationDuration::zero());
+    for (std::size_t i = 0; i < m_iteration_number; ++i) {
+        buildIteration();
+        m_current_step_error[i] = solve();
+        m_current_step_duration[i] = std::chrono::high_resolution_clock::now() - t0;
+        updateIterationData();
+    }
+    updateScene();
+    m_current_time += m_time_step;
+}
+
 SimulationFrictionEstimationHIP::SimulationFrictionEstimationHIP(PhysicScene& scene, const SimulationParameters& p)
-    : m_scene(scene), m_iterations(p.local_global_iterations) {
+    : SimulationBase(scene, p.time_step, p.local_global_iterations, p.air_damping, p.self_collision,
+                     p.self_collision_tolerance) {
     auto mesh = scene.getMesh(0);
     const auto& rest = mesh->getRestPositions();
     const auto& tris = mesh->getTriangles();
@@ -76,33 +107,64 @@ SimulationFrictionEstimationHIP::SimulationFrictionEstimationHIP(PhysicScene& sc
     if (pdfc_get_info(m_sim, &info) != PDFC_OK) throw std::runtime_error(pdfc_last_error());
     m_lhs_times.push_back(1e6 * info.lhs_seconds);
     m_cholesky_times.push_back(1e6 * info.factor_seconds);
-    m_self = p.self_collision;
-    // SimulationSpeed::solve returns 0 (SimulationSpeed.cpp:172-187): the per-iteration
-    // "errors" of the reference are zeros
-    m_current_step_error.assign(m_iterations, 0.);
-    m_current_step_duration.assign(m_iterations, IterationDuration::zero());
-    m_current_step_number_of_collisions.assign(m_iterations, 0);
+    m_levels_swept = (std::size_t)info.n_levels_swept;
+    m_phase_timing = p.phase_timing;
+    if (m_phase_timing && pdfc_set_kernel_timing(m_sim, 1) != PDFC_OK) throw std::runtime_error(pdfc_last_error());
 }
 
 SimulationFrictionEstimationHIP::~SimulationFrictionEstimationHIP() { pdfc_destroy(m_sim); }
 
-void SimulationFrictionEstimationHIP::step() noexcept {
-    if (pdfc_step(m_sim, 1) != PDFC_OK) fail_stop("step");
+// initializeStep + the device's detection (SimulationSpeed.cpp:62-76, SimulationBase.cpp:333-413,
+// SimulationFrictionEstimation.cpp:107-159).
+void SimulationFrictionEstimationHIP::initializeStep() {
+    if (pdfc_step_begin(m_sim) != PDFC_OK) fail_stop("step_begin");
+}
+
+// One local-global iteration on the device (buildIteration + solve); SimulationSpeed::solve
+// returns 0 (SimulationSpeed.cpp:172-187), so the per-iteration "errors" are zeros.
+double SimulationFrictionEstimationHIP::solve() {
+    if (pdfc_step_iterate(m_sim, 1) != PDFC_OK) fail_stop("step_iterate");
+    return 0.;
+}
+
+// updateScene (SimulationFrictionEstimation.cpp:64-87) and the stats bookkeeping of
+// SimulationBase::step / baseUpdateColisions (SimulationBase.cpp:103-173, 333-413), in device
+// time: the step, its prologue (detection), and per iteration its mean. With phase timing the
+// local kernel (base right-hand side and obstacle friction, fused), the self-contact levels and
+// the substitution are timed apart as well.
+void SimulationFrictionEstimationHIP::updateScene() {
+    if (pdfc_step_end(m_sim) != PDFC_OK) fail_stop("step_end");
     pdfc_phase_stats ps{};
     pdfc_info info{};
     if (pdfc_get_phase_stats(m_sim, &ps) != PDFC_OK || pdfc_get_info(m_sim, &info) != PDFC_OK) fail_stop("stats");
-    // SimulationBase::step / baseUpdateColisions bookkeeping (SimulationBase.cpp:103-173,
-    // 333-413): device times per step; each iteration gets the step's mean iteration time
     m_step_times.push_back(ps.step_us);
-    const std::size_t K = ps.steps > 0 ? (std::size_t)(ps.iterations / ps.steps) : m_iterations;
+    const std::size_t K = ps.iterations > 0 ? (std::size_t)ps.iterations : m_iteration_number;
     if (ps.timed_steps > 0) {
         m_collision_detection_times.push_back(ps.collision_us);
         for (std::size_t k = 0; k < K; ++k) m_iteration_times.push_back(ps.iterations_us / (double)std::max<std::size_t>(1, K));
     }
     if (info.n_contacts > 0) m_collision_numbers.push_back((double)info.n_contacts);
-    if (m_self) {
+    if (m_handle_self_collision) {
         m_self_collision_detection_times.push_back(ps.detection_us);
-        if (info.n_self_contacts > 0) m_self_collision_numbers.push_back((double)info.n_self_contacts);
+        if (info.n_self_contacts > 0) {
+            m_self_collision_numbers.push_back((double)info.n_self_contacts);
+            m_self_collision_ordering_times.push_back(ps.order_us);
+        }
+    }
+    if (m_phase_timing) {
+        pdfc_kernel_times kt{};
+        if (pdfc_get_kernel_times(m_sim, &kt) != PDFC_OK) fail_stop("kernel_times");
+        const double sweeps = m_levels_swept > 0 ? (double)m_levels_swept : 1.;
+        const double solver = (kt.forward_sweep_us * (double)kt.forward_launches / sweeps +
+                               kt.backward_sweep_us * (double)kt.backward_launches / sweeps +
+                               kt.top_us * (double)kt.top_launches) / (double)K;
+        for (std::size_t k = 0; k < K; ++k) {
+            m_global_times.push_back(solver);
+            m_rhs_base_times.push_back(kt.local_us);
+            m_rhs_times.push_back(kt.local_us + (kt.self_iterations > 0 ? kt.self_us : 0.));
+            if (kt.self_iterations > 0) m_self_friction_times.push_back(kt.self_us);
+        }
+        if (pdfc_set_kernel_timing(m_sim, 1) != PDFC_OK) fail_stop("kernel_timing");  // next step's sums
     }
     m_current_step_error.assign(K, 0.);
     m_current_step_number_of_collisions.assign(K, (std::size_t)info.n_contacts);
@@ -155,14 +217,17 @@ std::size_t SimulationFrictionEstimationHIP::getStepsDone() const {
 // (src/physics/SimulationFrictionEstimation.cpp:755-771, with SimulationBase::exportStats,
 // src/physics/SimulationBase.cpp:473-494, in the middle). LHS: host build of A^T A / K;
 // Cholesky: host factorisation; iteration and collision-detection times: device time per
-// step (HIP events); self-collision detection: host round trip and pairing. The solver,
-// RHS and friction phases run fused on the device and are not timed apart: "(No data)".
+// step (HIP events); self-collision detection: host round trip and pairing; ordering: host
+// level ordering. With phase_timing, solver (the substitution), full RHS (local kernel plus
+// self-contact levels), RHS base (the local kernel, which fuses the obstacle friction) and
+// self collision times come from device events around those launches; the obstacle friction
+// alone is not timed apart from the local kernel, so "RHS friction overhead" is "(No data)".
 void SimulationFrictionEstimationHIP::exportStats(std::ofstream& of) const noexcept {
     computeAndWriteStats("LHS computation time (µs)", of, m_lhs_times);
     computeAndWriteStats("Cholesky computation time (µs)", of, m_cholesky_times);
     computeAndWriteStats("Iteration time (µs)", of, m_iteration_times);
     computeAndWriteStats("Collision detection time (µs)", of, m_collision_detection_times);
-    if (m_self) {
+    if (m_handle_self_collision) {
         computeAndWriteStats("Number of collisions", of, m_collision_numbers);
         computeAndWriteStats("Self collision detection time (µs)", of, m_self_collision_detection_times);
     }
@@ -172,7 +237,7 @@ void SimulationFrictionEstimationHIP::exportStats(std::ofstream& of) const noexc
     computeAndWriteStats("Step time (µs)", of, m_step_times);
     computeAndWriteStats("RHS base computation time (µs)", of, m_rhs_base_times);
     computeAndWriteStats("RHS friction overhead (µs)", of, m_friction_times);
-    if (m_self) {
+    if (m_handle_self_collision) {
         computeAndWriteStats("Self collision time (µs)", of, m_self_friction_times);
         computeAndWriteStats("Self collision ordering time (µs)", of, m_self_collision_ordering_times);
     }

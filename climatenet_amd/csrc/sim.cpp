@@ -177,10 +177,11 @@ struct pdfc_sim {
     bool top_fold = true;           // the top block's finish inside the first backward level
     bool top_fold_allowed = true;   // PDFC_TOP_FOLD=0: never folded
     std::vector<hipEvent_t> kev;        // event pool, two per timed launch
-    std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level, 3 top block
+    std::vector<int> kfamily;           // 0 local, 1 forward level, 2 backward level, 3 top block,
+                                        // 4 self-contact friction (all levels of one iteration)
     size_t kused = 0;
-    double ksum[4] = {0., 0., 0., 0.};
-    int64_t kcount[4] = {0, 0, 0, 0};
+    double ksum[5] = {0., 0., 0., 0., 0.};
+    int64_t kcount[5] = {0, 0, 0, 0, 0};
     int64_t ksolves = 0;
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
@@ -346,6 +347,17 @@ struct pdfc_sim {
     std::vector<unsigned char> checkpoint_bytes();
     void restore_checkpoint(const unsigned char* p, size_t size);
     void solve_inplace(double* X);
+    // one step's state between its prologue, iterations and epilogue
+    struct StepPhase {
+        pdfc::StepArgs a{};
+        pdfc::LocalSelf ls{};
+        int mode = PDFC_WARM_PLAIN, hold = 1, nc = -1, iterations = 0;
+    };
+    void step_prologue(StepPhase& ph);
+    void step_fixed_iteration(StepPhase& ph);
+    void step_epilogue(StepPhase& ph, int K, int base, int to_tol, bool fell_back, bool check);
+    StepPhase phase;        // the step opened by pdfc_step_begin
+    bool phase_open = false;
     void run_step();
     void detect();
     void launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls);
@@ -1617,11 +1629,13 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     ls.SCFc = ds.SCFc;
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
-        for (int l = 0; l < n_self_levels; ++l) {
-            pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
-            pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
-        }
-        pdfc::launch_self_finalize(ds, vnext, n, stream);
+        timed(4, [&] {
+            for (int l = 0; l < n_self_levels; ++l) {
+                pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
+                pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+            }
+            pdfc::launch_self_finalize(ds, vnext, n, stream);
+        });
         std::swap(ds.CFc, ds.CFn);
         std::swap(ds.SCFc, ds.SCFn);
         std::swap(ds.rem, ds.addv);
@@ -1629,9 +1643,13 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     solve_inplace(vnext);
 }
 
-void pdfc_sim::run_step() {
+// Prologue of a step (initializeStep + baseUpdateColisions + updateCollision,
+// SimulationBase.cpp:103-116, 333-413): the per-step constant, obstacle and self-collision
+// detection, pairing and levels, warm-start features. Fills ph for the iterations.
+void pdfc_sim::step_prologue(StepPhase& ph) {
     const bool adaptive = prm.residual_tolerance > 0.;
-    pdfc::StepArgs a{};
+    pdfc::StepArgs& a = ph.a;
+    a = pdfc::StepArgs{};
     a.dt = prm.time_step;
     for (int k = 0; k < 3; ++k) a.g[k] = prm.gravity[k];
     a.damping = std::max(prm.air_damping, 0.);
@@ -1690,18 +1708,73 @@ void pdfc_sim::run_step() {
         a.warm_y = warm_out;
         a.warm_stride = 3;
     }
-    pdfc::LocalSelf ls{};
-    ls.selfmode = nsc > 0 ? 1 : 0;
-    ls.vslot = vslot;
+    ph.ls = pdfc::LocalSelf{};
+    ph.ls.selfmode = nsc > 0 ? 1 : 0;
+    ph.ls.vslot = vslot;
+    ph.mode = mode;
+    ph.hold = hold;
+    ph.nc = nc;
+    ph.iterations = 0;
+    step_event(1);
+}
 
+// Epilogue (updateScene, SimulationFrictionEstimation.cpp:64-87): q, v and the residual of
+// the last of the K iterations, clocks, per-step statistics.
+void pdfc_sim::step_epilogue(StepPhase& ph, int K, int base, int to_tol, bool fell_back, bool check) {
+    pdfc::StepArgs& a = ph.a;
+    const int total = std::max(ph.nc, 0) + nsc;
+    last_records = K;
+    pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, dob, a, K - 1, stream);
+    if (nsc > 0) {
+        pdfc::launch_self_ac_final(ds, V[K & 1], n, a, K - 1, stream);
+        pdfc::launch_self_release(ds, vslot, stream);
+    }
+    if (check && to_tol < 0) {
+        HIP_CHECK(hipMemcpyAsync(host_ac + (K - 1), ac_sum + (K - 1), sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (host_ac[K - 1] / total < prm.residual_tolerance) to_tol = base + K;
+    }
+    step_event(2);
+    HIP_CHECK(hipGetLastError());
+    mesh_advance(prm.time_step);
+    stats.iterations += base + K;
+    if (total > 0) {
+        stats.contact_steps += 1;
+        if (prm.residual_tolerance > 0.) stats.iterations_to_tolerance += to_tol > 0 ? to_tol : prm.max_iterations;
+        if (to_tol > 0) stats.converged_steps += 1;
+    }
+    if (fell_back) stats.fallbacks += 1;
+}
+
+// One iteration of a fixed-count step: the warm reaction for the first `hold` iterations.
+void pdfc_sim::step_fixed_iteration(StepPhase& ph) {
+    const int it = ph.iterations++;
+    const int hold_end = ph.mode != PDFC_WARM_PLAIN ? ph.hold : 0;
+    ph.a.warm_mode = it < hold_end ? (ph.mode == PDFC_WARM_LEARNED ? 2 : 1) : 0;
+    launch_iteration(it, ph.a, ph.ls);
+}
+
+void pdfc_sim::run_step() {
+    const bool adaptive = prm.residual_tolerance > 0.;
+    StepPhase ph;
+    step_prologue(ph);
+    pdfc::StepArgs& a = ph.a;
+    pdfc::LocalSelf& ls = ph.ls;
+    int mode = ph.mode;
+    const int hold = ph.hold;
+    const int nc = ph.nc;
     const int total = std::max(nc, 0) + nsc;
     const bool check = adaptive && total > 0;
-    int limit = check ? prm.max_iterations : prm.local_global_iterations;
+    if (!check) {
+        for (int k = 0; k < prm.local_global_iterations; ++k) step_fixed_iteration(ph);
+        step_epilogue(ph, ph.iterations, 0, -1, false, false);
+        return;
+    }
+    int limit = prm.max_iterations;
     int base = 0;         // iterations spent on an abandoned learned warm start
     int to_tol = -1;      // iterations to reach the tolerance (counting abandoned ones)
     bool fell_back = false;
     int it = 0;
-    step_event(1);
     for (;;) {
         int hold_end = mode != PDFC_WARM_PLAIN ? hold : 0;  // iterations < hold_end use the warm reaction
         int copied = 0;                   // residuals [0, copied) requested from the device
@@ -1728,7 +1801,7 @@ void pdfc_sim::run_step() {
         for (it = 0; it < limit && !stop; ++it) {
             a.warm_mode = it < hold_end ? (mode == PDFC_WARM_LEARNED ? 2 : 1) : 0;
             launch_iteration(it, a, ls);
-            if (!check || (it + 1) % kCheckChunk != 0) continue;
+            if ((it + 1) % kCheckChunk != 0) continue;
             // residuals [copied, it) are complete once iteration it's local kernels have run
             const int ev = pending ? 1 - pend_ev : 0;
             HIP_CHECK(hipMemcpyAsync(host_ac + copied, ac_sum + copied, (it - copied) * sizeof(double),
@@ -1745,7 +1818,7 @@ void pdfc_sim::run_step() {
             copied = it;
         }
         const int K = it;  // iterations launched in this attempt
-        if (check && !stop) {
+        if (!stop) {
             // the last residuals: wait for everything enqueued so far
             if (K - 1 > copied)
                 HIP_CHECK(hipMemcpyAsync(host_ac + copied, ac_sum + copied, (K - 1 - copied) * sizeof(double),
@@ -1767,28 +1840,7 @@ void pdfc_sim::run_step() {
         pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);  // also clears the residual records
         if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
     }
-    const int K = it;
-    last_records = K;
-    pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, dob, a, K - 1, stream);
-    if (nsc > 0) {
-        pdfc::launch_self_ac_final(ds, V[K & 1], n, a, K - 1, stream);
-        pdfc::launch_self_release(ds, vslot, stream);
-    }
-    if (check && to_tol < 0) {
-        HIP_CHECK(hipMemcpyAsync(host_ac + (K - 1), ac_sum + (K - 1), sizeof(double), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        if (host_ac[K - 1] / total < prm.residual_tolerance) to_tol = base + K;
-    }
-    step_event(2);
-    HIP_CHECK(hipGetLastError());
-    mesh_advance(prm.time_step);
-    stats.iterations += base + K;
-    if (total > 0) {
-        stats.contact_steps += 1;
-        if (adaptive) stats.iterations_to_tolerance += to_tol > 0 ? to_tol : prm.max_iterations;
-        if (to_tol > 0) stats.converged_steps += 1;
-    }
-    if (fell_back) stats.fallbacks += 1;
+    step_epilogue(ph, it, base, to_tol, fell_back, true);
 }
 
 // The first call captures run_step() on the stream (kernels only: no copies, memsets or
@@ -1883,22 +1935,61 @@ int pdfc_create_batch(const pdfc_scene* scenes, size_t count, const pdfc_params*
 
 void pdfc_destroy(pdfc_sim* sim) { delete sim; }
 
+namespace {
+// Bookkeeping around the steps of one call (pdfc_step, or pdfc_step_begin .. pdfc_step_end).
+void calls_begin(pdfc_sim* sim) {
+    HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
+    sim->self_host_us = 0.;
+    sim->pairing_host_us = 0.;
+    sim->order_host_us = 0.;
+    sim->upload_host_us = 0.;
+    sim->nc_known = true;
+    sim->stats.contacts_sum = sim->stats.self_contacts_sum = sim->stats.self_levels_max = 0;
+    sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
+    sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
+    sim->stats.timed_steps = 0;
+    sim->step_ev_used = 0;
+    sim->sum_collision_ms = sim->sum_iterations_ms = 0.;
+}
+void calls_end(pdfc_sim* sim, int nsteps) {
+    HIP_CHECK(hipMemcpyAsync(sim->host_nonfinite, sim->d_nonfinite, sizeof(int), hipMemcpyDeviceToHost, sim->stream));
+    HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
+    HIP_CHECK(hipEventSynchronize(sim->ev[2]));
+    float ms_total = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
+    sim->stats.step_us = 1e3 * ms_total / nsteps;
+    sim->flush_step_events();
+    const double ts = (double)std::max<int64_t>(1, sim->stats.timed_steps);
+    sim->stats.collision_us = 1e3 * sim->sum_collision_ms / ts;
+    sim->stats.iterations_us = 1e3 * sim->sum_iterations_ms / ts;
+    sim->stats.detection_us = sim->self_host_us / nsteps;
+    sim->stats.pairing_us = sim->pairing_host_us / nsteps;
+    sim->stats.order_us = sim->order_host_us / nsteps;
+    sim->stats.upload_us = sim->upload_host_us / nsteps;
+    if (!sim->nc_known) sim->stats.contacts_sum = -1;
+    sim->stats.steps = nsteps;
+    sim->steps_done += nsteps;
+    if (*sim->host_nonfinite) {
+        // cleared for the next call; the state stays as computed
+        HIP_CHECK(hipMemsetAsync(sim->d_nonfinite, 0, sizeof(int), sim->stream));
+        HIP_CHECK(hipStreamSynchronize(sim->stream));
+        *sim->host_nonfinite = 0;
+        throw NumericError("non-finite positions or velocities within steps " +
+                           std::to_string(sim->steps_done - nsteps + 1) + ".." + std::to_string(sim->steps_done));
+    }
+}
+void no_open_step(const pdfc_sim* sim) {
+    if (sim && sim->phase_open)
+        throw std::invalid_argument("a step is open (pdfc_step_begin without pdfc_step_end)");
+}
+}  // namespace
+
 int pdfc_step(pdfc_sim* sim, int nsteps) {
     return guarded([&] {
         if (!sim || nsteps < 0) throw std::invalid_argument("bad argument");
+        no_open_step(sim);
         if (nsteps == 0) return;
-        HIP_CHECK(hipEventRecord(sim->ev[0], sim->stream));
-        sim->self_host_us = 0.;
-        sim->pairing_host_us = 0.;
-        sim->order_host_us = 0.;
-        sim->upload_host_us = 0.;
-        sim->nc_known = true;
-        sim->stats.contacts_sum = sim->stats.self_contacts_sum = sim->stats.self_levels_max = 0;
-        sim->stats.iterations = sim->stats.iterations_to_tolerance = 0;
-        sim->stats.contact_steps = sim->stats.converged_steps = sim->stats.fallbacks = 0;
-        sim->stats.timed_steps = 0;
-        sim->step_ev_used = 0;
-        sim->sum_collision_ms = sim->sum_iterations_ms = 0.;
+        calls_begin(sim);
         const bool graph = sim->graph_capturable();
         for (int s = 0; s < nsteps; ++s) {
             if (graph)
@@ -1907,32 +1998,40 @@ int pdfc_step(pdfc_sim* sim, int nsteps) {
                 sim->run_step();
             if (sim->ktiming) sim->collect_kernel_times();
         }
-        HIP_CHECK(hipMemcpyAsync(sim->host_nonfinite, sim->d_nonfinite, sizeof(int), hipMemcpyDeviceToHost,
-                                 sim->stream));
-        HIP_CHECK(hipEventRecord(sim->ev[2], sim->stream));
-        HIP_CHECK(hipEventSynchronize(sim->ev[2]));
-        float ms_total = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms_total, sim->ev[0], sim->ev[2]));
-        sim->stats.step_us = 1e3 * ms_total / nsteps;
-        sim->flush_step_events();
-        const double ts = (double)std::max<int64_t>(1, sim->stats.timed_steps);
-        sim->stats.collision_us = 1e3 * sim->sum_collision_ms / ts;
-        sim->stats.iterations_us = 1e3 * sim->sum_iterations_ms / ts;
-        sim->stats.detection_us = sim->self_host_us / nsteps;
-        sim->stats.pairing_us = sim->pairing_host_us / nsteps;
-        sim->stats.order_us = sim->order_host_us / nsteps;
-        sim->stats.upload_us = sim->upload_host_us / nsteps;
-        if (!sim->nc_known) sim->stats.contacts_sum = -1;
-        sim->stats.steps = nsteps;
-        sim->steps_done += nsteps;
-        if (*sim->host_nonfinite) {
-            // cleared for the next call; the state stays as computed
-            HIP_CHECK(hipMemsetAsync(sim->d_nonfinite, 0, sizeof(int), sim->stream));
-            HIP_CHECK(hipStreamSynchronize(sim->stream));
-            *sim->host_nonfinite = 0;
-            throw NumericError("non-finite positions or velocities within steps " +
-                               std::to_string(sim->steps_done - nsteps + 1) + ".." + std::to_string(sim->steps_done));
-        }
+        calls_end(sim, nsteps);
+    });
+}
+
+int pdfc_step_begin(pdfc_sim* sim) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        no_open_step(sim);
+        if (sim->prm.residual_tolerance > 0.)
+            throw std::invalid_argument("pdfc_step_begin: the adaptive iteration count runs through pdfc_step only");
+        calls_begin(sim);
+        sim->step_prologue(sim->phase);
+        sim->phase_open = true;
+    });
+}
+
+int pdfc_step_iterate(pdfc_sim* sim, int n) {
+    return guarded([&] {
+        if (!sim || n < 0) throw std::invalid_argument("bad argument");
+        if (!sim->phase_open) throw std::invalid_argument("pdfc_step_iterate: no step is open (pdfc_step_begin)");
+        for (int k = 0; k < n; ++k) sim->step_fixed_iteration(sim->phase);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int pdfc_step_end(pdfc_sim* sim) {
+    return guarded([&] {
+        if (!sim) throw std::invalid_argument("null sim");
+        if (!sim->phase_open) throw std::invalid_argument("pdfc_step_end: no step is open (pdfc_step_begin)");
+        if (sim->phase.iterations < 1) throw std::invalid_argument("pdfc_step_end: no iteration since pdfc_step_begin");
+        sim->phase_open = false;
+        sim->step_epilogue(sim->phase, sim->phase.iterations, 0, -1, false, false);
+        if (sim->ktiming) sim->collect_kernel_times();
+        calls_end(sim, 1);
     });
 }
 
@@ -1946,6 +2045,7 @@ int pdfc_get_state(pdfc_sim* sim, double* q, double* v) {
 
 int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim) throw std::invalid_argument("null sim");
         for (const double* a : {q, v}) {
             if (!a) continue;
@@ -1965,6 +2065,7 @@ int pdfc_set_state(pdfc_sim* sim, const double* q, const double* v) {
 
 int pdfc_save_checkpoint(pdfc_sim* sim, void* buf, size_t cap, size_t* n) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim || !n) throw std::invalid_argument("null argument");
         std::vector<unsigned char> b = sim->checkpoint_bytes();
         *n = b.size();
@@ -1975,6 +2076,7 @@ int pdfc_save_checkpoint(pdfc_sim* sim, void* buf, size_t cap, size_t* n) {
 
 int pdfc_load_checkpoint(pdfc_sim* sim, const void* buf, size_t n) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim || !buf) throw std::invalid_argument("null argument");
         sim->restore_checkpoint(static_cast<const unsigned char*>(buf), n);
     });
@@ -1982,6 +2084,7 @@ int pdfc_load_checkpoint(pdfc_sim* sim, const void* buf, size_t n) {
 
 int pdfc_detect(pdfc_sim* sim) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim) throw std::invalid_argument("null sim");
         sim->detect();
         if (sim->self_on) sim->self_detect_and_pair(true);
@@ -2123,6 +2226,7 @@ int pdfc_get_iteration_stats(pdfc_sim* sim, double* ac_mean, double* ac_max, siz
 
 int pdfc_set_warm_reactions(pdfc_sim* sim, const pdfc_contact* contacts, size_t n, int hold) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim || (n && !contacts)) throw std::invalid_argument("null argument");
         const int nv = sim->n;
         const size_t slots = std::max(1, sim->max_local_obs);
@@ -2274,7 +2378,7 @@ int pdfc_set_kernel_timing(pdfc_sim* sim, int enable) {
         if (!sim) throw std::invalid_argument("null sim");
         sim->collect_kernel_times();
         sim->ktiming = enable != 0;
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 5; ++k) {
             sim->ksum[k] = 0.;
             sim->kcount[k] = 0;
         }
@@ -2300,12 +2404,15 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         t.backward_us = per_sweep ? t.backward_sweep_us / per_sweep : 0.;
         t.top_us = sim->kcount[3] ? sim->ksum[3] / sim->kcount[3] : 0.;
         t.top_launches = sim->kcount[3];
+        t.self_us = sim->kcount[4] ? sim->ksum[4] / sim->kcount[4] : 0.;
+        t.self_iterations = sim->kcount[4];
         *out = t;
     });
 }
 
 int pdfc_solve(pdfc_sim* sim, const double* b, double* x) {
     return guarded([&] {
+        no_open_step(sim);
         if (!sim || !b || !x) throw std::invalid_argument("null argument");
         sim->to_device_vec(b, sim->V[1]);
         sim->solve_inplace(sim->V[1]);

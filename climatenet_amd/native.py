@@ -20,7 +20,8 @@ EXPORTS = [
     "pdfc_set_kernel_timing", "pdfc_get_kernel_times", "pdfc_set_mlp", "pdfc_get_warm_data",
     "pdfc_set_warm_reactions", "pdfc_save_checkpoint", "pdfc_load_checkpoint",
     "pdfc_mlp_create", "pdfc_mlp_destroy", "pdfc_mlp_set_params", "pdfc_mlp_get_params",
-    "pdfc_mlp_forward", "pdfc_mlp_gradient", "pdfc_mlp_train",
+    "pdfc_mlp_forward", "pdfc_mlp_gradient", "pdfc_mlp_train", "pdfc_mlp_gradient_noalloc",
+    "pdfc_step_begin", "pdfc_step_iterate", "pdfc_step_end",
 ]
 
 
@@ -42,6 +43,9 @@ def lib():
         L.pdfc_create.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.Params), C.POINTER(P)]
         L.pdfc_create_batch.argtypes = [C.POINTER(abi.Scene), C.c_size_t, C.POINTER(abi.Params), C.POINTER(P)]
         L.pdfc_step.argtypes = [P, C.c_int]
+        L.pdfc_step_begin.argtypes = [P]
+        L.pdfc_step_iterate.argtypes = [P, C.c_int]
+        L.pdfc_step_end.argtypes = [P]
         L.pdfc_get_state.argtypes = [P, D, D]
         L.pdfc_set_state.argtypes = [P, D, D]
         L.pdfc_get_contacts.argtypes = [P, C.POINTER(abi.Contact), C.c_size_t, SZ]
@@ -134,6 +138,15 @@ class Simulation:
 
     def step(self, n=1):
         _check(lib().pdfc_step(self.h, n))
+
+    def step_phases(self, iterations=None):
+        """One step through pdfc_step_begin / pdfc_step_iterate / pdfc_step_end (the
+        reference's hook sequence), `iterations` iterations (default: the scene's)."""
+        k = self.spec.iterations if iterations is None else int(iterations)
+        _check(lib().pdfc_step_begin(self.h))
+        for _ in range(k):
+            _check(lib().pdfc_step_iterate(self.h, 1))
+        _check(lib().pdfc_step_end(self.h))
 
     def detect(self):
         _check(lib().pdfc_detect(self.h))

@@ -246,6 +246,8 @@ typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over a
     double forward_sweep_us;         /* one forward sweep, one event pair around all its launches */
     double backward_sweep_us;
     int64_t local_launches, forward_launches, backward_launches, top_launches;
+    double self_us;                  /* self-contact friction of one iteration (all its level launches) */
+    int64_t self_iterations;         /* iterations with self contacts that self_us averages over */
 } pdfc_kernel_times;
 
 /* Build the simulation: topology, masses, constraints, K, host FP64 factorisation, device upload. */
@@ -263,6 +265,23 @@ int pdfc_create_batch(const pdfc_scene* scenes, size_t count, const pdfc_params*
  * Returns PDFC_ERR_NUMERIC when a position or velocity became non-finite during the call (the
  * state is left as computed; the reference steps on silently). */
 int pdfc_step(pdfc_sim* sim, int nsteps);
+
+/* One step in the reference's phases, for a caller that drives SimulationBase::step()'s
+ * protected hooks (include/physics/SimulationBase.hpp:123-131; SimulationBase.cpp:103-173):
+ *   pdfc_step_begin    initializeStep + baseUpdateColisions + updateCollision: per-step constant,
+ *                      obstacle and self-collision detection, pairing, levels (returns once the
+ *                      pairing is done);
+ *   pdfc_step_iterate  buildIteration + solve, n times: local step, friction, substitution
+ *                      (enqueued, asynchronous);
+ *   pdfc_step_end      updateScene: the new state; returns when the step is done.
+ * pdfc_step_begin, pdfc_step_iterate(local_global_iterations), pdfc_step_end launches the kernels
+ * of pdfc_step(sim, 1) in the same order (bit-identical states). Fixed iteration counts only
+ * (residual_tolerance 0: PDFC_ERR_INVALID otherwise); any count >= 1 between begin and end. While
+ * a step is open, pdfc_step, pdfc_set_state, pdfc_detect, pdfc_solve, pdfc_set_warm_reactions and
+ * the checkpoint calls fail with PDFC_ERR_INVALID. */
+int pdfc_step_begin(pdfc_sim* sim);
+int pdfc_step_iterate(pdfc_sim* sim, int n);
+int pdfc_step_end(pdfc_sim* sim);
 
 /* Copy the generalized positions / velocities (3n each) to host memory; either may be NULL. */
 int pdfc_get_state(pdfc_sim* sim, double* q, double* v);

@@ -379,8 +379,8 @@ void build_constraints(orc_sim& S) {
         S.bend_idx.push_back(idx);
         S.bend_coef.push_back(coef);
         S.bend_w.push_back(wb);
-        // BendingConstraint.cpp:140 evaluates the rest Laplacian before the
-        // coefficient list is filled (:142-150), so the stored rest norm is 0.
+        // BendingConstraint.cpp:71 evaluates the rest Laplacian before the
+        // coefficient list is filled (:73-81), so the stored rest norm is 0.
         S.bend_rest_norm.push_back(0.);
     }
 }

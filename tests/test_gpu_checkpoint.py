@@ -192,3 +192,73 @@ def test_non_finite_scene_input_is_rejected(gpu):
     idx, pos, w = s.attachments
     s.attachments = (idx, pos, np.array([100.0, np.inf]))
     assert "attachment" in rejects(s)
+
+
+def test_driver_phase_timing_fills_the_table2_sections(gpu, tmp_path):
+    """--phase-timing: stats.txt gets device times for the sections the reference's Table 2
+    recipe reads (replicability/README.md:64-89): solver, full RHS, RHS base, self collision,
+    self-collision ordering; the obstacle friction is fused into the local kernel and stays
+    "(No data)". The run steps through the phase entry points like every driver run."""
+    p0, t0 = scenes.square_grid(12, 0.3, 0.006)
+    p1, t1 = scenes.square_grid(12, 0.3, 0.009, x0=0.17, y0=0.013)  # in self contact from the start
+    spec = scenes.SceneSpec("two_sheets", np.concatenate([p0, p1]),
+                            np.concatenate([t0, t1 + p0.size // 3]).astype(np.int32), None,
+                            [scenes.ObstacleSpec("plane", (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))], np.array([[0.3]]),
+                            time_step=0.005, iterations=30, self_collision=True, self_tolerance=4e-3)
+    spec.positions = spec.rest.copy()
+    scene = write_scene(str(tmp_path), spec, frames=2, steps_per_frame=3)
+    run_main(str(tmp_path), scene, "--phase-timing")
+    text = open(tmp_path / "data0" / "stats.txt").read()
+
+    def section(name):
+        i = text.index(name + "\n")
+        return text[i:].split("\n---")[0] if "\n---" in text[i:] else text[i:]
+
+    for name in ("Solver time (µs)", "Full RHS computation time (µs)", "RHS base computation time (µs)",
+                 "Self collision time (µs)", "Self collision ordering time (µs)", "Iteration time (µs)",
+                 "Step time (µs)"):
+        sec = section(name)
+        assert "(No data)" not in sec.split("\n")[1], (name, sec)
+    assert "(No data)" in section("RHS friction overhead (µs)")
+    # per-iteration sections: 6 steps x 30 iterations
+    assert "Solver time (µs)\n\t Number      : 180\n" in text
+    assert "Full RHS computation time (µs)\n\t Number      : 180\n" in text
+
+
+def test_step_phases_bit_identical_to_step(gpu):
+    """pdfc_step_begin / pdfc_step_iterate (one per iteration) / pdfc_step_end launch the kernels of
+    pdfc_step(sim, 1): bit-identical states over steps with obstacle and self contacts, and with a
+    warm start held for several iterations."""
+    from tests.test_gpu_self import stacked_small
+    for spec, kw in ((stacked_small(16), {}), (scenes.sphere_drape_small(), {"warm_start": abi.WARM_PREVIOUS,
+                                                                            "warm_hold": 4})):
+        a = gpu.Simulation(spec, **kw)
+        os.environ["PDFC_TUNE"] = a.tuning()
+        try:
+            b = gpu.Simulation(spec, **kw)
+        finally:
+            del os.environ["PDFC_TUNE"]
+        selfs = contacts = 0
+        for s in range(60):
+            a.step(1)
+            b.step_phases()
+            qa, va = a.state()
+            qb, vb = b.state()
+            assert np.array_equal(qa, qb) and np.array_equal(va, vb), (spec.name, s)
+            contacts += len(a.contacts())
+            selfs += len(a.self_contacts()) if spec.self_collision else 0
+        assert contacts > 0
+        if spec.self_collision:
+            assert selfs > 0
+    # misuse is rejected without side effects
+    sim = gpu.Simulation(scenes.sphere_drape_small())
+    with pytest.raises(native.PdfcError):
+        native._check(native.lib().pdfc_step_iterate(sim.h, 1))   # no open step
+    native._check(native.lib().pdfc_step_begin(sim.h))
+    with pytest.raises(native.PdfcError):
+        sim.step(1)                                               # a step is open
+    with pytest.raises(native.PdfcError):
+        native._check(native.lib().pdfc_step_end(sim.h))          # no iteration yet
+    native._check(native.lib().pdfc_step_iterate(sim.h, 10))
+    native._check(native.lib().pdfc_step_end(sim.h))
+    sim.step(1)
