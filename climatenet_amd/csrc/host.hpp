@@ -126,13 +126,32 @@ struct SelfPair {
 // Contact basis rows [n, t, b] (computeBasis, SimulationFrictionEstimation.cpp:89-105).
 void host_contact_basis(const double n[3], double B[9]);
 
-// Greedy node-node pairing over candidates sorted by (vertex, triangle).
+// Host workspace of the per-step self-contact pairing and level ordering: flat arrays sized
+// once per model and reused, so a step allocates nothing once they have grown.
+struct SelfWorkspace {
+    static constexpr int kRegCap = 16;  // registered partners per vertex held in place
+    int n = 0;
+    std::vector<int> nbr_ptr, nbr_idx;  // one ring, CSR (ascending)
+    std::vector<int> reg_cnt, reg;      // registered partners: count and kRegCap slots per vertex
+    std::vector<int> touched;           // vertices with a non-zero count
+    std::vector<int> vid;               // level ordering: vertex -> graph id (-1 outside the graph)
+    std::vector<unsigned long long> vbits;  // graph vertices, for ascending order
+    std::vector<int> verts, adj_ptr, adj_fill, adj_v, adj_s, frontier, next;
+    std::vector<char> visited;
+    void init(const Model& M);
+};
+
+// Greedy node-node pairing over candidates sorted by (vertex, triangle), general form.
 void pair_self_contacts(const Model& M, const SelfCandidate* cand, size_t ncand, std::vector<SelfPair>& out);
+// The same pairing on the workspace's flat arrays; falls back to the general form when a vertex
+// collects more than kRegCap registered partners (identical results either way).
+void pair_self_contacts(SelfWorkspace& W, const Model& M, const SelfCandidate* cand, size_t ncand,
+                        std::vector<SelfPair>& out);
 // Computation levels: BFS from the obstacle-contact vertices (is_seed), then from degree-1
-// and remaining vertices in ascending order. Fills pair.level and the per-level lists
-// (contact indices in BFS order); returns the number of levels. Work is O(pairs log pairs):
-// is_seed is asked only for vertices of the pairs.
-int order_self_contacts(const std::function<bool(int)>& is_seed, std::vector<SelfPair>& pairs,
-                        std::vector<std::vector<int>>& levels);
+// and remaining vertices in ascending order. Fills pair.level, the contact indices level by
+// level (order) and level_ptr (levels + 1 entries); returns the number of levels. Work is
+// O(pairs + n / 64): is_seed is asked only for vertices of the pairs.
+int order_self_contacts(SelfWorkspace& W, const std::function<bool(int)>& is_seed, std::vector<SelfPair>& pairs,
+                        std::vector<int>& order, std::vector<int>& level_ptr);
 
 }  // namespace pdfc

@@ -257,10 +257,9 @@ struct pdfc_sim {
     Pinned<unsigned long long> host_bits;          // seed bits, n / 64 + 1 words
     Pinned<pdfc::SelfCandidate> host_filt;         // filtered candidates
     std::vector<pdfc::SelfPair> pairs;             // pairing order
-    std::vector<std::vector<int>> levels;          // pair indices per level
-    std::vector<int> sorted_pair;                  // device order -> pair index
-    std::vector<int> up_int;                       // the per-step upload, assembled
-    std::vector<double> up_dbl;
+    pdfc::SelfWorkspace self_ws;                   // pairing and level ordering arrays
+    std::vector<int> level_order, level_ptr;       // pair indices level by level
+    std::vector<int> up_gidx, up_gcount, up_gfill, up_gstart;  // upload packing, reused
     Pinned<int> pin_int;                           // and staged for the copy (read by the copy until
     Pinned<double> pin_dbl;                        // the next step's detection synchronises)
     int* d_self_int = nullptr;
@@ -1457,15 +1456,16 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
     }
     filt_guess = nf + nf / 2 + 256;
     const auto t_pair0 = std::chrono::steady_clock::now();
-    pdfc::pair_self_contacts(model, host_filt.p, nf, pairs);
+    pdfc::pair_self_contacts(self_ws, model, host_filt.p, nf, pairs);
     const auto t_order0 = std::chrono::steady_clock::now();
     const unsigned long long* bits = host_bits.p;
     n_self_levels = pdfc::order_self_contacts(
+        self_ws,
         [&](int v) {
             const int i = fac.iperm[v];
             return ((bits[i >> 6] >> (i & 63)) & 1ull) != 0;
         },
-        pairs, levels);
+        pairs, level_order, level_ptr);
     nsc = (int)pairs.size();
     const auto t_up0 = std::chrono::steady_clock::now();
     if (nsc > 0) self_upload();
@@ -1476,67 +1476,78 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
     upload_host_us += std::chrono::duration<double, std::micro>(t_end - t_up0).count();
 }
 
-// Packs the self contacts sorted by level, the involved-vertex slots and the
-// per-level apply groups, and uploads them in one copy per type.
+// Packs the self contacts sorted by level, the involved-vertex slots and the per-level apply
+// groups straight into the pinned staging buffers, and uploads them in one copy per type.
+// Integer layout: a, b, sa, sb (nsc each), lvl_ptr, grp_ptr (L + 1 each), ent_contact,
+// ent_sign (2 nsc each), slot_vertex (nslots), grp_slot (G), ent_ptr (G + 1).
 void pdfc_sim::self_upload() {
-    const int L = (int)levels.size();
-    sorted_pair.clear();
-    for (const auto& lv : levels) sorted_pair.insert(sorted_pair.end(), lv.begin(), lv.end());
+    const int L = n_self_levels;
+    const size_t max_int = 14 * (size_t)nsc + 2 * (size_t)L + 3;
+    pin_int.grow(max_int);
+    pin_dbl.grow(10 * (size_t)nsc);
+    int* const P = pin_int.p;
+    int *A = P, *B = P + nsc, *SA = P + 2 * nsc, *SB = P + 3 * nsc;
+    int* lvl_ptr = P + 4 * nsc;
+    int* grp_ptr = lvl_ptr + (L + 1);
+    int* ent_contact = grp_ptr + (L + 1);
+    int* ent_sign = ent_contact + 2 * nsc;
+    int* slot_vertex = ent_sign + 2 * nsc;
+    double* basis = pin_dbl.p;
+    double* mu = pin_dbl.p + 9 * (size_t)nsc;
     // involved vertices: those of the self contacts, in order of first appearance (the local
     // kernel keeps the last obstacle reaction of every other contact vertex itself)
-    std::vector<int> slot_vertex;
     std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
     if ((int)slot_of.size() != n) slot_of.assign(n, -1);
-    std::vector<int> A(nsc), B(nsc), SA(nsc), SB(nsc);
-    std::vector<double> basis(9 * (size_t)nsc), mu(nsc, self_mu);
+    int nslots = 0;
     for (int j = 0; j < nsc; ++j) {
-        const pdfc::SelfPair& p = pairs[sorted_pair[j]];
+        const pdfc::SelfPair& p = pairs[level_order[j]];
         A[j] = fac.iperm[p.v];
         B[j] = fac.iperm[p.partner];
         for (int x : {A[j], B[j]})
             if (slot_of[x] < 0) {
-                slot_of[x] = (int)slot_vertex.size();
-                slot_vertex.push_back(x);
+                slot_of[x] = nslots;
+                slot_vertex[nslots++] = x;
             }
         SA[j] = slot_of[A[j]];
         SB[j] = slot_of[B[j]];
         const double nr[3] = {p.normal.x, p.normal.y, p.normal.z};
         pdfc::host_contact_basis(nr, &basis[9 * (size_t)j]);
+        mu[j] = self_mu;
     }
-    const int nslots = (int)slot_vertex.size();
-    for (int x : slot_vertex) slot_of[x] = -1;
-    // per-level groups: each touched slot once, entries in the level's contact order
-    std::vector<int> lvl_ptr(L + 1, 0), grp_ptr(L + 1, 0), grp_slot, ent_ptr(1, 0), ent_contact, ent_sign;
-    // groups in order of first appearance, entries of a group in the level's contact order:
-    // a counting pass, then a placing pass (no per-group containers)
-    std::vector<int> gidx(nslots, -1), gs, gcount, gfill;
-    ent_contact.resize(2 * (size_t)nsc);
-    ent_sign.resize(2 * (size_t)nsc);
+    for (int k = 0; k < nslots; ++k) slot_of[slot_vertex[k]] = -1;
+    // per-level groups: each touched slot once, in order of first appearance; entries of a group
+    // in the level's contact order (a counting pass, then a placing pass)
+    int* grp_slot = slot_vertex + nslots;
+    std::vector<int>& gidx = up_gidx;
+    if ((int)gidx.size() < nslots) gidx.assign(nslots, -1);
+    std::vector<int>& gcount = up_gcount;
+    std::vector<int>& gfill = up_gfill;
     lvl_counts.assign(L, 0);
     grp_counts.assign(L, 0);
-    int j0 = 0, e0 = 0;
+    lvl_ptr[0] = 0;
+    grp_ptr[0] = 0;
+    int G = 0, e0 = 0;
+    up_gstart.clear();
     for (int l = 0; l < L; ++l) {
-        const int cnt = (int)levels[l].size();
-        lvl_ptr[l + 1] = lvl_ptr[l] + cnt;
-        gs.clear();
+        const int j0 = level_ptr[l], cnt = level_ptr[l + 1] - level_ptr[l];
+        lvl_ptr[l + 1] = j0 + cnt;
+        const int g0 = G;
         gcount.clear();
         for (int j = j0; j < j0 + cnt; ++j)
-            for (int side = 0; side < 2; ++side) {
-                const int sl = side == 0 ? SA[j] : SB[j];
+            for (int sl : {SA[j], SB[j]}) {
                 if (gidx[sl] < 0) {
-                    gidx[sl] = (int)gs.size();
-                    gs.push_back(sl);
+                    gidx[sl] = G - g0;
+                    grp_slot[G++] = sl;
                     gcount.push_back(0);
                 }
                 ++gcount[gidx[sl]];
             }
-        const int ng = (int)gs.size();
-        gfill.assign(ng, 0);
+        const int ng = G - g0;
+        gfill.resize(ng);
         for (int g = 0, acc = e0; g < ng; ++g) {
             gfill[g] = acc;
+            up_gstart.push_back(acc);
             acc += gcount[g];
-            grp_slot.push_back(gs[g]);
-            ent_ptr.push_back(acc);
         }
         for (int j = j0; j < j0 + cnt; ++j)
             for (int side = 0; side < 2; ++side) {
@@ -1544,33 +1555,26 @@ void pdfc_sim::self_upload() {
                 ent_contact[e] = j;
                 ent_sign[e] = side == 0 ? 1 : -1;
             }
-        for (int sl : gs) gidx[sl] = -1;
-        grp_ptr[l + 1] = grp_ptr[l] + ng;
+        for (int g = g0; g < G; ++g) gidx[grp_slot[g]] = -1;
+        grp_ptr[l + 1] = G;
         lvl_counts[l] = cnt;
         grp_counts[l] = ng;
-        j0 += cnt;
         e0 += 2 * cnt;
     }
-    // staging: ints then doubles
-    up_int.clear();
-    auto put = [&](const std::vector<int>& v) {
-        size_t off = up_int.size();
-        up_int.insert(up_int.end(), v.begin(), v.end());
-        return off;
-    };
-    size_t oA = put(A), oB = put(B), oSA = put(SA), oSB = put(SB), oL = put(lvl_ptr), oG = put(grp_ptr),
-           oGS = put(grp_slot), oEP = put(ent_ptr), oEC = put(ent_contact), oES = put(ent_sign), oSV = put(slot_vertex);
-    up_dbl.clear();
-    up_dbl.insert(up_dbl.end(), basis.begin(), basis.end());
-    up_dbl.insert(up_dbl.end(), mu.begin(), mu.end());
-    if (up_int.size() > cap_self_int) {
+    // group g's entries run from its start to the next group's start
+    int* ent_ptr = grp_slot + G;
+    std::copy(up_gstart.begin(), up_gstart.end(), ent_ptr);
+    ent_ptr[G] = e0;
+    const size_t n_int = (size_t)(ent_ptr - P) + (size_t)G + 1;
+    const size_t n_dbl = 10 * (size_t)nsc;
+    if (n_int > cap_self_int) {
         free_ptr(d_self_int);
-        cap_self_int = up_int.size() * 2;
+        cap_self_int = n_int * 2;
         d_self_int = dalloc<int>(cap_self_int);
     }
-    if (up_dbl.size() > cap_self_dbl) {
+    if (n_dbl > cap_self_dbl) {
         free_ptr(d_self_dbl);
-        cap_self_dbl = up_dbl.size() * 2;
+        cap_self_dbl = n_dbl * 2;
         d_self_dbl = dalloc<double>(cap_self_dbl);
     }
     if ((size_t)nsc > cap_self_contacts || (size_t)nslots > cap_self_slots) {
@@ -1583,12 +1587,11 @@ void pdfc_sim::self_upload() {
     }
     // staged in pinned memory: the copies run asynchronously; the staging is next written by
     // the next step's upload, after that step's detection has synchronised the stream
-    pin_int.grow(up_int.size());
-    pin_dbl.grow(up_dbl.size());
-    std::memcpy(pin_int.p, up_int.data(), up_int.size() * sizeof(int));
-    std::memcpy(pin_dbl.p, up_dbl.data(), up_dbl.size() * sizeof(double));
-    HIP_CHECK(hipMemcpyAsync(d_self_int, pin_int.p, up_int.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_self_dbl, pin_dbl.p, up_dbl.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_self_int, pin_int.p, n_int * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_self_dbl, pin_dbl.p, n_dbl * sizeof(double), hipMemcpyHostToDevice, stream));
+    const size_t oA = 0, oB = nsc, oSA = 2 * (size_t)nsc, oSB = 3 * (size_t)nsc, oL = 4 * (size_t)nsc,
+                 oG = oL + L + 1, oEC = oG + L + 1, oES = oEC + 2 * (size_t)nsc, oSV = oES + 2 * (size_t)nsc,
+                 oGS = oSV + nslots, oEP = oGS + G;
     pdfc::DevSelf S{};
     S.nsc = nsc;
     S.nslots = nslots;
@@ -2152,7 +2155,7 @@ int pdfc_get_self_contacts(pdfc_sim* sim, pdfc_self_contact* out, size_t cap, si
         HIP_CHECK(hipMemcpyAsync(act.data(), sim->ds.active, ns * sizeof(int), hipMemcpyDeviceToHost, sim->stream));
         HIP_CHECK(hipStreamSynchronize(sim->stream));
         for (int j = 0; j < ns; ++j) {
-            const int p = sim->sorted_pair[j];
+            const int p = sim->level_order[j];
             const pdfc::SelfPair& sp = sim->pairs[p];
             pdfc_self_contact& o = out[p];
             o.vertex = sp.v;
