@@ -244,6 +244,12 @@ void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st);
 void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
                        const StepArgs& a, int iteration, hipStream_t st);
 void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st);
+// All self-contact levels of one iteration in a single workgroup (for few self contacts: one
+// launch instead of two per level and one to finish).
+constexpr int kSelfFusedThreads = 1024;
+constexpr int kSelfFusedMaxContacts = 4096;
+void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
+                       int iteration, double* rhs, hipStream_t st);
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st);
 void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, size_t count, hipStream_t st);

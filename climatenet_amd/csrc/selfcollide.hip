@@ -458,58 +458,53 @@ __device__ void ac_block(double s, double m, double* sum_slot, unsigned long lon
     }
 }
 
-// One level: new reaction of every self contact of the level from the state at the
-// level's start (computed-then-applied, as in the reference's two parallel loops).
-__global__ void k_self_level(DevSelf S, int l, const double* __restrict__ v, int n, const double* __restrict__ mass,
-                             StepArgs a, int iteration) {
-    int s = S.lvl_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
-    double acs = 0., acm = 0.;
-    if (s < S.lvl_ptr[l + 1]) {
-        const double* B = &S.basis[9 * s];
-        P3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
-        double* r = &S.r[3 * s];
-        const int pa = S.a[s], pb = S.b[s];
-        if (a.record && iteration > 0) {
-            P3 u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
-            acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, r, S.mu[s]);
-            acm = acs;
-        }
-        const int sa = S.sa[s], sb = S.sb[s];
-        P3 rem = ld_slot(S.rem, s);
-        P3 fA = psub(padd(padd(padd(ld_slot(S.Fc, sa), ld_slot(S.CFn, sa)), ld_slot(S.SCFc, sa)), ld_slot(S.SCFn, sa)), rem);
-        P3 fB = padd(padd(padd(padd(ld_slot(S.Fc, sb), ld_slot(S.CFn, sb)), ld_slot(S.SCFc, sb)), ld_slot(S.SCFn, sb)), rem);
-        const double mA = mass[pa], mB = mass[pb];
-        P3 f = psub(pscale(1. / mA, fA), pscale(1. / mB, fB));
-        P3 fl = {pdot(bn, f), pdot(bt, f), pdot(bb, f)};
-        P3 add = {0., 0., 0.};
-        if (fl.x > 0.) {
-            r[0] = 0.; r[1] = 0.; r[2] = 0.;
-            S.active[s] = 0;
-        } else {
-            double mu = S.mu[s];
-            double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
-            double rt = sqrt(r1 * r1 + r2 * r2);
-            if (rt > mu * r0) {
-                r1 *= mu * r0 / rt;
-                r2 *= mu * r0 / rt;
-            }
-            double w = (mA * mB) / (mA + mB);
-            r0 *= w; r1 *= w; r2 *= w;
-            r[0] = r0; r[1] = r1; r[2] = r2;
-            S.active[s] = 1;
-            add = {(bn.x * r0 + bt.x * r1) + bb.x * r2, (bn.y * r0 + bt.y * r1) + bb.y * r2,
-                   (bn.z * r0 + bt.z * r1) + bb.z * r2};
-        }
-        st_slot(S.addv, s, add);
+// One self contact's new reaction from the state at its level's start (the reference's first
+// parallel loop, SimulationFrictionEstimation.cpp:431-520); accumulates its friction residual.
+__device__ __forceinline__ void self_contact_update(const DevSelf& S, int s, const double* __restrict__ v, int n,
+                                                    const double* __restrict__ mass, const StepArgs& a, int iteration,
+                                                    double& acs, double& acm) {
+    const double* B = &S.basis[9 * s];
+    P3 bn = {B[0], B[1], B[2]}, bt = {B[3], B[4], B[5]}, bb = {B[6], B[7], B[8]};
+    double* r = &S.r[3 * s];
+    const int pa = S.a[s], pb = S.b[s];
+    if (a.record && iteration > 0) {
+        P3 u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
+        const double an = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, r, S.mu[s]);
+        acs += an;
+        acm = fmax(acm, an);
     }
-    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+    const int sa = S.sa[s], sb = S.sb[s];
+    P3 rem = ld_slot(S.rem, s);
+    P3 fA = psub(padd(padd(padd(ld_slot(S.Fc, sa), ld_slot(S.CFn, sa)), ld_slot(S.SCFc, sa)), ld_slot(S.SCFn, sa)), rem);
+    P3 fB = padd(padd(padd(padd(ld_slot(S.Fc, sb), ld_slot(S.CFn, sb)), ld_slot(S.SCFc, sb)), ld_slot(S.SCFn, sb)), rem);
+    const double mA = mass[pa], mB = mass[pb];
+    P3 f = psub(pscale(1. / mA, fA), pscale(1. / mB, fB));
+    P3 fl = {pdot(bn, f), pdot(bt, f), pdot(bb, f)};
+    P3 add = {0., 0., 0.};
+    if (fl.x > 0.) {
+        r[0] = 0.; r[1] = 0.; r[2] = 0.;
+        S.active[s] = 0;
+    } else {
+        double mu = S.mu[s];
+        double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
+        double rt = sqrt(r1 * r1 + r2 * r2);
+        if (rt > mu * r0) {
+            r1 *= mu * r0 / rt;
+            r2 *= mu * r0 / rt;
+        }
+        double w = (mA * mB) / (mA + mB);
+        r0 *= w; r1 *= w; r2 *= w;
+        r[0] = r0; r[1] = r1; r[2] = r2;
+        S.active[s] = 1;
+        add = {(bn.x * r0 + bt.x * r1) + bb.x * r2, (bn.y * r0 + bt.y * r1) + bb.y * r2,
+               (bn.z * r0 + bt.z * r1) + bb.z * r2};
+    }
+    st_slot(S.addv, s, add);
 }
 
-// Apply one level: per touched vertex, in the level's contact order:
-//   SCF_cur -= sign * previous force, SCF_next += sign * new force.
-__global__ void k_self_apply(DevSelf S, int l) {
-    int g = S.grp_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= S.grp_ptr[l + 1]) return;
+// Apply group g of a level (the second parallel loop, :522-560): per touched vertex, in the
+// level's contact order, SCF_cur -= sign * previous force, SCF_next += sign * new force.
+__device__ __forceinline__ void self_group_apply(const DevSelf& S, int g) {
     const int slot = S.grp_slot[g];
     P3 cur = ld_slot(S.SCFc, slot), nxt = ld_slot(S.SCFn, slot);
     for (int e = S.ent_ptr[g]; e < S.ent_ptr[g + 1]; ++e) {
@@ -527,11 +522,9 @@ __global__ void k_self_apply(DevSelf S, int l) {
     st_slot(S.SCFn, slot, nxt);
 }
 
-// b += CF_next + SCF_next at the involved vertices; clear the buffers that become
+// b += CF_next + SCF_next at an involved vertex (:566-568); clear the buffers that become
 // "next" after the swap.
-__global__ void k_self_finalize(DevSelf S, double* __restrict__ rhs, int n) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S.nslots) return;
+__device__ __forceinline__ void self_slot_finalize(const DevSelf& S, int s, double* __restrict__ rhs, int n) {
     int v = S.slot_vertex[s];
     P3 c = ld_slot(S.CFn, s), sc = ld_slot(S.SCFn, s);
     rhs[v] += c.x + sc.x;
@@ -539,6 +532,47 @@ __global__ void k_self_finalize(DevSelf S, double* __restrict__ rhs, int n) {
     rhs[2 * n + v] += c.z + sc.z;
     st_slot(S.CFc, s, P3{0., 0., 0.});
     st_slot(S.SCFc, s, P3{0., 0., 0.});
+}
+
+// One level: new reaction of every self contact of the level from the state at the
+// level's start (computed-then-applied, as in the reference's two parallel loops).
+__global__ void k_self_level(DevSelf S, int l, const double* __restrict__ v, int n, const double* __restrict__ mass,
+                             StepArgs a, int iteration) {
+    int s = S.lvl_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
+    double acs = 0., acm = 0.;
+    if (s < S.lvl_ptr[l + 1]) self_contact_update(S, s, v, n, mass, a, iteration, acs, acm);
+    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+}
+
+__global__ void k_self_apply(DevSelf S, int l) {
+    int g = S.grp_ptr[l] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < S.grp_ptr[l + 1]) self_group_apply(S, g);
+}
+
+__global__ void k_self_finalize(DevSelf S, double* __restrict__ rhs, int n) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S.nslots) self_slot_finalize(S, s, rhs, n);
+}
+
+// All levels of one iteration in one workgroup (few self contacts): each level's updates,
+// a barrier, its applies, a barrier; then the right-hand side. The barriers order the
+// workgroup's global-memory writes and reads as the kernel boundaries of the per-level
+// launches do; the arithmetic per contact, group and vertex is theirs.
+__global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused(DevSelf S, int nlev, const double* __restrict__ v,
+                                                                 int n, const double* __restrict__ mass, StepArgs a,
+                                                                 int iteration, double* __restrict__ rhs) {
+    double acs = 0., acm = 0.;
+    for (int l = 0; l < nlev; ++l) {
+        const int s1 = S.lvl_ptr[l + 1];
+        for (int s = S.lvl_ptr[l] + (int)threadIdx.x; s < s1; s += kSelfFusedThreads)
+            self_contact_update(S, s, v, n, mass, a, iteration, acs, acm);
+        __syncthreads();
+        const int g1 = S.grp_ptr[l + 1];
+        for (int g = S.grp_ptr[l] + (int)threadIdx.x; g < g1; g += kSelfFusedThreads) self_group_apply(S, g);
+        __syncthreads();
+    }
+    for (int s = threadIdx.x; s < S.nslots; s += kSelfFusedThreads) self_slot_finalize(S, s, rhs, n);
+    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
 }
 
 __global__ void k_self_ac_final(DevSelf S, const double* __restrict__ v, int n, StepArgs a, int iteration) {
@@ -823,6 +857,10 @@ void launch_self_level(const DevSelf& S, int l, int count, const double* v, int 
 }
 void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st) {
     if (groups > 0) hipLaunchKernelGGL(k_self_apply, dim3(nblk(groups)), dim3(kB), 0, st, S, l);
+}
+void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
+                       int iteration, double* rhs, hipStream_t st) {
+    hipLaunchKernelGGL(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
 }
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st) {
     if (S.nslots > 0) hipLaunchKernelGGL(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);

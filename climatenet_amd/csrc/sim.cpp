@@ -1633,11 +1633,15 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
         timed(4, [&] {
-            for (int l = 0; l < n_self_levels; ++l) {
-                pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
-                pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+            if (nsc <= pdfc::kSelfFusedMaxContacts) {
+                pdfc::launch_self_fused(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
+            } else {
+                for (int l = 0; l < n_self_levels; ++l) {
+                    pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
+                    pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
+                }
+                pdfc::launch_self_finalize(ds, vnext, n, stream);
             }
-            pdfc::launch_self_finalize(ds, vnext, n, stream);
         });
         std::swap(ds.CFc, ds.CFn);
         std::swap(ds.SCFc, ds.SCFn);
