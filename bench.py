@@ -336,7 +336,7 @@ def main():
                     help="drape: untimed steps before the warm-up (110: the cloth is on the sphere and "
                          "folds onto itself, the self-contact phase)")
     ap.add_argument("--no-full-drape", action="store_true",
-                    help="drape: skip the untimed-window-independent run of all 500 steps (profiling the window)")
+                    help="drape: skip timing the whole 500-step drape (to profile the window alone)")
     ap.add_argument("--no-self-collision", action="store_true",
                     help="disable self-collision (not the benchmark configuration)")
     ap.add_argument("--workload", choices=["drape", "batched", "stacked", "meshdrape", "mlp"], default="drape",
