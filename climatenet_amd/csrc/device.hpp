@@ -248,6 +248,12 @@ void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st);
 // launch instead of two per level and one to finish).
 constexpr int kSelfFusedThreads = 1024;
 constexpr int kSelfFusedMaxContacts = 4096;
+// The same with the level-to-level state in LDS: one contact per thread, 6 doubles per
+// involved vertex and per contact (144 KiB at the limits).
+constexpr int kSelfLdsMaxContacts = kSelfFusedThreads;
+constexpr int kSelfLdsMaxSlots = 2 * kSelfLdsMaxContacts;
+void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
+                           int iteration, double* rhs, hipStream_t st);
 void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                        int iteration, double* rhs, hipStream_t st);
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st);

@@ -554,6 +554,137 @@ __global__ void k_self_finalize(DevSelf S, double* __restrict__ rhs, int n) {
     if (s < S.nslots) self_slot_finalize(S, s, rhs, n);
 }
 
+// The new reaction of a self contact from its inputs (the arithmetic of self_contact_update):
+// f = B^T (fA / mA - fB / mB), take-off if f_N > 0, else the cone projection of -f scaled by
+// mA mB / (mA + mB). Writes r and active; returns the global force to apply.
+__device__ __forceinline__ P3 self_reaction(const DevSelf& S, int s, P3 bn, P3 bt, P3 bb, P3 fA, P3 fB, double mA,
+                                            double mB, double mu) {
+    double* r = &S.r[3 * s];
+    P3 f = psub(pscale(1. / mA, fA), pscale(1. / mB, fB));
+    P3 fl = {pdot(bn, f), pdot(bt, f), pdot(bb, f)};
+    if (fl.x > 0.) {
+        r[0] = 0.; r[1] = 0.; r[2] = 0.;
+        S.active[s] = 0;
+        return P3{0., 0., 0.};
+    }
+    double r0 = -fl.x, r1 = -fl.y, r2 = -fl.z;
+    double rt = sqrt(r1 * r1 + r2 * r2);
+    if (rt > mu * r0) {
+        r1 *= mu * r0 / rt;
+        r2 *= mu * r0 / rt;
+    }
+    double w = (mA * mB) / (mA + mB);
+    r0 *= w; r1 *= w; r2 *= w;
+    r[0] = r0; r[1] = r1; r[2] = r2;
+    S.active[s] = 1;
+    return P3{(bn.x * r0 + bt.x * r1) + bb.x * r2, (bn.y * r0 + bt.y * r1) + bb.y * r2,
+              (bn.z * r0 + bt.z * r1) + bb.z * r2};
+}
+
+// All levels of one iteration in one workgroup with the level-to-level state in LDS (at most
+// kSelfLdsMaxContacts self contacts, one per thread, and kSelfLdsMaxSlots involved vertices):
+// the self-contact forces of the involved vertices (current and next buffers) and each
+// contact's previous and new force live in LDS; each thread loads its contact's read-only data
+// (basis, vertices, masses, F and the obstacle reaction at both vertices, residual inputs) once.
+// A level is then its updates from LDS, a barrier, the per-vertex applies in LDS, a barrier.
+// Same operations in the same order as self_contact_update / self_group_apply /
+// self_slot_finalize, so the results are those of the other paths bit for bit.
+__global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S, int nlev,
+                                                                     const double* __restrict__ v, int n,
+                                                                     const double* __restrict__ mass, StepArgs a,
+                                                                     int iteration, double* __restrict__ rhs) {
+    __shared__ double sh[6 * (kSelfLdsMaxSlots + kSelfLdsMaxContacts)];
+    double* scf_c = sh;                                  // 3 per slot
+    double* scf_n = sh + 3 * kSelfLdsMaxSlots;           // 3 per slot
+    double* c_rem = sh + 6 * kSelfLdsMaxSlots;           // 3 per contact
+    double* c_add = c_rem + 3 * kSelfLdsMaxContacts;     // 3 per contact
+    const int t = threadIdx.x;
+    for (int k = t; k < 3 * S.nslots; k += kSelfFusedThreads) {
+        scf_c[k] = S.SCFc[k];
+        scf_n[k] = S.SCFn[k];
+    }
+    const int s = t;
+    const bool mine = s < S.nsc;
+    int lev = -1, sa = 0, sb = 0;
+    double acs = 0., acm = 0., mA = 1., mB = 1., mu = 0.;
+    P3 bn{}, bt{}, bb{}, fa{}, fb{}, rem{};
+    if (mine) {
+        for (int l = 0; l < nlev; ++l)
+            if (s >= S.lvl_ptr[l] && s < S.lvl_ptr[l + 1]) lev = l;
+        const double* B = &S.basis[9 * s];
+        bn = P3{B[0], B[1], B[2]};
+        bt = P3{B[3], B[4], B[5]};
+        bb = P3{B[6], B[7], B[8]};
+        const int pa = S.a[s], pb = S.b[s];
+        sa = S.sa[s];
+        sb = S.sb[s];
+        mu = S.mu[s];
+        mA = mass[pa];
+        mB = mass[pb];
+        fa = padd(ld_slot(S.Fc, sa), ld_slot(S.CFn, sa));  // (F + CF) before the self terms, as in fA
+        fb = padd(ld_slot(S.Fc, sb), ld_slot(S.CFn, sb));
+        rem = ld_slot(S.rem, s);
+        c_rem[3 * s] = rem.x;
+        c_rem[3 * s + 1] = rem.y;
+        c_rem[3 * s + 2] = rem.z;
+        if (a.record && iteration > 0) {
+            P3 u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
+            acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, &S.r[3 * s], mu);
+            acm = acs;
+        }
+    }
+    __syncthreads();
+    for (int l = 0; l < nlev; ++l) {
+        if (lev == l) {
+            const P3 ca = {scf_c[3 * sa], scf_c[3 * sa + 1], scf_c[3 * sa + 2]};
+            const P3 na = {scf_n[3 * sa], scf_n[3 * sa + 1], scf_n[3 * sa + 2]};
+            const P3 cb = {scf_c[3 * sb], scf_c[3 * sb + 1], scf_c[3 * sb + 2]};
+            const P3 nb = {scf_n[3 * sb], scf_n[3 * sb + 1], scf_n[3 * sb + 2]};
+            const P3 fA = psub(padd(padd(fa, ca), na), rem);
+            const P3 fB = padd(padd(padd(fb, cb), nb), rem);
+            const P3 add = self_reaction(S, s, bn, bt, bb, fA, fB, mA, mB, mu);
+            c_add[3 * s] = add.x;
+            c_add[3 * s + 1] = add.y;
+            c_add[3 * s + 2] = add.z;
+        }
+        __syncthreads();
+        const int g1 = S.grp_ptr[l + 1];
+        for (int g = S.grp_ptr[l] + t; g < g1; g += kSelfFusedThreads) {
+            const int slot = S.grp_slot[g];
+            P3 cur = {scf_c[3 * slot], scf_c[3 * slot + 1], scf_c[3 * slot + 2]};
+            P3 nxt = {scf_n[3 * slot], scf_n[3 * slot + 1], scf_n[3 * slot + 2]};
+            for (int e = S.ent_ptr[g]; e < S.ent_ptr[g + 1]; ++e) {
+                const int c = S.ent_contact[e];
+                const P3 prev = {c_rem[3 * c], c_rem[3 * c + 1], c_rem[3 * c + 2]};
+                const P3 add = {c_add[3 * c], c_add[3 * c + 1], c_add[3 * c + 2]};
+                if (S.ent_sign[e] > 0) {
+                    cur = psub(cur, prev);
+                    nxt = padd(nxt, add);
+                } else {
+                    cur = padd(cur, prev);
+                    nxt = psub(nxt, add);
+                }
+            }
+            scf_c[3 * slot] = cur.x; scf_c[3 * slot + 1] = cur.y; scf_c[3 * slot + 2] = cur.z;
+            scf_n[3 * slot] = nxt.x; scf_n[3 * slot + 1] = nxt.y; scf_n[3 * slot + 2] = nxt.z;
+        }
+        __syncthreads();
+    }
+    if (mine) st_slot(S.addv, s, P3{c_add[3 * s], c_add[3 * s + 1], c_add[3 * s + 2]});
+    for (int sl = t; sl < S.nslots; sl += kSelfFusedThreads) {
+        const int vv = S.slot_vertex[sl];
+        const P3 c = ld_slot(S.CFn, sl);
+        const P3 sc = {scf_n[3 * sl], scf_n[3 * sl + 1], scf_n[3 * sl + 2]};
+        rhs[vv] += c.x + sc.x;
+        rhs[n + vv] += c.y + sc.y;
+        rhs[2 * n + vv] += c.z + sc.z;
+        st_slot(S.SCFn, sl, sc);
+        st_slot(S.CFc, sl, P3{0., 0., 0.});
+        st_slot(S.SCFc, sl, P3{0., 0., 0.});
+    }
+    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+}
+
 // All levels of one iteration in one workgroup (few self contacts): each level's updates,
 // a barrier, its applies, a barrier; then the right-hand side. The barriers order the
 // workgroup's global-memory writes and reads as the kernel boundaries of the per-level
@@ -861,6 +992,11 @@ void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st) {
 void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                        int iteration, double* rhs, hipStream_t st) {
     hipLaunchKernelGGL(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
+}
+void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
+                           int iteration, double* rhs, hipStream_t st) {
+    hipLaunchKernelGGL(k_self_fused_lds, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration,
+                       rhs);
 }
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st) {
     if (S.nslots > 0) hipLaunchKernelGGL(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);

@@ -272,6 +272,9 @@ struct pdfc_sim {
     int nsc = 0, n_self_levels = 0;
     std::vector<int> lvl_counts, grp_counts;       // per level
     bool keep_full_candidates = false;
+    // self-contact levels of an iteration: 2 one workgroup with LDS (the default, when they fit),
+    // 1 one workgroup through global memory, 0 two launches per level (PDFC_SELF_PATH overrides)
+    int self_path = 2;
     size_t filt_guess = 4096;
     double self_host_us = 0.;   // host time in detection round trips + pairing, accumulated per pdfc_step
     double pairing_host_us = 0.; // the part after the detection results arrived
@@ -490,6 +493,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     // dynamic LDS (right-hand side of one supernode) + the 24 KiB partial-sum buffer
     if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
+    if (const char* e = std::getenv("PDFC_SELF_PATH")) self_path = std::max(0, std::min(2, std::atoi(e)));
     {
         double top_mb = 160.;  // full S^-1; the symmetric tiles read about half of it
         if (const char* e = std::getenv("PDFC_TOP_MB")) top_mb = std::atof(e);
@@ -1633,7 +1637,9 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
         timed(4, [&] {
-            if (nsc <= pdfc::kSelfFusedMaxContacts) {
+            if (self_path >= 2 && nsc <= pdfc::kSelfLdsMaxContacts && ds.nslots <= pdfc::kSelfLdsMaxSlots) {
+                pdfc::launch_self_fused_lds(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
+            } else if (self_path >= 1 && nsc <= pdfc::kSelfFusedMaxContacts) {
                 pdfc::launch_self_fused(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
             } else {
                 for (int l = 0; l < n_self_levels; ++l) {

@@ -183,3 +183,33 @@ def test_double_obstacle_contacts_with_self_contacts(gpu):
     print("worst", worst, "double-contact vertices outside self contacts", doubles, "self contacts", selfs)
     assert doubles > 10 and selfs > 10
     assert max(worst.values()) < 1e-9, worst
+
+
+@pytest.mark.parametrize("scene", ["stacked_small", "drape_40k_step_118"])
+def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
+    """The self-contact levels of an iteration run as two launches per level (PDFC_SELF_PATH=0),
+    in one workgroup through global memory (1) or in one workgroup with the level-to-level state in
+    LDS (2, the default when it fits). Same arithmetic in the same order: bit-identical states."""
+    if scene == "stacked_small":
+        spec, first, steps = stacked_small(16), 30, 20
+    else:
+        spec, first, steps = scenes.sphere_drape_40k(), 118, 3
+    base = gpu.Simulation(spec)
+    print(scene, ": %d steps to a self-contact state" % first, flush=True)
+    base.step(first)
+    q0, v0 = base.state()
+    monkeypatch.setenv("PDFC_TUNE", base.tuning())
+    out, selfs = [], 0
+    for path in ("0", "1", "2"):
+        monkeypatch.setenv("PDFC_SELF_PATH", path)
+        sim = gpu.Simulation(spec)
+        sim.set_state(q0, v0)
+        for _ in range(steps):
+            sim.step(1)
+            selfs = max(selfs, len(sim.self_contacts()))
+        out.append(sim.state())
+        sim.close()
+    print(scene, "self contacts (max over steps)", selfs)
+    assert selfs > 0
+    for q, v in out[1:]:
+        assert np.array_equal(q, out[0][0]) and np.array_equal(v, out[0][1])
