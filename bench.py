@@ -26,12 +26,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(kernel, info):
+def measured_traffic(kernel, info, name="drape40k"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for
-    a device copy of known size), when it was measured on this workload, kernel and level
-    structure; otherwise None."""
-    path = os.path.join(ROOT, "profiles", "drape40k_pmc_traffic.json")
+    profiles/<name>_pmc_traffic.json (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    the same counter's reading for a device copy of known size), when it was measured on this
+    workload, kernel and level structure; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "%s_pmc_traffic.json" % name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -169,6 +169,64 @@ def warm_start_comparison(spec, sim, prec, steps=2, cap=2000, tol=1e-6):
     return {"drape_40k": big, "drape_20x20": res}
 
 
+def kernel_roofline(sim, info, timing_steps, traffic_name):
+    """Average launch time of each kernel family over `timing_steps` more steps with HIP events
+    around every launch (the local kernel, each whole sweep divided by its launches, the top
+    block), the dominant family by total time, and its HBM roofline: algorithmic bytes per launch
+    (pdfc_info) over the live average launch time."""
+    sim.set_kernel_timing(True)
+    sim.step(timing_steps)
+    kt = sim.kernel_times()
+    sim.set_kernel_timing(False)
+    per_sweep = max(1, info["n_levels_swept"])
+    fam = {
+        "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
+        "k_forward": (kt["forward_us"], info["bytes_forward"] / per_sweep, kt["forward_launches"]),
+        "k_backward": (kt["backward_us"], info["bytes_backward"] / per_sweep, kt["backward_launches"]),
+        "k_top": (kt["top_us"], info["bytes_top"], kt["top_launches"]),
+    }
+    dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
+    us, bytes_per_launch, _ = fam[dom]
+    achieved = bytes_per_launch / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, info, traffic_name),
+                "kernel": dom, "avg_launch_us": round(us, 3), "algorithmic_bytes_per_launch": bytes_per_launch}
+    return roofline, {k: round(v[0], 3) for k, v in fam.items()}
+
+
+def cpu_baseline_batched(specs, q, v, ranges, budget_s=20.0, runs=3):
+    """The 64 scenes one after another on the CPU oracle, each from its part of the batch's
+    state: aggregate instance-steps/s over k steps of every scene (k cut to the budget), median
+    of `runs` runs from the same states."""
+    from oracle import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orcs = [po.Oracle(sp) for sp in specs]
+
+    def reset():
+        for o, (a, b) in zip(orcs, ranges):
+            o.set_state(q[3 * a:3 * b].copy(), v[3 * a:3 * b].copy())
+
+    reset()
+    t0 = time.perf_counter()
+    for o in orcs:
+        o.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(20, int(budget_s / runs / max(one, 1e-6))))
+    rates = []
+    for _ in range(runs):
+        reset()
+        t0 = time.perf_counter()
+        for o in orcs:
+            o.step(k)
+        rates.append(len(orcs) * k / (time.perf_counter() - t0))
+    return {"value": float(np.median(rates)), "unit": "instance-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "runs": [round(x, 3) for x in rates],
+            "sample": "%d steps of each of the 64 scenes, one scene after another, from the batch's state after "
+                      "its timed steps; %d OpenMP threads; median of %d runs; factorisation excluded"
+                      % (k, threads, runs)}
+
+
 def bench_batched(args):
     """64 independent cloths (N ~ U{32..128}, one sphere each) advanced as one batch, against
     the same 64 scenes run one after another; instance-steps/s of each."""
@@ -182,8 +240,10 @@ def bench_batched(args):
     sim.step(args.steps)
     batched_s = time.perf_counter() - t0
     info = sim.info()
+    q_end, v_end = sim.state()
+    roofline, ktimes = kernel_roofline(sim, info, args.timing_steps, "batched64")
     seq_s = 0.0
-    for sp in specs:
+    for sp in ([] if args.no_sequential else specs):
         one = native.Simulation(sp, record_iterations=False, use_graph=not args.no_graph)
         one.step(args.warmup)
         t1 = time.perf_counter()
@@ -201,10 +261,16 @@ def bench_batched(args):
                    "total_vertices": info["n_vertices"], "levels": info["n_levels"],
                    "levels_swept": info["n_levels_swept"], "top_block_rows": info["top_m"],
                    "graph_replay": not args.no_graph, "create_seconds": round(create_s, 2),
+                   "tuning": sim.tuning(), "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
+                   "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
+                   "top_chunks": info["top_chunks"], "top_fold": info["top_fold"], "nnz_F": info["nnz_F"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
-        "sequential": {"value": round(len(specs) * args.steps / seq_s, 3), "unit": "instance-steps/s",
-                       "note": "the same 64 scenes, one simulation after another on the GPU"},
-        "speedup_vs_sequential": round(seq_s / batched_s, 3),
+        "sequential": None if args.no_sequential else {
+            "value": round(len(specs) * args.steps / seq_s, 3), "unit": "instance-steps/s",
+            "note": "the same 64 scenes, one simulation after another on the GPU"},
+        "speedup_vs_sequential": None if args.no_sequential else round(seq_s / batched_s, 3),
+        "roofline": roofline,
+        "kernel_times_us": ktimes,
         # whole batch step: algorithmic bytes of its 15 iterations at the measured rate
         "step_roofline": {"bytes_per_step": info["bytes_per_iteration"] * specs[0].iterations,
                           "achieved": round(info["bytes_per_iteration"] * specs[0].iterations * args.steps
@@ -213,6 +279,8 @@ def bench_batched(args):
                           "frac": round(info["bytes_per_iteration"] * specs[0].iterations * args.steps
                                         / batched_s / 1e9 / HBM_PEAK_GBS, 6)},
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batched(specs, q_end, v_end, sim.ranges)
     print(json.dumps(out), flush=True)
 
 
@@ -344,6 +412,8 @@ def main():
                          "stacked sheets, FP32 + FP64 refinement; meshdrape: cloth on a key-framed mesh sphere; "
                          "mlp: the warm-start network alone")
     ap.add_argument("--no-graph", action="store_true", help="batched: launch kernels eagerly")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched: skip the 64 scenes run one after another on the GPU (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -398,26 +468,10 @@ def main():
     first_timed = args.first_step + args.warmup + 1
 
     # kernel times: instrumented continuation of the same run (events around every launch)
-    sim.set_kernel_timing(True)
-    sim.step(args.timing_steps)
-    kt = sim.kernel_times()
-    sim.set_kernel_timing(False)
     info = sim.info()
     K = spec.iterations
-    # one launch per swept level
-    per_sweep = max(1, info["n_levels_swept"])
-    fam = {
-        "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
-        "k_forward": (kt["forward_us"], info["bytes_forward"] / per_sweep, kt["forward_launches"]),
-        "k_backward": (kt["backward_us"], info["bytes_backward"] / per_sweep, kt["backward_launches"]),
-        "k_top": (kt["top_us"], info["bytes_top"], kt["top_launches"]),
-    }
-    dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
-    us, bytes_per_launch, _ = fam[dom]
-    achieved = bytes_per_launch / (us * 1e-6) / 1e9 if us > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, info), "kernel": dom,
-                "avg_launch_us": round(us, 3), "algorithmic_bytes_per_launch": bytes_per_launch}
+    roofline, ktimes = kernel_roofline(sim, info, args.timing_steps, "drape40k")
+    info = sim.info()
 
     out = {
         "metric": "steps/s on 40k-vertex cloth drape",
@@ -453,7 +507,7 @@ def main():
                           "achieved": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9, 3),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(info["bytes_per_iteration"] * K * steps_per_s / 1e9 / HBM_PEAK_GBS, 6)},
-        "kernel_times_us": {k: round(v[0], 3) for k, v in fam.items()},
+        "kernel_times_us": ktimes,
         "self_collision_host_us_per_step": {"round_trip": round(phases["detection_us"], 1),
                                             "pairing": round(phases["pairing_us"], 1),
                                             "of_which_levels": round(phases["order_us"], 1),
