@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
                     r0 = mi * (double)y[0];
                     r1 = mi * (double)y[1];
                     r2 = mi * (double)y[2];
-                    warm = a.warm_mode == 2 || r0 != 0. || r1 != 0. || r2 != 0.;
+                    warm = r0 != 0. || r1 != 0. || r2 != 0.;  // no warm value: the contact starts plain
                 }
                 if (warm ? !(r0 > 0.) : fl.x > 0.) {
                     r[0] = 0.; r[1] = 0.; r[2] = 0.;

@@ -338,6 +338,21 @@ __global__ void k_mlp_refresh(const float* __restrict__ w, MlpDev P) {
 
 }  // namespace
 
+// Learned warm start: y (the network's change of the reaction per unit mass) += the previous
+// step's reaction per unit mass, features 6..8 of each contact. A contact without a previous
+// reaction gets y = 0: it starts plain, as with the previous-step warm start.
+__global__ void k_mlp_add_previous(const float* __restrict__ feat, int rows, float* __restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float* p = feat + (size_t)r * kMlpFeatures + 6;
+    const bool has_prev = p[0] != 0.f || p[1] != 0.f || p[2] != 0.f;
+    for (int k = 0; k < 3; ++k) y[3 * (size_t)r + k] = has_prev ? y[3 * (size_t)r + k] + p[k] : 0.f;
+}
+
+void launch_mlp_add_previous(const float* feat, int rows, float* y, hipStream_t st) {
+    if (rows > 0) hipLaunchKernelGGL(k_mlp_add_previous, dim3((rows + 255) / 256), dim3(256), 0, st, feat, rows, y);
+}
+
 void launch_mlp_forward(const MlpDev& P, const float* x, int rows, float* y, hipStream_t st) {
     if (rows > 0)
         hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min((rows + 15) / 16, kMlpForwardGrid)), dim3(64), 0, st,

@@ -45,6 +45,7 @@ static_assert(kMlpNorm == PDFC_MLP_NORM, "normalisation size of the C ABI");
 static_assert(kMlpFeatures == PDFC_MLP_FEATURES && kMlpOut == PDFC_MLP_OUTPUTS, "shapes of the C ABI");
 
 // y[rows x 3] = y_scale * MLP((x - mean) / std), one wave per 16 rows, on stream st.
+void launch_mlp_add_previous(const float* feat, int rows, float* y, hipStream_t st);
 void launch_mlp_forward(const MlpDev& P, const float* x, int rows, float* y, hipStream_t st);
 // Message of the last failure, read by pdfc_last_error() (defined with the simulation).
 void set_last_error(const char* msg);

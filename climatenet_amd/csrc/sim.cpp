@@ -85,10 +85,9 @@ void pdfc::set_last_error(const char* msg) { g_last_error = msg; }
 constexpr int kCheckChunk = 8;
 constexpr int kMaxTopChunks = 16;  // reduction split of the dense top-block matvec
 static_assert(kMaxTopChunks <= 16, "the folded top finish (top_sum) reads at most 16 chunks");
-// Dependency-driven substitution sweeps (one launch per sweep): switched off, see autotune_waves().
 constexpr int kReleaseWindow = 50;
 constexpr double kReleaseFactor = 0.9;
-constexpr int kStagnationWindow = 200;
+constexpr int kStagnationWindow = 1000;  // the 40k drape's tail falls about 10x per 6,000 iterations
 constexpr double kStagnationFactor = 0.9;
 
 struct pdfc_sim {
@@ -1711,7 +1710,10 @@ void pdfc_sim::step_prologue(StepPhase& ph) {
     if (features && nc > 0) {
         pdfc::launch_warm_features(dm, qn, V[0], cvec, ncontacts, (int)std::min<size_t>(cap_contacts, nc), dc, dob, a,
                                    feat, stream);
-        if (mode == PDFC_WARM_LEARNED) pdfc::launch_mlp_forward(*pdfc_mlp_device(mlp), feat, nc, warm_out, stream);
+        if (mode == PDFC_WARM_LEARNED) {  // the previous step's reaction plus the network's change
+            pdfc::launch_mlp_forward(*pdfc_mlp_device(mlp), feat, nc, warm_out, stream);
+            pdfc::launch_mlp_add_previous(feat, nc, warm_out, stream);
+        }
     }
     if (nc == 0) mode = PDFC_WARM_PLAIN;
     if (mode == PDFC_WARM_PREVIOUS) {

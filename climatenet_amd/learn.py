@@ -5,12 +5,12 @@ library's own MLP kernels, and the comparison against the plain warm start.
     python -m climatenet_amd.learn train --data data.npz --out contact_mlp.npz
     python -m climatenet_amd.learn evaluate --model contact_mlp.npz
 
-Data: runs of the 20x20 sphere drape (jitter seeds 0..4, 200 steps, FP64) and of the
-three stacked 150x150 sheets (FP32 substitution + FP64 refinement, from the first
-contacts), every step iterated until the mean friction (Alart-Curnier) residual is
-below 1e-6 or the iteration cap is reached. A sample is one obstacle contact: the 12
-features at the start of its step and the final reaction per unit mass in the contact
-basis (pdfc_get_warm_data), with the residual its step reached.
+Data (the configured set): about 1e6 converged obstacle contacts from runs of the 20x20
+sphere drape (jitter seeds 0, 1, ..., 200 steps each) and of the three stacked 150x150
+sheets (from their first contacts), FP64, every step iterated until the mean friction
+(Alart-Curnier) residual is below 1e-6 or 1,000 iterations have run. A sample is one
+obstacle contact: the 12 features at the start of its step and the final reaction per
+unit mass in the contact basis (pdfc_get_warm_data), with the residual its step reached.
 """
 import argparse
 import json
@@ -24,6 +24,9 @@ from . import abi, native, scenes
 
 TOL = 1e-6
 CAP = 8000
+DATA_CAP = 1000          # iterations per step when generating the training data (at this cap no step of
+                         # either scene reaches 1e-6: the targets are the reactions after 1,000 iterations)
+DATA_SAMPLES = 1000000   # contacts in the training data
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL = os.path.join(HERE, "data", "contact_mlp.npz")
 
@@ -55,11 +58,36 @@ def run_for_data(spec, steps, precision=abi.FP64, start_steps=0, cap=CAP, tol=TO
     return np.concatenate(X), np.concatenate(Y), np.concatenate(R)
 
 
-def generate(small_seeds=5, stacked_steps=25, stacked_cap=4000, log=print):
-    parts = [run_for_data(scenes.sphere_drape_small(jitter_seed=s), 200, log=log) for s in range(small_seeds)]
-    parts.append(run_for_data(scenes.stacked_layers(), stacked_steps, abi.FP32_REFINE64, start_steps=40,
-                              cap=stacked_cap, log=log))
-    return tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
+def generate(samples=DATA_SAMPLES, small_seeds=5, cap=DATA_CAP, log=print):
+    """The configured training set: the 20x20 drape for `small_seeds` seeds (200 steps each),
+    then the stacked sheets from their first contacts until `samples` contacts are collected."""
+    parts = [run_for_data(scenes.sphere_drape_small(jitter_seed=s), 200, cap=cap, log=log) for s in range(small_seeds)]
+    have = sum(len(p[0]) for p in parts)
+    spec = scenes.stacked_layers()
+    sim = native.Simulation(spec)
+    sim.step(30)  # the sheets reach the sphere
+    q, v = sim.state()
+    sim.close()
+    sim = native.Simulation(spec, residual_tolerance=TOL, max_iterations=cap)
+    sim.set_state(q, v)
+    X, Y, R = [], [], []
+    t0 = time.perf_counter()
+    while have < samples:
+        sim.step(1)
+        if not sim.phase_stats()["contact_steps"]:
+            continue
+        x, y = sim.warm_data()
+        m, _ = sim.iteration_stats()
+        X.append(x)
+        Y.append(y)
+        R.append(np.full(len(x), m[-1], dtype=np.float32))
+        have += len(x)
+    log("  %s: %d samples over %d steps, %.1f s" % (spec.name, sum(len(x) for x in X), len(X),
+                                                    time.perf_counter() - t0))
+    sim.close()
+    parts.append((np.concatenate(X), np.concatenate(Y), np.concatenate(R)))
+    out = tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
+    return tuple(a[:samples] for a in out)
 
 
 def normalisation(X, Y):
@@ -73,9 +101,21 @@ def normalisation(X, Y):
     return norm
 
 
+def target(X, Y):
+    """What the network learns: the change of the converged reaction per unit mass from the
+    previous step's (features 6..8); the device adds the previous reaction back."""
+    return (Y - X[:, 6:9]).astype(np.float32)
+
+
+def predict(mlp, X):
+    """The warm-start reaction per unit mass the device uses: previous + predicted change."""
+    return mlp.forward(X) + X[:, 6:9]
+
+
 def train(X, Y, epochs=30, batch=512, lr=2e-3, seed=0, log=print):
+    T = target(X, Y)
     mlp = native.ContactMLP(seed=seed)
-    mlp.set_params(None, normalisation(X, Y))
+    mlp.set_params(None, normalisation(X, T))
     t0 = time.perf_counter()
     losses = []
     # step decay: three stages of the learning rate
@@ -83,10 +123,34 @@ def train(X, Y, epochs=30, batch=512, lr=2e-3, seed=0, log=print):
     for k, (r, e) in enumerate(stages):
         if e <= 0:
             continue
-        losses += list(mlp.train(X, Y, epochs=e, batch=batch, lr=r, seed=seed + k))
+        losses += list(mlp.train(X, T, epochs=e, batch=batch, lr=r, seed=seed + k))
     dt = time.perf_counter() - t0
     log("  trained %d epochs on %d samples in %.1f s, loss %.4g -> %.4g" % (epochs, len(X), dt, losses[0], losses[-1]))
     return mlp, np.array(losses), dt
+
+
+# output layer of the parameter layout (climatenet_amd/csrc/mlp.hpp): W4 is 64 x 16 row-major
+# at 9408, b4 16 entries after it (only the first 3 outputs are live)
+_W4, _B4 = slice(9408, 9408 + 64 * 16), slice(9408 + 64 * 16, 9408 + 64 * 16 + 16)
+
+
+def shrink(mlp, X, Y, grid=np.linspace(0.0, 1.0, 21)):
+    """Scales the network's predicted change by the alpha in [0, 1] that minimises the mean
+    relative error of previous + alpha * change on held-out contacts (X, Y): where the change
+    is not predictable from the features, alpha -> 0 and the learned warm start falls back to
+    the previous step's reaction. Returns alpha."""
+    d = mlp.forward(X)
+    prev = X[:, 6:9]
+    scale = np.linalg.norm(Y, axis=1).mean()
+    den = np.linalg.norm(Y, axis=1) + 0.1 * scale
+    errs = [float(np.mean(np.linalg.norm(prev + a * d - Y, axis=1) / den)) for a in grid]
+    alpha = float(grid[int(np.argmin(errs))])
+    p, z = mlp.params()
+    p = p.copy()
+    p[_W4] *= alpha
+    p[_B4] *= alpha
+    mlp.set_params(p, z)
+    return alpha
 
 
 def save(path, mlp):
@@ -138,13 +202,70 @@ def evaluate(spec, mlp, steps, warmup_steps=0, modes=(abi.WARM_PLAIN, abi.WARM_P
     return out
 
 
+def evaluate_drape40k(mlp, start=150, steps=2, cap=40000, tol=TOL, self_collision=True, log=print):
+    """The learned warm start on its configured workload: the 40k-vertex drape from its state at
+    step `start` (self-collision as configured), `steps` steps run to the tolerance with each
+    warm start from the same state: plain (the reference), previous (last step's converged
+    reactions), learned (the network), and exact (the plain run's converged reactions of the same
+    step, held as long as the residual keeps falling): the bound any reaction warm start of the
+    obstacle contacts can reach. Self contacts start plain in every mode."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = self_collision
+    base = native.Simulation(spec, record_iterations=False)
+    base.step(start)
+    q0, v0 = base.state()
+    base.close()
+    out = {"scene": "40k drape, %d steps from step %d, self-collision %s" % (steps, start, "on" if self_collision else "off"),
+           "tolerance": tol, "max_iterations": cap, "steps": steps}
+    plain_contacts, plain_states = [], []
+    for name, mode in (("plain", abi.WARM_PLAIN), ("previous", abi.WARM_PREVIOUS), ("learned", abi.WARM_LEARNED),
+                       ("exact", None)):
+        sim = native.Simulation(spec, residual_tolerance=tol, max_iterations=cap,
+                                warm_start=abi.WARM_PLAIN if mode is None else mode, warm_hold=cap,
+                                mlp=mlp if mode == abi.WARM_LEARNED else None)
+        sim.set_state(q0, v0)
+        it, conv, fb, t, res = 0, 0, 0, 0.0, []
+        for k in range(steps):
+            if mode is None:
+                sim.set_state(*plain_states[k])
+                sim.set_warm_reactions(plain_contacts[k], cap)
+            elif mode == abi.WARM_PLAIN:
+                plain_states.append(sim.state())
+            t0 = time.perf_counter()
+            sim.step(1)
+            t += time.perf_counter() - t0
+            ps = sim.phase_stats()
+            it += ps["iterations_to_tolerance"]
+            conv += ps["converged_steps"]
+            fb += ps["fallbacks"]
+            res.append(float(sim.iteration_stats()[0][-1]))
+            if mode == abi.WARM_PLAIN:
+                plain_contacts.append(sim.contacts())
+        out[name] = {"iterations_to_tolerance": it / steps, "converged_steps": conv, "fallbacks": fb,
+                     "final_residual": res, "seconds_per_step": round(t / steps, 3)}
+        if name == "plain":
+            out["final_state_plain"] = sim.state()
+        if name == "learned":
+            q, v = sim.state()
+            qp, vp = out["final_state_plain"]
+            out["learned"]["max_rel_diff_q_vs_plain"] = float(np.abs(q - qp).max() / np.abs(qp).max())
+        log("  %-8s iterations to %.0e: %.0f, converged %d/%d, fallbacks %d, %.2f s/step"
+            % (name, tol, it / steps, conv, steps, fb, t / steps))
+        sim.close()
+    del out["final_state_plain"]
+    P = out["plain"]["iterations_to_tolerance"]
+    for name in ("previous", "learned", "exact"):
+        out["iteration_ratio_plain_over_" + name] = round(P / max(1e-9, out[name]["iterations_to_tolerance"]), 4)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     g = sub.add_parser("generate")
     g.add_argument("--out", required=True)
+    g.add_argument("--samples", type=int, default=DATA_SAMPLES)
     g.add_argument("--small-seeds", type=int, default=5)
-    g.add_argument("--stacked-steps", type=int, default=25)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--out", default=MODEL)
@@ -157,7 +278,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "generate":
-        X, Y, R = generate(a.small_seeds, a.stacked_steps, log=log)
+        X, Y, R = generate(a.samples, a.small_seeds, log=log)
         np.savez(a.out, X=X, Y=Y, R=R)
         log("%d samples -> %s (residual reached: median %.2e, max %.2e)" % (len(X), a.out, np.median(R), R.max()))
     elif a.cmd == "train":
@@ -167,7 +288,7 @@ def main(argv=None):
         save(a.out, mlp)
         X, Y = d["X"], d["Y"]
         scale = np.linalg.norm(Y, axis=1).mean()
-        err = np.linalg.norm(mlp.forward(X) - Y, axis=1) / (np.linalg.norm(Y, axis=1) + 0.1 * scale)
+        err = np.linalg.norm(predict(mlp, X) - Y, axis=1) / (np.linalg.norm(Y, axis=1) + 0.1 * scale)
         log(json.dumps({"epochs": a.epochs, "samples": int(len(X)), "seconds": round(dt, 2),
                         "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                         "relative_error_median": float(np.median(err)),

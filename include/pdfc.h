@@ -68,8 +68,9 @@ enum pdfc_warm_start {
     PDFC_WARM_PLAIN = 0,     /* the reference: r^0 is the cone projection of the contact force at v^0 */
     PDFC_WARM_PREVIOUS = 1,  /* r^0 = the previous step's reaction of the same (obstacle, vertex) contact,
                                 plain where there was none */
-    PDFC_WARM_LEARNED = 2    /* r^0 predicted by the contact MLP (pdfc_set_mlp); the step restarts with the
-                                plain warm start when the friction residual stagnates */
+    PDFC_WARM_LEARNED = 2    /* r^0 = the previous step's reaction plus the change the contact MLP predicts
+                                (pdfc_set_mlp); the step restarts with the plain warm start when the
+                                friction residual stagnates */
 };
 
 /* Contact MLP (learned warm start; no counterpart in the reference). */
@@ -347,7 +348,8 @@ const char* pdfc_last_error(void);
  * [6..8] previous-step reaction of the same (obstacle, vertex) pair per unit mass, in B; [9] mu;
  * [10] gap (signed distance of q_n,i to the obstacle surface); [11] normal component of the
  * contact force at v^0 per unit mass (the plain warm start's own estimate). The network predicts
- * the converged reaction per unit mass in B; the step uses m_i times it, projected on the cone. */
+ * the change of the converged reaction per unit mass in B from the previous step's (features
+ * 6..8); the step uses m_i times (previous + predicted change), projected on the cone. */
 typedef struct pdfc_mlp pdfc_mlp;
 
 /* 12 -> 64 -> 64 -> 64 -> 3, SiLU, Glorot-uniform weights from seed, identity normalisation. */

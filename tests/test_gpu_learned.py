@@ -1,64 +1,44 @@
-"""GPU: what the trained contact network (climatenet_amd/data/contact_mlp.npz, trained on the
-GPU by the library's own kernels on converged contacts of the 20x20 drape and the stacked
-sheets) gains over the plain warm start, in iterations to a 1e-6 friction residual.
+"""GPU: the learned warm start on its configured workload, the 40k-vertex drape.
 
-Measured on three steps of the 20x20 drape (jitter seed 7, a trajectory not in the
-training data) from the same state: plain, learned, and the exact converged reactions
-of the same step held for 1000 iterations (the bound on what any reaction warm start
-can give). The printed numbers are the result; the assertions are the guarantees:
-the learned start converges to the plain start's solution (no accuracy loss), costs
-at most the fallback's stagnation window more than plain, and the bound shows the
-warm-start path itself works.
+The committed network (climatenet_amd/data/contact_mlp.npz) was trained on the GPU by the
+library's own kernels on the configured data set: 1e6 obstacle contacts from runs of the 20x20
+drape and of the three stacked sheets, each step iterated to a 1e-6 friction residual or 1,000
+iterations (climatenet_amd/learn.py). It predicts the change of each contact's reaction from the
+previous step's; the change it predicts is scaled by the factor that fits held-out contacts best.
+
+From the drape's state at step 150 (self-collision on, as configured), one step is run to a
+1e-6 mean Alart-Curnier residual (coulomb_util.cpp:101-152) with each warm start: plain (the
+reference), previous (last step's converged reactions), learned, and exact (the plain run's
+converged reactions of the same step, held while the residual keeps falling), the bound of any
+warm start of the obstacle reactions. The printed numbers are the result; the assertions are the
+guarantees: every start reaches the tolerance, the learned start ends at the plain start's
+solution (no accuracy loss) and costs at most 2% more iterations than plain, and the exact
+reactions do save iterations (the warm-start path works).
 """
 import os
 
 import numpy as np
 import pytest
 
-from climatenet_amd import abi, learn, scenes
+from climatenet_amd import learn
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-6
-CAP = 8000
 
-
-def rel_err(a, b):
-    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
-
-
-def test_committed_network_against_plain_and_exact(gpu):
+def test_learned_warm_start_on_the_40k_drape(gpu):
     assert os.path.exists(learn.MODEL)
     mlp = learn.load(learn.MODEL)
-    spec = scenes.sphere_drape_small(jitter_seed=7)
-    base = gpu.Simulation(spec)
-    base.step(60)
-    q, v = base.state()
-    plain = gpu.Simulation(spec, residual_tolerance=TOL, max_iterations=CAP)
-    learned = gpu.Simulation(spec, residual_tolerance=TOL, max_iterations=CAP, warm_start=abi.WARM_LEARNED, mlp=mlp)
-    exact = gpu.Simulation(spec, residual_tolerance=TOL, max_iterations=CAP)
-    plain.set_state(q, v)
-    learned.set_state(q, v)
-    rows = []
-    for s in range(3):
-        q0, v0 = plain.state()
-        plain.step(1)
-        learned.step(1)  # its own trajectory: its previous-step reactions are its own
-        exact.set_state(q0, v0)
-        exact.set_warm_reactions(plain.contacts(), 1000)
-        exact.step(1)
-        pp, pl, pe = plain.phase_stats(), learned.phase_stats(), exact.phase_stats()
-        for ps in (pp, pl, pe):
-            assert ps["contact_steps"] == 1 and ps["converged_steps"] == 1, ps
-        rows.append((pp["iterations_to_tolerance"], pl["iterations_to_tolerance"], pe["iterations_to_tolerance"],
-                     pl["fallbacks"]))
-        # no accuracy loss: both at the tolerance, the same fixed point every step
-        qp, vp = plain.state()
-        ql, vl = learned.state()
-        assert rel_err(ql, qp) < 1e-5 and rel_err(vl, vp) < 1e-2
-    P, L, E = (np.mean([r[k] for r in rows]) for k in range(3))
-    print("iterations to %.0e: plain %.0f, learned %.0f (fallbacks %d), exact reactions held 1000: %.0f"
-          % (TOL, P, L, sum(r[3] for r in rows), E))
-    print("learned / plain = %.3f, exact / plain = %.3f" % (L / P, E / P))
-    assert L <= P + 1000
-    assert E < 0.6 * P
+    print("40k drape: 150 steps, then one step per warm start to 1e-6", flush=True)
+    res = learn.evaluate_drape40k(mlp, start=150, steps=1, cap=40000)
+    P = res["plain"]["iterations_to_tolerance"]
+    L = res["learned"]["iterations_to_tolerance"]
+    E = res["exact"]["iterations_to_tolerance"]
+    V = res["previous"]["iterations_to_tolerance"]
+    print("iterations to 1e-6: plain %.0f, previous %.0f, learned %.0f, exact reactions %.0f" % (P, V, L, E))
+    print("plain / learned = %.4f (target 1.5), plain / previous = %.4f, plain / exact = %.4f (bound)"
+          % (P / L, P / V, P / E))
+    for mode in ("plain", "previous", "learned", "exact"):
+        assert res[mode]["converged_steps"] == 1, (mode, res[mode])
+    assert res["learned"]["max_rel_diff_q_vs_plain"] < 1e-9
+    assert L <= 1.02 * P
+    assert E < P
