@@ -30,7 +30,8 @@ def measured_traffic(kernel, info, name="drape40k"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     profiles/<name>_pmc_traffic.json (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
     the same counter's reading for a device copy of known size), when it was measured on this
-    workload, kernel and level structure; otherwise None."""
+    workload, kernel and level structure (a summary without levels per launch was measured with
+    one launch per level); otherwise None."""
     path = os.path.join(ROOT, "profiles", "%s_pmc_traffic.json" % name)
     if not os.path.exists(path):
         return None
@@ -38,7 +39,9 @@ def measured_traffic(kernel, info, name="drape40k"):
         d = json.load(f)
     if d.get("kernel") != kernel or d.get("levels_swept") != info["n_levels_swept"] or \
             d.get("n_vertices") != info["n_vertices"] or d.get("nnz_F") != info["nnz_F"] or \
-            d.get("top_fold") != info["top_fold"] or d.get("top_chunks") != info["top_chunks"]:
+            d.get("top_fold") != info["top_fold"] or d.get("top_chunks") != info["top_chunks"] or \
+            d.get("parts_backward", [1] * info["n_levels_swept"]) != info["parts_backward"][:info["launches_backward"]] or \
+            d.get("parts_forward", [1] * info["n_levels_swept"]) != info["parts_forward"][:info["launches_forward"]]:
         return None
     return d.get("traffic_bytes_per_launch")
 
@@ -120,11 +123,12 @@ def kernel_roofline(sim, info, timing_steps, traffic_name):
     sim.step(timing_steps)
     kt = sim.kernel_times()
     sim.set_kernel_timing(False)
-    per_sweep = max(1, info["n_levels_swept"])
+    per_fwd = max(1, info["launches_forward"])
+    per_bwd = max(1, info["launches_backward"])
     fam = {
         "k_local": (kt["local_us"], info["bytes_local"], kt["local_launches"]),
-        "k_forward": (kt["forward_us"], info["bytes_forward"] / per_sweep, kt["forward_launches"]),
-        "k_backward": (kt["backward_us"], info["bytes_backward"] / per_sweep, kt["backward_launches"]),
+        "k_forward": (kt["forward_us"], info["bytes_forward"] / per_fwd, kt["forward_launches"]),
+        "k_backward": (kt["backward_us"], info["bytes_backward"] / per_bwd, kt["backward_launches"]),
         "k_top": (kt["top_us"], info["bytes_top"], kt["top_launches"]),
     }
     dom = max(fam, key=lambda k: fam[k][0] * fam[k][2])
@@ -205,6 +209,8 @@ def bench_batched(args):
                    "graph_replay": not args.no_graph, "create_seconds": round(create_s, 2),
                    "tuning": sim.tuning(), "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
+                   "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
+                   "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
                    "top_chunks": info["top_chunks"], "top_fold": info["top_fold"], "nnz_F": info["nnz_F"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "sequential": None if args.no_sequential else {
@@ -441,6 +447,8 @@ def main():
                    "top_block_rows": info["top_m"], "contacts_last_step": info["n_contacts"],
                    "waves_forward": info["waves_forward"][:info["n_levels_swept"]],
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
+                   "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
+                   "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
                    "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"], "top_fold": info["top_fold"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,

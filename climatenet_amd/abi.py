@@ -162,6 +162,10 @@ class Info(C.Structure):
         ("lhs_seconds", C.c_double),
         ("steps_done", C.c_int64),
         ("top_fold", C.c_int32),
+        ("launches_forward", C.c_int32),
+        ("launches_backward", C.c_int32),
+        ("parts_forward", C.c_int32 * 32),
+        ("parts_backward", C.c_int32 * 32),
         ("reserved_info", C.c_int32),
     ]
 

@@ -1112,6 +1112,200 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
     backward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Y, X, lds, part, fo);
 }
 
+// ---------------------------------------------------------------- merged levels
+// Consecutive levels in one launch, one 16-wave workgroup per group: the supernodes of the
+// range under one root (their highest ancestor inside the range). A group's supernode reads
+// its children (forward) or ancestors (backward) only inside the group or outside the range,
+// where an earlier launch wrote them, so the levels are separated by the workgroup barrier
+// alone and every group's data stays in its CU's cache hierarchy. At each level the group's
+// tasks run W waves each, 16 / W at a time ("slots"); a slot computes exactly what the level
+// kernel's task with W waves computes, in the same operation order, so results are
+// bit-identical to the level-by-level sweep with the same waves per task. Idle slots (past
+// the level's last task) take part in every barrier.
+
+// forward_task for one slot: W waves (runtime), wv = wave within the slot, task < 0 idle.
+template <typename T>
+__device__ __forceinline__ void forward_slot(const DevFactor& Fa, int task, int W, int wv, const double* __restrict__ Bv,
+                                             double* __restrict__ Y, double* lds, double (*part)[3][64]) {
+    TaskDesc d{};
+    if (task >= 0) d = Fa.desc[task];
+    const int ns = d.ns, f = d.f, c0 = d.c0, r0 = d.r0;
+    const int n = Fa.n;
+    double* bx = lds;
+    double* by = lds + ns;
+    double* bz = lds + 2 * ns;
+    const int lane = threadIdx.x & 63, tid = wv * 64 + lane, nthr = 64 * W;
+    const int* pp = Fa.pull_ptr + d.pull_base;
+    const int my_row = r0 + lane;
+    const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
+    double pre[kPrefetch];
+#pragma unroll
+    for (int q = 0; q < kPrefetch; ++q) {
+        const int j = wv + q * W;
+        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * f]) : 0.;
+    }
+    for (int t = tid; t < ns; t += nthr) {
+        double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
+        pull_entries<true, 2>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
+        bx[t] = x;
+        by[t] = y;
+        bz[t] = z;
+    }
+    double ux = 0., uy = 0., uz = 0.;
+    if (wv == 0 && my_row >= ns && my_row < f) pull_entries<false, 2>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
+    __syncthreads();
+    double ax = 0., ay = 0., az = 0.;
+    if (my_row < f) {
+#pragma unroll
+        for (int q = 0; q < kPrefetch; ++q) {
+            const int j = wv + q * W;
+            if (j < ns) {
+                ax = fma(pre[q], bx[j], ax);
+                ay = fma(pre[q], by[j], ay);
+                az = fma(pre[q], bz[j], az);
+            }
+        }
+#pragma unroll 8
+        for (int j = wv + kPrefetch * W; j < ns; j += W) {
+            double fv = as_d(Fs[(size_t)j * f]);
+            ax = fma(fv, bx[j], ax);
+            ay = fma(fv, by[j], ay);
+            az = fma(fv, bz[j], az);
+        }
+    }
+    if (W > 1) {
+        part[wv][0][lane] = ax;
+        part[wv][1][lane] = ay;
+        part[wv][2][lane] = az;
+        __syncthreads();
+        if (wv == 0) {
+            ax = part[0][0][lane];
+            ay = part[0][1][lane];
+            az = part[0][2][lane];
+            for (int w = 1; w < W; ++w) {
+                ax += part[w][0][lane];
+                ay += part[w][1][lane];
+                az += part[w][2][lane];
+            }
+        }
+    }
+    if (wv == 0 && my_row < f) {
+        if (my_row < ns) {
+            Y[c0 + my_row] = ax;
+            Y[n + c0 + my_row] = ay;
+            Y[2 * n + c0 + my_row] = az;
+        } else {
+            double* u = Fa.U + 3 * (d.uoff + my_row - ns);
+            u[0] = ax + ux;
+            u[1] = ay + uy;
+            u[2] = az + uz;
+        }
+    }
+}
+
+// backward_task for one slot (no folded top-block finish: the top rows are in X).
+template <typename T>
+__device__ __forceinline__ void backward_slot(const DevFactor& Fa, int task, int W, int wv, const double* __restrict__ Y,
+                                              double* __restrict__ X, double* lds, double (*part)[3][64]) {
+    TaskDesc d{};
+    if (task >= 0) d = Fa.desc[task];
+    const int ns = d.ns, nb = d.nb, f = d.f, c0 = d.c0, j0 = d.r0;
+    const int n = Fa.n;
+    double* gx = lds;
+    double* gy = lds + f;
+    double* gz = lds + 2 * f;
+    const int lane = threadIdx.x & 63, tid = wv * 64 + lane, nthr = 64 * W;
+    const int j = j0 + lane;
+    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
+    double pre[kPrefetch];
+#pragma unroll
+    for (int q = 0; q < kPrefetch; ++q) {
+        const int i = wv + q * W;
+        pre[q] = (j < ns && i < f) ? as_d(Ft[(size_t)i * ns]) : 0.;
+    }
+    for (int t = tid; t < ns; t += nthr) {
+        gx[t] = Y[c0 + t];
+        gy[t] = Y[n + c0 + t];
+        gz[t] = Y[2 * n + c0 + t];
+    }
+    const int* R = Fa.rows + d.roff;
+    for (int t = tid; t < nb; t += nthr) {
+        const int r = R[t];
+        gx[ns + t] = -X[r];
+        gy[ns + t] = -X[n + r];
+        gz[ns + t] = -X[2 * n + r];
+    }
+    __syncthreads();
+    double ax = 0., ay = 0., az = 0.;
+    if (j < ns) {
+#pragma unroll
+        for (int q = 0; q < kPrefetch; ++q) {
+            const int i = wv + q * W;
+            if (i < f) {
+                ax = fma(pre[q], gx[i], ax);
+                ay = fma(pre[q], gy[i], ay);
+                az = fma(pre[q], gz[i], az);
+            }
+        }
+#pragma unroll 8
+        for (int i = wv + kPrefetch * W; i < f; i += W) {
+            double fv = as_d(Ft[(size_t)i * ns]);
+            ax = fma(fv, gx[i], ax);
+            ay = fma(fv, gy[i], ay);
+            az = fma(fv, gz[i], az);
+        }
+    }
+    if (W > 1) {
+        part[wv][0][lane] = ax;
+        part[wv][1][lane] = ay;
+        part[wv][2][lane] = az;
+        __syncthreads();
+        if (wv == 0) {
+            ax = part[0][0][lane];
+            ay = part[0][1][lane];
+            az = part[0][2][lane];
+            for (int w = 1; w < W; ++w) {
+                ax += part[w][0][lane];
+                ay += part[w][1][lane];
+                az += part[w][2][lane];
+            }
+        }
+    }
+    if (wv == 0 && j < ns) {
+        X[c0 + j] = ax;
+        X[n + c0 + j] = ay;
+        X[2 * n + c0 + j] = az;
+    }
+}
+
+// The control flow below is uniform across the workgroup (group offsets and level info are
+// the same for every wave), so every wave meets the same barriers.
+template <typename T, bool kForward>
+__global__ void __launch_bounds__(1024) k_sweep_merged(DevFactor Fa, MergedRange R, const double* __restrict__ In,
+                                                       double* __restrict__ Out) {
+    extern __shared__ double lds[];
+    __shared__ double part[16][3][64];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int* gp = R.ptr + (size_t)blockIdx.x * (R.nl + 1);
+    for (int k = 0; k < R.nl; ++k) {
+        const int W = R.info[2 * k], stride = R.info[2 * k + 1];
+        const int per = 16 / W;
+        const int slot = wave / W, wv = wave - slot * W;
+        const int a = gp[k], b = gp[k + 1];
+        for (int base = a; base < b; base += per) {
+            const int t = base + slot;
+            const int task = t < b ? R.task[t] : -1;
+            if (kForward)
+                forward_slot<T>(Fa, task, W, wv, In, Out, lds + (size_t)slot * stride, part + slot * W);
+            else
+                backward_slot<T>(Fa, task, W, wv, In, Out, lds + (size_t)slot * stride, part + slot * W);
+            // tasks of one level are independent; a slot's LDS and partial sums are reused only
+            // after its own waves passed the slot's last barrier (W = 1: one wave, in order)
+        }
+        __syncthreads();  // this level's results are read by the next level's tasks
+    }
+}
+
 // ---------------------------------------------------------------- top block
 __global__ void k_top_gather(DevTop T, const double* __restrict__ U, const double* __restrict__ B, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1436,6 +1630,22 @@ void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, co
         backward_dispatch<float>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
     else
         backward_dispatch<double>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
+}
+void launch_sweep_merged(const DevFactor& F, bool fp32, bool forward, const MergedRange& R, const double* In,
+                         double* Out, hipStream_t st) {
+    if (R.groups <= 0) return;
+    const dim3 g(R.groups), b(1024);
+    if (forward) {
+        if (fp32)
+            hipLaunchKernelGGL((k_sweep_merged<float, true>), g, b, R.lds_bytes, st, F, R, In, Out);
+        else
+            hipLaunchKernelGGL((k_sweep_merged<double, true>), g, b, R.lds_bytes, st, F, R, In, Out);
+    } else {
+        if (fp32)
+            hipLaunchKernelGGL((k_sweep_merged<float, false>), g, b, R.lds_bytes, st, F, R, In, Out);
+        else
+            hipLaunchKernelGGL((k_sweep_merged<double, false>), g, b, R.lds_bytes, st, F, R, In, Out);
+    }
 }
 void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
                 bool finish) {

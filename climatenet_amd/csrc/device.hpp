@@ -302,6 +302,20 @@ void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, co
                            int max_front, int waves, hipStream_t st, const TopFold* fold = nullptr);
 // Heuristic wave count: chains of about 8 loads, at most 16 waves.
 int default_waves(int kmax);
+
+// Consecutive levels of one sweep in a single launch, one 16-wave workgroup per group (the
+// supernodes of the range under one root inside the range).
+constexpr int kMergedLdsMax = 40960;  // dynamic LDS bytes of a merged launch (24 KiB static besides)
+struct MergedRange {
+    int nl;            // levels in the range, in sweep order
+    int groups;        // workgroups
+    const int* ptr;    // groups x (nl + 1): group g's tasks at its k-th level are
+    const int* task;   //   task[ptr[g (nl + 1) + k] .. ptr[g (nl + 1) + k + 1]) (indices into DevFactor::desc)
+    const int* info;   // 2 nl: waves per task (1, 2, 4, 8, 16) and LDS doubles per slot, per level
+    size_t lds_bytes;  // (16 / waves) x slot doubles x 8 at the widest level, <= kMergedLdsMax
+};
+void launch_sweep_merged(const DevFactor& F, bool fp32, bool forward, const MergedRange& R, const double* In,
+                         double* Out, hipStream_t st);
 // Dense top block of the substitution (see TopPlan).
 struct DevTop {
     int m;

@@ -358,4 +358,37 @@ void plan_sweeps(const Factor& F, int chunk, int top_level, SweepPlan& fwd, Swee
     }
 }
 
+void plan_merged(const Factor& F, const SweepPlan& P, bool forward, int first, int nl, MergedGroups& out) {
+    const int L = (int)P.level_ptr.size() - 1;
+    if (first < 0 || nl < 1 || first + nl > L) throw std::invalid_argument("merged range outside the sweep");
+    auto level_of = [&](int l) { return forward ? l : L - 1 - l; };
+    const int hi = std::max(level_of(first), level_of(first + nl - 1));
+    std::vector<int> gid(F.ns.size(), -1);  // root supernode -> group
+    std::vector<std::vector<std::vector<int>>> gt;  // group -> level in range -> tasks
+    for (int k = 0; k < nl; ++k)
+        for (int t = P.level_ptr[first + k]; t < P.level_ptr[first + k + 1]; ++t) {
+            int r = P.task_snode[t];
+            while (F.parent[r] >= 0 && F.level[F.parent[r]] <= hi) r = F.parent[r];
+            if (gid[r] < 0) {
+                gid[r] = (int)gt.size();
+                gt.emplace_back(nl);
+            }
+            gt[gid[r]][k].push_back(t);
+        }
+    out = MergedGroups();
+    out.groups = (int)gt.size();
+    out.tmax.assign(nl, 0);
+    int acc = 0;
+    for (const auto& g : gt)
+        for (int k = 0; k <= nl; ++k) {
+            out.ptr.push_back(acc);
+            if (k < nl) {
+                acc += (int)g[k].size();
+                out.tmax[k] = std::max(out.tmax[k], (int)g[k].size());
+            }
+        }
+    for (const auto& g : gt)
+        for (int k = 0; k < nl; ++k) out.task.insert(out.task.end(), g[k].begin(), g[k].end());
+}
+
 }  // namespace pdfc

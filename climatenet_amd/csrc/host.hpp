@@ -92,6 +92,20 @@ struct SweepPlan {
 // Levels below `top_level` only; supernodes at levels >= top_level form the dense top block.
 void plan_sweeps(const Factor& F, int chunk, int top_level, SweepPlan& fwd, SweepPlan& bwd);
 
+// Groups of one merged launch over the sweep levels [first, first + nl) of P (forward: leaves
+// first; backward: root side first). A group is the supernodes of the range under one root,
+// their highest ancestor whose level is inside the range; groups are numbered in order of
+// first appearance in P's task order and keep P's task order at each level. Every child
+// (forward) or ancestor (backward) a task depends on is then in the task's group at an
+// earlier level of the range, or outside the range.
+struct MergedGroups {
+    int groups = 0;
+    std::vector<int> ptr;    // groups x (nl + 1): group g's tasks at level k are task[ptr[g (nl+1) + k] ..
+    std::vector<int> task;   //   ptr[g (nl+1) + k + 1]) (indices into P's task list)
+    std::vector<int> tmax;   // nl: most tasks of one group at each level
+};
+void plan_merged(const Factor& F, const SweepPlan& P, bool forward, int first, int nl, MergedGroups& out);
+
 // Dense top block: the columns T of all supernodes at levels >= level. Both
 // substitutions over T collapse into x_T = S^-1 b~_T with S the Schur complement
 // of K on T (S^-1 = (K^-1)_TT), where b~_T subtracts the update vectors of the

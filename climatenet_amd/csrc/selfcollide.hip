@@ -586,6 +586,9 @@ __device__ __forceinline__ P3 self_reaction(const DevSelf& S, int s, P3 bn, P3 b
 // the self-contact forces of the involved vertices (current and next buffers) and each
 // contact's previous and new force live in LDS; each thread loads its contact's read-only data
 // (basis, vertices, masses, F and the obstacle reaction at both vertices, residual inputs) once.
+// The per-vertex groups of every level (their slots and entries, 16-bit) are copied to LDS
+// with those loads, and each thread's involved vertices for the right-hand side are read there
+// too, so no global load waits inside the level loop or before the final update.
 // A level is then its updates from LDS, a barrier, the per-vertex applies in LDS, a barrier.
 // Same operations in the same order as self_contact_update / self_group_apply /
 // self_slot_finalize, so the results are those of the other paths bit for bit.
@@ -594,6 +597,10 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
                                                                      const double* __restrict__ mass, StepArgs a,
                                                                      int iteration, double* __restrict__ rhs) {
     __shared__ double sh[6 * (kSelfLdsMaxSlots + kSelfLdsMaxContacts)];
+    // groups of all levels: at most one per (contact, side), so 2 kSelfLdsMaxContacts
+    __shared__ unsigned short g_slot[2 * kSelfLdsMaxContacts];
+    __shared__ unsigned short g_ent[2 * kSelfLdsMaxContacts + 1];
+    __shared__ unsigned short e_code[2 * kSelfLdsMaxContacts];  // contact * 2 + (sign > 0)
     double* scf_c = sh;                                  // 3 per slot
     double* scf_n = sh + 3 * kSelfLdsMaxSlots;           // 3 per slot
     double* c_rem = sh + 6 * kSelfLdsMaxSlots;           // 3 per contact
@@ -602,6 +609,20 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     for (int k = t; k < 3 * S.nslots; k += kSelfFusedThreads) {
         scf_c[k] = S.SCFc[k];
         scf_n[k] = S.SCFn[k];
+    }
+    const int G = S.grp_ptr[nlev];
+    for (int k = t; k <= G; k += kSelfFusedThreads) {
+        if (k < G) g_slot[k] = (unsigned short)S.grp_slot[k];
+        g_ent[k] = (unsigned short)S.ent_ptr[k];
+    }
+    for (int e = t; e < 2 * S.nsc; e += kSelfFusedThreads)
+        e_code[e] = (unsigned short)(2 * S.ent_contact[e] + (S.ent_sign[e] > 0 ? 1 : 0));
+    static_assert(kSelfLdsMaxSlots <= 2 * kSelfFusedThreads, "two involved vertices per thread");
+    int vv[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int sl = t + q * kSelfFusedThreads;
+        vv[q] = sl < S.nslots ? S.slot_vertex[sl] : 0;
     }
     const int s = t;
     const bool mine = s < S.nsc;
@@ -650,14 +671,15 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         __syncthreads();
         const int g1 = S.grp_ptr[l + 1];
         for (int g = S.grp_ptr[l] + t; g < g1; g += kSelfFusedThreads) {
-            const int slot = S.grp_slot[g];
+            const int slot = g_slot[g];
             P3 cur = {scf_c[3 * slot], scf_c[3 * slot + 1], scf_c[3 * slot + 2]};
             P3 nxt = {scf_n[3 * slot], scf_n[3 * slot + 1], scf_n[3 * slot + 2]};
-            for (int e = S.ent_ptr[g]; e < S.ent_ptr[g + 1]; ++e) {
-                const int c = S.ent_contact[e];
+            for (int e = g_ent[g]; e < g_ent[g + 1]; ++e) {
+                const int code = e_code[e];
+                const int c = code >> 1;
                 const P3 prev = {c_rem[3 * c], c_rem[3 * c + 1], c_rem[3 * c + 2]};
                 const P3 add = {c_add[3 * c], c_add[3 * c + 1], c_add[3 * c + 2]};
-                if (S.ent_sign[e] > 0) {
+                if (code & 1) {
                     cur = psub(cur, prev);
                     nxt = padd(nxt, add);
                 } else {
@@ -671,13 +693,16 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         __syncthreads();
     }
     if (mine) st_slot(S.addv, s, P3{c_add[3 * s], c_add[3 * s + 1], c_add[3 * s + 2]});
-    for (int sl = t; sl < S.nslots; sl += kSelfFusedThreads) {
-        const int vv = S.slot_vertex[sl];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int sl = t + q * kSelfFusedThreads;
+        if (sl >= S.nslots) break;
+        const int w = vv[q];
         const P3 c = ld_slot(S.CFn, sl);
         const P3 sc = {scf_n[3 * sl], scf_n[3 * sl + 1], scf_n[3 * sl + 2]};
-        rhs[vv] += c.x + sc.x;
-        rhs[n + vv] += c.y + sc.y;
-        rhs[2 * n + vv] += c.z + sc.z;
+        rhs[w] += c.x + sc.x;
+        rhs[n + w] += c.y + sc.y;
+        rhs[2 * n + w] += c.z + sc.z;
         st_slot(S.SCFn, sl, sc);
         st_slot(S.CFc, sl, P3{0., 0., 0.});
         st_slot(S.SCFc, sl, P3{0., 0., 0.});

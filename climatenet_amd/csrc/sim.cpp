@@ -154,7 +154,20 @@ struct pdfc_sim {
     int* task_buf = nullptr;
     pdfc::TaskDesc* desc_buf = nullptr;
     std::vector<int> fwd_waves, bwd_waves;  // waves per task, per level (autotuned)
-    // dependency-driven sweeps (one launch per sweep), chosen by the autotuner when faster
+    // launches of each sweep: consecutive levels per launch, in sweep order (1: the level
+    // kernel; more: one merged launch, a workgroup per subtree group); autotuned
+    std::vector<int> fwd_parts, bwd_parts;
+    struct MergedLaunch {
+        int first;              // first level of the range (sweep order)
+        pdfc::MergedRange R;    // R.nl == 1: level kernel
+    };
+    std::vector<MergedLaunch> fwd_launches, bwd_launches;
+    int* merged_buf = nullptr;
+    bool build_merged(bool choose_waves);
+    // the top block's finish runs in the first backward launch (a level kernel)
+    bool folded() const {
+        return use_top && top_fold && !bwd_launches.empty() && bwd_launches[0].R.nl == 1;
+    }
     void apply_pinned_tuning(const char* spec);
     double autotune_ms = 0.;
     void autotune_waves();
@@ -866,6 +879,89 @@ void pdfc_sim::upload_sweeps(int top_level) {
     desc_buf = upload(df);
     dfwd.desc = desc_buf;
     dbwd.desc = desc_buf + nf;
+    const size_t L = fwd.level_ptr.size() - 1;
+    fwd_parts.assign(L, 1);
+    bwd_parts.assign(L, 1);
+    build_merged(false);
+}
+
+// Launch lists of both sweeps from fwd_parts / bwd_parts. A range of several levels becomes
+// one merged launch over the groups of pdfc::plan_merged. With choose_waves the
+// waves per task of a merged level are set so that a group's tasks of that level run at
+// once (16 / waves slots), widened while the slots' LDS exceeds the launch's budget; else
+// the given waves are used. Returns false when a level does not fit the LDS budget.
+bool pdfc_sim::build_merged(bool choose_waves) {
+    if (merged_buf) free_ptr(merged_buf);
+    merged_buf = nullptr;
+    const int L = (int)fwd.level_ptr.size() - 1;
+    std::vector<int> host;  // every merged range's ptr, task and info arrays
+    struct Off {
+        size_t ptr, task, info;
+    };
+    std::vector<Off> fix;
+    bool ok = true;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool forward = pass == 0;
+        const std::vector<int>& parts = forward ? fwd_parts : bwd_parts;
+        const pdfc::SweepPlan& P = forward ? fwd : bwd;
+        std::vector<int>& waves = forward ? fwd_waves : bwd_waves;
+        std::vector<MergedLaunch>& out = forward ? fwd_launches : bwd_launches;
+        out.clear();
+        out.reserve(parts.size());
+        int first = 0;
+        for (int nl : parts) {
+            MergedLaunch ml{first, {}};
+            ml.R.nl = nl;
+            if (nl > 1) {
+                pdfc::MergedGroups mg;
+                pdfc::plan_merged(fac, P, forward, first, nl, mg);
+                const int G = mg.groups;
+                Off o{host.size(), 0, 0};
+                host.insert(host.end(), mg.ptr.begin(), mg.ptr.end());
+                o.task = host.size();
+                host.insert(host.end(), mg.task.begin(), mg.task.end());
+                o.info = host.size();
+                size_t lds = 0;
+                for (int k = 0; k < nl; ++k) {
+                    const int l = first + k;
+                    const int stride = 3 * P.level_kmax[l];  // forward: 3 ns, backward: 3 (ns + nb)
+                    if (choose_waves) {
+                        int slots = 1;
+                        while (slots < 16 && slots < mg.tmax[k]) slots *= 2;
+                        int W = 16 / slots;
+                        while (W < 16 && (size_t)(16 / W) * stride * 8 > (size_t)pdfc::kMergedLdsMax) W *= 2;
+                        waves[l] = W;
+                    }
+                    const size_t need = (size_t)(16 / waves[l]) * stride * 8;
+                    if (need > (size_t)pdfc::kMergedLdsMax) ok = false;
+                    lds = std::max(lds, need);
+                    host.push_back(waves[l]);
+                    host.push_back(stride);
+                }
+                ml.R.groups = G;
+                ml.R.lds_bytes = lds;
+                out.push_back(ml);
+                fix.push_back(o);
+            } else {
+                out.push_back(ml);
+            }
+            first += nl;
+        }
+        if (first != L) throw std::logic_error("sweep partition does not cover the levels");
+    }
+    // device copies, then the ranges' pointers into them (fix holds the merged ranges in
+    // launch-list order: forward first, then backward)
+    if (!host.empty()) merged_buf = upload(host);
+    size_t q = 0;
+    for (std::vector<MergedLaunch>* out : {&fwd_launches, &bwd_launches})
+        for (MergedLaunch& ml : *out)
+            if (ml.R.nl > 1) {
+                const Off& o = fix[q++];
+                ml.R.ptr = merged_buf + o.ptr;
+                ml.R.task = merged_buf + o.task;
+                ml.R.info = merged_buf + o.info;
+            }
+    return ok;
 }
 
 
@@ -917,6 +1013,28 @@ void pdfc_sim::apply_pinned_tuning(const char* spec) {
             throw std::invalid_argument("PDFC_TUNE: symmetric tiles are off (PDFC_TOP_SYM=0, or more than 128 row blocks)");
         set_top_mode(t[0]);
     }
+    // g / h: levels per forward / backward launch in sweep order (a partition of the levels)
+    std::vector<int> g, h;
+    const bool pg = parse_list('g', g), ph = parse_list('h', h);
+    const int L = (int)fwd_waves.size();
+    auto check_parts = [&](const std::vector<int>& p, const char* what) {
+        int sum = 0;
+        for (int x : p) {
+            if (x < 1) throw std::invalid_argument(std::string("PDFC_TUNE: ") + what + " parts must be >= 1");
+            sum += x;
+        }
+        if (sum != L) throw std::invalid_argument(std::string("PDFC_TUNE: ") + what + " parts must cover the levels");
+    };
+    if (pg) {
+        check_parts(g, "forward");
+        fwd_parts = g;
+    }
+    if (ph) {
+        check_parts(h, "backward");
+        bwd_parts = h;
+    }
+    if (!build_merged(false))
+        throw std::invalid_argument("PDFC_TUNE: a merged level needs more LDS than a launch has; give it more waves");
 }
 
 // Chooses the waves per task of every substitution level by timing each level alone
@@ -1063,6 +1181,46 @@ void pdfc_sim::autotune_waves() {
             }
             waves[l] = best_w;
         }
+    }
+    // Merged launches (PDFC_MERGE=1 offers them): every partition of a sweep's levels into at
+    // most three ranges of consecutive levels, a range of several levels running as one launch
+    // with the waves per task that start a group's tasks of a level at once. One sweep at a
+    // time, the other fixed; a partition replaces the level-by-level sweep only when clearly
+    // faster. Off by default: on the 40k drape, the 64-cloth batch and the mesh drape no
+    // partition was faster (a group's levels run on one CU, whose load throughput bounds them),
+    // and the search takes the 64-cloth batch's tuning from 290 to 700 ms.
+    const char* me = std::getenv("PDFC_MERGE");
+    const bool try_merge = me && std::atoi(me) != 0;
+    for (int pass = 0; pass < 2 && try_merge; ++pass) {
+        std::vector<int>& parts = pass == 0 ? fwd_parts : bwd_parts;
+        std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
+        const int L = (int)waves.size();
+        if (L < 2) continue;
+        const std::vector<int> keep_parts = parts, keep_waves = waves;
+        build_merged(false);
+        float best = solve_ms();
+        std::vector<int> best_parts = parts, best_waves = waves;
+        std::vector<std::vector<int>> seen{keep_parts};
+        for (int c1 = 1; c1 <= L; ++c1)
+            for (int c2 = c1; c2 <= L; ++c2) {
+                std::vector<int> p{c1};
+                if (c2 > c1) p.push_back(c2 - c1);
+                if (L > c2) p.push_back(L - c2);
+                if (std::find(seen.begin(), seen.end(), p) != seen.end()) continue;
+                seen.push_back(p);
+                parts = p;
+                waves = keep_waves;
+                if (!build_merged(true)) continue;
+                float ms = solve_ms();
+                if (ms < kSwitch * best) {
+                    best = ms;
+                    best_parts = parts;
+                    best_waves = waves;
+                }
+            }
+        parts = best_parts;
+        waves = best_waves;
+        build_merged(false);
     }
     HIP_CHECK(hipEventDestroy(a));
     HIP_CHECK(hipEventDestroy(b));
@@ -1297,18 +1455,29 @@ void pdfc_sim::solve_inplace(double* X) {
         // one event pair around the whole sweep: per-launch events would add their own
         // overhead to every few-microsecond level kernel
         timed(1, [&] {
-            for (size_t l = 0; l + 1 < fwd.level_ptr.size(); ++l)
-                pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
-                                           inout, Y, fac.max_ns, fwd_waves[l], stream);
+            for (const MergedLaunch& ml : fwd_launches) {
+                const int l = ml.first;
+                if (ml.R.nl > 1)
+                    pdfc::launch_sweep_merged(dfwd, fp32, true, ml.R, inout, Y, stream);
+                else
+                    pdfc::launch_forward_level(dfwd, fp32, fwd.level_ptr[l], fwd.level_ptr[l + 1] - fwd.level_ptr[l],
+                                               inout, Y, fac.max_ns, fwd_waves[l], stream);
+            }
         });
         // the top block's ordered finish runs inside the first backward level when there is one
-        const bool fold = use_top && top_fold && bwd.level_ptr.size() > 1;
+        const bool fold = folded();
         if (use_top) timed(3, [&] { pdfc::launch_top(dtop, fp32, dfwd.U, inout, inout, n, stream, !fold); });
         pdfc::TopFold tf{dtop.part, dtop.rows, dtop.top_of, dtop.m, dtop.kchunks, 0, 0, dtop.nb};
         timed(2, [&] {
-            for (size_t l = 0; l + 1 < bwd.level_ptr.size(); ++l)
-                pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l], Y,
-                                            inout, fac.max_front, bwd_waves[l], stream, fold && l == 0 ? &tf : nullptr);
+            for (const MergedLaunch& ml : bwd_launches) {
+                const int l = ml.first;
+                if (ml.R.nl > 1)
+                    pdfc::launch_sweep_merged(dbwd, fp32, false, ml.R, Y, inout, stream);
+                else
+                    pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
+                                                Y, inout, fac.max_front, bwd_waves[l], stream,
+                                                fold && l == 0 ? &tf : nullptr);
+            }
         });
         if (ktiming) ++ksolves;
     };
@@ -2344,7 +2513,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.factor_seconds = sim->fac.seconds;
         I.lhs_seconds = sim->lhs_seconds;
         I.steps_done = sim->steps_done;
-        I.top_fold = (sim->use_top && sim->top_fold && sim->bwd.level_ptr.size() > 1) ? 1 : 0;
+        I.top_fold = sim->folded() ? 1 : 0;
         // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
         //   local kernel: read v, q_n, c and write b (4 x 24 B per vertex); per spring end an
         //   index and kappa (4 + 8 B); per contact the basis (72 B), mu (8 B), r read+write
@@ -2359,7 +2528,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         const double fwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 2. * 24. * nbtot + 4. * nbtot;
         const double mt = sim->use_top ? (double)sim->top.m : 0.;
         // with the top block's finish folded into the first backward level, x_T is written there
-        const bool folded = sim->use_top && sim->top_fold && sim->bwd.level_ptr.size() > 1;
+        const bool folded = sim->folded();
         const double bwd = ent * (double)sim->nnzF_lower + 2. * 24. * n + 28. * nbtot + (folded ? 24. * mt : 0.);
         // top block: S^-1 once (its upper block triangle with symmetric tiles), b_T in, x_T out
         // (unless folded), gathered update entries (index + 24 B)
@@ -2378,6 +2547,10 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.top_chunks = sim->use_top && sim->dtop.nb == 0 ? sim->dtop.kchunks : 0;
         I.top_m = (int64_t)mt;
         I.n_levels_swept = (int64_t)(sim->fwd.level_ptr.size() - 1);
+        I.launches_forward = (int32_t)sim->fwd_launches.size();
+        I.launches_backward = (int32_t)sim->bwd_launches.size();
+        for (size_t k = 0; k < sim->fwd_parts.size() && k < 32; ++k) I.parts_forward[k] = sim->fwd_parts[k];
+        for (size_t k = 0; k < sim->bwd_parts.size() && k < 32; ++k) I.parts_backward[k] = sim->bwd_parts[k];
         I.bytes_local = local;
         I.bytes_forward = fwd;
         I.bytes_backward = bwd;
@@ -2407,16 +2580,16 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         sim->collect_kernel_times();
         pdfc_kernel_times t{};
         // sweeps are timed whole (families 1 and 2 count sweeps); a sweep is one launch per
-        // swept level
-        const int64_t per_sweep = (int64_t)(sim->fwd.level_ptr.size() - 1);
+        // level, or per range of merged levels
+        const int64_t per_fwd = (int64_t)sim->fwd_launches.size(), per_bwd = (int64_t)sim->bwd_launches.size();
         t.local_launches = sim->kcount[0];
-        t.forward_launches = sim->kcount[1] * per_sweep;
-        t.backward_launches = sim->kcount[2] * per_sweep;
+        t.forward_launches = sim->kcount[1] * per_fwd;
+        t.backward_launches = sim->kcount[2] * per_bwd;
         t.local_us = sim->kcount[0] ? sim->ksum[0] / sim->kcount[0] : 0.;
         t.forward_sweep_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
         t.backward_sweep_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
-        t.forward_us = per_sweep ? t.forward_sweep_us / per_sweep : 0.;
-        t.backward_us = per_sweep ? t.backward_sweep_us / per_sweep : 0.;
+        t.forward_us = per_fwd ? t.forward_sweep_us / per_fwd : 0.;
+        t.backward_us = per_bwd ? t.backward_sweep_us / per_bwd : 0.;
         t.top_us = sim->kcount[3] ? sim->ksum[3] / sim->kcount[3] : 0.;
         t.top_launches = sim->kcount[3];
         t.self_us = sim->kcount[4] ? sim->ksum[4] / sim->kcount[4] : 0.;

@@ -221,8 +221,14 @@ class Simulation:
             raise ValueError("more levels than pdfc_info reports")
         f = ",".join(str(w) for w in I["waves_forward"][:nl])
         b = ",".join(str(w) for w in I["waves_backward"][:nl])
-        # t=0: symmetric tiles (or no top block); o: the top block's finish folded or not
-        return "f=%s;b=%s;t=%d;o=%d" % (f, b, I["top_chunks"], I["top_fold"])
+        g = ",".join(str(x) for x in I["parts_forward"][:I["launches_forward"]])
+        h = ",".join(str(x) for x in I["parts_backward"][:I["launches_backward"]])
+        # t=0: symmetric tiles (or no top block); o: the top block's finish folded or not;
+        # g, h: levels per launch of each sweep (merged ranges)
+        out = "f=%s;b=%s;t=%d;o=%d" % (f, b, I["top_chunks"], I["top_fold"])
+        if nl > 0:
+            out += ";g=%s;h=%s" % (g, h)
+        return out
 
     def phase_stats(self):
         P = abi.PhaseStats()

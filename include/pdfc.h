@@ -209,6 +209,11 @@ typedef struct pdfc_info {
     double lhs_seconds;              /* host time to build topology, masses, constraints and A^T A (K) */
     int64_t steps_done;              /* steps taken since creation (restored by pdfc_load_checkpoint) */
     int32_t top_fold;                /* 1: the top block's ordered finish runs in the first backward level */
+    int32_t launches_forward;        /* kernel launches per forward sweep (levels, or ranges of merged levels) */
+    int32_t launches_backward;       /* same for the backward sweep */
+    int32_t parts_forward[32];       /* levels per forward launch in sweep order (1: a level kernel; more: one
+                                        launch, a workgroup per subtree of the range; autotuned) */
+    int32_t parts_backward[32];      /* same for the backward sweep (root side first) */
     int32_t reserved_info;
 } pdfc_info;
 

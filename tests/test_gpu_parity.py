@@ -178,8 +178,9 @@ def test_fp32_refinement_error_40k(gpu, solve_mode):
 
 
 def pinned(probe, top_mode, fold):
-    """probe's substitution schedule with the top block's mode and fold set."""
-    base = re.sub(r";t=\d+;o=\d", "", probe.tuning())
+    """probe's substitution schedule with the top block's mode and fold set, every level in a
+    launch of its own (the fold runs in the first backward level kernel)."""
+    base = re.sub(r";t=\d+;o=\d|;g=[\d,]+|;h=[\d,]+", "", probe.tuning())
     return "%s;t=%d;o=%d" % (base, top_mode, fold)
 
 
@@ -301,3 +302,32 @@ def test_drape_40k_full_step_parity_in_self_contact_phase(gpu):
         assert len(ko) > 1000 and len(so) > 100
     print("worst relative errors", worst)
     assert max(worst.values()) < REL, worst
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_REFINE64])
+def test_merged_levels_are_bit_identical(gpu, monkeypatch, precision):
+    """Consecutive substitution levels merged into one launch (a workgroup per subtree group,
+    levels separated by the workgroup barrier) compute every task as the level kernel does with
+    the same waves per task: states after 5 steps of the 40k drape are identical for the
+    level-by-level sweeps, two and three launches per sweep, and one launch per sweep."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    probe = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    L = probe.info()["n_levels_swept"]
+    assert L >= 3
+    waves = ",".join(["8"] * L)  # 2 slots per merged level: fits the merged launch's LDS everywhere
+    parts = [",".join(["1"] * L), "1,%d" % (L - 1), "%d,1" % (L - 1), "1,1,%d" % (L - 2), str(L)]
+    states = []
+    for g in parts:
+        h = ",".join(reversed(g.split(",")))
+        monkeypatch.setenv("PDFC_TUNE", "f=%s;b=%s;t=0;o=0;g=%s;h=%s" % (waves, waves, g, h))
+        s = gpu.Simulation(spec, precision=precision, record_iterations=False)
+        I = s.info()
+        assert I["launches_forward"] == len(g.split(",")) and I["launches_backward"] == len(h.split(","))
+        print("g=%s h=%s: %d + %d launches per solve" % (g, h, I["launches_forward"], I["launches_backward"]),
+              flush=True)
+        s.step(5)
+        states.append(s.state())
+    q0, v0 = states[0]
+    for (q, v), g in zip(states[1:], parts[1:]):
+        assert np.array_equal(q, q0) and np.array_equal(v, v0), g

@@ -25,3 +25,18 @@ def test_supernodal_solve_residual(checker, N, leaf):
     r = subprocess.run([checker, str(N), str(leaf)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_merged_launch_plans(tmp_path):
+    """Every partition of each sweep into at most three ranges of consecutive levels: the
+    merged-launch groups list each task once, and each task's children (forward) or below-row
+    owners (backward) are done before it inside its own workgroup or by an earlier launch."""
+    exe = str(tmp_path / "check_merged")
+    src = [os.path.join(ROOT, "tests", "native", "check_merged.cpp"),
+           os.path.join(ROOT, "climatenet_amd", "csrc", "model.cpp"),
+           os.path.join(ROOT, "climatenet_amd", "csrc", "factor.cpp")]
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include")] + src + ["-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok (0 failures)" in r.stdout
