@@ -101,7 +101,8 @@ def cpu_baseline(spec, budget_s=20.0, start=2, state=None, window=None, runs=3):
 
 def warm_start_comparison(steps=2, cap=40000):
     """The learned warm start on its configured workload: the 40k drape from its state at step
-    150 (self-collision on), `steps` steps run to a 1e-6 mean friction residual with the plain,
+    150 (self-collision on), step 151 untimed (it gives the previous and learned starts their
+    previous reactions), then `steps` steps run to a 1e-6 mean friction residual with the plain,
     previous-step, learned and exact warm starts (climatenet_amd.learn.evaluate_drape40k); the
     exact converged reactions of the same step are the bound any obstacle-reaction warm start
     can reach."""

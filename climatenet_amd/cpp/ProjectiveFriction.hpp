@@ -213,7 +213,6 @@ private:
     pdfc_sim* m_sim = nullptr;
     bool m_mirror = true;
     bool m_phase_timing = false;
-    std::size_t m_levels_swept = 0;
     // the vectors of SimulationSpeed/SimulationBase/SimulationFrictionEstimation that
     // exportStats writes; the ones this build does not measure stay empty ("(No data)")
     std::vector<double> m_lhs_times, m_cholesky_times, m_iteration_times, m_collision_detection_times;

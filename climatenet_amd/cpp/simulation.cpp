@@ -107,7 +107,6 @@ SimulationFrictionEstimationHIP::SimulationFrictionEstimationHIP(PhysicScene& sc
     if (pdfc_get_info(m_sim, &info) != PDFC_OK) throw std::runtime_error(pdfc_last_error());
     m_lhs_times.push_back(1e6 * info.lhs_seconds);
     m_cholesky_times.push_back(1e6 * info.factor_seconds);
-    m_levels_swept = (std::size_t)info.n_levels_swept;
     m_phase_timing = p.phase_timing;
     if (m_phase_timing && pdfc_set_kernel_timing(m_sim, 1) != PDFC_OK) throw std::runtime_error(pdfc_last_error());
 }
@@ -154,9 +153,9 @@ void SimulationFrictionEstimationHIP::updateScene() {
     if (m_phase_timing) {
         pdfc_kernel_times kt{};
         if (pdfc_get_kernel_times(m_sim, &kt) != PDFC_OK) fail_stop("kernel_times");
-        const double sweeps = m_levels_swept > 0 ? (double)m_levels_swept : 1.;
-        const double solver = (kt.forward_sweep_us * (double)kt.forward_launches / sweeps +
-                               kt.backward_sweep_us * (double)kt.backward_launches / sweeps +
+        // device time of all the step's substitution launches (per-launch mean times launches)
+        const double solver = (kt.forward_us * (double)kt.forward_launches +
+                               kt.backward_us * (double)kt.backward_launches +
                                kt.top_us * (double)kt.top_launches) / (double)K;
         for (std::size_t k = 0; k < K; ++k) {
             m_global_times.push_back(solver);

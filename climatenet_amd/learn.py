@@ -1,6 +1,7 @@
 """Learned warm start: contact data from the simulator, training on the GPU with the
 library's own MLP kernels, and the comparison against the plain warm start.
 
+    python -m climatenet_amd.learn build                 # the shipped network (data/contact_mlp.npz)
     python -m climatenet_amd.learn generate --out data.npz
     python -m climatenet_amd.learn train --data data.npz --out contact_mlp.npz
     python -m climatenet_amd.learn evaluate --model contact_mlp.npz
@@ -153,6 +154,100 @@ def shrink(mlp, X, Y, grid=np.linspace(0.0, 1.0, 21)):
     return alpha
 
 
+def scale_change(mlp, factor):
+    """Multiplies the network's predicted change by `factor` (its last layer is linear)."""
+    p, z = mlp.params()
+    p = p.copy()
+    p[_W4] *= factor
+    p[_B4] *= factor
+    mlp.set_params(p, z)
+
+
+def held_start_iterations(mlp, spec, warmup_steps=60, steps=3, tol=TOL, cap=CAP):
+    """Mean iterations to the tolerance of the learned warm start over `steps` steps of `spec`
+    after `warmup_steps` fixed-iteration steps and one untimed adaptive step (which gives the
+    warm start its previous reactions)."""
+    base = native.Simulation(spec)
+    base.step(warmup_steps)
+    q0, v0 = base.state()
+    tuning = base.tuning()
+    base.close()
+    saved = os.environ.get("PDFC_TUNE")
+    os.environ["PDFC_TUNE"] = tuning
+    try:
+        sim = native.Simulation(spec, residual_tolerance=tol, max_iterations=cap, warm_start=abi.WARM_LEARNED,
+                                warm_hold=cap, mlp=mlp)
+        sim.set_state(q0, v0)
+        sim.step(1)
+        its = []
+        for _ in range(steps):
+            sim.step(1)
+            its.append(sim.phase_stats()["iterations_to_tolerance"])
+        sim.close()
+    finally:
+        if saved is None:
+            del os.environ["PDFC_TUNE"]
+        else:
+            os.environ["PDFC_TUNE"] = saved
+    return float(np.mean(its))
+
+
+def select_alpha(mlp, factors=(0.0, 0.5, 1.0, 2.0, 4.0), seeds=(7, 8), min_gain=0.03, log=print):
+    """The held-start objective: the iterations the learned warm start needs to reach 1e-6 on
+    20x20 drapes outside the training data (jitter seeds 7 and 8, steps 62-64), for the network's
+    change scaled by each factor. Factor 0 (the previous step's reactions alone) is kept unless
+    another factor needs at least `min_gain` fewer iterations; then the factor with the fewest
+    (the smaller on a tie). Returns (factor, {factor: iterations})."""
+    p0, z0 = mlp.params()
+    p0 = p0.copy()
+    res = {}
+    for f in factors:
+        mlp.set_params(p0, z0)
+        scale_change(mlp, f)
+        res[f] = float(np.mean([held_start_iterations(mlp, scenes.sphere_drape_small(jitter_seed=s)) for s in seeds]))
+        log("  change x %.2f: %.1f iterations to 1e-6 per step" % (f, res[f]))
+    best = min(factors, key=lambda f: (res[f], f))
+    if 0.0 in res and res[best] > (1.0 - min_gain) * res[0.0]:
+        best = 0.0
+    mlp.set_params(p0, z0)
+    scale_change(mlp, best)
+    return best, res
+
+
+def build(out=MODEL, epochs=60, log=print):
+    """The shipped network: the configured data set (generate), 5% of it held out, training
+    (train), the change scaled by the factor that fits the held-out contacts best (shrink), then
+    by the held-start objective (select_alpha). Returns the statistics."""
+    t0 = time.perf_counter()
+    X, Y, R = generate(log=log)
+    gen_s = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    idx = rng.permutation(len(X))
+    nv = len(X) // 20
+    va, tr = idx[:nv], idx[nv:]
+    mlp, losses, dt = train(X[tr], Y[tr], epochs=epochs, batch=512, lr=2e-3, log=log)
+
+    def relerr(Ys, pred):
+        scale = np.linalg.norm(Ys, axis=1).mean()
+        return np.linalg.norm(pred - Ys, axis=1) / (np.linalg.norm(Ys, axis=1) + 0.1 * scale)
+
+    e_net = float(np.median(relerr(Y[va], predict(mlp, X[va]))))
+    alpha = shrink(mlp, X[va], Y[va])
+    e_shrunk = float(np.median(relerr(Y[va], predict(mlp, X[va]))))
+    factor, held = select_alpha(mlp, log=log)
+    e_final = float(np.median(relerr(Y[va], predict(mlp, X[va]))))
+    e_prev = float(np.median(relerr(Y[va], X[va][:, 6:9])))
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    save(out, mlp)
+    return {"samples": int(len(X)), "generate_seconds": round(gen_s, 1),
+            "residual_reached_median": float(np.median(R)), "fraction_converged": float((R < TOL).mean()),
+            "epochs": epochs, "train_seconds": round(dt, 1), "loss_first": float(losses[0]),
+            "loss_last": float(losses[-1]), "val_rel_err_median_network": e_net, "alpha_shrink": alpha,
+            "val_rel_err_median_shrunk": e_shrunk, "held_start_iterations": {str(k): v for k, v in held.items()},
+            "held_start_factor": factor, "val_rel_err_median_shipped": e_final,
+            "val_rel_err_median_previous_reaction": e_prev}
+
+
 def save(path, mlp):
     p, z = mlp.params()
     np.savez(path, params=p, norm=z)
@@ -204,26 +299,54 @@ def evaluate(spec, mlp, steps, warmup_steps=0, modes=(abi.WARM_PLAIN, abi.WARM_P
 
 def evaluate_drape40k(mlp, start=150, steps=2, cap=40000, tol=TOL, self_collision=True, log=print):
     """The learned warm start on its configured workload: the 40k-vertex drape from its state at
-    step `start` (self-collision as configured), `steps` steps run to the tolerance with each
-    warm start from the same state: plain (the reference), previous (last step's converged
-    reactions), learned (the network), and exact (the plain run's converged reactions of the same
-    step, held as long as the residual keeps falling): the bound any reaction warm start of the
-    obstacle contacts can reach. Self contacts start plain in every mode."""
+    step `start` (self-collision as configured). Each warm start first runs step start + 1 to the
+    tolerance untimed (with no earlier step every contact starts plain, so this step is the same
+    in every mode; it gives the previous and learned starts their previous reactions), then
+    `steps` counted steps to the tolerance: plain (the reference), previous (last step's
+    converged reactions), learned (the network), and exact (the plain run's converged reactions
+    of the same step, held as long as the residual keeps falling): the bound any reaction warm
+    start of the obstacle contacts can reach. Self contacts start plain in every mode.
+    iterations_to_tolerance is the mean over the counted steps."""
     spec = scenes.sphere_drape_40k()
     spec.self_collision = self_collision
     base = native.Simulation(spec, record_iterations=False)
     base.step(start)
     q0, v0 = base.state()
+    tuning = base.tuning()
     base.close()
-    out = {"scene": "40k drape, %d steps from step %d, self-collision %s" % (steps, start, "on" if self_collision else "off"),
+    # every mode on the same substitution schedule (each simulation would otherwise tune its own,
+    # whose summation order differs in the last bits)
+    saved = os.environ.get("PDFC_TUNE")
+    os.environ["PDFC_TUNE"] = tuning
+    try:
+        return _evaluate_modes(spec, mlp, q0, v0, start, steps, cap, tol, self_collision, log)
+    finally:
+        if saved is None:
+            del os.environ["PDFC_TUNE"]
+        else:
+            os.environ["PDFC_TUNE"] = saved
+
+
+def _evaluate_modes(spec, mlp, q0, v0, start, steps, cap, tol, self_collision, log):
+    out = {"scene": "40k drape, step %d untimed then %d counted steps from step %d, self-collision %s"
+                    % (start + 1, steps, start + 2, "on" if self_collision else "off"),
            "tolerance": tol, "max_iterations": cap, "steps": steps}
     plain_contacts, plain_states = [], []
+    first_state = None
     for name, mode in (("plain", abi.WARM_PLAIN), ("previous", abi.WARM_PREVIOUS), ("learned", abi.WARM_LEARNED),
                        ("exact", None)):
         sim = native.Simulation(spec, residual_tolerance=tol, max_iterations=cap,
                                 warm_start=abi.WARM_PLAIN if mode is None else mode, warm_hold=cap,
                                 mlp=mlp if mode == abi.WARM_LEARNED else None)
         sim.set_state(q0, v0)
+        if mode is not None:
+            sim.step(1)  # untimed step start + 1
+            qs, vs = sim.state()
+            if first_state is None:
+                first_state = (qs, vs)
+            out.setdefault("same_state_after_first_step", True)
+            out["same_state_after_first_step"] &= bool(np.array_equal(qs, first_state[0]) and
+                                                       np.array_equal(vs, first_state[1]))
         it, conv, fb, t, res = 0, 0, 0, 0.0, []
         for k in range(steps):
             if mode is None:
@@ -245,10 +368,10 @@ def evaluate_drape40k(mlp, start=150, steps=2, cap=40000, tol=TOL, self_collisio
                      "final_residual": res, "seconds_per_step": round(t / steps, 3)}
         if name == "plain":
             out["final_state_plain"] = sim.state()
-        if name == "learned":
+        else:
             q, v = sim.state()
             qp, vp = out["final_state_plain"]
-            out["learned"]["max_rel_diff_q_vs_plain"] = float(np.abs(q - qp).max() / np.abs(qp).max())
+            out[name]["max_rel_diff_q_vs_plain"] = float(np.abs(q - qp).max() / np.abs(qp).max())
         log("  %-8s iterations to %.0e: %.0f, converged %d/%d, fallbacks %d, %.2f s/step"
             % (name, tol, it / steps, conv, steps, fb, t / steps))
         sim.close()
@@ -272,6 +395,9 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
+    b = sub.add_parser("build", help="generate, train, shrink and select the shipped network")
+    b.add_argument("--out", default=MODEL)
+    b.add_argument("--epochs", type=int, default=60)
     e = sub.add_parser("evaluate")
     e.add_argument("--model", default=MODEL)
     e.add_argument("--steps", type=int, default=10)
@@ -293,6 +419,8 @@ def main(argv=None):
                         "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                         "relative_error_median": float(np.median(err)),
                         "relative_error_p90": float(np.percentile(err, 90))}))
+    elif a.cmd == "build":
+        log(json.dumps(build(a.out, a.epochs, log=log)))
     else:
         mlp = load(a.model)
         res = evaluate(scenes.sphere_drape_small(jitter_seed=7), mlp, a.steps, 60, log=log)
