@@ -30,8 +30,8 @@ def measured_traffic(kernel, info, name="drape40k"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     profiles/<name>_pmc_traffic.json (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
     the same counter's reading for a device copy of known size), when it was measured on this
-    workload, kernel and level structure (a summary without levels per launch was measured with
-    one launch per level); otherwise None."""
+    workload, kernel and level structure (a summary without levels per launch or split counts was
+    measured with one launch per level and unsplit reductions); otherwise None."""
     path = os.path.join(ROOT, "profiles", "%s_pmc_traffic.json" % name)
     if not os.path.exists(path):
         return None
@@ -41,7 +41,8 @@ def measured_traffic(kernel, info, name="drape40k"):
             d.get("n_vertices") != info["n_vertices"] or d.get("nnz_F") != info["nnz_F"] or \
             d.get("top_fold") != info["top_fold"] or d.get("top_chunks") != info["top_chunks"] or \
             d.get("parts_backward", [1] * info["n_levels_swept"]) != info["parts_backward"][:info["launches_backward"]] or \
-            d.get("parts_forward", [1] * info["n_levels_swept"]) != info["parts_forward"][:info["launches_forward"]]:
+            d.get("parts_forward", [1] * info["n_levels_swept"]) != info["parts_forward"][:info["launches_forward"]] or \
+            d.get("split_backward", [1] * info["n_levels_swept"]) != info["split_backward"][:info["n_levels_swept"]]:
         return None
     return d.get("traffic_bytes_per_launch")
 
@@ -212,6 +213,7 @@ def bench_batched(args):
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
                    "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
                    "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
+                   "split_backward": info["split_backward"][:info["n_levels_swept"]],
                    "top_chunks": info["top_chunks"], "top_fold": info["top_fold"], "nnz_F": info["nnz_F"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "sequential": None if args.no_sequential else {
@@ -450,6 +452,7 @@ def main():
                    "waves_backward": info["waves_backward"][:info["n_levels_swept"]],
                    "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
                    "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
+                   "split_backward": info["split_backward"][:info["n_levels_swept"]],
                    "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"], "top_fold": info["top_fold"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "roofline": roofline,

@@ -166,6 +166,7 @@ class Info(C.Structure):
         ("launches_backward", C.c_int32),
         ("parts_forward", C.c_int32 * 32),
         ("parts_backward", C.c_int32 * 32),
+        ("split_backward", C.c_int32 * 32),
         ("reserved_info", C.c_int32),
     ]
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
 #include "device.hpp"
 
@@ -1010,63 +1011,98 @@ __device__ __forceinline__ void top_sum(const TopFold& fo, int t, double& x, dou
         }
 }
 
+// Ordered sum of the partials of row r (parts >= 2): XP[0] + XP[1] + ... per component.
+__device__ __forceinline__ void split_sum(const double* __restrict__ XP, int n, int parts, int r, double& x, double& y,
+                                          double& z) {
+    const size_t n3 = 3 * (size_t)n;
+    double px[kMaxSplit], py[kMaxSplit], pz[kMaxSplit];
+#pragma unroll
+    for (int p = 0; p < kMaxSplit; ++p)
+        if (p < parts) {
+            px[p] = XP[p * n3 + r];
+            py[p] = XP[p * n3 + n + r];
+            pz[p] = XP[p * n3 + 2 * n + r];
+        }
+    x = px[0];
+    y = py[0];
+    z = pz[0];
+#pragma unroll
+    for (int p = 1; p < kMaxSplit; ++p)
+        if (p < parts) {
+            x += px[p];
+            y += py[p];
+            z += pz[p];
+        }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDesc& d, const double* __restrict__ Y,
                                               double* __restrict__ X, double* lds, double (*part)[3][64],
                                               const TopFold& fo) {
-    const int ns = d.ns, nb = d.nb, f = d.f, c0 = d.c0, j0 = d.r0;
+    const int ns = d.ns, f = d.f, c0 = d.c0, j0 = d.r0;
     const int n = Fa.n;
+    // rows [i0, i1) of the front: all of them, or this part's share of a split reduction
+    const int parts = d.split >> 16, mypart = d.split & 0xffff;
+    const int i0 = parts ? (int)(d.rng & 0xffffffff) : 0;
+    const int i1 = parts ? (int)(d.rng >> 32) : f;
+    const int len = i1 - i0;
     double* gx = lds;
-    double* gy = lds + f;
-    double* gz = lds + 2 * f;
+    double* gy = lds + len;
+    double* gz = lds + 2 * len;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = j0 + lane;
     const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
     double pre[kPrefetch];  // operator entries independent of the gather, issued first
 #pragma unroll
     for (int q = 0; q < kPrefetch; ++q) {
-        const int i = wave + q * W;
-        pre[q] = (j < ns && i < f) ? as_d(Ft[(size_t)i * ns]) : 0.;
+        const int i = i0 + wave + q * W;
+        pre[q] = (j < ns && i < i1) ? as_d(Ft[(size_t)i * ns]) : 0.;
     }
-    for (int t = tid; t < ns; t += 64 * W) {
-        gx[t] = Y[c0 + t];
-        gy[t] = Y[n + c0 + t];
-        gz[t] = Y[2 * n + c0 + t];
+    const int a1 = min(i1, ns);
+    for (int i = i0 + tid; i < a1; i += 64 * W) {
+        gx[i - i0] = Y[c0 + i];
+        gy[i - i0] = Y[n + c0 + i];
+        gz[i - i0] = Y[2 * n + c0 + i];
     }
     const int* R = Fa.rows + d.roff;
-    for (int t = tid; t < nb; t += 64 * W) {
-        int r = R[t];
+    const int* RP = Fa.rparts ? Fa.rparts + d.roff : nullptr;
+    for (int i = max(i0, ns) + tid; i < i1; i += 64 * W) {
+        const int t = i - ns;
+        const int r = R[t];
+        const int np = RP ? RP[t] : 0;
         const int tr = fo.part ? fo.top_of[r] : -1;
+        double x, y, z;
         if (tr >= 0) {
-            double x, y, z;
             top_sum(fo, tr, x, y, z);
-            gx[ns + t] = -x;
-            gy[ns + t] = -y;
-            gz[ns + t] = -z;
+        } else if (np > 1) {
+            split_sum(Fa.XP, n, np, r, x, y, z);
         } else {
-            gx[ns + t] = -X[r];
-            gy[ns + t] = -X[n + r];
-            gz[ns + t] = -X[2 * n + r];
+            x = X[r];
+            y = X[n + r];
+            z = X[2 * n + r];
         }
+        gx[i - i0] = -x;
+        gy[i - i0] = -y;
+        gz[i - i0] = -z;
     }
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (j < ns) {
 #pragma unroll
         for (int q = 0; q < kPrefetch; ++q) {
-            const int i = wave + q * W;
-            if (i < f) {
-                ax = fma(pre[q], gx[i], ax);
-                ay = fma(pre[q], gy[i], ay);
-                az = fma(pre[q], gz[i], az);
+            const int i = i0 + wave + q * W;
+            if (i < i1) {
+                ax = fma(pre[q], gx[i - i0], ax);
+                ay = fma(pre[q], gy[i - i0], ay);
+                az = fma(pre[q], gz[i - i0], az);
             }
         }
 #pragma unroll 8
-        for (int i = wave + kPrefetch * W; i < f; i += W) {
+        for (int i = i0 + wave + kPrefetch * W; i < i1; i += W) {
             double fv = as_d(Ft[(size_t)i * ns]);
-            ax = fma(fv, gx[i], ax);
-            ay = fma(fv, gy[i], ay);
-            az = fma(fv, gz[i], az);
+            ax = fma(fv, gx[i - i0], ax);
+            ay = fma(fv, gy[i - i0], ay);
+            az = fma(fv, gz[i - i0], az);
         }
     }
     if (W > 1) {
@@ -1086,15 +1122,16 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
         }
     }
     if (wave == 0 && j < ns) {
-        X[c0 + j] = ax;
-        X[n + c0 + j] = ay;
-        X[2 * n + c0 + j] = az;
+        double* out = parts ? Fa.XP + (size_t)mypart * 3 * n : X;
+        out[c0 + j] = ax;
+        out[n + c0 + j] = ay;
+        out[2 * n + c0 + j] = az;
     }
 }
 
 template <typename T, int W>
 __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const double* __restrict__ Y,
-                                                     double* __restrict__ X, TopFold fo) {
+                                                     double* __restrict__ X, TopFold fo, SplitFin fin) {
     extern __shared__ double lds[];
     __shared__ double part[W > 1 ? W : 1][3][64];
     if (fo.part && (int)blockIdx.x >= fo.ntasks) {  // folded top-block finish: write the top rows
@@ -1103,6 +1140,18 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
             double x, y, z;
             top_sum(fo, t, x, y, z);
             const int r = fo.rows[t];
+            X[r] = x;
+            X[Fa.n + r] = y;
+            X[2 * Fa.n + r] = z;
+        }
+        return;
+    }
+    if (fin.count > 0 && (int)blockIdx.x >= fin.first) {  // final values of the previous level's split rows
+        const int t = ((int)blockIdx.x - fin.first) * (int)blockDim.x + (int)threadIdx.x;
+        if (t < fin.count) {
+            const int r = fin.rows[t];
+            double x, y, z;
+            split_sum(Fa.XP, Fa.n, fin.parts, r, x, y, z);
             X[r] = x;
             X[Fa.n + r] = y;
             X[2 * Fa.n + r] = z;
@@ -1597,13 +1646,13 @@ static void forward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const De
 }
 template <typename T>
 static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* Y,
-                              double* X, const TopFold& fo) {
+                              double* X, const TopFold& fo, const SplitFin& fin) {
     switch (W) {
-        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X, fo); break;
-        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X, fo); break;
-        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X, fo); break;
-        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X, fo); break;
-        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X, fo); break;
+        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X, fo, fin); break;
+        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X, fo, fin); break;
+        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X, fo, fin); break;
+        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X, fo, fin); break;
+        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X, fo, fin); break;
     }
 }
 
@@ -1617,19 +1666,29 @@ void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, con
         forward_dispatch<double>(W, dim3(ntasks), lds, st, F, t0, B, Y);
 }
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int W, hipStream_t st, const TopFold* fold) {
+                           int max_front, int W, hipStream_t st, const TopFold* fold, const SplitFin* fin) {
     TopFold fo{};
     if (fold && fold->part && fold->m > 0) {
         fo = *fold;
         fo.ntasks = ntasks;
         fo.extra = (fo.m + 64 * W - 1) / (64 * W);
     }
-    if (ntasks + fo.extra <= 0) return;
+    SplitFin sf{};
+    int fin_extra = 0;
+    if (fin && fin->count > 0) {
+        if (fo.extra > 0) throw std::logic_error("a backward launch has either the top fold or split rows to finish");
+        if (fin->parts < 2 || fin->parts > kMaxSplit) throw std::logic_error("split parts out of range");
+        sf = *fin;
+        sf.first = ntasks;
+        fin_extra = (sf.count + 64 * W - 1) / (64 * W);
+    }
+    const int blocks = ntasks + fo.extra + fin_extra;
+    if (blocks <= 0) return;
     size_t lds = (size_t)3 * max_front * sizeof(double);
     if (fp32)
-        backward_dispatch<float>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
+        backward_dispatch<float>(W, dim3(blocks), lds, st, F, t0, Y, X, fo, sf);
     else
-        backward_dispatch<double>(W, dim3(ntasks + fo.extra), lds, st, F, t0, Y, X, fo);
+        backward_dispatch<double>(W, dim3(blocks), lds, st, F, t0, Y, X, fo, sf);
 }
 void launch_sweep_merged(const DevFactor& F, bool fp32, bool forward, const MergedRange& R, const double* In,
                          double* Out, hipStream_t st) {

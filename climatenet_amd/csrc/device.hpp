@@ -115,14 +115,16 @@ struct DevModel {
 };
 
 // One substitution task (supernode s, rows or columns r0 .. r0+63), everything the
-// kernel needs in one 64-byte load.
+// kernel needs in one 64-byte load. A backward task whose reduction is split over several
+// workgroups ("parts") covers rows [i0, i1) of the front and writes a partial result:
+// split = part index | parts << 16 (0: the whole reduction), rng = i0 | i1 << 32.
 struct TaskDesc {
     int s, r0, ns, nb;
-    int c0, f, pull_base, pad;
+    int c0, f, pull_base, split;
     int64_t foff;
     int64_t uoff;
     int64_t roff;
-    int64_t spare;
+    int64_t rng;
 };
 
 // Supernodal substitution operators (see factor.cpp).
@@ -146,6 +148,10 @@ struct DevFactor {
     const TaskDesc* desc;   // per task of this sweep
     const int* pull_ptr;    // per supernode, f + 1 entries from pull_base: children's update
     const int* pull_idx;    //   entries (index into U) contributing to each front row, child order
+    // backward sweep with split reductions: partial results XP[p][c][row] (parts x 3n), and per
+    // below row (parallel to rows) the number of partials to sum for it (0: its value is in X)
+    double* XP;
+    const int* rparts;
 };
 
 struct StepArgs {
@@ -294,12 +300,20 @@ struct TopFold {
     int m, kchunks, ntasks, extra;
     int nb;               // DevTop::nb
 };
+// Rows of the previous backward level's split supernodes, whose final values (the ordered sum
+// of their partials) extra workgroups of the next backward launch write into X.
+struct SplitFin {
+    const int* rows;
+    int count, parts;
+    int first;            // first extra workgroup (set by the launcher)
+};
 // One level of the forward (B -> Y, U) or backward (Y -> X) substitution.
 // waves: 1, 2, 4, 8 or 16 waves per task splitting the reduction (ns forward, ns + nb backward).
 void launch_forward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* B, double* Y,
                           int max_ns, int waves, hipStream_t st);
 void launch_backward_level(const DevFactor& F, bool fp32, int t0, int ntasks, const double* Y, double* X,
-                           int max_front, int waves, hipStream_t st, const TopFold* fold = nullptr);
+                           int max_front, int waves, hipStream_t st, const TopFold* fold = nullptr,
+                           const SplitFin* fin = nullptr);
 // Heuristic wave count: chains of about 8 loads, at most 16 waves.
 int default_waves(int kmax);
 

@@ -391,4 +391,51 @@ void plan_merged(const Factor& F, const SweepPlan& P, bool forward, int first, i
         for (int k = 0; k < nl; ++k) out.task.insert(out.task.end(), g[k].begin(), g[k].end());
 }
 
+void plan_backward_split(const Factor& F, const SweepPlan& bwd, const std::vector<int>& split, SplitPlan& out) {
+    const int L = (int)bwd.level_ptr.size() - 1;
+    if ((int)split.size() != L) throw std::invalid_argument("one split count per backward level");
+    for (int l = 0; l < L; ++l)
+        if (split[l] < 1 || split[l] > kMaxSplit || (l == L - 1 && split[l] != 1))
+            throw std::invalid_argument("split parts must be 1..kMaxSplit, and 1 for the last level");
+    out = SplitPlan();
+    out.tasks.level_ptr.push_back(0);
+    out.tasks.level_kmax = bwd.level_kmax;
+    for (int l = 0; l < L; ++l) {
+        const int P = split[l];
+        for (int t = bwd.level_ptr[l]; t < bwd.level_ptr[l + 1]; ++t) {
+            const int s = bwd.task_snode[t];
+            const int f = F.ns[s] + F.nb[s];
+            const int chunk = (f + P - 1) / P;
+            for (int p = 0; p < P; ++p) {
+                out.tasks.task_snode.push_back(s);
+                out.tasks.task_row0.push_back(bwd.task_row0[t]);
+                out.part.push_back({p, P, std::min(f, p * chunk), std::min(f, (p + 1) * chunk)});
+            }
+        }
+        out.tasks.level_ptr.push_back((int)out.tasks.task_snode.size());
+    }
+    out.rparts.assign(F.rows.size(), 0);
+    out.fin_rows.assign(L, {});
+    out.fin_parts.assign(L, 0);
+    // backward sweep index l holds level L - 1 - l
+    std::vector<int> owner(F.n, -1);
+    for (size_t s = 0; s < F.ns.size(); ++s)
+        for (int k = 0; k < F.ns[s]; ++k) owner[F.c0[s] + k] = (int)s;
+    for (size_t s = 0; s < F.ns.size(); ++s) {
+        const int lv = F.level[s];
+        if (lv + 1 >= L) continue;  // not swept, or its parent level is not
+        for (int t = 0; t < F.nb[s]; ++t) {
+            const int a = owner[F.rows[F.roff[s] + t]];
+            if (F.level[a] == lv + 1 && split[L - 1 - (lv + 1)] > 1) out.rparts[F.roff[s] + t] = split[L - 1 - (lv + 1)];
+        }
+    }
+    for (int l = 1; l < L; ++l) {
+        if (split[l - 1] < 2) continue;
+        out.fin_parts[l] = split[l - 1];
+        for (size_t s = 0; s < F.ns.size(); ++s)
+            if (F.level[s] == L - l)
+                for (int k = 0; k < F.ns[s]; ++k) out.fin_rows[l].push_back(F.c0[s] + k);
+    }
+}
+
 }  // namespace pdfc

@@ -106,6 +106,20 @@ struct MergedGroups {
 };
 void plan_merged(const Factor& F, const SweepPlan& P, bool forward, int first, int nl, MergedGroups& out);
 
+// Backward sweep with split reductions: level l's tasks (supernode, 64 columns) each become
+// split[l] tasks ("parts") over contiguous ranges of the front's rows, whose partial results
+// the next level's tasks sum where they gather those rows and whose final values extra
+// workgroups of the next level's launch write. The last level (the leaves) is never split.
+constexpr int kMaxSplit = 4;  // parts of a split backward reduction
+struct SplitPlan {
+    SweepPlan tasks;                          // the tasks as launched, grouped by level as bwd
+    std::vector<std::array<int, 4>> part;     // per task: part index, parts, first row, end row
+    std::vector<int> rparts;                  // parallel to Factor::rows: partials to sum (0: final)
+    std::vector<std::vector<int>> fin_rows;   // per level: rows the launch finishes (previous level's)
+    std::vector<int> fin_parts;               // per level: their parts
+};
+void plan_backward_split(const Factor& F, const SweepPlan& bwd, const std::vector<int>& split, SplitPlan& out);
+
 // Dense top block: the columns T of all supernodes at levels >= level. Both
 // substitutions over T collapse into x_T = S^-1 b~_T with S the Schur complement
 // of K on T (S^-1 = (K^-1)_TT), where b~_T subtracts the update vectors of the

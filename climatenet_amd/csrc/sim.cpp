@@ -154,6 +154,21 @@ struct pdfc_sim {
     int* task_buf = nullptr;
     pdfc::TaskDesc* desc_buf = nullptr;
     std::vector<int> fwd_waves, bwd_waves;  // waves per task, per level (autotuned)
+    // backward reductions split over several workgroups per task (parts per level, autotuned)
+    std::vector<int> bwd_split;
+    pdfc::SweepPlan bwdx;               // the backward tasks as launched (with their parts)
+    std::vector<int> pull_base_of;      // per supernode: its pull-list base
+    pdfc::TaskDesc* bdesc_buf = nullptr;
+    int* split_buf = nullptr;           // rows' partial counts, then the rows each launch finishes
+    double* xpart = nullptr;            // kMaxSplit x 3n partial results
+    std::vector<pdfc::SplitFin> bwd_fin;
+    void build_backward();
+    bool any_split() const {
+        for (int p : bwd_split)
+            if (p > 1) return true;
+        return false;
+    }
+    std::vector<pdfc::TaskDesc> task_descs(const pdfc::SweepPlan& P, const std::vector<std::array<int, 4>>& split);
     // launches of each sweep: consecutive levels per launch, in sweep order (1: the level
     // kernel; more: one merged launch, a workgroup per subtree group); autotuned
     std::vector<int> fwd_parts, bwd_parts;
@@ -849,40 +864,79 @@ void pdfc_sim::upload_sweeps(int top_level) {
             pull_ptr.push_back((int)pull_idx.size());
         }
     }
-    auto descs = [&](const pdfc::SweepPlan& P) {
-        std::vector<pdfc::TaskDesc> D(P.task_snode.size());
-        for (size_t t = 0; t < D.size(); ++t) {
-            int s2 = P.task_snode[t];
-            pdfc::TaskDesc& d = D[t];
-            d.s = s2;
-            d.r0 = P.task_row0[t];
-            d.ns = fac.ns[s2];
-            d.nb = fac.nb[s2];
-            d.c0 = fac.c0[s2];
-            d.f = fac.ns[s2] + fac.nb[s2];
-            d.pull_base = base[s2];
-            d.pad = 0;
-            d.foff = fac.foff[s2];
-            d.uoff = fac.uoff[s2];
-            d.roff = fac.roff[s2];
-            d.spare = 0;
-        }
-        return D;
-    };
+    pull_base_of = base;
     if (!dfwd.pull_ptr) {
         dfwd.pull_ptr = dbwd.pull_ptr = upload(pull_ptr);
         dfwd.pull_idx = dbwd.pull_idx = upload(pull_idx);
     }
     if (desc_buf) free_ptr(desc_buf);
-    std::vector<pdfc::TaskDesc> df = descs(fwd), db = descs(bwd);
-    df.insert(df.end(), db.begin(), db.end());
-    desc_buf = upload(df);
+    desc_buf = upload(task_descs(fwd, {}));
     dfwd.desc = desc_buf;
-    dbwd.desc = desc_buf + nf;
     const size_t L = fwd.level_ptr.size() - 1;
+    bwd_split.assign(L, 1);
+    build_backward();
     fwd_parts.assign(L, 1);
     bwd_parts.assign(L, 1);
     build_merged(false);
+}
+
+// Descriptors of a sweep's tasks; `split` (backward, optional) holds per task its part index
+// and parts (0: the whole reduction) and its row range.
+std::vector<pdfc::TaskDesc> pdfc_sim::task_descs(const pdfc::SweepPlan& P, const std::vector<std::array<int, 4>>& split) {
+    std::vector<pdfc::TaskDesc> D(P.task_snode.size());
+    for (size_t t = 0; t < D.size(); ++t) {
+        const int s2 = P.task_snode[t];
+        pdfc::TaskDesc& d = D[t];
+        d.s = s2;
+        d.r0 = P.task_row0[t];
+        d.ns = fac.ns[s2];
+        d.nb = fac.nb[s2];
+        d.c0 = fac.c0[s2];
+        d.f = fac.ns[s2] + fac.nb[s2];
+        d.pull_base = pull_base_of[s2];
+        d.foff = fac.foff[s2];
+        d.uoff = fac.uoff[s2];
+        d.roff = fac.roff[s2];
+        d.split = 0;
+        d.rng = 0;
+        if (!split.empty() && split[t][1] > 1) {
+            d.split = split[t][0] | (split[t][1] << 16);
+            d.rng = (int64_t)(uint32_t)split[t][2] | ((int64_t)split[t][3] << 32);
+        }
+    }
+    return D;
+}
+
+// The backward sweep's launch plan from bwd_split (pdfc::plan_backward_split): the tasks with
+// their parts and descriptors, the partial counts of the below rows, and per launch the rows
+// whose final values its extra workgroups write.
+void pdfc_sim::build_backward() {
+    const int L = (int)bwd.level_ptr.size() - 1;
+    if ((int)bwd_split.size() != L) bwd_split.assign(L, 1);
+    pdfc::SplitPlan sp;
+    pdfc::plan_backward_split(fac, bwd, bwd_split, sp);
+    bwdx = sp.tasks;
+    if (bdesc_buf) free_ptr(bdesc_buf);
+    bdesc_buf = upload(task_descs(bwdx, sp.part));
+    dbwd.desc = bdesc_buf;
+    if (split_buf) free_ptr(split_buf);
+    split_buf = nullptr;
+    dbwd.rparts = nullptr;
+    bwd_fin.assign(L, pdfc::SplitFin{});
+    if (!any_split()) return;
+    if (!xpart) xpart = dalloc<double>((size_t)pdfc::kMaxSplit * 3 * n);
+    dbwd.XP = xpart;
+    std::vector<int> host = sp.rparts;
+    std::vector<size_t> at(L, 0);
+    for (int l = 0; l < L; ++l) {
+        at[l] = host.size();
+        host.insert(host.end(), sp.fin_rows[l].begin(), sp.fin_rows[l].end());
+    }
+    split_buf = upload(host);
+    dbwd.rparts = split_buf;
+    for (int l = 0; l < L; ++l)
+        if (!sp.fin_rows[l].empty())
+            bwd_fin[l] = pdfc::SplitFin{split_buf + at[l], (int)sp.fin_rows[l].size(), sp.fin_parts[l], 0};
 }
 
 // Launch lists of both sweeps from fwd_parts / bwd_parts. A range of several levels becomes
@@ -913,6 +967,7 @@ bool pdfc_sim::build_merged(bool choose_waves) {
             MergedLaunch ml{first, {}};
             ml.R.nl = nl;
             if (nl > 1) {
+                if (!forward && any_split()) ok = false;  // merged backward tasks index the unsplit plan
                 pdfc::MergedGroups mg;
                 pdfc::plan_merged(fac, P, forward, first, nl, mg);
                 const int G = mg.groups;
@@ -1033,8 +1088,19 @@ void pdfc_sim::apply_pinned_tuning(const char* spec) {
         check_parts(h, "backward");
         bwd_parts = h;
     }
+    // p: parts of each backward level's split reductions (1: whole; the last level is never split)
+    std::vector<int> sp;
+    if (parse_list('p', sp)) {
+        if (sp.size() != bwd_waves.size()) throw std::invalid_argument("PDFC_TUNE: backward split count differs");
+        for (int x : sp)
+            if (x != 1 && x != 2 && x != 4) throw std::invalid_argument("PDFC_TUNE: split parts must be 1, 2 or 4");
+        if (!sp.empty() && sp.back() != 1) throw std::invalid_argument("PDFC_TUNE: the last backward level is not split");
+        bwd_split = sp;
+        build_backward();
+    }
     if (!build_merged(false))
-        throw std::invalid_argument("PDFC_TUNE: a merged level needs more LDS than a launch has; give it more waves");
+        throw std::invalid_argument("PDFC_TUNE: a merged level needs more LDS than a launch has (give it more waves), "
+                                    "or a merged backward range meets split reductions");
 }
 
 // Chooses the waves per task of every substitution level by timing each level alone
@@ -1182,6 +1248,27 @@ void pdfc_sim::autotune_waves() {
             waves[l] = best_w;
         }
     }
+    // Split backward reductions: each backward level but the last, with its tasks' reductions
+    // split over 2 or 4 workgroups (the upper levels have fewer tasks than the part has CUs),
+    // kept when 1.5% faster on whole solves; one level at a time, root side first.
+    {
+        const int L = (int)bwd_split.size();
+        for (int l = 0; l + 1 < L; ++l) {
+            float best = solve_ms();
+            int best_p = 1;
+            for (int P : {2, 4}) {
+                bwd_split[l] = P;
+                build_backward();
+                float ms = solve_ms();
+                if (ms < kSwitchLevels * best) {
+                    best = ms;
+                    best_p = P;
+                }
+            }
+            bwd_split[l] = best_p;
+            build_backward();
+        }
+    }
     // Merged launches (PDFC_MERGE=1 offers them): every partition of a sweep's levels into at
     // most three ranges of consecutive levels, a range of several levels running as one launch
     // with the waves per task that start a group's tasks of a level at once. One sweep at a
@@ -1192,6 +1279,7 @@ void pdfc_sim::autotune_waves() {
     const char* me = std::getenv("PDFC_MERGE");
     const bool try_merge = me && std::atoi(me) != 0;
     for (int pass = 0; pass < 2 && try_merge; ++pass) {
+        if (pass == 1 && any_split()) continue;  // merged backward ranges need the unsplit plan
         std::vector<int>& parts = pass == 0 ? fwd_parts : bwd_parts;
         std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
         const int L = (int)waves.size();
@@ -1474,9 +1562,9 @@ void pdfc_sim::solve_inplace(double* X) {
                 if (ml.R.nl > 1)
                     pdfc::launch_sweep_merged(dbwd, fp32, false, ml.R, Y, inout, stream);
                 else
-                    pdfc::launch_backward_level(dbwd, fp32, bwd.level_ptr[l], bwd.level_ptr[l + 1] - bwd.level_ptr[l],
+                    pdfc::launch_backward_level(dbwd, fp32, bwdx.level_ptr[l], bwdx.level_ptr[l + 1] - bwdx.level_ptr[l],
                                                 Y, inout, fac.max_front, bwd_waves[l], stream,
-                                                fold && l == 0 ? &tf : nullptr);
+                                                fold && l == 0 ? &tf : nullptr, &bwd_fin[l]);
             }
         });
         if (ktiming) ++ksolves;
@@ -2551,6 +2639,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.launches_backward = (int32_t)sim->bwd_launches.size();
         for (size_t k = 0; k < sim->fwd_parts.size() && k < 32; ++k) I.parts_forward[k] = sim->fwd_parts[k];
         for (size_t k = 0; k < sim->bwd_parts.size() && k < 32; ++k) I.parts_backward[k] = sim->bwd_parts[k];
+        for (size_t k = 0; k < sim->bwd_split.size() && k < 32; ++k) I.split_backward[k] = sim->bwd_split[k];
         I.bytes_local = local;
         I.bytes_forward = fwd;
         I.bytes_backward = bwd;

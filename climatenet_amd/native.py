@@ -227,7 +227,7 @@ class Simulation:
         # g, h: levels per launch of each sweep (merged ranges)
         out = "f=%s;b=%s;t=%d;o=%d" % (f, b, I["top_chunks"], I["top_fold"])
         if nl > 0:
-            out += ";g=%s;h=%s" % (g, h)
+            out += ";g=%s;h=%s;p=%s" % (g, h, ",".join(str(x) for x in I["split_backward"][:nl]))
         return out
 
     def phase_stats(self):

@@ -214,6 +214,8 @@ typedef struct pdfc_info {
     int32_t parts_forward[32];       /* levels per forward launch in sweep order (1: a level kernel; more: one
                                         launch, a workgroup per subtree of the range; autotuned) */
     int32_t parts_backward[32];      /* same for the backward sweep (root side first) */
+    int32_t split_backward[32];      /* workgroups sharing each backward task's reduction, per backward level
+                                        (1: one; 2 or 4: partial sums finished by the next level; autotuned) */
     int32_t reserved_info;
 } pdfc_info;
 

@@ -331,3 +331,33 @@ def test_merged_levels_are_bit_identical(gpu, monkeypatch, precision):
     q0, v0 = states[0]
     for (q, v), g in zip(states[1:], parts[1:]):
         assert np.array_equal(q, q0) and np.array_equal(v, v0), g
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_REFINE64])
+def test_split_backward_reductions_match(gpu, monkeypatch, precision):
+    """Backward reductions split over 2 or 4 workgroups per task (partials summed in part order by
+    the next level, which also writes their final values) solve K x = b as the unsplit sweep
+    does, to rounding, and 5 steps of the 40k drape agree to 1e-10."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    probe = gpu.Simulation(spec, precision=precision, record_iterations=False)
+    L = probe.info()["n_levels_swept"]
+    assert L >= 3
+    base = re.sub(r";p=[\d,]+", "", probe.tuning())
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(3 * spec.n_vertices)
+    out = {}
+    for name, parts in (("whole", [1] * L), ("two", [2] * (L - 1) + [1]), ("four", [4] * (L - 1) + [1]),
+                        ("mixed", [4, 1, 2] + [1] * (L - 3))):
+        monkeypatch.setenv("PDFC_TUNE", base + ";p=" + ",".join(map(str, parts)))
+        s = gpu.Simulation(spec, precision=precision, record_iterations=False)
+        assert s.info()["split_backward"][:L] == parts
+        x = s.solve(b)
+        s.step(5)
+        out[name] = (x, s.state())
+    xw, (qw, vw) = out["whole"]
+    for name in ("two", "four", "mixed"):
+        x, (q, v) = out[name]
+        print(name, "solve rel diff %.3e, q %.3e, v %.3e" % (rel_err(x, xw), rel_err(q, qw), rel_err(v, vw)), flush=True)
+        assert rel_err(x, xw) < 1e-12
+        assert rel_err(q, qw) < 1e-10 and rel_err(v, vw) < 1e-10

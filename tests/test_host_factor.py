@@ -40,3 +40,18 @@ def test_merged_launch_plans(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ok (0 failures)" in r.stdout
+
+
+def test_split_backward_plans(tmp_path):
+    """Backward reductions split over 2 or 4 workgroups: a launch-by-launch replay of the split
+    plans reads only values completed by earlier launches, keeps every index in range, finishes
+    every row, and equals the unsplit sweep to rounding."""
+    exe = str(tmp_path / "check_split")
+    src = [os.path.join(ROOT, "tests", "native", "check_split.cpp"),
+           os.path.join(ROOT, "climatenet_amd", "csrc", "model.cpp"),
+           os.path.join(ROOT, "climatenet_amd", "csrc", "factor.cpp")]
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include")] + src + ["-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok (0 failures)" in r.stdout
