@@ -139,7 +139,10 @@ def kernel_roofline(sim, info, timing_steps, traffic_name):
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, info, traffic_name),
                 "kernel": dom, "avg_launch_us": round(us, 3), "algorithmic_bytes_per_launch": bytes_per_launch}
-    return roofline, {k: round(v[0], 3) for k, v in fam.items()}
+    times = {k: round(v[0], 3) for k, v in fam.items()}
+    if kt["self_iterations"] > 0:
+        times["k_self_levels"] = round(kt["self_us"], 3)  # all self-contact levels of an iteration
+    return roofline, times
 
 
 def cpu_baseline_batched(specs, q, v, ranges, budget_s=20.0, runs=3):

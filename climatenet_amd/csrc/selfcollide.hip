@@ -606,23 +606,49 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     double* c_rem = sh + 6 * kSelfLdsMaxSlots;           // 3 per contact
     double* c_add = c_rem + 3 * kSelfLdsMaxContacts;     // 3 per contact
     const int t = threadIdx.x;
-    for (int k = t; k < 3 * S.nslots; k += kSelfFusedThreads) {
-        scf_c[k] = S.SCFc[k];
-        scf_n[k] = S.SCFn[k];
+    // Every load of the LDS-resident data is issued before the first LDS store (a loop that
+    // stores each value as it arrives waits a memory round trip per element it handles):
+    // at most 6 force components per buffer, 2 groups and 2 entries per thread.
+    static_assert(3 * kSelfLdsMaxSlots <= 6 * kSelfFusedThreads && 2 * kSelfLdsMaxContacts <= 2 * kSelfFusedThreads,
+                  "LDS-resident data per thread");
+    const int n3s = 3 * S.nslots, G = S.grp_ptr[nlev], E = 2 * S.nsc;
+    double fc[6], fn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int k = t + q * kSelfFusedThreads;
+        fc[q] = k < n3s ? S.SCFc[k] : 0.;
+        fn[q] = k < n3s ? S.SCFn[k] : 0.;
     }
-    const int G = S.grp_ptr[nlev];
-    for (int k = t; k <= G; k += kSelfFusedThreads) {
-        if (k < G) g_slot[k] = (unsigned short)S.grp_slot[k];
-        g_ent[k] = (unsigned short)S.ent_ptr[k];
+    int gs[2], ge[2], ec[2], es[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k = t + q * kSelfFusedThreads;
+        gs[q] = k < G ? S.grp_slot[k] : 0;
+        ge[q] = k <= G ? S.ent_ptr[k] : 0;
+        ec[q] = k < E ? S.ent_contact[k] : 0;
+        es[q] = k < E ? S.ent_sign[k] : 0;
     }
-    for (int e = t; e < 2 * S.nsc; e += kSelfFusedThreads)
-        e_code[e] = (unsigned short)(2 * S.ent_contact[e] + (S.ent_sign[e] > 0 ? 1 : 0));
     static_assert(kSelfLdsMaxSlots <= 2 * kSelfFusedThreads, "two involved vertices per thread");
     int vv[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int sl = t + q * kSelfFusedThreads;
         vv[q] = sl < S.nslots ? S.slot_vertex[sl] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int k = t + q * kSelfFusedThreads;
+        if (k < n3s) {
+            scf_c[k] = fc[q];
+            scf_n[k] = fn[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k = t + q * kSelfFusedThreads;
+        if (k < G) g_slot[k] = (unsigned short)gs[q];
+        if (k <= G) g_ent[k] = (unsigned short)ge[q];
+        if (k < E) e_code[k] = (unsigned short)(2 * ec[q] + (es[q] > 0 ? 1 : 0));
     }
     const int s = t;
     const bool mine = s < S.nsc;
@@ -654,7 +680,17 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
             acm = acs;
         }
     }
+    // the first involved vertex's right-hand side and obstacle reaction, read now so that the
+    // final update waits for no load (they do not change during the levels)
+    P3 rhs0{}, cf0{};
+    if (t < S.nslots) {
+        const int w = vv[0];
+        rhs0 = P3{rhs[w], rhs[n + w], rhs[2 * n + w]};
+        cf0 = ld_slot(S.CFn, t);
+    }
     __syncthreads();
+    // the residual's atomics complete while the levels run
+    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
     for (int l = 0; l < nlev; ++l) {
         if (lev == l) {
             const P3 ca = {scf_c[3 * sa], scf_c[3 * sa + 1], scf_c[3 * sa + 2]};
@@ -698,16 +734,16 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         const int sl = t + q * kSelfFusedThreads;
         if (sl >= S.nslots) break;
         const int w = vv[q];
-        const P3 c = ld_slot(S.CFn, sl);
+        const P3 c = q == 0 ? cf0 : ld_slot(S.CFn, sl);
+        const P3 b0 = q == 0 ? rhs0 : P3{rhs[w], rhs[n + w], rhs[2 * n + w]};
         const P3 sc = {scf_n[3 * sl], scf_n[3 * sl + 1], scf_n[3 * sl + 2]};
-        rhs[w] += c.x + sc.x;
-        rhs[n + w] += c.y + sc.y;
-        rhs[2 * n + w] += c.z + sc.z;
+        rhs[w] = b0.x + (c.x + sc.x);
+        rhs[n + w] = b0.y + (c.y + sc.y);
+        rhs[2 * n + w] = b0.z + (c.z + sc.z);
         st_slot(S.SCFn, sl, sc);
         st_slot(S.CFc, sl, P3{0., 0., 0.});
         st_slot(S.SCFc, sl, P3{0., 0., 0.});
     }
-    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
 }
 
 // All levels of one iteration in one workgroup (few self contacts): each level's updates,
