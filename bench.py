@@ -217,6 +217,7 @@ def bench_batched(args):
                    "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
                    "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
                    "split_backward": info["split_backward"][:info["n_levels_swept"]],
+                   "top_pregather": info["top_pregather"],
                    "top_chunks": info["top_chunks"], "top_fold": info["top_fold"], "nnz_F": info["nnz_F"],
                    "autotune_ms": round(info["autotune_ms"], 1)},
         "sequential": None if args.no_sequential else {
@@ -456,8 +457,9 @@ def main():
                    "levels_per_launch_forward": info["parts_forward"][:info["launches_forward"]],
                    "levels_per_launch_backward": info["parts_backward"][:info["launches_backward"]],
                    "split_backward": info["split_backward"][:info["n_levels_swept"]],
+                   "top_pregather": info["top_pregather"],
                    "leaf_size": info["leaf_size"], "top_chunks": info["top_chunks"], "top_fold": info["top_fold"],
-                   "autotune_ms": round(info["autotune_ms"], 1)},
+                   "autotune_ms": round(info["autotune_ms"], 1), "tuning": sim.tuning()},
         "roofline": roofline,
         # whole step: the algorithmic bytes of its iterations at the measured rate
         "step_roofline": {"bytes_per_step": info["bytes_per_iteration"] * K,

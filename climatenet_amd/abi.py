@@ -167,7 +167,7 @@ class Info(C.Structure):
         ("parts_forward", C.c_int32 * 32),
         ("parts_backward", C.c_int32 * 32),
         ("split_backward", C.c_int32 * 32),
-        ("reserved_info", C.c_int32),
+        ("top_pregather", C.c_int32),
     ]
 
 

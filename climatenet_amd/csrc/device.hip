@@ -1428,7 +1428,11 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
     if (wave < 2 && (wave == 0 || I != J)) {
         const int t = (wave == 0 ? r0 : c0) + lane;
         double bx = 0., by = 0., bz = 0.;
-        if (t < m) {
+        if (t < m && Tp.pregather) {
+            bx = Tp.bt[t];
+            by = Tp.bt[m + t];
+            bz = Tp.bt[2 * m + t];
+        } else if (t < m) {
             const int rw = Tp.rows[t];
             bx = B[rw];
             by = B[n + rw];
@@ -1710,6 +1714,7 @@ void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, do
                 bool finish) {
     if (T.m <= 0) return;
     if (T.nb > 0) {
+        if (T.pregather) hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
         if (fp32)
             hipLaunchKernelGGL(k_top_tile<float>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
         else

@@ -347,6 +347,7 @@ struct DevTop {
     int ntiles;             // nb (nb + 1) / 2 block pairs I <= J
     const int* tiles;       // 2 per tile: I, J
     const void* Stile;      // the tiles' blocks of S^-1, 4096 entries each (double or float)
+    int pregather;          // tiles: b_T gathered into bt by its own launch first (else by every tile)
 };
 // x_T = S^-1 (b_T - lower updates). Split matvec: gather, partial products over kchunks
 // column ranges, ordered reduction + scatter. Symmetric tiles: one workgroup per block pair

@@ -1068,6 +1068,12 @@ void pdfc_sim::apply_pinned_tuning(const char* spec) {
             throw std::invalid_argument("PDFC_TUNE: symmetric tiles are off (PDFC_TOP_SYM=0, or more than 128 row blocks)");
         set_top_mode(t[0]);
     }
+    // q: symmetric tiles read b_T gathered by a launch of its own (1) or gather it themselves (0)
+    std::vector<int> qv;
+    if (parse_list('q', qv) && use_top) {
+        if (qv.size() != 1 || (qv[0] != 0 && qv[0] != 1)) throw std::invalid_argument("PDFC_TUNE: q=0 or q=1");
+        dtop.pregather = qv[0];
+    }
     // g / h: levels per forward / backward launch in sweep order (a partition of the levels)
     std::vector<int> g, h;
     const bool pg = parse_list('g', g), ph = parse_list('h', h);
@@ -1227,6 +1233,11 @@ void pdfc_sim::autotune_waves() {
         }
         set_top_mode(chosen.mode);
         top_fold = chosen.fold;
+        // symmetric tiles: b_T gathered once by a launch of its own instead of by every tile
+        if (dtop.nb > 0) {
+            dtop.pregather = 1;
+            if (!(solve_ms() < kSwitchLevels * best)) dtop.pregather = 0;
+        }
     }
     for (int pass = 0; pass < 2; ++pass) {
         std::vector<int>& waves = pass == 0 ? fwd_waves : bwd_waves;
@@ -1267,6 +1278,14 @@ void pdfc_sim::autotune_waves() {
             }
             bwd_split[l] = best_p;
             build_backward();
+        }
+        // a split first level gathers fewer top rows per task, which changes what folding the
+        // top block's finish into it costs: decide the fold again
+        if (use_top && top_fold_allowed && any_split()) {
+            const bool keep = top_fold;
+            float best = solve_ms();
+            top_fold = !keep;
+            if (!(solve_ms() < kSwitch * best)) top_fold = keep;
         }
     }
     // Merged launches (PDFC_MERGE=1 offers them): every partition of a sweep's levels into at
@@ -2602,6 +2621,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.lhs_seconds = sim->lhs_seconds;
         I.steps_done = sim->steps_done;
         I.top_fold = sim->folded() ? 1 : 0;
+        I.top_pregather = sim->use_top && sim->dtop.nb > 0 && sim->dtop.pregather ? 1 : 0;
         // Algorithmic HBM bytes (FP64 vectors, 32-bit indices):
         //   local kernel: read v, q_n, c and write b (4 x 24 B per vertex); per spring end an
         //   index and kappa (4 + 8 B); per contact the basis (72 B), mu (8 B), r read+write

@@ -225,7 +225,7 @@ class Simulation:
         h = ",".join(str(x) for x in I["parts_backward"][:I["launches_backward"]])
         # t=0: symmetric tiles (or no top block); o: the top block's finish folded or not;
         # g, h: levels per launch of each sweep (merged ranges)
-        out = "f=%s;b=%s;t=%d;o=%d" % (f, b, I["top_chunks"], I["top_fold"])
+        out = "f=%s;b=%s;t=%d;o=%d;q=%d" % (f, b, I["top_chunks"], I["top_fold"], I["top_pregather"])
         if nl > 0:
             out += ";g=%s;h=%s;p=%s" % (g, h, ",".join(str(x) for x in I["split_backward"][:nl]))
         return out

@@ -216,7 +216,7 @@ typedef struct pdfc_info {
     int32_t parts_backward[32];      /* same for the backward sweep (root side first) */
     int32_t split_backward[32];      /* workgroups sharing each backward task's reduction, per backward level
                                         (1: one; 2 or 4: partial sums finished by the next level; autotuned) */
-    int32_t reserved_info;
+    int32_t top_pregather;           /* 1: the symmetric tiles read b_T gathered by a launch of its own */
 } pdfc_info;
 
 typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, microseconds */

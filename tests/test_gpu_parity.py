@@ -361,3 +361,20 @@ def test_split_backward_reductions_match(gpu, monkeypatch, precision):
         print(name, "solve rel diff %.3e, q %.3e, v %.3e" % (rel_err(x, xw), rel_err(q, qw), rel_err(v, vw)), flush=True)
         assert rel_err(x, xw) < 1e-12
         assert rel_err(q, qw) < 1e-10 and rel_err(v, vw) < 1e-10
+
+
+def test_pregathered_tiles_are_bit_identical(gpu, monkeypatch):
+    """The symmetric tiles reading b_T from a gather launch of its own compute exactly what the
+    tiles gathering b_T themselves compute (same pull order): identical states after 5 steps."""
+    spec = scenes.sphere_drape_40k()
+    spec.self_collision = False
+    probe = gpu.Simulation(spec, record_iterations=False)
+    base = re.sub(r";t=\d+;o=\d;q=\d", "", probe.tuning())
+    states = []
+    for q in (0, 1):
+        monkeypatch.setenv("PDFC_TUNE", "%s;t=0;o=0;q=%d" % (base, q))
+        s = gpu.Simulation(spec, record_iterations=False)
+        assert s.info()["top_pregather"] == q
+        s.step(5)
+        states.append(s.state())
+    assert np.array_equal(states[0][0], states[1][0]) and np.array_equal(states[0][1], states[1][1])
