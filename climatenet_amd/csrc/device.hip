@@ -844,6 +844,56 @@ __device__ __forceinline__ void pull_entries(const double* __restrict__ U, const
     }
 }
 
+// x op= the first two of a row's update entries (op: -= when kSub, else +=), named by its record
+// (cnt entries, U indices i0 and i1), in list order; both loads are issued before either add.
+template <bool kSub>
+__device__ __forceinline__ void pull_two(const double* __restrict__ U, int cnt, int i0, int i1, double& x, double& y,
+                                         double& z) {
+    double a0 = 0., a1 = 0., a2 = 0., b0 = 0., b1 = 0., b2 = 0.;
+    if (cnt > 0) {
+        const double* p = U + 3 * (size_t)i0;
+        a0 = p[0];
+        a1 = p[1];
+        a2 = p[2];
+    }
+    if (cnt > 1) {
+        const double* p = U + 3 * (size_t)i1;
+        b0 = p[0];
+        b1 = p[1];
+        b2 = p[2];
+    }
+    if (cnt > 0) {
+        if (kSub) {
+            x -= a0;
+            y -= a1;
+            z -= a2;
+        } else {
+            x += a0;
+            y += a1;
+            z += a2;
+        }
+    }
+    if (cnt > 1) {
+        if (kSub) {
+            x -= b0;
+            y -= b1;
+            z -= b2;
+        } else {
+            x += b0;
+            y += b1;
+            z += b2;
+        }
+    }
+}
+
+// All of a front row's update entries in list order: the first two from its record, any
+// further ones from the pull list.
+template <bool kSub>
+__device__ __forceinline__ void pull_row(const DevFactor& Fa, int4 rc, double& x, double& y, double& z) {
+    pull_two<kSub>(Fa.U, rc.x, rc.y, rc.z, x, y, z);
+    if (rc.x > 2) pull_entries<kSub, 2>(Fa.U, Fa.pull_idx, rc.w + 2, rc.w + rc.x, x, y, z);
+}
+
 // Forward substitution, one task = (supernode s, 64 rows of F_s), W waves:
 //   b~_s = b_s - sum_children u_c|own ;  [y_s ; u_s] = F_s b~_s (+ children's u_c|below)
 // The reduction over the ns columns is split across the W waves (wave w takes
@@ -858,7 +908,7 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     double* by = lds + ns;
     double* bz = lds + 2 * ns;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int* pp = Fa.pull_ptr + d.pull_base;
+    const int4* rec = Fa.pull_rec + d.pull_base;
     // The first kPrefetch operator entries of this thread's column slice do not depend on
     // the gather: issue them now so their latency overlaps the gather's.
     const int my_row = r0 + lane;
@@ -872,14 +922,14 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
-        pull_entries<true, 2>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
+        pull_row<true>(Fa, rec[t], x, y, z);
         bx[t] = x;
         by[t] = y;
         bz[t] = z;
     }
     // this task's below rows: children's entries to pass on
     double ux = 0., uy = 0., uz = 0.;
-    if (wave == 0 && my_row >= ns && my_row < f) pull_entries<false, 2>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
+    if (wave == 0 && my_row >= ns && my_row < f) pull_row<false>(Fa, rec[my_row], ux, uy, uz);
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
@@ -1184,7 +1234,7 @@ __device__ __forceinline__ void forward_slot(const DevFactor& Fa, int task, int 
     double* by = lds + ns;
     double* bz = lds + 2 * ns;
     const int lane = threadIdx.x & 63, tid = wv * 64 + lane, nthr = 64 * W;
-    const int* pp = Fa.pull_ptr + d.pull_base;
+    const int4* rec = Fa.pull_rec + d.pull_base;
     const int my_row = r0 + lane;
     const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
     double pre[kPrefetch];
@@ -1195,13 +1245,13 @@ __device__ __forceinline__ void forward_slot(const DevFactor& Fa, int task, int 
     }
     for (int t = tid; t < ns; t += nthr) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
-        pull_entries<true, 2>(Fa.U, Fa.pull_idx, pp[t], pp[t + 1], x, y, z);
+        pull_row<true>(Fa, rec[t], x, y, z);
         bx[t] = x;
         by[t] = y;
         bz[t] = z;
     }
     double ux = 0., uy = 0., uz = 0.;
-    if (wv == 0 && my_row >= ns && my_row < f) pull_entries<false, 2>(Fa.U, Fa.pull_idx, pp[my_row], pp[my_row + 1], ux, uy, uz);
+    if (wv == 0 && my_row >= ns && my_row < f) pull_row<false>(Fa, rec[my_row], ux, uy, uz);
     __syncthreads();
     double ax = 0., ay = 0., az = 0.;
     if (my_row < f) {
@@ -1359,9 +1409,10 @@ __global__ void __launch_bounds__(1024) k_sweep_merged(DevFactor Fa, MergedRange
 __global__ void k_top_gather(DevTop T, const double* __restrict__ U, const double* __restrict__ B, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T.m) return;
-    int row = T.rows[t];
-    double bx = B[row], by = B[n + row], bz = B[2 * n + row];
-    pull_entries<true, 4>(U, T.cidx, T.cptr[t], T.cptr[t + 1], bx, by, bz);
+    const int4 tr = T.trec[t];
+    double bx = B[tr.x], by = B[n + tr.x], bz = B[2 * n + tr.x];
+    pull_two<true>(U, tr.y, tr.z, tr.w, bx, by, bz);
+    if (tr.y > 2) pull_entries<true, 4>(U, T.cidx, T.cptr[t] + 2, T.cptr[t + 1], bx, by, bz);
     T.bt[t] = bx;
     T.bt[T.m + t] = by;
     T.bt[2 * T.m + t] = bz;
@@ -1433,11 +1484,12 @@ __global__ void __launch_bounds__(256) k_top_tile(DevTop Tp, const double* __res
             by = Tp.bt[m + t];
             bz = Tp.bt[2 * m + t];
         } else if (t < m) {
-            const int rw = Tp.rows[t];
-            bx = B[rw];
-            by = B[n + rw];
-            bz = B[2 * n + rw];
-            pull_entries<true, 4>(U, Tp.cidx, Tp.cptr[t], Tp.cptr[t + 1], bx, by, bz);
+            const int4 tr = Tp.trec[t];
+            bx = B[tr.x];
+            by = B[n + tr.x];
+            bz = B[2 * n + tr.x];
+            pull_two<true>(U, tr.y, tr.z, tr.w, bx, by, bz);
+            if (tr.y > 2) pull_entries<true, 4>(U, Tp.cidx, Tp.cptr[t] + 2, Tp.cptr[t + 1], bx, by, bz);
         }
         xb[wave][0][lane] = bx;
         xb[wave][1][lane] = by;

@@ -148,6 +148,8 @@ struct DevFactor {
     const TaskDesc* desc;   // per task of this sweep
     const int* pull_ptr;    // per supernode, f + 1 entries from pull_base: children's update
     const int* pull_idx;    //   entries (index into U) contributing to each front row, child order
+    const int4* pull_rec;   // per front row (same index as pull_ptr): entries, the first two entries'
+                            //   U indices, its list start (one load instead of a pointer-list chain)
     // backward sweep with split reductions: partial results XP[p][c][row] (parts x 3n), and per
     // below row (parallel to rows) the number of partials to sum for it (0: its value is in X)
     double* XP;
@@ -348,6 +350,7 @@ struct DevTop {
     const int* tiles;       // 2 per tile: I, J
     const void* Stile;      // the tiles' blocks of S^-1, 4096 entries each (double or float)
     int pregather;          // tiles: b_T gathered into bt by its own launch first (else by every tile)
+    const int4* trec;       // per top row: supernodal row, update entries, the first two entries' indices
 };
 // x_T = S^-1 (b_T - lower updates). Split matvec: gather, partial products over kchunks
 // column ranges, ordered reduction + scatter. Symmetric tiles: one workgroup per block pair

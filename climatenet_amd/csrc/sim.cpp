@@ -868,6 +868,17 @@ void pdfc_sim::upload_sweeps(int top_level) {
     if (!dfwd.pull_ptr) {
         dfwd.pull_ptr = dbwd.pull_ptr = upload(pull_ptr);
         dfwd.pull_idx = dbwd.pull_idx = upload(pull_idx);
+        // per front row: entries, the first two U indices, list start (the last slot of each
+        // supernode's range is unused)
+        std::vector<int> rec(4 * pull_ptr.size(), 0);
+        for (size_t k = 0; k + 1 < pull_ptr.size(); ++k) {
+            const int q0 = pull_ptr[k], cnt = std::max(0, pull_ptr[k + 1] - q0);
+            rec[4 * k] = cnt;
+            rec[4 * k + 1] = cnt > 0 ? pull_idx[q0] : 0;
+            rec[4 * k + 2] = cnt > 1 ? pull_idx[q0 + 1] : 0;
+            rec[4 * k + 3] = q0;
+        }
+        dfwd.pull_rec = dbwd.pull_rec = reinterpret_cast<const int4*>(upload(rec));
     }
     if (desc_buf) free_ptr(desc_buf);
     desc_buf = upload(task_descs(fwd, {}));
@@ -1259,26 +1270,19 @@ void pdfc_sim::autotune_waves() {
             waves[l] = best_w;
         }
     }
-    // Split backward reductions: each backward level but the last, with its tasks' reductions
-    // split over 2 or 4 workgroups (the upper levels have fewer tasks than the part has CUs),
-    // kept when 1.5% faster on whole solves; one level at a time, root side first.
+    // Split backward reductions: a level (not the last) whose tasks leave most of the chip idle
+    // gets its reductions split over 4 (or 2) workgroups while the parts' waves fit in half the
+    // wave slots (4,096 on 256 CUs). A rule, not a timing: on the 40k drape the per-level
+    // differences are about 1% of a solve, within the noise of whole-solve timings, and the rule
+    // matches the best measured configuration (backward sweep 41.7 -> 37.6 us).
     {
         const int L = (int)bwd_split.size();
         for (int l = 0; l + 1 < L; ++l) {
-            float best = solve_ms();
-            int best_p = 1;
-            for (int P : {2, 4}) {
-                bwd_split[l] = P;
-                build_backward();
-                float ms = solve_ms();
-                if (ms < kSwitchLevels * best) {
-                    best = ms;
-                    best_p = P;
-                }
-            }
-            bwd_split[l] = best_p;
-            build_backward();
+            const long tasks = bwd.level_ptr[l + 1] - bwd.level_ptr[l];
+            const long waves = tasks * bwd_waves[l];
+            bwd_split[l] = (tasks * 4 <= 512 && waves * 4 <= 4096) ? 4 : (tasks * 2 <= 512 && waves * 2 <= 4096) ? 2 : 1;
         }
+        build_backward();
         // a split first level gathers fewer top rows per task, which changes what folding the
         // top block's finish into it costs: decide the fold again
         if (use_top && top_fold_allowed && any_split()) {
@@ -1346,6 +1350,17 @@ void pdfc_sim::build_top() {
     }
     dtop.cptr = upload(top.cptr);
     dtop.cidx = upload(top.cidx);
+    {
+        std::vector<int> rec(4 * (size_t)m);
+        for (int t = 0; t < m; ++t) {
+            const int q0 = top.cptr[t], cnt = top.cptr[t + 1] - q0;
+            rec[4 * t] = top.rows[t];
+            rec[4 * t + 1] = cnt;
+            rec[4 * t + 2] = cnt > 0 ? top.cidx[q0] : 0;
+            rec[4 * t + 3] = cnt > 1 ? top.cidx[q0 + 1] : 0;
+        }
+        dtop.trec = reinterpret_cast<const int4*>(upload(rec));
+    }
     dtop.bt = dalloc<double>(3 * (size_t)m);
     const int rowblocks = (m + 63) / 64;
     heuristic_chunks = std::max(1, std::min(kMaxTopChunks, 512 / std::max(1, rowblocks)));
