@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 
 #include "device.hpp"
@@ -1661,10 +1662,15 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
                   const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,
                   const LocalSelf& ls, hipStream_t st) {
     const unsigned blocks = blocks_for(M.n);
-    if (blocks <= kLocalLowLatencyBlocks)
-        hipLaunchKernelGGL(k_local<true>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
-                           iteration, ls);
-    else
+    if (blocks <= kLocalLowLatencyBlocks) {
+        static const int lb = [] {
+            const char* e = std::getenv("PDFC_LOCAL_BLOCK");
+            const int v = e ? std::atoi(e) : 64;
+            return (v == 64 || v == 128 || v == 256) ? v : 64;
+        }();
+        hipLaunchKernelGGL(k_local<true>, dim3((M.n + lb - 1) / lb), dim3(lb), 0, st, M, qn, vcur, cvec, rhs, ccnt,
+                           cstart, C, a, iteration, ls);
+    } else
         hipLaunchKernelGGL(k_local<false>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C,
                            a, iteration, ls);
 }
