@@ -1564,6 +1564,9 @@ void pdfc_sim::restore_checkpoint(const unsigned char* p, size_t size) {
                     for (int d = 0; d < 3; ++d) dev[3 * (sl * n + k) + d] = g[3 * (sl * n + fac.perm[k]) + d];
         HIP_CHECK(hipMemcpyAsync(prev_g, dev.data(), dev.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     }
+    // restored mesh positions reach the device (and a static mesh's grid is rebuilt) now, as at
+    // creation, so that no step (nor a graph capture) has uploads left to do
+    mesh_refresh();
     HIP_CHECK(hipStreamSynchronize(stream));
     given_pending = false;
     steps_done = csteps;
