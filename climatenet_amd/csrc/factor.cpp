@@ -29,18 +29,10 @@ struct NDState {
 
 double coord(const Model& M, int v, int ax) { return M.rest[3 * v + ax]; }
 
-// Recursive coordinate bisection on the graph of K (the pattern of A^T A). The
-// smaller of the two one-sided boundaries becomes the separator; separators and
-// leaves become supernodes, emitted in post-order.
-void dissect(NDState& st, std::vector<int>& set) {
+// One coordinate bisection of `set` on the graph of K: the smaller of the two one-sided
+// boundaries becomes the separator `sep`, the rest of each side L and R. `set` is consumed.
+void bisect(NDState& st, std::vector<int>& set, std::vector<int>& L, std::vector<int>& R, std::vector<int>& sep) {
     const Model& M = *st.M;
-    if ((int)set.size() <= st.leaf) {
-        if (set.empty()) return;
-        std::sort(set.begin(), set.end());
-        st.order.insert(st.order.end(), set.begin(), set.end());
-        st.bounds.push_back((int)st.order.size());
-        return;
-    }
     double lo[3], hi[3];
     for (int k = 0; k < 3; ++k) lo[k] = hi[k] = coord(M, set[0], k);
     for (int v : set)
@@ -56,14 +48,15 @@ void dissect(NDState& st, std::vector<int>& set) {
     };
     size_t half = set.size() / 2;
     std::nth_element(set.begin(), set.begin() + half, set.end(), less);
-    std::vector<int> L(set.begin(), set.begin() + half), R(set.begin() + half, set.end());
+    L.assign(set.begin(), set.begin() + half);
+    R.assign(set.begin() + half, set.end());
     set.clear();
     set.shrink_to_fit();
-    auto boundary = [&](const std::vector<int>& A, char other, std::vector<int>& sep, std::vector<int>& rest) {
+    auto boundary = [&](const std::vector<int>& A, char other, std::vector<int>& sp, std::vector<int>& rest) {
         for (int v : A) {
             bool touch = false;
             for (int64_t p = M.ata_ptr[v]; p < M.ata_ptr[v + 1] && !touch; ++p) touch = st.side[M.ata_idx[p]] == other;
-            (touch ? sep : rest).push_back(v);
+            (touch ? sp : rest).push_back(v);
         }
     };
     for (int v : L) st.side[v] = 1;
@@ -73,7 +66,7 @@ void dissect(NDState& st, std::vector<int>& set) {
     boundary(R, 1, sepR, restR);
     for (int v : L) st.side[v] = 0;
     for (int v : R) st.side[v] = 0;
-    std::vector<int> sep;
+    sep.clear();
     if (sepL.size() <= sepR.size()) {
         sep.swap(sepL);
         L.swap(restL);
@@ -81,12 +74,61 @@ void dissect(NDState& st, std::vector<int>& set) {
         sep.swap(sepR);
         R.swap(restR);
     }
-    restL.clear(); restR.clear(); sepL.clear(); sepR.clear();
-    dissect(st, L);
-    dissect(st, R);
-    if (!sep.empty()) {
-        std::sort(sep.begin(), sep.end());
-        st.order.insert(st.order.end(), sep.begin(), sep.end());
+}
+
+// Node of the dissection tree: a leaf set, or a separator with the two halves it splits.
+struct NDNode {
+    std::vector<int> cols;  // ascending
+    int kid[2] = {-1, -1};
+    bool leaf = false;
+    int height = 0;         // leaves 0, a separator one above its higher half
+    bool absorb = false;    // its supernode also takes its halves' separators (a cross)
+};
+
+// Recursive coordinate bisection into st-owned nodes; returns the node of `set` (-1 if empty).
+int build_nd(NDState& st, std::vector<NDNode>& T, std::vector<int>& set) {
+    if (set.empty()) return -1;
+    if ((int)set.size() <= st.leaf) {
+        NDNode nd;
+        nd.leaf = true;
+        nd.cols.swap(set);
+        std::sort(nd.cols.begin(), nd.cols.end());
+        T.push_back(std::move(nd));
+        return (int)T.size() - 1;
+    }
+    std::vector<int> L, R, sep;
+    bisect(st, set, L, R, sep);
+    const int a = build_nd(st, T, L);
+    const int b = build_nd(st, T, R);
+    NDNode nd;
+    nd.cols.swap(sep);
+    std::sort(nd.cols.begin(), nd.cols.end());
+    nd.kid[0] = a;
+    nd.kid[1] = b;
+    for (int k : {a, b})
+        if (k >= 0) nd.height = std::max(nd.height, T[k].height + 1);
+    T.push_back(std::move(nd));
+    return (int)T.size() - 1;
+}
+
+// Post-order emission of node i's subtree: leaves and separators become supernodes; a node
+// that absorbs gives one supernode to its halves' separators (their units) and its own. With
+// `out`, node i's unit is appended there instead (it joins its parent's supernode).
+void emit_nd(NDState& st, const std::vector<NDNode>& T, int i, std::vector<int>* out) {
+    const NDNode& nd = T[i];
+    if (nd.leaf) {
+        st.order.insert(st.order.end(), nd.cols.begin(), nd.cols.end());
+        st.bounds.push_back((int)st.order.size());
+        return;
+    }
+    std::vector<int> unit;
+    for (int k : nd.kid)
+        if (k >= 0) emit_nd(st, T, k, nd.absorb && !T[k].leaf ? &unit : nullptr);
+    unit.insert(unit.end(), nd.cols.begin(), nd.cols.end());
+    if (out) {
+        out->insert(out->end(), unit.begin(), unit.end());
+    } else if (!unit.empty()) {
+        st.order.insert(st.order.end(), unit.begin(), unit.end());
         st.bounds.push_back((int)st.order.size());
     }
 }
@@ -108,7 +150,7 @@ void lower_inverse(const double* Lm, int ld, int n, std::vector<double>& inv) {
 
 }  // namespace
 
-void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
+void factorize(const Model& M, double dt2, Factor& F, int leaf_size, double cross_top_bytes, int entry_bytes) {
     auto t0 = std::chrono::steady_clock::now();
     const int n = M.n;
     F.n = n;
@@ -117,6 +159,8 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
     st.side.assign(n, 0);
     st.leaf = std::max(8, leaf_size);
     st.bounds.push_back(0);
+    std::vector<NDNode> T;
+    std::vector<int> roots;
     {
         // Connected components of the graph of K (independent cloth pieces, batched
         // instances) are dissected one after another, so each component gets the
@@ -140,9 +184,33 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size) {
                 }
             }
             std::sort(set.begin(), set.end());
-            dissect(st, set);
+            roots.push_back(build_nd(st, T, set));
         }
     }
+    if (cross_top_bytes > 0.) {
+        // Crosses below the dense top block. The top block will hold the highest levels whose
+        // columns m fit m^2 * entry_bytes <= cross_top_bytes (as plan_top decides, here on
+        // the dissection's heights); the swept separator levels under it are paired from the
+        // top down and each pair becomes one level: a separator at height h absorbs its
+        // halves' separators (height h - 1) for h = top - 1, top - 3, ... >= 2. Fewer levels
+        // mean fewer latency-bound launches per substitution for somewhat denser operators.
+        int H = 0;
+        for (const NDNode& nd : T) H = std::max(H, nd.height);
+        std::vector<double> cols_at(H + 1, 0.);
+        for (const NDNode& nd : T) cols_at[nd.height] += (double)nd.cols.size();
+        int top = H + 1;
+        double m = 0.;
+        for (int h = H; h >= 0; --h) {
+            m += cols_at[h];
+            if (m * m * entry_bytes <= cross_top_bytes) top = h;
+            else break;
+        }
+        for (NDNode& nd : T)
+            nd.absorb = !nd.leaf && !nd.cols.empty() && nd.height >= 2 && nd.height < top &&
+                        (top - 1 - nd.height) % 2 == 0;
+    }
+    for (int r : roots)
+        if (r >= 0) emit_nd(st, T, r, nullptr);
     if ((int)st.order.size() != n) throw std::logic_error("ordering lost vertices");
     F.perm = st.order;
     F.iperm.assign(n, 0);

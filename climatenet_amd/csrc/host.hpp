@@ -80,7 +80,10 @@ struct Factor {
 
 // Nested dissection (coordinate bisection on the graph of K), symbolic and
 // numeric multifrontal factorisation. K = diag(mass) + dt2 * A^T A.
-void factorize(const Model& M, double dt2, Factor& F, int leaf_size = 64);
+// With cross_top_bytes > 0, pairs of separator levels below the dense top block of that budget
+// (entries of entry_bytes) are merged into one level of cross-shaped supernodes.
+void factorize(const Model& M, double dt2, Factor& F, int leaf_size = 64, double cross_top_bytes = 0.,
+               int entry_bytes = 8);
 
 // Work lists of one substitution sweep: (supernode, first row) tasks of at most
 // `chunk` rows, grouped by level (forward: leaves first; backward: root first).

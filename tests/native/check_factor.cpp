@@ -15,6 +15,7 @@ using namespace pdfc;
 int main(int argc, char** argv) {
     int N = argc > 1 ? atoi(argv[1]) : 20;
     int leaf = argc > 2 ? atoi(argv[2]) : 64;
+    double cross_mb = argc > 3 ? atof(argv[3]) : 0.;  // > 0: crosses below a top block of this budget
     double side = 1.0, dt = 0.005;
     int nv = (N + 1) * (N + 1);
     std::vector<double> pos(3 * nv);
@@ -38,7 +39,7 @@ int main(int argc, char** argv) {
     Model M;
     build_model(sc, M);
     Factor F;
-    factorize(M, dt * dt, F, leaf);
+    factorize(M, dt * dt, F, leaf, cross_mb * 1048576.);
     const int n = nv, S = (int)F.ns.size();
     std::mt19937_64 rng(1);
     std::uniform_real_distribution<double> U(-1, 1);

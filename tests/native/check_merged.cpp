@@ -127,8 +127,10 @@ int main() {
         sc.stretching = 2e2;
         Model M;
         build_model(sc, M);
+        for (int cross : {0, 1}) {
+        // cross: separator levels below a 1 MiB top block merged in pairs, as the library orders them
         Factor F;
-        factorize(M, 0.005 * 0.005, F, cs.leaf);
+        factorize(M, 0.005 * 0.005, F, cs.leaf, cross ? 1048576. : 0.);
         std::vector<int> owner(F.n);
         for (size_t s = 0; s < F.ns.size(); ++s)
             for (int j = 0; j < F.ns[s]; ++j) owner[F.c0[s] + j] = (int)s;
@@ -152,8 +154,9 @@ int main() {
                         }
                     }
                 }
-            printf("%s: %zu supernodes, %d levels, top at %d: %d swept levels, %d ranges checked\n", cs.name,
-                   F.ns.size(), F.n_levels, top_level, L, ranges);
+            printf("%s%s: %zu supernodes, %d levels, top at %d: %d swept levels, %d ranges checked\n", cs.name,
+                   cross ? " (crosses)" : "", F.ns.size(), F.n_levels, top_level, L, ranges);
+        }
         }
     }
     printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);

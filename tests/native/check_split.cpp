@@ -93,8 +93,10 @@ int main() {
         sc.stretching = 2e2;
         Model M;
         build_model(sc, M);
+        for (int cross : {0, 1}) {
+        // cross: separator levels below a 1 MiB top block merged in pairs, as the library orders them
         Factor F;
-        factorize(M, 0.005 * 0.005, F, cs.leaf);
+        factorize(M, 0.005 * 0.005, F, cs.leaf, cross ? 1048576. : 0.);
         const int n = F.n;
         for (int drop : {0, 3}) {
             const int top_level = F.n_levels - drop;
@@ -198,8 +200,9 @@ int main() {
                 CHECK(err < 1e-12, "split result differs from the unsplit sweep by %.3e", err);
                 ++checked;
             }
-            printf("%s: top at %d of %d levels, %d swept, %d split configurations replayed\n", cs.name, top_level,
-                   F.n_levels, L, checked);
+            printf("%s%s: top at %d of %d levels, %d swept, %d split configurations replayed\n", cs.name,
+                   cross ? " (crosses)" : "", top_level, F.n_levels, L, checked);
+        }
         }
     }
     printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);

@@ -347,8 +347,9 @@ def test_split_backward_reductions_match(gpu, monkeypatch, precision):
     rng = np.random.default_rng(12)
     b = rng.standard_normal(3 * spec.n_vertices)
     out = {}
+    mixed = ([4, 1, 2][:L - 1] + [1] * L)[:L - 1] + [1]  # the last (leaf) level is never split
     for name, parts in (("whole", [1] * L), ("two", [2] * (L - 1) + [1]), ("four", [4] * (L - 1) + [1]),
-                        ("mixed", [4, 1, 2] + [1] * (L - 3))):
+                        ("mixed", mixed)):
         monkeypatch.setenv("PDFC_TUNE", base + ";p=" + ",".join(map(str, parts)))
         s = gpu.Simulation(spec, precision=precision, record_iterations=False)
         assert s.info()["split_backward"][:L] == parts

@@ -20,9 +20,12 @@ def checker(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("N,leaf", [(1, 8), (3, 8), (20, 32), (20, 64), (60, 32), (60, 128)])
-def test_supernodal_solve_residual(checker, N, leaf):
-    r = subprocess.run([checker, str(N), str(leaf)], capture_output=True, text=True)
+@pytest.mark.parametrize("N,leaf,cross_mb", [(1, 8, 0), (3, 8, 0), (20, 32, 0), (20, 64, 0), (60, 32, 0), (60, 128, 0),
+                                             (20, 8, 0.1), (60, 32, 1), (90, 64, 2)])
+def test_supernodal_solve_residual(checker, N, leaf, cross_mb):
+    """cross_mb > 0: separator levels below a top block of that budget merged in pairs (cross-shaped
+    supernodes, the library's default ordering)."""
+    r = subprocess.run([checker, str(N), str(leaf), str(cross_mb)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
 
