@@ -83,6 +83,7 @@ struct NDNode {
     bool leaf = false;
     int height = 0;         // leaves 0, a separator one above its higher half
     bool absorb = false;    // its supernode also takes its halves' separators (a cross)
+    int snode = -1;         // its supernode in the ordering without crosses (chooser's scratch)
 };
 
 // Recursive coordinate bisection into st-owned nodes; returns the node of `set` (-1 if empty).
@@ -133,6 +134,218 @@ void emit_nd(NDState& st, const std::vector<NDNode>& T, int i, std::vector<int>*
     }
 }
 
+// Below-row sets of the supernodes of an ordering (perm: new -> old, bounds: supernode column
+// boundaries), in the new numbering; the symbolic step of the factorisation.
+void below_rows(const Model& M, const std::vector<int>& perm, const std::vector<int>& bounds,
+                std::vector<std::vector<int>>& R) {
+    const int n = (int)perm.size(), S = (int)bounds.size() - 1;
+    std::vector<int> iperm(n), snode_of(n), mark(n, -1);
+    for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
+    for (int s = 0; s < S; ++s)
+        for (int j = bounds[s]; j < bounds[s + 1]; ++j) snode_of[j] = s;
+    std::vector<std::vector<int>> kids(S);
+    R.assign(S, {});
+    for (int s = 0; s < S; ++s) {
+        const int cend = bounds[s + 1];
+        std::vector<int>& buf = R[s];
+        for (int j = bounds[s]; j < cend; ++j)
+            for (int64_t p = M.ata_ptr[perm[j]]; p < M.ata_ptr[perm[j] + 1]; ++p) {
+                int i = iperm[M.ata_idx[p]];
+                if (i >= cend && mark[i] != s) { mark[i] = s; buf.push_back(i); }
+            }
+        for (int c : kids[s])
+            for (int i : R[c])
+                if (i >= cend && mark[i] != s) { mark[i] = s; buf.push_back(i); }
+        if (!buf.empty()) kids[snode_of[*std::min_element(buf.begin(), buf.end())]].push_back(s);
+    }
+}
+
+// Cost model of one substitution (forward + backward sweep and the dense top block) on MI355X,
+// in microseconds, fitted to measurements of the 40k-vertex drape's orderings (3 to 5 swept
+// levels, top blocks of 1,594 to 3,946 rows) and of the 64-cloth batch (6 and 9 levels): a swept
+// level costs a launch-and-dependent-load latency, plus its operator bytes (F forward, F^T
+// backward) at a rate that depends on where they stream from. When the sweeps' working set
+// (F, F^T and the top block's tiles) fits the 256 MB last-level cache it is replayed from there
+// every iteration (the drape: 0.33 us per MB of F); far beyond it, from HBM (the batch: 0.56).
+// The top block costs a fixed part (gather, tiles and finish launches) plus its tiles' bytes.
+constexpr double kLevelUs = 12.0;           // forward + backward launch of one level
+constexpr double kLevelUsPerMBCached = 0.33;
+constexpr double kLevelUsPerMBHbm = 0.56;
+constexpr double kCachedMB = 192.;          // working set served from the last-level cache
+constexpr double kTopUs = 9.5;              // gather, tiles and finish launches
+constexpr double kTopUsPerMB = 0.25;
+
+// Chooses the crosses: the top-block height T (its columns m fit m^2 * e <= max_top_bytes) and
+// the grouping of the separator heights T - 1 .. 1 into levels (groups of at most 3 consecutive
+// heights, from the top; `forced` fixes the group sizes instead), minimising the cost model.
+// Marks the absorbing nodes and returns the top block's budget for that choice.
+double choose_crosses(const Model& M, int leaf, std::vector<NDNode>& T, const std::vector<int>& roots,
+                      double max_top_bytes, int e, const std::vector<int>& forced) {
+    int H = 0;
+    for (const NDNode& nd : T) H = std::max(H, nd.height);
+    std::vector<double> cols_at(H + 1, 0.);
+    for (const NDNode& nd : T) cols_at[nd.height] += (double)nd.cols.size();
+    // the ordering without crosses and its below-row sets
+    NDState st0;
+    st0.M = &M;
+    st0.leaf = leaf;
+    st0.bounds.push_back(0);
+    for (NDNode& nd : T) nd.absorb = false;
+    for (int r : roots)
+        if (r >= 0) emit_nd(st0, T, r, nullptr);
+    {
+        // node -> supernode of that ordering (emission order: post-order, non-empty nodes)
+        int next = 0;
+        std::function<void(int)> walk = [&](int i) {
+            for (int k : T[i].kid)
+                if (k >= 0) walk(k);
+            T[i].snode = T[i].cols.empty() ? -1 : next++;
+        };
+        for (int r : roots)
+            if (r >= 0) walk(r);
+    }
+    std::vector<std::vector<int>> R;
+    below_rows(M, st0.order, st0.bounds, R);
+    // operator entries (ns + nb) ns of node i's supernode when it absorbs g - 1 levels below it
+    const int n = M.n;
+    std::vector<int> stamp(n, -1);
+    int tick = 0;
+    auto group_entries = [&](int i, int g) -> double {
+        std::vector<int> members;
+        std::function<void(int, int)> collect = [&](int j, int left) {
+            if (T[j].leaf) return;
+            if (T[j].snode >= 0) members.push_back(T[j].snode);
+            if (left > 1)
+                for (int k : T[j].kid)
+                    if (k >= 0) collect(k, left - 1);
+        };
+        collect(i, g);
+        ++tick;
+        double ns = 0.;
+        for (int s : members) {
+            ns += st0.bounds[s + 1] - st0.bounds[s];
+            for (int c = st0.bounds[s]; c < st0.bounds[s + 1]; ++c) stamp[c] = tick;
+        }
+        double nb = 0.;
+        for (int s : members)
+            for (int r : R[s])
+                if (stamp[r] != tick) {
+                    stamp[r] = tick;
+                    nb += 1.;
+                }
+        return (ns + nb) * ns;
+    };
+    std::vector<std::array<double, 3>> ent(T.size(), {0., 0., 0.});
+    double leaf_entries = 0.;
+    for (size_t i = 0; i < T.size(); ++i) {
+        if (T[i].leaf) {
+            const int s = T[i].snode;
+            const double ns = st0.bounds[s + 1] - st0.bounds[s];
+            leaf_entries += (ns + (double)R[s].size()) * ns;
+            continue;
+        }
+        if (T[i].cols.empty()) continue;
+        for (int g = 1; g <= 3; ++g) ent[i][g - 1] = group_entries((int)i, g);
+    }
+    const double MB = e / 1e6;
+    {
+        // Merging levels trades operator bytes for launches, which pays while the levels are
+        // latency-bound: when the sweeps' working set is replayed from the last-level cache. A
+        // scene whose unmerged sweeps stream from HBM keeps one level per bisection (the 64-cloth
+        // batch, 1.1 GB per iteration, measured 2% slower with merged levels).
+        double mt = 0., bytes0 = leaf_entries;
+        int top0 = H + 1;
+        for (int h = H; h >= 0; --h) {
+            if ((mt + cols_at[h]) * (mt + cols_at[h]) * e > max_top_bytes) break;
+            mt += cols_at[h];
+            top0 = h;
+        }
+        for (size_t i = 0; i < T.size(); ++i)
+            if (!T[i].leaf && T[i].height < top0) bytes0 += ent[i][0];
+        const double nb0 = std::ceil(mt / 64.);
+        const double ws0 = 2. * bytes0 * MB + (mt > 0. ? 4096. * nb0 * (nb0 + 1.) / 2. * MB : 0.);
+        if (top0 == 0 || ws0 > kCachedMB) {
+            for (NDNode& nd : T) nd.absorb = false;
+            return max_top_bytes;
+        }
+    }
+    double best_cost = 1e300, best_bytes = 0.;
+    std::vector<int> best_tops;   // group-top heights
+    std::vector<int> best_sizes;  // their group sizes
+    // A scene whose top level fits the budget keeps a top block: on the 64-cloth batch, sweeping
+    // the tallest trees' roots instead (merged with the level below) measured 11% slower although
+    // the model saw one launch less.
+    const bool need_top = cols_at[H] * cols_at[H] * e <= max_top_bytes;
+    double m = 0.;
+    for (int top = H + 1; top >= 0; --top) {
+        if (top <= H) {
+            m += cols_at[top];
+            if (m * m * e > max_top_bytes) break;
+        } else if (need_top) {
+            continue;
+        }
+        const double nbk = std::ceil(m / 64.);
+        const double top_cost = m > 0. ? kTopUs + kTopUsPerMB * (4096. * nbk * (nbk + 1.) / 2.) * MB : 0.;
+        // compositions of the heights top - 1 .. 1 into groups of 1 to 3
+        std::vector<int> sizes;
+        std::function<void(int)> rec = [&](int h) {
+            if (h < 1) {
+                // top 0: every level in the top block, nothing swept
+                double bytes = top > 0 ? leaf_entries : 0., levels = top > 0 ? 1. : 0.;
+                std::vector<int> tops;
+                int hh = top - 1;
+                for (int g : sizes) {
+                    tops.push_back(hh);
+                    for (size_t i = 0; i < T.size(); ++i)
+                        if (!T[i].leaf && T[i].height == hh) bytes += ent[i][g - 1];
+                    levels += 1.;
+                    hh -= g;
+                }
+                const double tiles_mb = m > 0. ? 4096. * nbk * (nbk + 1.) / 2. * MB : 0.;
+                const double ws = 2. * bytes * MB + tiles_mb;
+                const double hit = ws > kCachedMB ? kCachedMB / ws : 1.;
+                const double per_mb = hit * kLevelUsPerMBCached + (1. - hit) * kLevelUsPerMBHbm;
+                const double cost = levels * kLevelUs + per_mb * bytes * MB + top_cost;
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_bytes = (m + 0.5) * (m + 0.5) * e;
+                    best_tops = tops;
+                    best_sizes = sizes;
+                }
+                return;
+            }
+            if (!forced.empty()) {
+                const int g = std::max(1, std::min(3, forced[std::min(sizes.size(), forced.size() - 1)]));
+                sizes.push_back(std::min(g, h));
+                rec(h - sizes.back());
+                sizes.pop_back();
+                return;
+            }
+            for (int g = 1; g <= std::min(3, h); ++g) {
+                sizes.push_back(g);
+                rec(h - g);
+                sizes.pop_back();
+            }
+        };
+        rec(top - 1);
+    }
+    // mark the absorbing nodes top-down: a node at a group-top height absorbs size - 1 levels of
+    // separators below it (each absorbed separator passes one level less on)
+    std::vector<int> size_at(H + 2, 0);
+    for (size_t k = 0; k < best_tops.size(); ++k) size_at[best_tops[k]] = best_sizes[k];
+    std::function<void(int, int)> mark = [&](int i, int inherited) {
+        NDNode& nd = T[i];
+        if (nd.leaf) return;
+        const int c = inherited >= 0 ? inherited : std::max(0, size_at[nd.height] - 1);
+        nd.absorb = c >= 1 && !nd.cols.empty();
+        for (int k : nd.kid)
+            if (k >= 0) mark(k, nd.absorb ? c - 1 : -1);
+    };
+    for (int r : roots)
+        if (r >= 0) mark(r, -1);
+    return best_bytes;
+}
+
 // Dense lower-triangular inverse, column-major ld x n block starting at a.
 void lower_inverse(const double* Lm, int ld, int n, std::vector<double>& inv) {
     inv.assign((size_t)n * n, 0.);
@@ -150,7 +363,8 @@ void lower_inverse(const double* Lm, int ld, int n, std::vector<double>& inv) {
 
 }  // namespace
 
-void factorize(const Model& M, double dt2, Factor& F, int leaf_size, double cross_top_bytes, int entry_bytes) {
+void factorize(const Model& M, double dt2, Factor& F, int leaf_size, double cross_top_bytes, int entry_bytes,
+               const std::vector<int>& groups) {
     auto t0 = std::chrono::steady_clock::now();
     const int n = M.n;
     F.n = n;
@@ -187,27 +401,14 @@ void factorize(const Model& M, double dt2, Factor& F, int leaf_size, double cros
             roots.push_back(build_nd(st, T, set));
         }
     }
+    F.top_bytes = 0.;
     if (cross_top_bytes > 0.) {
-        // Crosses below the dense top block. The top block will hold the highest levels whose
-        // columns m fit m^2 * entry_bytes <= cross_top_bytes (as plan_top decides, here on
-        // the dissection's heights); the swept separator levels under it are paired from the
-        // top down and each pair becomes one level: a separator at height h absorbs its
-        // halves' separators (height h - 1) for h = top - 1, top - 3, ... >= 2. Fewer levels
-        // mean fewer latency-bound launches per substitution for somewhat denser operators.
-        int H = 0;
-        for (const NDNode& nd : T) H = std::max(H, nd.height);
-        std::vector<double> cols_at(H + 1, 0.);
-        for (const NDNode& nd : T) cols_at[nd.height] += (double)nd.cols.size();
-        int top = H + 1;
-        double m = 0.;
-        for (int h = H; h >= 0; --h) {
-            m += cols_at[h];
-            if (m * m * entry_bytes <= cross_top_bytes) top = h;
-            else break;
-        }
-        for (NDNode& nd : T)
-            nd.absorb = !nd.leaf && !nd.cols.empty() && nd.height >= 2 && nd.height < top &&
-                        (top - 1 - nd.height) % 2 == 0;
+        // Crosses below the dense top block: the separator levels under it are merged in groups of
+        // consecutive levels (a separator absorbs its halves' separators, and theirs), each group
+        // one level of the substitution. Fewer levels mean fewer latency-bound launches per
+        // substitution for denser operators; the top block's size and the grouping minimise the
+        // cost model above.
+        F.top_bytes = choose_crosses(M, st.leaf, T, roots, cross_top_bytes, entry_bytes, groups);
     }
     for (int r : roots)
         if (r >= 0) emit_nd(st, T, r, nullptr);

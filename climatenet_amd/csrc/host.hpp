@@ -75,15 +75,19 @@ struct Factor {
     int64_t nnz_L = 0;
     int n_levels = 0;
     int max_ns = 0, max_front = 0;
+    double top_bytes = 0.;           // with crosses: the top-block budget the ordering was chosen for
     double seconds = 0.;
 };
 
 // Nested dissection (coordinate bisection on the graph of K), symbolic and
 // numeric multifrontal factorisation. K = diag(mass) + dt2 * A^T A.
-// With cross_top_bytes > 0, pairs of separator levels below the dense top block of that budget
-// (entries of entry_bytes) are merged into one level of cross-shaped supernodes.
+// With cross_top_bytes > 0, the separator levels below a dense top block of at most that budget
+// (entries of entry_bytes) are merged in groups of consecutive levels, each group one level of
+// cross-shaped supernodes; the top block's size and the groups are chosen by a cost model of the
+// substitution (F.top_bytes: the budget to plan the top block with). `groups` fixes the group
+// sizes from the top instead (the last repeating).
 void factorize(const Model& M, double dt2, Factor& F, int leaf_size = 64, double cross_top_bytes = 0.,
-               int entry_bytes = 8);
+               int entry_bytes = 8, const std::vector<int>& groups = {});
 
 // Work lists of one substitution sweep: (supernode, first row) tasks of at most
 // `chunk` rows, grouped by level (forward: leaves first; backward: root first).

@@ -517,12 +517,24 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         if (const char* e = std::getenv("PDFC_LEAF")) leaf = std::max(8, std::atoi(e));
         top_mb = 160.;  // full S^-1; the symmetric tiles read about half of it
         if (const char* e = std::getenv("PDFC_TOP_MB")) top_mb = std::atof(e);
-        // Separator levels below the top block merged in pairs (cross-shaped supernodes):
-        // measured on MI355X, the 40k-vertex drape's five swept levels become three,
-        // 364 -> 419 steps/s. PDFC_CROSS=0 keeps one level per bisection.
+        // Separator levels below the top block merged in groups (cross-shaped supernodes), the
+        // top block's size and the groups chosen by the substitution's cost model: measured on
+        // MI355X, the 40k-vertex drape's five swept levels become three. PDFC_CROSS=0 keeps one
+        // level per bisection; PDFC_CROSS_GROUPS fixes the group sizes.
         cross = true;
         if (const char* e = std::getenv("PDFC_CROSS")) cross = std::atoi(e) != 0;
-        pdfc::factorize(model, dt * dt, fac, leaf, cross ? top_mb * 1048576. : 0., fp32 ? 4 : 8);
+        std::vector<int> groups;
+        if (const char* e = std::getenv("PDFC_CROSS_GROUPS")) {
+            groups.clear();
+            for (const char* c = e; *c;) {
+                char* end = nullptr;
+                long g = std::strtol(c, &end, 10);
+                if (end == c) break;
+                groups.push_back((int)std::max(1L, std::min(8L, g)));
+                c = *end == ',' ? end + 1 : end;
+            }
+        }
+        pdfc::factorize(model, dt * dt, fac, leaf, cross ? top_mb * 1048576. : 0., fp32 ? 4 : 8, groups);
     } catch (const std::runtime_error& e) {
         throw NumericError(e.what());
     }
@@ -530,7 +542,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
     if (const char* e = std::getenv("PDFC_SELF_PATH")) self_path = std::max(0, std::min(2, std::atoi(e)));
-    pdfc::plan_top(fac, top_mb * 1048576., fp32 ? 4 : 8, top);
+    pdfc::plan_top(fac, cross ? fac.top_bytes : top_mb * 1048576., fp32 ? 4 : 8, top);
 
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));

@@ -103,6 +103,11 @@ int main(int argc, char** argv) {
         printf("top budget %.0f MB: level %d of %d, m=%d, contributions=%zu\n", mb, top.level, F.n_levels, top.m,
                top.cidx.size());
     }
+    if (cross_mb > 0.) {
+        TopPlan top;
+        plan_top(F, F.top_bytes, 8, top);
+        printf("chosen: top level %d of %d, m=%d\n", top.level, F.n_levels, top.m);
+    }
     printf("n=%d supernodes=%d levels=%d nnzL=%lld nnzF=%lld max_ns=%d max_front=%d factor_s=%.3f resid=%.3e\n",
            n, S, F.n_levels, (long long)F.nnz_L, (long long)F.foff[S], F.max_ns, F.max_front, F.seconds, std::sqrt(rn / bn));
     (void)maxlvl_tasks;

@@ -30,6 +30,17 @@ def test_supernodal_solve_residual(checker, N, leaf, cross_mb):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+def test_cross_choice_on_the_40k_grid(checker):
+    """The ordering the cost model picks for a 200x200-quad sheet with 96-vertex leaves and the
+    160 MiB top-block budget (the 40k drape's): a 2,388-row top block over three swept levels (the
+    leaves, then the five separator heights below the top block grouped 2 + 3 from the top), 7 levels in all instead of 10; the solve's
+    residual stays at rounding."""
+    r = subprocess.run([checker, "200", "96", "160"], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "chosen: top level 3 of 7, m=2388" in r.stdout
+
+
 def test_merged_launch_plans(tmp_path):
     """Every partition of each sweep into at most three ranges of consecutive levels: the
     merged-launch groups list each task once, and each task's children (forward) or below-row
