@@ -189,7 +189,8 @@ def test_double_obstacle_contacts_with_self_contacts(gpu):
 def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
     """The self-contact levels of an iteration run as two launches per level (PDFC_SELF_PATH=0),
     in one workgroup through global memory (1) or in one workgroup with the level-to-level state in
-    LDS (2, the default when it fits). Same arithmetic in the same order: bit-identical states."""
+    LDS (2, the default when it fits). Same arithmetic in the same order: bit-identical states and
+    reactions."""
     if scene == "stacked_small":
         spec, first, steps = stacked_small(16), 30, 20
     else:
@@ -207,9 +208,12 @@ def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
         for _ in range(steps):
             sim.step(1)
             selfs = max(selfs, len(sim.self_contacts()))
-        out.append(sim.state())
+        r = np.array([list(c.r_local) for c in sim.contacts()])
+        rs = np.array([list(c.r_local) for c in sim.self_contacts()])
+        out.append(sim.state() + (r, rs))
         sim.close()
     print(scene, "self contacts (max over steps)", selfs)
     assert selfs > 0
-    for q, v in out[1:]:
-        assert np.array_equal(q, out[0][0]) and np.array_equal(v, out[0][1])
+    for path, (q, v, r, rs) in zip(("1", "2"), out[1:]):
+        assert np.array_equal(q, out[0][0]) and np.array_equal(v, out[0][1]), path
+        assert np.array_equal(r, out[0][2]) and np.array_equal(rs, out[0][3]), path
