@@ -594,6 +594,15 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
         ContactIn first;
         if (contact && regs) ata_row_load(M, i, AR);
         if (kLowLatency && nc > 0) contact_load(C, c0, rec, first);
+        // a slot vertex's (contact, side) entries: their range and first two now, so that their
+        // stores after the contacts wait for no load
+        int se0 = 0, se1 = 0, ent0 = 0, ent1 = 0;
+        if (slot >= 0) {
+            se0 = ls.slot_ent_ptr[slot];
+            se1 = ls.slot_ent_ptr[slot + 1];
+            ent0 = ls.slot_ent[se0];  // every slot has an entry
+            ent1 = ls.slot_ent[min(se0 + 1, se1 - 1)];
+        }
         const D3 q = {qi.x + dt * v.x, qi.y + dt * v.y, qi.z + dt * v.z};
         const D3 s = regs ? spring_row_sum(SR, qn, vcur, n, i, dt, q) : spring_sum(M, qn, vcur, dt, i, q);
         double damp = contact ? 0. : a.damping;
@@ -603,6 +612,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
             // reaction reaches the right-hand side (m_contact_forces is assigned, not summed,
             // SimulationFrictionEstimation.cpp:410-418; rhs += CF + SCF with SCF = 0 there)
             D3 glast = {0., 0., 0.};
+            D3 cfl = {0., 0., 0.};  // slot vertex: its last active obstacle reaction
             const D3 av = regs ? ata_row_sum(AR, vcur, n, i) : ata_row(M, vcur, i);
             const double dt2 = dt * dt;
             D3 F = {b.x - dt2 * av.x, b.y - dt2 * av.y, b.z - dt2 * av.z};
@@ -668,6 +678,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
                     ls.CFn[3 * slot] = g.x;
                     ls.CFn[3 * slot + 1] = g.y;
                     ls.CFn[3 * slot + 2] = g.z;
+                    cfl = g;
                 } else if (ls.selfmode) {
                     glast = g;
                 } else {
@@ -680,6 +691,22 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
                 b.x += glast.x;
                 b.y += glast.y;
                 b.z += glast.z;
+            }
+            if (slot >= 0) {
+                // F + CF for each of the slot's (contact, side) entries and the right-hand side,
+                // so that the self-contact levels read their inputs by contact and by slot
+                // without a dependent load
+                const D3 fcf = {F.x + cfl.x, F.y + cfl.y, F.z + cfl.z};
+                for (int e = se0; e < se1; ++e) {
+                    const int code = e == se0 ? ent0 : (e == se0 + 1 ? ent1 : ls.slot_ent[e]);
+                    double* x = ls.x2 + 3 * (size_t)code;
+                    x[0] = fcf.x;
+                    x[1] = fcf.y;
+                    x[2] = fcf.z;
+                }
+                ls.b0[3 * slot] = b.x;
+                ls.b0[3 * slot + 1] = b.y;
+                ls.b0[3 * slot + 2] = b.z;
             }
         }
         st3(rhs, n, i, b);

@@ -230,6 +230,11 @@ struct DevSelf {
     double* SCFc;           // self contact forces, current / next buffer
     double* SCFn;
     const int* slot_vertex; // slot -> supernodal vertex
+    const double* mass2;    // per contact: the masses of its two vertices
+    double* x2;             // per contact: F + CF at its two vertices (3 each), written by the local kernel
+    double* b0;             // per slot: the local kernel's right-hand side (3)
+    const int* slot_ent_ptr;  // per slot: its (contact, side) entries slot_ent[ptr[slot] .. ptr[slot + 1])
+    const int* slot_ent;      //   as 2 contact + side
     const int* lvl_ptr;     // contacts of level l: [lvl_ptr[l], lvl_ptr[l+1])
     const int* grp_ptr;     // vertex groups of level l
     const int* grp_slot;
@@ -279,6 +284,10 @@ struct LocalSelf {
     double* Fc;
     double* CFn;
     const double* SCFc;
+    double* x2;             // DevSelf::x2, b0, slot_ent_ptr, slot_ent
+    double* b0;
+    const int* slot_ent_ptr;
+    const int* slot_ent;
 };
 void launch_local(const DevModel& M, const double* qn, const double* vcur, const double* cvec, double* rhs,
                   const int* ccnt, const int* cstart, DevContacts C, const StepArgs& a, int iteration,

@@ -650,11 +650,15 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         if (k <= G) g_ent[k] = (unsigned short)ge[q];
         if (k < E) e_code[k] = (unsigned short)(2 * ec[q] + (es[q] > 0 ? 1 : 0));
     }
+    // Every per-contact and per-slot input is indexed by the thread (one round of loads): the
+    // masses of the contact's vertices, F + CF at both (stored by contact by the local kernel,
+    // the value fA / fB start from), and the first slot's right-hand side and obstacle reaction
+    // for the final update (which then waits for no load; they do not change during the levels).
     const int s = t;
     const bool mine = s < S.nsc;
-    int lev = -1, sa = 0, sb = 0;
-    double acs = 0., acm = 0., mA = 1., mB = 1., mu = 0.;
-    P3 bn{}, bt{}, bb{}, fa{}, fb{}, rem{};
+    int lev = -1, sa = 0, sb = 0, pa = 0, pb = 0;
+    double mA = 1., mB = 1., mu = 0.;
+    P3 bn{}, bt{}, bb{}, fa{}, fb{}, rem{}, rprev{};
     if (mine) {
         for (int l = 0; l < nlev; ++l)
             if (s >= S.lvl_ptr[l] && s < S.lvl_ptr[l + 1]) lev = l;
@@ -662,35 +666,31 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         bn = P3{B[0], B[1], B[2]};
         bt = P3{B[3], B[4], B[5]};
         bb = P3{B[6], B[7], B[8]};
-        const int pa = S.a[s], pb = S.b[s];
+        pa = S.a[s];
+        pb = S.b[s];
         sa = S.sa[s];
         sb = S.sb[s];
         mu = S.mu[s];
-        mA = mass[pa];
-        mB = mass[pb];
-        fa = padd(ld_slot(S.Fc, sa), ld_slot(S.CFn, sa));  // (F + CF) before the self terms, as in fA
-        fb = padd(ld_slot(S.Fc, sb), ld_slot(S.CFn, sb));
+        mA = S.mass2[2 * s];
+        mB = S.mass2[2 * s + 1];
+        fa = ld_slot(S.x2, 2 * s);
+        fb = ld_slot(S.x2, 2 * s + 1);
         rem = ld_slot(S.rem, s);
+        rprev = ld_slot(S.r, s);
         c_rem[3 * s] = rem.x;
         c_rem[3 * s + 1] = rem.y;
         c_rem[3 * s + 2] = rem.z;
-        if (a.record && iteration > 0) {
-            P3 u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
-            acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, &S.r[3 * s], mu);
-            acm = acs;
-        }
     }
-    // the first involved vertex's right-hand side and obstacle reaction, read now so that the
-    // final update waits for no load (they do not change during the levels)
     P3 rhs0{}, cf0{};
     if (t < S.nslots) {
-        const int w = vv[0];
-        rhs0 = P3{rhs[w], rhs[n + w], rhs[2 * n + w]};
+        rhs0 = ld_slot(S.b0, t);
         cf0 = ld_slot(S.CFn, t);
     }
+    // the residual's relative velocity (a dependent load) is used after the levels
+    P3 u{};
+    const bool res = mine && a.record && iteration > 0;
+    if (res) u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
     __syncthreads();
-    // the residual's atomics complete while the levels run
-    if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
     for (int l = 0; l < nlev; ++l) {
         if (lev == l) {
             const P3 ca = {scf_c[3 * sa], scf_c[3 * sa + 1], scf_c[3 * sa + 2]};
@@ -729,13 +729,22 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         __syncthreads();
     }
     if (mine) st_slot(S.addv, s, P3{c_add[3 * s], c_add[3 * s + 1], c_add[3 * s + 2]});
+    if (a.record && iteration > 0) {
+        // the friction residual of the previous iteration (r^{k-1}, v^k), as self_contact_update
+        double acs = 0.;
+        if (res) {
+            const double rp[3] = {rprev.x, rprev.y, rprev.z};
+            acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, rp, mu);
+        }
+        ac_block(acs, acs, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int sl = t + q * kSelfFusedThreads;
         if (sl >= S.nslots) break;
         const int w = vv[q];
         const P3 c = q == 0 ? cf0 : ld_slot(S.CFn, sl);
-        const P3 b0 = q == 0 ? rhs0 : P3{rhs[w], rhs[n + w], rhs[2 * n + w]};
+        const P3 b0 = q == 0 ? rhs0 : ld_slot(S.b0, sl);
         const P3 sc = {scf_n[3 * sl], scf_n[3 * sl + 1], scf_n[3 * sl + 2]};
         rhs[w] = b0.x + (c.x + sc.x);
         rhs[n + w] = b0.y + (c.y + sc.y);

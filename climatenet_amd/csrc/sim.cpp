@@ -1796,9 +1796,9 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
 // ent_sign (2 nsc each), slot_vertex (nslots), grp_slot (G), ent_ptr (G + 1).
 void pdfc_sim::self_upload() {
     const int L = n_self_levels;
-    const size_t max_int = 14 * (size_t)nsc + 2 * (size_t)L + 3;
+    const size_t max_int = 18 * (size_t)nsc + 2 * (size_t)L + 4;
     pin_int.grow(max_int);
-    pin_dbl.grow(10 * (size_t)nsc);
+    pin_dbl.grow(12 * (size_t)nsc);
     int* const P = pin_int.p;
     int *A = P, *B = P + nsc, *SA = P + 2 * nsc, *SB = P + 3 * nsc;
     int* lvl_ptr = P + 4 * nsc;
@@ -1808,6 +1808,7 @@ void pdfc_sim::self_upload() {
     int* slot_vertex = ent_sign + 2 * nsc;
     double* basis = pin_dbl.p;
     double* mu = pin_dbl.p + 9 * (size_t)nsc;
+    double* mass2 = pin_dbl.p + 10 * (size_t)nsc;
     // involved vertices: those of the self contacts, in order of first appearance (the local
     // kernel keeps the last obstacle reaction of every other contact vertex itself)
     std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
@@ -1827,6 +1828,8 @@ void pdfc_sim::self_upload() {
         const double nr[3] = {p.normal.x, p.normal.y, p.normal.z};
         pdfc::host_contact_basis(nr, &basis[9 * (size_t)j]);
         mu[j] = self_mu;
+        mass2[2 * (size_t)j] = model.mass[p.v];
+        mass2[2 * (size_t)j + 1] = model.mass[p.partner];
     }
     for (int k = 0; k < nslots; ++k) slot_of[slot_vertex[k]] = -1;
     // per-level groups: each touched slot once, in order of first appearance; entries of a group
@@ -1879,8 +1882,25 @@ void pdfc_sim::self_upload() {
     int* ent_ptr = grp_slot + G;
     std::copy(up_gstart.begin(), up_gstart.end(), ent_ptr);
     ent_ptr[G] = e0;
-    const size_t n_int = (size_t)(ent_ptr - P) + (size_t)G + 1;
-    const size_t n_dbl = 10 * (size_t)nsc;
+    // each slot's (contact, side) entries, 2 contact + side, in contact order (counting pass)
+    int* slot_ent_ptr = ent_ptr + G + 1;
+    int* slot_ent = slot_ent_ptr + nslots + 1;
+    std::fill(slot_ent_ptr, slot_ent_ptr + nslots + 1, 0);
+    for (int j = 0; j < nsc; ++j) {
+        ++slot_ent_ptr[SA[j] + 1];
+        ++slot_ent_ptr[SB[j] + 1];
+    }
+    for (int k = 0; k < nslots; ++k) slot_ent_ptr[k + 1] += slot_ent_ptr[k];
+    {
+        std::vector<int>& fill = up_gfill;
+        fill.assign(slot_ent_ptr, slot_ent_ptr + nslots);
+        for (int j = 0; j < nsc; ++j) {
+            slot_ent[fill[SA[j]]++] = 2 * j;
+            slot_ent[fill[SB[j]]++] = 2 * j + 1;
+        }
+    }
+    const size_t n_int = (size_t)(slot_ent - P) + 2 * (size_t)nsc;
+    const size_t n_dbl = 12 * (size_t)nsc;
     if (n_int > cap_self_int) {
         free_ptr(d_self_int);
         cap_self_int = n_int * 2;
@@ -1896,7 +1916,7 @@ void pdfc_sim::self_upload() {
         free_ptr(d_self_active);
         cap_self_contacts = std::max(cap_self_contacts, (size_t)nsc * 2);
         cap_self_slots = std::max(cap_self_slots, (size_t)nslots * 2);
-        d_self_work = dalloc<double>(9 * cap_self_contacts + 15 * cap_self_slots);
+        d_self_work = dalloc<double>(15 * cap_self_contacts + 18 * cap_self_slots);
         d_self_active = dalloc<int>(cap_self_contacts);
     }
     // staged in pinned memory: the copies run asynchronously; the staging is next written by
@@ -1905,7 +1925,7 @@ void pdfc_sim::self_upload() {
     HIP_CHECK(hipMemcpyAsync(d_self_dbl, pin_dbl.p, n_dbl * sizeof(double), hipMemcpyHostToDevice, stream));
     const size_t oA = 0, oB = nsc, oSA = 2 * (size_t)nsc, oSB = 3 * (size_t)nsc, oL = 4 * (size_t)nsc,
                  oG = oL + L + 1, oEC = oG + L + 1, oES = oEC + 2 * (size_t)nsc, oSV = oES + 2 * (size_t)nsc,
-                 oGS = oSV + nslots, oEP = oGS + G;
+                 oGS = oSV + nslots, oEP = oGS + G, oSEP = oEP + G + 1, oSEN = oSEP + nslots + 1;
     pdfc::DevSelf S{};
     S.nsc = nsc;
     S.nslots = nslots;
@@ -1920,18 +1940,23 @@ void pdfc_sim::self_upload() {
     S.ent_contact = d_self_int + oEC;
     S.ent_sign = d_self_int + oES;
     S.slot_vertex = d_self_int + oSV;
+    S.slot_ent_ptr = d_self_int + oSEP;
+    S.slot_ent = d_self_int + oSEN;
     S.basis = d_self_dbl;
     S.mu = d_self_dbl + 9 * (size_t)nsc;
+    S.mass2 = d_self_dbl + 10 * (size_t)nsc;
     double* w = d_self_work;
     S.r = w;
     S.rem = w + 3 * cap_self_contacts;
     S.addv = w + 6 * cap_self_contacts;
-    double* ws = w + 9 * cap_self_contacts;
+    S.x2 = w + 9 * cap_self_contacts;
+    double* ws = w + 15 * cap_self_contacts;
     S.Fc = ws;
     S.CFc = ws + 3 * cap_self_slots;
     S.CFn = ws + 6 * cap_self_slots;
     S.SCFc = ws + 9 * cap_self_slots;
     S.SCFn = ws + 12 * cap_self_slots;
+    S.b0 = ws + 15 * cap_self_slots;
     S.active = d_self_active;
     ds = S;
 }
@@ -1944,6 +1969,10 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     ls.Fc = ds.Fc;
     ls.CFn = ds.CFn;
     ls.SCFc = ds.SCFc;
+    ls.x2 = ds.x2;
+    ls.b0 = ds.b0;
+    ls.slot_ent_ptr = ds.slot_ent_ptr;
+    ls.slot_ent = ds.slot_ent;
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
         timed(4, [&] {
