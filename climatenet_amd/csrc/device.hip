@@ -940,12 +940,13 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
     // The first kPrefetch operator entries of this thread's column slice do not depend on
     // the gather: issue them now so their latency overlaps the gather's.
     const int my_row = r0 + lane;
+    const int ldf = op_ld(f, sizeof(T));
     const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
     double pre[kPrefetch];
 #pragma unroll
     for (int q = 0; q < kPrefetch; ++q) {
         const int j = wave + q * W;
-        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * f]) : 0.;
+        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * ldf]) : 0.;
     }
     // own rows: b~ = b - sum of the children's update entries (pull, child order)
     for (int t = tid; t < ns; t += 64 * W) {
@@ -972,7 +973,7 @@ __device__ __forceinline__ void forward_task(const DevFactor& Fa, const TaskDesc
         }
 #pragma unroll 8
         for (int j = wave + kPrefetch * W; j < ns; j += W) {
-            double fv = as_d(Fs[(size_t)j * f]);
+            double fv = as_d(Fs[(size_t)j * ldf]);
             ax = fma(fv, bx[j], ax);
             ay = fma(fv, by[j], ay);
             az = fma(fv, bz[j], az);
@@ -1129,12 +1130,13 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
     double* gz = lds + 2 * len;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = j0 + lane;
-    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
+    const int ldt = op_ld(ns, sizeof(T));
+    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + op_t_offset(ns, f, sizeof(T)) + j;
     double pre[kPrefetch];  // operator entries independent of the gather, issued first
 #pragma unroll
     for (int q = 0; q < kPrefetch; ++q) {
         const int i = i0 + wave + q * W;
-        pre[q] = (j < ns && i < i1) ? as_d(Ft[(size_t)i * ns]) : 0.;
+        pre[q] = (j < ns && i < i1) ? as_d(Ft[(size_t)i * ldt]) : 0.;
     }
     const int a1 = min(i1, ns);
     for (int i = i0 + tid; i < a1; i += 64 * W) {
@@ -1177,7 +1179,7 @@ __device__ __forceinline__ void backward_task(const DevFactor& Fa, const TaskDes
         }
 #pragma unroll 8
         for (int i = i0 + wave + kPrefetch * W; i < i1; i += W) {
-            double fv = as_d(Ft[(size_t)i * ns]);
+            double fv = as_d(Ft[(size_t)i * ldt]);
             ax = fma(fv, gx[i - i0], ax);
             ay = fma(fv, gy[i - i0], ay);
             az = fma(fv, gz[i - i0], az);
@@ -1264,12 +1266,13 @@ __device__ __forceinline__ void forward_slot(const DevFactor& Fa, int task, int 
     const int lane = threadIdx.x & 63, tid = wv * 64 + lane, nthr = 64 * W;
     const int4* rec = Fa.pull_rec + d.pull_base;
     const int my_row = r0 + lane;
+    const int ldf = op_ld(f, sizeof(T));
     const T* Fs = static_cast<const T*>(Fa.F) + d.foff + my_row;
     double pre[kPrefetch];
 #pragma unroll
     for (int q = 0; q < kPrefetch; ++q) {
         const int j = wv + q * W;
-        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * f]) : 0.;
+        pre[q] = (my_row < f && j < ns) ? as_d(Fs[(size_t)j * ldf]) : 0.;
     }
     for (int t = tid; t < ns; t += nthr) {
         double x = Bv[c0 + t], y = Bv[n + c0 + t], z = Bv[2 * n + c0 + t];
@@ -1294,7 +1297,7 @@ __device__ __forceinline__ void forward_slot(const DevFactor& Fa, int task, int 
         }
 #pragma unroll 8
         for (int j = wv + kPrefetch * W; j < ns; j += W) {
-            double fv = as_d(Fs[(size_t)j * f]);
+            double fv = as_d(Fs[(size_t)j * ldf]);
             ax = fma(fv, bx[j], ax);
             ay = fma(fv, by[j], ay);
             az = fma(fv, bz[j], az);
@@ -1343,12 +1346,13 @@ __device__ __forceinline__ void backward_slot(const DevFactor& Fa, int task, int
     double* gz = lds + 2 * f;
     const int lane = threadIdx.x & 63, tid = wv * 64 + lane, nthr = 64 * W;
     const int j = j0 + lane;
-    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + j;
+    const int ldt = op_ld(ns, sizeof(T));
+    const T* Ft = static_cast<const T*>(Fa.FT) + d.foff + op_t_offset(ns, f, sizeof(T)) + j;
     double pre[kPrefetch];
 #pragma unroll
     for (int q = 0; q < kPrefetch; ++q) {
         const int i = wv + q * W;
-        pre[q] = (j < ns && i < f) ? as_d(Ft[(size_t)i * ns]) : 0.;
+        pre[q] = (j < ns && i < f) ? as_d(Ft[(size_t)i * ldt]) : 0.;
     }
     for (int t = tid; t < ns; t += nthr) {
         gx[t] = Y[c0 + t];
@@ -1376,7 +1380,7 @@ __device__ __forceinline__ void backward_slot(const DevFactor& Fa, int task, int
         }
 #pragma unroll 8
         for (int i = wv + kPrefetch * W; i < f; i += W) {
-            double fv = as_d(Ft[(size_t)i * ns]);
+            double fv = as_d(Ft[(size_t)i * ldt]);
             ax = fma(fv, gx[i], ax);
             ay = fma(fv, gy[i], ay);
             az = fma(fv, gz[i], az);

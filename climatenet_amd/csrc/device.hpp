@@ -127,6 +127,24 @@ struct TaskDesc {
     int64_t rng;
 };
 
+// Device layout of the substitution operators: per supernode its F_s block, column-major with
+// leading dimension op_ld(ns + nb), then its F_s^T block, column-major with leading dimension
+// op_ld(ns), each block starting on a 128-byte boundary (pad entries zero). A dimension above 64
+// is split over several 64-row tasks, so it is rounded up to 128 bytes: every task's 64 entries
+// of a column (or of a row) then start on a cache-line boundary and no line is fetched by two
+// tasks; a dimension of at most 64 is read whole by one task and stays unpadded.
+__host__ __device__ inline int op_round(int64_t x, int entry_bytes) {
+    const int a = 128 / entry_bytes;
+    return (int)((x + a - 1) / a * a);
+}
+__host__ __device__ inline int op_ld(int x, int entry_bytes) { return x > 64 ? op_round(x, entry_bytes) : x; }
+// offset of the F_s^T block inside a supernode's blocks
+__host__ __device__ inline int64_t op_t_offset(int ns, int f, int entry_bytes) {
+    const int a = 128 / entry_bytes;
+    const int64_t b = (int64_t)op_ld(f, entry_bytes) * ns;
+    return (b + a - 1) / a * a;
+}
+
 // Supernodal substitution operators (see factor.cpp).
 struct DevFactor {
     int n;
@@ -140,8 +158,8 @@ struct DevFactor {
     const int* children;
     const int* relmap;   // indexed by uoff of the child
     const int* rows;
-    const void* F;       // double or float
-    const void* FT;
+    const void* F;       // double or float: the padded F_s / F_s^T blocks (TaskDesc::foff)
+    const void* FT;      //   the same array
     double* U;           // 3 * total below rows, interleaved
     const int* task_s;
     const int* task_r0;

@@ -127,6 +127,7 @@ struct pdfc_sim {
     std::vector<int> scene_v0;          // per scene: first vertex
     int n_scenes = 1, max_local_obs = 0;
     int leaf = 64;                      // leaf supernode size of the dissection
+    std::vector<int64_t> dev_foff;      // per supernode: offset of its padded F_s / F_s^T blocks
     bool cross = true;                  // separator levels below the top block merged in pairs
     double top_mb = 160.;               // dense top-block budget (MiB of S^-1)
     size_t cap_contacts = 1;
@@ -634,13 +635,38 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     df.children = upload(fac.children);
     df.relmap = upload(fac.relmap);
     df.rows = upload(fac.rows);
-    if (fp32) {
-        std::vector<float> f32(fac.F.begin(), fac.F.end()), ft32(fac.FT.begin(), fac.FT.end());
-        df.F = upload(f32);
-        df.FT = upload(ft32);
-    } else {
-        df.F = upload(fac.F);
-        df.FT = upload(fac.FT);
+    {
+        // padded device layout (pdfc::op_ld): per supernode F_s, then F_s^T
+        const int eb = fp32 ? 4 : 8;
+        dev_foff.assign(S + 1, 0);
+        for (int s2 = 0; s2 < S; ++s2) {
+            const int ns = fac.ns[s2], f = ns + fac.nb[s2];
+            const int64_t tb = (int64_t)pdfc::op_ld(ns, eb) * f, a = 128 / eb;
+            dev_foff[s2 + 1] = dev_foff[s2] + pdfc::op_t_offset(ns, f, eb) + (tb + a - 1) / a * a;
+        }
+        auto pack = [&](auto* out) {
+            for (int s2 = 0; s2 < S; ++s2) {
+                const int ns = fac.ns[s2], f = ns + fac.nb[s2];
+                const int ldf = pdfc::op_ld(f, eb), ldt = pdfc::op_ld(ns, eb);
+                const double* F = &fac.F[fac.foff[s2]];
+                const double* FT = &fac.FT[fac.foff[s2]];
+                auto* o = out + dev_foff[s2];
+                for (int j = 0; j < ns; ++j)
+                    for (int r = 0; r < f; ++r) o[(size_t)j * ldf + r] = F[(size_t)j * f + r];
+                o += pdfc::op_t_offset(ns, f, eb);
+                for (int i = 0; i < f; ++i)
+                    for (int j = 0; j < ns; ++j) o[(size_t)i * ldt + j] = FT[(size_t)i * ns + j];
+            }
+        };
+        if (fp32) {
+            std::vector<float> op((size_t)dev_foff[S], 0.f);
+            pack(op.data());
+            df.F = df.FT = upload(op);
+        } else {
+            std::vector<double> op((size_t)dev_foff[S], 0.);
+            pack(op.data());
+            df.F = df.FT = upload(op);
+        }
     }
     df.U = dalloc<double>(3 * (size_t)fac.uoff[S]);
     dfwd = df;
@@ -922,7 +948,7 @@ std::vector<pdfc::TaskDesc> pdfc_sim::task_descs(const pdfc::SweepPlan& P, const
         d.c0 = fac.c0[s2];
         d.f = fac.ns[s2] + fac.nb[s2];
         d.pull_base = pull_base_of[s2];
-        d.foff = fac.foff[s2];
+        d.foff = dev_foff[s2];
         d.uoff = fac.uoff[s2];
         d.roff = fac.roff[s2];
         d.split = 0;
