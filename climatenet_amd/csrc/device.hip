@@ -15,6 +15,7 @@
 #include "device.hpp"
 
 namespace pdfc {
+thread_local LaunchTimer* g_launch_timer = nullptr;
 namespace {
 
 constexpr int kBlock = 256;
@@ -1699,11 +1700,11 @@ void launch_local(const DevModel& M, const double* qn, const double* vcur, const
             const int v = e ? std::atoi(e) : 64;
             return (v == 64 || v == 128 || v == 256) ? v : 64;
         }();
-        hipLaunchKernelGGL(k_local<true>, dim3((M.n + lb - 1) / lb), dim3(lb), 0, st, M, qn, vcur, cvec, rhs, ccnt,
-                           cstart, C, a, iteration, ls);
+        launch_k(k_local<true>, dim3((M.n + lb - 1) / lb), dim3(lb), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
+                 iteration, ls);
     } else
-        hipLaunchKernelGGL(k_local<false>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C,
-                           a, iteration, ls);
+        launch_k(k_local<false>, dim3(blocks), dim3(kBlock), 0, st, M, qn, vcur, cvec, rhs, ccnt, cstart, C, a,
+                 iteration, ls);
 }
 void launch_finish(const DevModel& M, double* qn, const double* vlast, double* vstate, const int* ccnt,
                    const int* cstart, DevContacts C, const DevObstacles& ob, const StepArgs& a, int iteration,
@@ -1730,22 +1731,22 @@ template <typename T>
 static void forward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* B,
                              double* Y) {
     switch (W) {
-        case 1: hipLaunchKernelGGL((k_forward<T, 1>), g, dim3(64), lds, st, F, t0, B, Y); break;
-        case 2: hipLaunchKernelGGL((k_forward<T, 2>), g, dim3(128), lds, st, F, t0, B, Y); break;
-        case 4: hipLaunchKernelGGL((k_forward<T, 4>), g, dim3(256), lds, st, F, t0, B, Y); break;
-        case 8: hipLaunchKernelGGL((k_forward<T, 8>), g, dim3(512), lds, st, F, t0, B, Y); break;
-        default: hipLaunchKernelGGL((k_forward<T, 16>), g, dim3(1024), lds, st, F, t0, B, Y); break;
+        case 1: launch_k(k_forward<T, 1>, g, dim3(64), lds, st, F, t0, B, Y); break;
+        case 2: launch_k(k_forward<T, 2>, g, dim3(128), lds, st, F, t0, B, Y); break;
+        case 4: launch_k(k_forward<T, 4>, g, dim3(256), lds, st, F, t0, B, Y); break;
+        case 8: launch_k(k_forward<T, 8>, g, dim3(512), lds, st, F, t0, B, Y); break;
+        default: launch_k(k_forward<T, 16>, g, dim3(1024), lds, st, F, t0, B, Y); break;
     }
 }
 template <typename T>
 static void backward_dispatch(int W, dim3 g, size_t lds, hipStream_t st, const DevFactor& F, int t0, const double* Y,
                               double* X, const TopFold& fo, const SplitFin& fin) {
     switch (W) {
-        case 1: hipLaunchKernelGGL((k_backward<T, 1>), g, dim3(64), lds, st, F, t0, Y, X, fo, fin); break;
-        case 2: hipLaunchKernelGGL((k_backward<T, 2>), g, dim3(128), lds, st, F, t0, Y, X, fo, fin); break;
-        case 4: hipLaunchKernelGGL((k_backward<T, 4>), g, dim3(256), lds, st, F, t0, Y, X, fo, fin); break;
-        case 8: hipLaunchKernelGGL((k_backward<T, 8>), g, dim3(512), lds, st, F, t0, Y, X, fo, fin); break;
-        default: hipLaunchKernelGGL((k_backward<T, 16>), g, dim3(1024), lds, st, F, t0, Y, X, fo, fin); break;
+        case 1: launch_k(k_backward<T, 1>, g, dim3(64), lds, st, F, t0, Y, X, fo, fin); break;
+        case 2: launch_k(k_backward<T, 2>, g, dim3(128), lds, st, F, t0, Y, X, fo, fin); break;
+        case 4: launch_k(k_backward<T, 4>, g, dim3(256), lds, st, F, t0, Y, X, fo, fin); break;
+        case 8: launch_k(k_backward<T, 8>, g, dim3(512), lds, st, F, t0, Y, X, fo, fin); break;
+        default: launch_k(k_backward<T, 16>, g, dim3(1024), lds, st, F, t0, Y, X, fo, fin); break;
     }
 }
 
@@ -1789,38 +1790,38 @@ void launch_sweep_merged(const DevFactor& F, bool fp32, bool forward, const Merg
     const dim3 g(R.groups), b(1024);
     if (forward) {
         if (fp32)
-            hipLaunchKernelGGL((k_sweep_merged<float, true>), g, b, R.lds_bytes, st, F, R, In, Out);
+            launch_k(k_sweep_merged<float, true>, g, b, R.lds_bytes, st, F, R, In, Out);
         else
-            hipLaunchKernelGGL((k_sweep_merged<double, true>), g, b, R.lds_bytes, st, F, R, In, Out);
+            launch_k(k_sweep_merged<double, true>, g, b, R.lds_bytes, st, F, R, In, Out);
     } else {
         if (fp32)
-            hipLaunchKernelGGL((k_sweep_merged<float, false>), g, b, R.lds_bytes, st, F, R, In, Out);
+            launch_k(k_sweep_merged<float, false>, g, b, R.lds_bytes, st, F, R, In, Out);
         else
-            hipLaunchKernelGGL((k_sweep_merged<double, false>), g, b, R.lds_bytes, st, F, R, In, Out);
+            launch_k(k_sweep_merged<double, false>, g, b, R.lds_bytes, st, F, R, In, Out);
     }
 }
 void launch_top(const DevTop& T, bool fp32, const double* U, const double* B, double* X, int n, hipStream_t st,
                 bool finish) {
     if (T.m <= 0) return;
     if (T.nb > 0) {
-        if (T.pregather) hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
+        if (T.pregather) launch_k(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
         if (fp32)
-            hipLaunchKernelGGL(k_top_tile<float>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
+            launch_k(k_top_tile<float>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
         else
-            hipLaunchKernelGGL(k_top_tile<double>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
+            launch_k(k_top_tile<double>, dim3(T.ntiles), dim3(256), 0, st, T, U, B, n);
         // nb <= kFinishLanes * kTopChunksMax (the host offers the tiles for m <= 8192 only)
         static_assert(64 % kFinishLanes == 0 && kFinishLanes * kTopChunksMax == 128, "finish lanes");
         if (finish)
-            hipLaunchKernelGGL(k_top_finish_blocks, dim3(T.nb), dim3(64 * kFinishLanes), 0, st, T, X, n);
+            launch_k(k_top_finish_blocks, dim3(T.nb), dim3(64 * kFinishLanes), 0, st, T, X, n);
         return;
     }
-    hipLaunchKernelGGL(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
+    launch_k(k_top_gather, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, U, B, n);
     dim3 g((T.m + 63) / 64, T.kchunks);
     if (fp32)
-        hipLaunchKernelGGL(k_top_matvec<float>, g, dim3(1024), 0, st, T);
+        launch_k(k_top_matvec<float>, g, dim3(1024), 0, st, T);
     else
-        hipLaunchKernelGGL(k_top_matvec<double>, g, dim3(1024), 0, st, T);
-    if (finish) hipLaunchKernelGGL(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
+        launch_k(k_top_matvec<double>, g, dim3(1024), 0, st, T);
+    if (finish) launch_k(k_top_finish, dim3(blocks_for(T.m)), dim3(kBlock), 0, st, T, X, n);
 }
 void launch_unit3(double* B, int n, const int* rows, int m, int j, hipStream_t st) {
     hipLaunchKernelGGL(k_unit3, dim3(blocks_for(3 * (size_t)n)), dim3(kBlock), 0, st, B, n, rows, m, j);

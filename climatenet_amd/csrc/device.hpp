@@ -7,8 +7,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "host.hpp"
 
@@ -126,6 +128,42 @@ struct TaskDesc {
     int64_t roff;
     int64_t rng;
 };
+
+// Kernel-duration timing. While a launch timer is current on the thread and has a family set,
+// the launches made through launch_k record their own start and end (the events that
+// hipExtLaunchKernelGGL attaches to the dispatch itself, so no separate markers are timed) under
+// that family; the owner reads them after synchronising.
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev;  // pairs
+    std::vector<int> fam;        // family of each recorded pair
+    size_t used = 0;             // events in use
+    int family = -1;             // < 0: launches are not timed
+};
+extern thread_local LaunchTimer* g_launch_timer;
+inline void launch_timer_events(hipEvent_t& a, hipEvent_t& b) {
+    a = b = nullptr;
+    LaunchTimer* t = g_launch_timer;
+    if (!t || t->family < 0) return;
+    if (t->used + 2 > t->ev.size())
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            t->ev.push_back(e);
+        }
+    a = t->ev[t->used];
+    b = t->ev[t->used + 1];
+    t->used += 2;
+    t->fam.push_back(t->family);
+}
+template <typename... KArgs, typename... Args>
+inline void launch_k(void (*kernel)(KArgs...), dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args) {
+    hipEvent_t e0, e1;
+    launch_timer_events(e0, e1);
+    if (e0)
+        hipExtLaunchKernelGGL(kernel, g, b, (std::uint32_t)lds, st, e0, e1, 0u, static_cast<KArgs>(args)...);
+    else
+        hipLaunchKernelGGL(kernel, g, b, lds, st, static_cast<KArgs>(args)...);
+}
 
 // Device layout of the substitution operators: per supernode its F_s block, column-major with
 // leading dimension op_ld(ns + nb), then its F_s^T block, column-major with leading dimension

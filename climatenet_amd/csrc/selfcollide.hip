@@ -1054,22 +1054,21 @@ void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st) {
 }
 void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
                        const StepArgs& a, int iteration, hipStream_t st) {
-    if (count > 0) hipLaunchKernelGGL(k_self_level, dim3(nblk(count)), dim3(kB), 0, st, S, l, v, n, mass, a, iteration);
+    if (count > 0) launch_k(k_self_level, dim3(nblk(count)), dim3(kB), 0, st, S, l, v, n, mass, a, iteration);
 }
 void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st) {
-    if (groups > 0) hipLaunchKernelGGL(k_self_apply, dim3(nblk(groups)), dim3(kB), 0, st, S, l);
+    if (groups > 0) launch_k(k_self_apply, dim3(nblk(groups)), dim3(kB), 0, st, S, l);
 }
 void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                        int iteration, double* rhs, hipStream_t st) {
-    hipLaunchKernelGGL(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
+    launch_k(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
 }
 void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                            int iteration, double* rhs, hipStream_t st) {
-    hipLaunchKernelGGL(k_self_fused_lds, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration,
-                       rhs);
+    launch_k(k_self_fused_lds, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
 }
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st) {
-    if (S.nslots > 0) hipLaunchKernelGGL(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);
+    if (S.nslots > 0) launch_k(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);
 }
 void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st) {
     if (S.nsc > 0) hipLaunchKernelGGL(k_self_ac_final, dim3(nblk(S.nsc)), dim3(kB), 0, st, S, v, n, a, iteration);

@@ -212,6 +212,9 @@ struct pdfc_sim {
     size_t kused = 0;
     double ksum[5] = {0., 0., 0., 0., 0.};
     int64_t kcount[5] = {0, 0, 0, 0, 0};
+    pdfc::LaunchTimer ltimer;           // the timed intervals' launches, each with its own events
+    double kdur[5] = {0., 0., 0., 0., 0.};       // kernel durations per family
+    int64_t kdur_count[5] = {0, 0, 0, 0, 0};     // launches per family
     int64_t ksolves = 0;
     pdfc_phase_stats stats{};
     int64_t steps_done = 0;
@@ -349,6 +352,7 @@ struct pdfc_sim {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : kev) (void)hipEventDestroy(e);
+        for (auto& e : ltimer.ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -368,7 +372,11 @@ struct pdfc_sim {
             }
         }
         HIP_CHECK(hipEventRecord(kev[kused], stream));
+        pdfc::g_launch_timer = &ltimer;
+        ltimer.family = family;
         launch();
+        ltimer.family = -1;
+        pdfc::g_launch_timer = nullptr;
         HIP_CHECK(hipEventRecord(kev[kused + 1], stream));
         kfamily.push_back(family);
         kused += 2;
@@ -2258,6 +2266,14 @@ void pdfc_sim::collect_kernel_times() {
     }
     kfamily.clear();
     kused = 0;
+    for (size_t k = 0; k < ltimer.fam.size(); ++k) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ltimer.ev[2 * k], ltimer.ev[2 * k + 1]));
+        kdur[ltimer.fam[k]] += 1e3 * ms;
+        kdur_count[ltimer.fam[k]]++;
+    }
+    ltimer.fam.clear();
+    ltimer.used = 0;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -2768,6 +2784,8 @@ int pdfc_set_kernel_timing(pdfc_sim* sim, int enable) {
         for (int k = 0; k < 5; ++k) {
             sim->ksum[k] = 0.;
             sim->kcount[k] = 0;
+            sim->kdur[k] = 0.;
+            sim->kdur_count[k] = 0;
         }
         sim->ksolves = 0;
     });
@@ -2778,20 +2796,20 @@ int pdfc_get_kernel_times(pdfc_sim* sim, pdfc_kernel_times* out) {
         if (!sim || !out) throw std::invalid_argument("null argument");
         sim->collect_kernel_times();
         pdfc_kernel_times t{};
-        // sweeps are timed whole (families 1 and 2 count sweeps); a sweep is one launch per
-        // level, or per range of merged levels
-        const int64_t per_fwd = (int64_t)sim->fwd_launches.size(), per_bwd = (int64_t)sim->bwd_launches.size();
-        t.local_launches = sim->kcount[0];
-        t.forward_launches = sim->kcount[1] * per_fwd;
-        t.backward_launches = sim->kcount[2] * per_bwd;
-        t.local_us = sim->kcount[0] ? sim->ksum[0] / sim->kcount[0] : 0.;
-        t.forward_sweep_us = sim->kcount[1] ? sim->ksum[1] / sim->kcount[1] : 0.;
-        t.backward_sweep_us = sim->kcount[2] ? sim->ksum[2] / sim->kcount[2] : 0.;
-        t.forward_us = per_fwd ? t.forward_sweep_us / per_fwd : 0.;
-        t.backward_us = per_bwd ? t.backward_sweep_us / per_bwd : 0.;
-        t.top_us = sim->kcount[3] ? sim->ksum[3] / sim->kcount[3] : 0.;
+        // kernel durations: each launch's own start and end (the events attached to the dispatch);
+        // the sweeps also as whole intervals between events around them (launch gaps included)
+        auto per = [](double sum, int64_t count) { return count ? sum / (double)count : 0.; };
+        t.local_launches = sim->kdur_count[0];
+        t.forward_launches = sim->kdur_count[1];
+        t.backward_launches = sim->kdur_count[2];
+        t.local_us = per(sim->kdur[0], sim->kdur_count[0]);
+        t.forward_us = per(sim->kdur[1], sim->kdur_count[1]);
+        t.backward_us = per(sim->kdur[2], sim->kdur_count[2]);
+        t.forward_sweep_us = per(sim->ksum[1], sim->kcount[1]);
+        t.backward_sweep_us = per(sim->ksum[2], sim->kcount[2]);
+        t.top_us = per(sim->kdur[3], sim->kcount[3]);
         t.top_launches = sim->kcount[3];
-        t.self_us = sim->kcount[4] ? sim->ksum[4] / sim->kcount[4] : 0.;
+        t.self_us = per(sim->kdur[4], sim->kcount[4]);
         t.self_iterations = sim->kcount[4];
         *out = t;
     });

@@ -246,14 +246,16 @@ typedef struct pdfc_phase_stats {    /* device time of the last pdfc_step call, 
     int64_t self_levels_max;         /* most self-contact levels of one step in the call */
 } pdfc_phase_stats;
 
-typedef struct pdfc_kernel_times {  /* device time (HIP events), averaged over an instrumented run */
+typedef struct pdfc_kernel_times {  /* device time averaged over an instrumented run: kernel durations from
+                                        the start and end events attached to each launch (hipExtLaunchKernel),
+                                        sweeps also as event-pair intervals */
     double local_us;                 /* local step + RHS + obstacle friction kernel, per launch */
-    double forward_us;               /* one forward-substitution level launch: sweep time / launches per sweep */
-    double backward_us;              /* one backward-substitution level launch: sweep time / launches per sweep */
-    double top_us;                   /* dense top block (gather + matvec + scatter) */
-    double forward_sweep_us;         /* one forward sweep, one event pair around all its launches */
+    double forward_us;               /* one forward-substitution level launch */
+    double backward_us;              /* one backward-substitution level launch */
+    double top_us;                   /* dense top block (gather, tiles or matvec, finish) per solve */
+    double forward_sweep_us;         /* one forward sweep, one event pair around all its launches (gaps included) */
     double backward_sweep_us;
-    int64_t local_launches, forward_launches, backward_launches, top_launches;
+    int64_t local_launches, forward_launches, backward_launches, top_launches;  /* top: solves */
     double self_us;                  /* self-contact friction of one iteration (all its level launches) */
     int64_t self_iterations;         /* iterations with self contacts that self_us averages over */
 } pdfc_kernel_times;
