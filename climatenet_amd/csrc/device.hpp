@@ -230,6 +230,13 @@ struct StepArgs {
 };
 
 // Self-collision detection buffers (scene numbering for vertices and triangles).
+// A bucket slot's prefilter data: the cell of the triangle box's lower corner (21 bits per
+// axis) and the box rounded outwards to float, so a test that the double box passes also passes.
+struct alignas(16) BoxKey {
+    long long key;
+    float lo[3], hi[3];
+};
+
 struct DevSelfDetect {
     int n, T;
     const double* q;        // positions, supernodal planar layout
@@ -256,9 +263,9 @@ struct DevSelfDetect {
     SelfCandidate* filt;
     int cap_full, cap_filt;
     int write_full;         // also write the complete candidate list
-    // single-pass path (filtered list only): triangle data sorted by bucket and a
-    // fixed number of candidate slots per vertex
-    double* bk_box;         // 6 per bucket slot
+    // single-pass path (filtered list only): triangle data sorted by bucket and a fixed number
+    // of candidate slots per vertex
+    struct BoxKey* bk_key;  // per bucket slot: the lower corner's cell and the box rounded outwards to float
     double* bk_pos;         // 9 per bucket slot: the three vertex positions
     int* bk_vid;            // 3 per bucket slot: scene vertex indices
     int* bk_tri;            // triangle index per bucket slot

@@ -248,14 +248,28 @@ __global__ void k_vertex_query(DevSelfDetect D, int pass) {
 }
 
 // Bucket-sorted copy of the triangle data (box, vertex positions, vertex ids).
+// Single-pass path: triangle index, vertices and positions into the bucket of the cell of the
+// triangle box's lower corner (bucket slots sorted by bucket).
+__device__ __forceinline__ long long cell_key(long long ix, long long iy, long long iz) {
+    const long long m = (1ll << 21) - 1;
+    return ((ix & m) << 42) | ((iy & m) << 21) | (iz & m);
+}
+
 __global__ void k_tri_scatter_packed(DevSelfDetect D) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= D.T) return;
     double c = cell_size(D);
-    unsigned h = cell_hash(cell_of(D.tbox[6 * t], c), cell_of(D.tbox[6 * t + 1], c), cell_of(D.tbox[6 * t + 2], c), D.P);
+    const long long ix = cell_of(D.tbox[6 * t], c), iy = cell_of(D.tbox[6 * t + 1], c), iz = cell_of(D.tbox[6 * t + 2], c);
+    unsigned h = cell_hash(ix, iy, iz, D.P);
     int q = D.bucket_off[h] + atomicAdd(&D.bucket_fill[h], 1);
+    BoxKey e;
+    e.key = cell_key(ix, iy, iz);
+    for (int k = 0; k < 3; ++k) {
+        e.lo[k] = __double2float_rd(D.tbox[6 * t + k]);
+        e.hi[k] = __double2float_ru(D.tbox[6 * t + 3 + k]);
+    }
+    D.bk_key[q] = e;
     D.bk_tri[q] = t;
-    for (int k = 0; k < 6; ++k) D.bk_box[6 * q + k] = D.tbox[6 * t + k];
     for (int k = 0; k < 3; ++k) {
         int v = D.tris[3 * t + k];
         D.bk_vid[3 * q + k] = v;
@@ -266,86 +280,114 @@ __global__ void k_tri_scatter_packed(DevSelfDetect D) {
     }
 }
 
-// One pass over the grid: filtered candidates into the vertex's slots. Eight lanes per
-// vertex, one grid cell each (a query box spans 2 x 2 x 2 cells, rarely up to 3 per
-// axis); slot order is arbitrary, the compaction sorts by triangle.
-constexpr int kLanesPerVertex = 8;
-__global__ void k_vertex_query_slots(DevSelfDetect D) {
-    __shared__ int slot_count[kB / kLanesPerVertex];
-    const int local_v = threadIdx.x / kLanesPerVertex, sub = threadIdx.x % kLanesPerVertex;
-    const int v = blockIdx.x * (kB / kLanesPerVertex) + local_v;
+// One pass over the grid: filtered candidates into the vertex's slots. The vertex's query box
+// x - 1.1 tol - E .. x + 1.1 tol (E the largest triangle extent) is shorter than a cell, so it
+// spans at most 2 x 2 x 2 cells; their buckets are taken as one list that kFlatLanes lanes split
+// (lane j takes entries j, j + kFlatLanes, ...), so each lane's chain of dependent loads is a few
+// entries long whatever the buckets' sizes. A bucket entry counts for a cell only if its triangle's
+// lower corner lies in that cell (buckets shared by several cells through the hash), and only if
+// its float box, rounded outwards, meets the query box; then its positions are read and the
+// exact tests run on the double box (the min / max of the positions, the values k_tri_box
+// computes). Slot order is arbitrary, the compaction sorts by triangle.
+constexpr int kFlatLanes = 16;
+constexpr int kQueryCells = 8;
+__global__ void k_vertex_query_flat(DevSelfDetect D) {
+    __shared__ int slot_count[kB / kFlatLanes];
+    const int local_v = threadIdx.x / kFlatLanes, sub = threadIdx.x % kFlatLanes;
+    const int v = blockIdx.x * (kB / kFlatLanes) + local_v;
     if (sub == 0) slot_count[local_v] = 0;
     __syncthreads();
     if (v < D.n) {
         const double c = cell_size(D);
         const double E = __longlong_as_double((long long)*D.ext_bits) * (1. + 1e-9);
-        P3 x = at_perm(D.q, D.n, D.iperm[v]);
+        const P3 x = at_perm(D.q, D.n, D.iperm[v]);
         const double lo[3] = {x.x - D.t11, x.y - D.t11, x.z - D.t11};
         const double hi[3] = {x.x + D.t11, x.y + D.t11, x.z + D.t11};
-        long long c0[3], span[3];
-        bool sane = isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+        long long c0[3];
+        int span[3];
+        const bool sane = isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
         for (int k = 0; k < 3; ++k) {
             c0[k] = sane ? cell_of(lo[k] - E, c) : 0;
-            span[k] = sane ? cell_of(hi[k], c) - c0[k] + 1 : 0;
-            if (span[k] > 3 || span[k] < 0) span[k] = 0;
+            const long long sp = sane ? cell_of(hi[k], c) - c0[k] + 1 : 0;
+            span[k] = (sp < 0 || sp > 3) ? 0 : (int)sp;
         }
-        const int ncell = (int)(span[0] * span[1] * span[2]);
-        SelfCandidate* out = D.vslots + (size_t)v * kSlotsPerVertex;
-        for (int ci = sub; ci < ncell; ci += kLanesPerVertex) {
-            const long long ix = c0[0] + ci % span[0];
-            const long long iy = c0[1] + (ci / span[0]) % span[1];
-            const long long iz = c0[2] + ci / (span[0] * span[1]);
-            unsigned h = cell_hash(ix, iy, iz, D.P);
-            int b0 = D.bucket_off[h], b1 = D.bucket_off[h + 1];
-            // the next slot's box is loaded while this one is tested
-            double nbx[6];
+        int ncell = span[0] * span[1] * span[2];
+        if (ncell > kQueryCells) {  // not for a finite vertex (the box is shorter than a cell)
+            atomicOr(D.overflow, 1);
+            ncell = 0;
+        }
+        int bstart[kQueryCells], blen[kQueryCells];
+        int total = 0;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) nbx[k] = b0 < b1 ? D.bk_box[6 * (size_t)b0 + k] : 0.;
-            for (int q = b0; q < b1; ++q) {
-                double bx[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) bx[k] = nbx[k];
-                const int qn = min(q + 1, b1 - 1);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) nbx[k] = D.bk_box[6 * (size_t)qn + k];
-                if (cell_of(bx[0], c) != ix || cell_of(bx[1], c) != iy || cell_of(bx[2], c) != iz) continue;
-                if (!(bx[0] <= hi[0] && lo[0] <= bx[3] && bx[1] <= hi[1] && lo[1] <= bx[4] && bx[2] <= hi[2] &&
-                      lo[2] <= bx[5]))
-                    continue;
-                const int* vid = &D.bk_vid[3 * q];
-                if (vid[0] == v || vid[1] == v || vid[2] == v) continue;  // never kept by the pairing
-                const double* pp = &D.bk_pos[9 * q];
-                P3 a = {pp[0], pp[1], pp[2]}, b = {pp[3], pp[4], pp[5]}, cc = {pp[6], pp[7], pp[8]};
-                double bary[3];
-                P3 cp;
-                closest_point(x, a, b, cc, D.tol, bary, cp);
-                P3 d = psub(cp, x);
-                if (!(pdot(d, d) < D.tol_sq)) continue;
-                if (bary[0] == 0 || bary[1] == 0 || bary[2] == 0) continue;
-                int k = atomicAdd(&slot_count[local_v], 1);
-                if (k >= kSlotsPerVertex) {
-                    atomicOr(D.overflow, 1);
-                    continue;
-                }
-                SelfCandidate& o = out[k];
-                o.v = v;
-                o.t = D.bk_tri[q];
-                o.bary[0] = bary[0];
-                o.bary[1] = bary[1];
-                o.bary[2] = bary[2];
-                P3 cpb = {(a.x * bary[0] + b.x * bary[1]) + cc.x * bary[2],
-                          (a.y * bary[0] + b.y * bary[1]) + cc.y * bary[2],
-                          (a.z * bary[0] + b.z * bary[1]) + cc.z * bary[2]};
-                P3 nv = psub(x, cpb);
-                double z = pdot(nv, nv);
-                if (z > 0.) {
-                    double sq = sqrt(z);
-                    nv = {nv.x / sq, nv.y / sq, nv.z / sq};
-                }
-                o.normal[0] = nv.x;
-                o.normal[1] = nv.y;
-                o.normal[2] = nv.z;
+        for (int ci = 0; ci < kQueryCells; ++ci) {
+            bstart[ci] = 0;
+            blen[ci] = 0;
+            if (ci < ncell) {
+                const long long ix = c0[0] + ci % span[0], iy = c0[1] + (ci / span[0]) % span[1],
+                                iz = c0[2] + ci / (span[0] * span[1]);
+                const unsigned h = cell_hash(ix, iy, iz, D.P);
+                bstart[ci] = D.bucket_off[h];
+                blen[ci] = D.bucket_off[h + 1] - bstart[ci];
             }
+            total += blen[ci];
+        }
+        SelfCandidate* out = D.vslots + (size_t)v * kSlotsPerVertex;
+        for (int k = sub; k < total; k += kFlatLanes) {
+            int q = -1, cell = 0, base = 0;
+#pragma unroll
+            for (int ci = 0; ci < kQueryCells; ++ci) {
+                if (q < 0 && k < base + blen[ci]) {
+                    q = bstart[ci] + (k - base);
+                    cell = ci;
+                }
+                base += blen[ci];
+            }
+            const long long ix = c0[0] + cell % span[0], iy = c0[1] + (cell / span[0]) % span[1],
+                            iz = c0[2] + cell / (span[0] * span[1]);
+            // prefilter on 32 bytes: the entry's cell, then its box rounded outwards
+            const BoxKey e = D.bk_key[q];
+            if (e.key != cell_key(ix, iy, iz)) continue;
+            if (!((double)e.lo[0] <= hi[0] && lo[0] <= (double)e.hi[0] && (double)e.lo[1] <= hi[1] &&
+                  lo[1] <= (double)e.hi[1] && (double)e.lo[2] <= hi[2] && lo[2] <= (double)e.hi[2]))
+                continue;
+            const double* pp = &D.bk_pos[9 * (size_t)q];
+            const P3 a = {pp[0], pp[1], pp[2]}, b = {pp[3], pp[4], pp[5]}, cc = {pp[6], pp[7], pp[8]};
+            const double bx[6] = {fmin(fmin(a.x, b.x), cc.x), fmin(fmin(a.y, b.y), cc.y), fmin(fmin(a.z, b.z), cc.z),
+                                  fmax(fmax(a.x, b.x), cc.x), fmax(fmax(a.y, b.y), cc.y), fmax(fmax(a.z, b.z), cc.z)};
+            if (!(bx[0] <= hi[0] && lo[0] <= bx[3] && bx[1] <= hi[1] && lo[1] <= bx[4] && bx[2] <= hi[2] &&
+                  lo[2] <= bx[5]))
+                continue;
+            const int* vid = &D.bk_vid[3 * (size_t)q];
+            if (vid[0] == v || vid[1] == v || vid[2] == v) continue;  // never kept by the pairing
+            double bary[3];
+            P3 cp;
+            closest_point(x, a, b, cc, D.tol, bary, cp);
+            P3 d = psub(cp, x);
+            if (!(pdot(d, d) < D.tol_sq)) continue;
+            if (bary[0] == 0 || bary[1] == 0 || bary[2] == 0) continue;
+            int slot = atomicAdd(&slot_count[local_v], 1);
+            if (slot >= kSlotsPerVertex) {
+                atomicOr(D.overflow, 1);
+                continue;
+            }
+            SelfCandidate& o = out[slot];
+            o.v = v;
+            o.t = D.bk_tri[q];
+            o.bary[0] = bary[0];
+            o.bary[1] = bary[1];
+            o.bary[2] = bary[2];
+            P3 cpb = {(a.x * bary[0] + b.x * bary[1]) + cc.x * bary[2],
+                      (a.y * bary[0] + b.y * bary[1]) + cc.y * bary[2],
+                      (a.z * bary[0] + b.z * bary[1]) + cc.z * bary[2]};
+            P3 nv = psub(x, cpb);
+            double z = pdot(nv, nv);
+            if (z > 0.) {
+                double sq = sqrt(z);
+                nv = {nv.x / sq, nv.y / sq, nv.z / sq};
+            }
+            o.normal[0] = nv.x;
+            o.normal[1] = nv.y;
+            o.normal[2] = nv.z;
         }
     }
     __syncthreads();
@@ -1037,7 +1079,7 @@ void launch_self_detect_filtered(const DevSelfDetect& D, const int* ccnt, unsign
     launch_copy_int(D.bucket_off, D.bucket_cnt, D.P + 1, st);
     launch_scan_int(D.bucket_off, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
     hipLaunchKernelGGL(k_tri_scatter_packed, dim3(nblk(D.T)), dim3(kB), 0, st, D);
-    hipLaunchKernelGGL(k_vertex_query_slots, dim3((unsigned)((D.n + kB / kLanesPerVertex - 1) / (kB / kLanesPerVertex))),
+    hipLaunchKernelGGL(k_vertex_query_flat, dim3((unsigned)((D.n + kB / kFlatLanes - 1) / (kB / kFlatLanes))),
                        dim3(kB), 0, st, D);
     launch_copy_int(D.off_filt, D.cnt_filt, D.n, st);
     launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);

@@ -831,7 +831,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         sd.cap_full = 0;  // allocated on demand (pdfc_detect)
         sd.cap_filt = std::max(1024, n);
         sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
-        sd.bk_box = dalloc<double>(6 * (size_t)T);
+        sd.bk_key = dalloc<pdfc::BoxKey>(T);
         sd.bk_pos = dalloc<double>(9 * (size_t)T);
         sd.bk_vid = dalloc<int>(3 * (size_t)T);
         sd.bk_tri = dalloc<int>(T);
