@@ -1,5 +1,8 @@
 // Contact MLP for the learned warm start: 12 -> 64 -> 64 -> 64 -> 3, SiLU,
-// BF16 operands on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation.
+// BF16 operands on MFMA with FP32 accumulation: gfx950's v_mfma_f32_16x16x32_bf16 for the
+// 64-deep products (the hidden and output layers, the gradients through 64-wide layers), and
+// v_mfma_f32_16x16x16_bf16 where the depth is 16 (the 12 input features padded to 16, the
+// weight gradients summed over a 16-row tile, the gradient through the 16-wide padded output).
 //
 // One wave handles a tile of 16 samples through all layers; activations live
 // in LDS, weight operands are read from global memory (10 K parameters, cache
@@ -7,9 +10,10 @@
 // per-wave partial weight gradients, sums them in a fixed order and applies
 // Adam to FP32 master weights, from which the BF16 operand copies are refreshed.
 //
-// Operand placement follows the 16x16x16 MFMA layout: A (16 x K tile) lane l
-// holds row l%16, k = 4*(l/16)..+3; B (K x 16) lane l holds column l%16,
-// k = 4*(l/16)..+3; D lane l holds column l%16, rows 4*(l/16)..+3.
+// Operand placement: A (16 x K tile) lane l holds row l%16 and B (K x 16) lane l
+// column l%16, both for k = 4*(l/16)..+3 (16-deep) or k = 8*(l/16)..+7 (32-deep); A and B
+// take the same k of a lane, so the product does not depend on how the instruction orders
+// its k. D lane l holds column l%16, rows 4*(l/16)..+3 (both forms).
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -29,34 +33,47 @@ namespace pdfc {
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// round-to-nearest-even FP32 -> BF16 bit pattern (NaN kept quiet)
-__device__ __forceinline__ unsigned short to_bf16(float x) {
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+// FP32 -> BF16 bit pattern, round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned short to_bf16(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// sigmoid through the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32: 1 ulp each)
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    float s = 1.f / (1.f + __expf(-z));
+    float s = sigmoid(z);
     return s * (1.f + z * (1.f - s));
 }
 
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(s4 a, s4 b, f4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+}
+
 // D(16 x 16 tile n0) += A(16 x K, bf16 row-major in LDS, leading dim lda) * B, where the
-// B operand for column n, k-block kk comes from Bt (N x K row-major bf16, leading dim K).
+// B operand for column n comes from Bt (N x K row-major bf16, leading dim K). K is 16 or 64:
+// 32-deep steps while they fit, then a 16-deep one.
 __device__ __forceinline__ f4 mfma_tile(const unsigned short* A, int lda, const unsigned short* Bt, int K, int n0,
                                         f4 acc) {
     const int l = threadIdx.x & 63;
-    const int r = l & 15, kq = (l >> 4) * 4;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        // 4 consecutive bf16 (8-byte aligned: lda, K and k0 + kq are multiples of 4)
+    const int r = l & 15;
+    int k0 = 0;
+    for (; k0 + 32 <= K; k0 += 32) {
+        // 8 consecutive bf16 (16-byte aligned: lda, K, k0 and kq are multiples of 8)
+        const int kq = (l >> 4) * 8;
+        const bf8 a = *reinterpret_cast<const bf8*>(A + r * lda + k0 + kq);
+        const bf8 b = *reinterpret_cast<const bf8*>(Bt + (size_t)(n0 + r) * K + k0 + kq);
+        acc = mfma32(a, b, acc);
+    }
+    if (k0 < K) {
+        const int kq = (l >> 4) * 4;
         const s4 a = *reinterpret_cast<const s4*>(A + r * lda + k0 + kq);
         const s4 b = *reinterpret_cast<const s4*>(Bt + (size_t)(n0 + r) * K + k0 + kq);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        acc = mfma16(a, b, acc);
     }
     return acc;
 }
@@ -121,22 +138,25 @@ __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev P, const float* __res
     __shared__ alignas(16) unsigned short X[16 * 16];
     __shared__ alignas(16) unsigned short H[2][16 * 64];
     const int l = threadIdx.x;
-    const int r = l & 15, kq = (l >> 4) * 4, col = l & 15, rq = (l >> 4) * 4;
-    s4 w0[4], w1[4][4], w2[4][4], w3[4];
+    const int r = l & 15, kq = (l >> 4) * 4, kq8 = (l >> 4) * 8, col = l & 15, rq = (l >> 4) * 4;
+    s4 w0[4];
+    bf8 w1[4][2], w2[4][2], w3[2];
     float b0[4], b1[4], b2[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         w0[nt] = *reinterpret_cast<const s4*>(P.wt_bf[0] + (size_t)(nt * 16 + r) * kMlpIn + kq);
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            w1[nt][ks] = *reinterpret_cast<const s4*>(P.wt_bf[1] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 16 + kq);
-            w2[nt][ks] = *reinterpret_cast<const s4*>(P.wt_bf[2] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 16 + kq);
+        for (int ks = 0; ks < 2; ++ks) {
+            w1[nt][ks] = *reinterpret_cast<const bf8*>(P.wt_bf[1] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 32 + kq8);
+            w2[nt][ks] = *reinterpret_cast<const bf8*>(P.wt_bf[2] + (size_t)(nt * 16 + r) * kMlpHidden + ks * 32 + kq8);
         }
-        w3[nt] = *reinterpret_cast<const s4*>(P.wt_bf[3] + (size_t)r * kMlpHidden + nt * 16 + kq);
         b0[nt] = P.b[0][nt * 16 + col];
         b1[nt] = P.b[1][nt * 16 + col];
         b2[nt] = P.b[2][nt * 16 + col];
     }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+        w3[ks] = *reinterpret_cast<const bf8*>(P.wt_bf[3] + (size_t)r * kMlpHidden + ks * 32 + kq8);
     const float b3 = P.b[3][col];
     const float ys = col < kMlpOut ? P.y_scale[col] : 0.f;
     const int ntiles = (nrows + 15) / 16;
@@ -155,7 +175,7 @@ __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev P, const float* __res
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 f4 acc = {0.f, 0.f, 0.f, 0.f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w0[nt], acc, 0, 0, 0);
+                acc = mfma16(a, w0[nt], acc);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) H[0][(rq + j) * 64 + nt * 16 + col] = to_bf16(silu(acc[j] + b0[nt]));
             }
@@ -165,16 +185,14 @@ __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev P, const float* __res
         for (int layer = 1; layer < 3; ++layer) {  // layers 2 and 3
             const unsigned short* in = H[layer - 1];
             unsigned short* out = H[layer & 1 ? 1 : 0];
-            s4 a[4];
+            bf8 a[2];
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const s4*>(in + r * 64 + ks * 16 + kq);
+            for (int ks = 0; ks < 2; ++ks) a[ks] = *reinterpret_cast<const bf8*>(in + r * 64 + ks * 32 + kq8);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], layer == 1 ? w1[nt][ks] : w2[nt][ks], acc,
-                                                                    0, 0, 0);
+                for (int ks = 0; ks < 2; ++ks) acc = mfma32(a[ks], layer == 1 ? w1[nt][ks] : w2[nt][ks], acc);
                 const float bb = layer == 1 ? b1[nt] : b2[nt];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) out[(rq + j) * 64 + nt * 16 + col] = to_bf16(silu(acc[j] + bb));
@@ -182,12 +200,12 @@ __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev P, const float* __res
             __syncthreads();
         }
         {  // output layer from H[0] (layer 3's activations)
-            s4 a[4];
+            bf8 a[2];
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const s4*>(H[0] + r * 64 + ks * 16 + kq);
+            for (int ks = 0; ks < 2; ++ks) a[ks] = *reinterpret_cast<const bf8*>(H[0] + r * 64 + ks * 32 + kq8);
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], w3[ks], acc, 0, 0, 0);
+            for (int ks = 0; ks < 2; ++ks) acc = mfma32(a[ks], w3[ks], acc);
             if (col < kMlpOut) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -249,7 +267,7 @@ __global__ void __launch_bounds__(64) k_mlp_grad_tile(MlpDev P, const float* __r
                     b[j] = (short)T.G[(rq + j) * Nout + nt * 16 + col];    // B[k=row][n=feature]
                 }
                 f4 acc = {0.f, 0.f, 0.f, 0.f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+                acc = mfma16(a, b, acc);
                 for (int j = 0; j < 4; ++j) {
                     int i = mt * 16 + rq + j, n = nt * 16 + col;
                     float v = acc[j];
