@@ -246,8 +246,7 @@ struct DevSelfDetect {
     double* tbox;           // 6T: lo xyz, hi xyz
     unsigned long long* ext_bits;  // largest box extent (bit pattern)
     int P;                  // hash buckets (power of two)
-    int* bucket_cnt;        // P + 1
-    int* bucket_fill;       // P
+    int* bucket_cnt;        // P + 1; zero between detections
     int* bucket_off;        // P + 1
     int* bucket_tris;       // T
     int* cnt_full;
@@ -309,7 +308,7 @@ struct DevSelf {
 void launch_fill(double* p, size_t count, double value, hipStream_t st);
 void launch_fill_int(int* p, int count, int value, hipStream_t st);
 void launch_copy_int(int* dst, const int* src, int count, hipStream_t st);
-void launch_scan_int(int* a, int count, int* sums, int* total, int* tail, hipStream_t st);
+void launch_scan_int(int* dst, const int* src, int count, int* sums, int* total, int* tail, hipStream_t st);
 void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st);
 // Single pass: filtered candidates only (the hot path); falls back to launch_self_detect
 // when *overflow is set.

@@ -82,10 +82,13 @@ __global__ void k_copy_int(int* dst, const int* src, int count) {
     if (i < count) dst[i] = src[i];
 }
 
-// Triangle bounding boxes and the largest box extent (for the cell size).
+// Triangle bounding boxes and the largest box extent (for the cell size). Also clears the
+// slot-overflow flag of the query that follows (the largest extent starts from zero: the
+// previous detection's k_seed_bits, or the allocation, cleared it).
 __global__ void k_tri_box(DevSelfDetect D) {
     __shared__ double red[kB / 64];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *D.overflow = 0;
     double ext = 0.;
     if (t < D.T) {
         P3 a = at_perm(D.q, D.n, D.iperm[D.tris[3 * t]]);
@@ -122,7 +125,8 @@ __global__ void k_tri_scatter(DevSelfDetect D) {
     if (t >= D.T) return;
     double c = cell_size(D);
     unsigned h = cell_hash(cell_of(D.tbox[6 * t], c), cell_of(D.tbox[6 * t + 1], c), cell_of(D.tbox[6 * t + 2], c), D.P);
-    int slot = atomicAdd(&D.bucket_fill[h], 1);
+    // positions from the bucket's end down; the count returns to zero for the next detection
+    int slot = atomicSub(&D.bucket_cnt[h], 1) - 1;
     D.bucket_tris[D.bucket_off[h] + slot] = t;
 }
 
@@ -261,7 +265,8 @@ __global__ void k_tri_scatter_packed(DevSelfDetect D) {
     double c = cell_size(D);
     const long long ix = cell_of(D.tbox[6 * t], c), iy = cell_of(D.tbox[6 * t + 1], c), iz = cell_of(D.tbox[6 * t + 2], c);
     unsigned h = cell_hash(ix, iy, iz, D.P);
-    int q = D.bucket_off[h] + atomicAdd(&D.bucket_fill[h], 1);
+    // positions from the bucket's end down; the count returns to zero for the next detection
+    int q = D.bucket_off[h] + atomicSub(&D.bucket_cnt[h], 1) - 1;
     BoxKey e;
     e.key = cell_key(ix, iy, iz);
     for (int k = 0; k < 3; ++k) {
@@ -418,19 +423,23 @@ __global__ void k_compact_slots(DevSelfDetect D) {
     for (int i = 0; i < cnt; ++i) D.filt[base + i] = in[order[i]];
 }
 
-// Bitset (permuted numbering) of vertices with at least one obstacle contact.
-__global__ void k_seed_bits(const int* ccnt, int n, unsigned long long* bits) {
+// Bitset (permuted numbering) of vertices with at least one obstacle contact. The last kernel
+// of a detection: it also clears the largest box extent for the next one.
+__global__ void k_seed_bits(const int* ccnt, int n, unsigned long long* bits, unsigned long long* ext_bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *ext_bits = 0ull;
     bool on = i < n && ccnt[i] > 0;
     unsigned long long m = __ballot(on);
     if ((threadIdx.x & 63) == 0 && i < n) bits[i >> 6] = m;
 }
 
-// Block-wise exclusive scan in place; per-block totals into sums.
-__global__ void k_scan_local(int* a, int count, int* sums) {
+// Exclusive scan of int counts, dst = scan(src) (dst may be src); *total (and *tail when given)
+// = the sum. Block-wise scans with per-block totals, then each block adds the sum of the totals
+// of the blocks before it, which it reduces itself (the last block writes the sum).
+__global__ void k_scan_local(int* dst, const int* src, int count, int* sums) {
     __shared__ int buf[kB];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = i < count ? a[i] : 0;
+    int v = i < count ? src[i] : 0;
     buf[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < kB; off <<= 1) {
@@ -439,36 +448,27 @@ __global__ void k_scan_local(int* a, int count, int* sums) {
         buf[threadIdx.x] += t;
         __syncthreads();
     }
-    if (i < count) a[i] = buf[threadIdx.x] - v;
+    if (i < count) dst[i] = buf[threadIdx.x] - v;
     if (threadIdx.x == kB - 1) sums[blockIdx.x] = buf[kB - 1];
 }
-__global__ void k_scan_sums(int* sums, int nblocks, int* total) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+__global__ void k_scan_add(int* a, int count, const int* sums, int* total, int* tail) {
+    __shared__ int red[kB / 64];
+    const int b = blockIdx.x;
+    int s = 0;
+    for (int k = threadIdx.x; k < b; k += kB) s += sums[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int base = 0; base < nblocks; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < nblocks ? sums[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nblocks) sums[i] = carry + buf[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
+    s = 0;
+#pragma unroll
+    for (int k = 0; k < kB / 64; ++k) s += red[k];
+    const int i = b * kB + threadIdx.x;
+    if (i < count) a[i] += s;
+    if (b == (int)gridDim.x - 1 && threadIdx.x == 0) {
+        const int sum = s + sums[b];
+        *total = sum;
+        if (tail) *tail = sum;
     }
-    if (threadIdx.x == 0) *total = carry;
-}
-__global__ void k_scan_add(int* a, int count, const int* sums, const int* total, int* tail) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) a[i] += sums[blockIdx.x];
-    if (tail && i == 0) *tail = *total;
 }
 
 // ---------------------------------------------------------------- friction
@@ -1030,7 +1030,7 @@ void launch_mesh_build(const DevMeshBuild& B, hipStream_t st) {
         hipLaunchKernelGGL(k_mesh_box, dim3(nblk(B.T)), dim3(kB), 0, st, B);
         hipLaunchKernelGGL(k_mesh_count, dim3(nblk(B.T)), dim3(kB), 0, st, B);
     }
-    launch_scan_int(B.off, B.P + 1, B.scan_sums, B.scan_total, nullptr, st);
+    launch_scan_int(B.off, B.off, B.P + 1, B.scan_sums, B.scan_total, nullptr, st);
     if (B.T > 0) hipLaunchKernelGGL(k_mesh_scatter, dim3(nblk(B.T)), dim3(kB), 0, st, B);
 }
 
@@ -1042,49 +1042,38 @@ void launch_copy_int(int* dst, const int* src, int count, hipStream_t st) {
     if (count > 0) hipLaunchKernelGGL(k_copy_int, dim3(nblk(count)), dim3(kB), 0, st, dst, src, count);
 }
 
-void launch_scan_int(int* a, int count, int* sums, int* total, int* tail, hipStream_t st) {
-    unsigned nb = nblk(count);
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kB), 0, st, a, count, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, (int)nb, total);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kB), 0, st, a, count, sums, total, tail);
+void launch_scan_int(int* dst, const int* src, int count, int* sums, int* total, int* tail, hipStream_t st) {
+    const unsigned nb = nblk(count);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kB), 0, st, dst, src, count, sums);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kB), 0, st, dst, count, sums, total, tail);
 }
 
+// The bucket counts are zero on entry (the previous detection's scatter took them back to zero,
+// or the allocation cleared them), and so is the largest box extent (k_seed_bits).
 void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, reinterpret_cast<int*>(D.ext_bits), 2, 0);
     hipLaunchKernelGGL(k_tri_box, dim3(nblk(D.T)), dim3(kB), 0, st, D);
-    launch_fill_int(D.bucket_cnt, D.P + 1, 0, st);
-    launch_fill_int(D.bucket_fill, D.P, 0, st);
     hipLaunchKernelGGL(k_tri_count, dim3(nblk(D.T)), dim3(kB), 0, st, D);
     // bucket_off = exclusive scan of bucket_cnt (P+1 entries, the last one becomes T)
-    launch_copy_int(D.bucket_off, D.bucket_cnt, D.P + 1, st);
-    launch_scan_int(D.bucket_off, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
+    launch_scan_int(D.bucket_off, D.bucket_cnt, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
     hipLaunchKernelGGL(k_tri_scatter, dim3(nblk(D.T)), dim3(kB), 0, st, D);
     hipLaunchKernelGGL(k_vertex_query, dim3(nblk(D.n)), dim3(kB), 0, st, D, 0);
-    launch_copy_int(D.off_full, D.cnt_full, D.n, st);
-    launch_copy_int(D.off_filt, D.cnt_filt, D.n, st);
-    launch_scan_int(D.off_full, D.n, D.scan_sums, D.totals + 0, nullptr, st);
-    launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
+    launch_scan_int(D.off_full, D.cnt_full, D.n, D.scan_sums, D.totals + 0, nullptr, st);
+    launch_scan_int(D.off_filt, D.cnt_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
     hipLaunchKernelGGL(k_vertex_query, dim3(nblk(D.n)), dim3(kB), 0, st, D, 1);
-    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits);
+    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits, D.ext_bits);
 }
 
 void launch_self_detect_filtered(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits,
                                  hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, reinterpret_cast<int*>(D.ext_bits), 2, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, D.overflow, 1, 0);
     hipLaunchKernelGGL(k_tri_box, dim3(nblk(D.T)), dim3(kB), 0, st, D);
-    launch_fill_int(D.bucket_cnt, D.P + 1, 0, st);
-    launch_fill_int(D.bucket_fill, D.P, 0, st);
     hipLaunchKernelGGL(k_tri_count, dim3(nblk(D.T)), dim3(kB), 0, st, D);
-    launch_copy_int(D.bucket_off, D.bucket_cnt, D.P + 1, st);
-    launch_scan_int(D.bucket_off, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
+    launch_scan_int(D.bucket_off, D.bucket_cnt, D.P + 1, D.scan_sums, D.totals + 2, nullptr, st);
     hipLaunchKernelGGL(k_tri_scatter_packed, dim3(nblk(D.T)), dim3(kB), 0, st, D);
     hipLaunchKernelGGL(k_vertex_query_flat, dim3((unsigned)((D.n + kB / kFlatLanes - 1) / (kB / kFlatLanes))),
                        dim3(kB), 0, st, D);
-    launch_copy_int(D.off_filt, D.cnt_filt, D.n, st);
-    launch_scan_int(D.off_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
+    launch_scan_int(D.off_filt, D.cnt_filt, D.n, D.scan_sums2, D.totals + 1, nullptr, st);
     hipLaunchKernelGGL(k_compact_slots, dim3(nblk(D.n)), dim3(kB), 0, st, D);
-    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits);
+    hipLaunchKernelGGL(k_seed_bits, dim3(nblk(D.n)), dim3(kB), 0, st, ccnt, D.n, seed_bits, D.ext_bits);
 }
 
 void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st) {

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -285,10 +286,16 @@ struct pdfc_sim {
     pdfc::DevSelfDetect sd{};
     unsigned long long* seed_bits = nullptr;
     int* vslot = nullptr;
-    Pinned<int> host_small;                        // [0..2] detection totals, [3] slot overflow
+    // What a detection sends back, in one block so that one copy brings it: detection totals
+    // (3 ints) and the slot-overflow flag, the seed bits (n / 64 + 1 words) at kXferBits, the
+    // filtered candidates at xfer_filt. The device block and its pinned mirror share the layout.
+    static constexpr size_t kXferBits = 64;
+    size_t xfer_filt = 0;
+    unsigned char* d_xfer = nullptr;
+    Pinned<unsigned char> host_xfer;
+    void xfer_alloc(int cap_filt);
+    const int* host_totals() const { return reinterpret_cast<const int*>(host_xfer.p); }
     std::vector<int> slot_of_scratch;              // per vertex, -1 between uploads
-    Pinned<unsigned long long> host_bits;          // seed bits, n / 64 + 1 words
-    Pinned<pdfc::SelfCandidate> host_filt;         // filtered candidates
     std::vector<pdfc::SelfPair> pairs;             // pairing order
     pdfc::SelfWorkspace self_ws;                   // pairing and level ordering arrays
     std::vector<int> level_order, level_ptr;       // pair indices level by level
@@ -322,7 +329,7 @@ struct pdfc_sim {
         if (it != allocs.end()) allocs.erase(it);
         (void)hipFree(p);
     }
-    void self_detect_and_pair(bool write_full);
+    void self_detect_and_pair(bool write_full, const std::function<void()>& after_download = {});
     void self_upload();
 
     template <typename T>
@@ -817,29 +824,26 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
         while (P < 2 * T) P <<= 1;
         sd.P = P;
         sd.bucket_cnt = dalloc<int>(P + 1);
-        sd.bucket_fill = dalloc<int>(P);
+        // zero between detections from here on (each detection's scatter counts them back down,
+        // its last kernel clears the extent)
+        pdfc::launch_fill_int(sd.bucket_cnt, P + 1, 0, stream);
+        pdfc::launch_fill_int(reinterpret_cast<int*>(sd.ext_bits), 2, 0, stream);
         sd.bucket_off = dalloc<int>(P + 1);
         sd.bucket_tris = dalloc<int>(T);
         sd.cnt_full = dalloc<int>(n);
         sd.cnt_filt = dalloc<int>(n);
         sd.off_full = dalloc<int>(n);
         sd.off_filt = dalloc<int>(n);
-        sd.totals = dalloc<int>(3);
         int nbl = (std::max(P + 1, n) + 255) / 256 + 1;
         sd.scan_sums = dalloc<int>(nbl);
         sd.scan_sums2 = dalloc<int>(nbl);
         sd.cap_full = 0;  // allocated on demand (pdfc_detect)
-        sd.cap_filt = std::max(1024, n);
-        sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
+        xfer_alloc(std::max(1024, n));
         sd.bk_key = dalloc<pdfc::BoxKey>(T);
         sd.bk_pos = dalloc<double>(9 * (size_t)T);
         sd.bk_vid = dalloc<int>(3 * (size_t)T);
         sd.bk_tri = dalloc<int>(T);
         sd.vslots = dalloc<pdfc::SelfCandidate>((size_t)n * pdfc::kSlotsPerVertex);
-        sd.overflow = dalloc<int>(1);
-        seed_bits = dalloc<unsigned long long>(n / 64 + 1);
-        host_bits.grow(n / 64 + 1);
-        host_small.grow(4);
         vslot = dalloc<int>(n);
         pdfc::launch_fill_int(vslot, n, -1, stream);
     }
@@ -1739,51 +1743,57 @@ void pdfc_sim::mesh_advance(double dt) {
     }
 }
 
+// The detection's transfer block for up to cap_filt filtered candidates; points the
+// detection's totals, overflow flag, seed bits and candidate list into it.
+void pdfc_sim::xfer_alloc(int cap_filt) {
+    if (d_xfer) free_ptr(d_xfer);
+    const size_t bit_bytes = ((size_t)n / 64 + 1) * sizeof(unsigned long long);
+    xfer_filt = (kXferBits + bit_bytes + 63) / 64 * 64;
+    sd.cap_filt = cap_filt;
+    d_xfer = dalloc<unsigned char>(xfer_filt + (size_t)cap_filt * sizeof(pdfc::SelfCandidate));
+    sd.totals = reinterpret_cast<int*>(d_xfer);
+    sd.overflow = sd.totals + 3;
+    seed_bits = reinterpret_cast<unsigned long long*>(d_xfer + kXferBits);
+    sd.filt = reinterpret_cast<pdfc::SelfCandidate*>(d_xfer + xfer_filt);
+}
+
 // Device detection, host pairing and level ordering, upload of the step's self
 // contacts (geometry_util.cpp:94-206, SimulationFrictionEstimation.cpp:107-293).
-void pdfc_sim::self_detect_and_pair(bool write_full) {
+// after_download runs once the detection results are on the host, before the pairing: work
+// enqueued there keeps the device busy while the host pairs.
+void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>& after_download) {
     auto t_host0 = std::chrono::steady_clock::now();
-    // One synchronisation in the common case: counts, seed bits and a guessed prefix of
-    // the filtered list come back together; a second copy only if the guess was short.
+    // One synchronisation in the common case: totals, seed bits and a guessed prefix of the
+    // filtered list come back in one copy; a second copy only if the guess was short.
     size_t guess = std::min<size_t>((size_t)sd.cap_filt, (size_t)filt_guess);
-    host_filt.grow(guess);
-    int* const host_totals = host_small.p;
-    const size_t bit_words = (size_t)n / 64 + 1;
+    constexpr size_t cand = sizeof(pdfc::SelfCandidate);
     bool single_pass = !write_full;
     for (int attempt = 0; attempt < 4; ++attempt) {
+        host_xfer.grow(xfer_filt + guess * cand);
         sd.q = qn;
         sd.write_full = write_full ? 1 : 0;
         if (single_pass)
             pdfc::launch_self_detect_filtered(sd, ccnt, seed_bits, stream);
         else
             pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
-        HIP_CHECK(hipMemcpyAsync(host_totals, sd.totals, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        host_totals[3] = 0;
-        if (single_pass)
-            HIP_CHECK(hipMemcpyAsync(host_totals + 3, sd.overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipMemcpyAsync(host_bits.p, seed_bits, bit_words * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 stream));
-        if (guess > 0)
-            HIP_CHECK(hipMemcpyAsync(host_filt.p, sd.filt, guess * sizeof(pdfc::SelfCandidate), hipMemcpyDeviceToHost,
-                                     stream));
+        HIP_CHECK(hipMemcpyAsync(host_xfer.p, d_xfer, xfer_filt + guess * cand, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
-        if (single_pass && host_totals[3]) {  // a vertex has more candidates than slots: two-pass path
+        const int* tot = host_totals();
+        if (single_pass && tot[3]) {  // a vertex has more candidates than slots: two-pass path
             single_pass = false;
             guess = 0;
             continue;
         }
         bool grow = false;
-        if (host_totals[1] > sd.cap_filt) {
-            free_ptr(sd.filt);
-            sd.cap_filt = host_totals[1] + host_totals[1] / 2 + 1024;
-            sd.filt = dalloc<pdfc::SelfCandidate>(sd.cap_filt);
+        if (tot[1] > sd.cap_filt) {
+            xfer_alloc(tot[1] + tot[1] / 2 + 1024);
             grow = true;
         }
-        if (write_full && host_totals[0] > sd.cap_full) {
+        if (write_full && tot[0] > sd.cap_full) {
             free_ptr(sd.full_v);
             free_ptr(sd.full_t);
             free_ptr(sd.full_bary);
-            sd.cap_full = host_totals[0] + host_totals[0] / 4 + 1024;
+            sd.cap_full = tot[0] + tot[0] / 4 + 1024;
             sd.full_v = dalloc<int>(sd.cap_full);
             sd.full_t = dalloc<int>(sd.cap_full);
             sd.full_bary = dalloc<double>(3 * (size_t)sd.cap_full);
@@ -1793,20 +1803,23 @@ void pdfc_sim::self_detect_and_pair(bool write_full) {
         if (attempt == 3) throw DeviceError("self-collision candidate buffers could not be sized");
         guess = 0;
     }
+    const int* host_tot = host_totals();
     if (!write_full) n_full_candidates = -1;
-    else n_full_candidates = host_totals[0];
-    const size_t nf = (size_t)host_totals[1];
+    else n_full_candidates = host_tot[0];
+    const size_t nf = (size_t)host_tot[1];
     if (nf > guess) {
-        host_filt.grow(nf, guess);
-        HIP_CHECK(hipMemcpyAsync(host_filt.p + guess, sd.filt + guess, (nf - guess) * sizeof(pdfc::SelfCandidate),
-                                 hipMemcpyDeviceToHost, stream));
+        host_xfer.grow(xfer_filt + nf * cand, xfer_filt + guess * cand);
+        HIP_CHECK(hipMemcpyAsync(host_xfer.p + xfer_filt + guess * cand, d_xfer + xfer_filt + guess * cand,
+                                 (nf - guess) * cand, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
     filt_guess = nf + nf / 2 + 256;
+    if (after_download) after_download();
     const auto t_pair0 = std::chrono::steady_clock::now();
-    pdfc::pair_self_contacts(self_ws, model, host_filt.p, nf, pairs);
+    pdfc::pair_self_contacts(self_ws, model, reinterpret_cast<const pdfc::SelfCandidate*>(host_xfer.p + xfer_filt), nf,
+                             pairs);
     const auto t_order0 = std::chrono::steady_clock::now();
-    const unsigned long long* bits = host_bits.p;
+    const unsigned long long* bits = reinterpret_cast<const unsigned long long*>(host_xfer.p + kXferBits);
     n_self_levels = pdfc::order_self_contacts(
         self_ws,
         [&](int v) {
@@ -2057,7 +2070,11 @@ void pdfc_sim::step_prologue(StepPhase& ph) {
     if (mode == PDFC_WARM_LEARNED && !mlp)
         throw std::invalid_argument("learned warm start without a network (pdfc_set_mlp)");
     step_event(0);
-    pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream);  // also clears the residual records
+    // k_step_begin (which also clears the residual records) reads q_n and the state velocity and
+    // writes v^0 and the step's constant, which no detection kernel reads: with self collision it
+    // runs while the host pairs the candidates
+    auto step_begin = [&] { pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream); };
+    if (!self_on) step_begin();
     detect();
     const bool need_nc = adaptive || features;
     // the count comes back with the self-collision round trip's synchronisation at no extra cost
@@ -2065,7 +2082,7 @@ void pdfc_sim::step_prologue(StepPhase& ph) {
     if (copy_nc) HIP_CHECK(hipMemcpyAsync(host_nc, ncontacts, sizeof(int), hipMemcpyDeviceToHost, stream));
     nsc = 0;
     if (self_on) {
-        self_detect_and_pair(keep_full_candidates);  // synchronises the stream
+        self_detect_and_pair(keep_full_candidates, step_begin);  // synchronises the stream
         if (nsc > 0) pdfc::launch_self_prepare(ds, vslot, stream);
         stats.self_contacts_sum += nsc;
         stats.self_levels_max = std::max<int64_t>(stats.self_levels_max, n_self_levels);
@@ -2720,7 +2737,7 @@ int pdfc_get_info(pdfc_sim* sim, pdfc_info* out) {
         I.n_contacts = nc;
         I.n_self_contacts = sim->nsc;
         I.n_self_candidates = sim->n_full_candidates >= 0 ? sim->n_full_candidates
-                                                          : (sim->host_small.p ? sim->host_small.p[1] : 0);
+                                                          : (sim->host_xfer.p ? sim->host_totals()[1] : 0);
         I.n_self_levels = sim->n_self_levels;
         I.max_front = sim->fac.max_front;
         I.factor_seconds = sim->fac.seconds;
