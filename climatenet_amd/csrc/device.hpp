@@ -327,6 +327,7 @@ constexpr int kSelfFusedMaxContacts = 4096;
 // involved vertex and per contact (144 KiB at the limits).
 constexpr int kSelfLdsMaxContacts = kSelfFusedThreads;
 constexpr int kSelfLdsMaxSlots = 2 * kSelfLdsMaxContacts;
+constexpr int kSelfLdsMaxLevels = 255;  // the levels' contact and group offsets are held in LDS too
 void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                            int iteration, double* rhs, hipStream_t st);
 void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,

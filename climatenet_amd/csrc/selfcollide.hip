@@ -643,6 +643,10 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     __shared__ unsigned short g_slot[2 * kSelfLdsMaxContacts];
     __shared__ unsigned short g_ent[2 * kSelfLdsMaxContacts + 1];
     __shared__ unsigned short e_code[2 * kSelfLdsMaxContacts];  // contact * 2 + (sign > 0)
+    // each level's first contact and first group: read from LDS in the level loop (a global load
+    // there puts a memory round trip into every level)
+    __shared__ unsigned short l_ptr[kSelfLdsMaxLevels + 1];
+    __shared__ unsigned short l_grp[kSelfLdsMaxLevels + 1];
     double* scf_c = sh;                                  // 3 per slot
     double* scf_n = sh + 3 * kSelfLdsMaxSlots;           // 3 per slot
     double* c_rem = sh + 6 * kSelfLdsMaxSlots;           // 3 per contact
@@ -677,6 +681,12 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         const int sl = t + q * kSelfFusedThreads;
         vv[q] = sl < S.nslots ? S.slot_vertex[sl] : 0;
     }
+    static_assert(kSelfLdsMaxLevels < kSelfFusedThreads, "one level offset per thread");
+    int lp = 0, gp = 0;
+    if (t <= nlev) {
+        lp = S.lvl_ptr[t];
+        gp = S.grp_ptr[t];
+    }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int k = t + q * kSelfFusedThreads;
@@ -692,18 +702,20 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         if (k <= G) g_ent[k] = (unsigned short)ge[q];
         if (k < E) e_code[k] = (unsigned short)(2 * ec[q] + (es[q] > 0 ? 1 : 0));
     }
+    if (t <= nlev) {
+        l_ptr[t] = (unsigned short)lp;
+        l_grp[t] = (unsigned short)gp;
+    }
     // Every per-contact and per-slot input is indexed by the thread (one round of loads): the
     // masses of the contact's vertices, F + CF at both (stored by contact by the local kernel,
     // the value fA / fB start from), and the first slot's right-hand side and obstacle reaction
     // for the final update (which then waits for no load; they do not change during the levels).
     const int s = t;
     const bool mine = s < S.nsc;
-    int lev = -1, sa = 0, sb = 0, pa = 0, pb = 0;
+    int sa = 0, sb = 0, pa = 0, pb = 0;
     double mA = 1., mB = 1., mu = 0.;
     P3 bn{}, bt{}, bb{}, fa{}, fb{}, rem{}, rprev{};
     if (mine) {
-        for (int l = 0; l < nlev; ++l)
-            if (s >= S.lvl_ptr[l] && s < S.lvl_ptr[l + 1]) lev = l;
         const double* B = &S.basis[9 * s];
         bn = P3{B[0], B[1], B[2]};
         bt = P3{B[3], B[4], B[5]};
@@ -734,7 +746,7 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     if (res) u = psub(at_perm(v, n, pa), at_perm(v, n, pb));
     __syncthreads();
     for (int l = 0; l < nlev; ++l) {
-        if (lev == l) {
+        if (mine && s >= l_ptr[l] && s < l_ptr[l + 1]) {
             const P3 ca = {scf_c[3 * sa], scf_c[3 * sa + 1], scf_c[3 * sa + 2]};
             const P3 na = {scf_n[3 * sa], scf_n[3 * sa + 1], scf_n[3 * sa + 2]};
             const P3 cb = {scf_c[3 * sb], scf_c[3 * sb + 1], scf_c[3 * sb + 2]};
@@ -747,8 +759,8 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
             c_add[3 * s + 2] = add.z;
         }
         __syncthreads();
-        const int g1 = S.grp_ptr[l + 1];
-        for (int g = S.grp_ptr[l] + t; g < g1; g += kSelfFusedThreads) {
+        const int g1 = l_grp[l + 1];
+        for (int g = l_grp[l] + t; g < g1; g += kSelfFusedThreads) {
             const int slot = g_slot[g];
             P3 cur = {scf_c[3 * slot], scf_c[3 * slot + 1], scf_c[3 * slot + 2]};
             P3 nxt = {scf_n[3 * slot], scf_n[3 * slot + 1], scf_n[3 * slot + 2]};

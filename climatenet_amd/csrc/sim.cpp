@@ -2023,7 +2023,8 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
         timed(4, [&] {
-            if (self_path >= 2 && nsc <= pdfc::kSelfLdsMaxContacts && ds.nslots <= pdfc::kSelfLdsMaxSlots) {
+            if (self_path >= 2 && nsc <= pdfc::kSelfLdsMaxContacts && ds.nslots <= pdfc::kSelfLdsMaxSlots &&
+                n_self_levels <= pdfc::kSelfLdsMaxLevels) {
                 pdfc::launch_self_fused_lds(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
             } else if (self_path >= 1 && nsc <= pdfc::kSelfFusedMaxContacts) {
                 pdfc::launch_self_fused(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
