@@ -276,6 +276,7 @@ constexpr int kSlotsPerVertex = 8;
 // Self contacts of the current step, sorted by computation level.
 struct DevSelf {
     int nsc, nslots;
+    int ngroups;            // vertex groups over all levels (grp_ptr[levels])
     const int* a;           // contact vertex (supernodal index)
     const int* b;           // paired face vertex
     const int* sa;          // slot of a in the involved-vertex buffers

@@ -657,13 +657,15 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     // at most 6 force components per buffer, 2 groups and 2 entries per thread.
     static_assert(3 * kSelfLdsMaxSlots <= 6 * kSelfFusedThreads && 2 * kSelfLdsMaxContacts <= 2 * kSelfFusedThreads,
                   "LDS-resident data per thread");
-    const int n3s = 3 * S.nslots, G = S.grp_ptr[nlev], E = 2 * S.nsc;
-    double fc[6], fn[6];
+    // The "next" self-contact forces start at zero: k_self_prepare clears both buffers, and each
+    // launch of this kernel clears the one (SCFc) that the swap after it makes the next one. The
+    // group count comes with the arguments, so no load waits for it.
+    const int n3s = 3 * S.nslots, G = S.ngroups, E = 2 * S.nsc;
+    double fc[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int k = t + q * kSelfFusedThreads;
         fc[q] = k < n3s ? S.SCFc[k] : 0.;
-        fn[q] = k < n3s ? S.SCFn[k] : 0.;
     }
     int gs[2], ge[2], ec[2], es[2];
 #pragma unroll
@@ -692,7 +694,7 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
         const int k = t + q * kSelfFusedThreads;
         if (k < n3s) {
             scf_c[k] = fc[q];
-            scf_n[k] = fn[q];
+            scf_n[k] = 0.;
         }
     }
 #pragma unroll

@@ -1976,6 +1976,7 @@ void pdfc_sim::self_upload() {
     pdfc::DevSelf S{};
     S.nsc = nsc;
     S.nslots = nslots;
+    S.ngroups = G;
     S.a = d_self_int + oA;
     S.b = d_self_int + oB;
     S.sa = d_self_int + oSA;
