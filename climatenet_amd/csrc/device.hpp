@@ -281,7 +281,8 @@ struct DevSelf {
     const int* b;           // paired face vertex
     const int* sa;          // slot of a in the involved-vertex buffers
     const int* sb;
-    const double* basis;    // 9 per contact: n, t, b
+    const double* normal;   // 3 per contact, uploaded
+    double* basis;          // 9 per contact: n, t, b, built from the normal by k_self_prepare
     const double* mu;
     double* r;              // local reaction (mass-weighted)
     int* active;

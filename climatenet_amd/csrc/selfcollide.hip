@@ -845,6 +845,28 @@ __global__ void k_self_ac_final(DevSelf S, const double* __restrict__ v, int n, 
 }
 
 // Start of step: slot map on, buffers zeroed. End of step: slot map off.
+// computeBasis (SimulationFrictionEstimation.cpp:89-105): n, t = n x e_x (n x e_y when that is
+// shorter than 1e-15), b = n x t, t and b normalised; the operations of the host's and the
+// oracle's, in their order (IEEE square root and division).
+__device__ void self_contact_basis(P3 n, double* B) {
+    auto cross = [](P3 a, P3 b) { return P3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+    auto nrm = [](P3 a) {
+        const double z = (a.x * a.x + a.y * a.y) + a.z * a.z;
+        if (z > 0.) {
+            const double q = sqrt(z);
+            return P3{a.x / q, a.y / q, a.z / q};
+        }
+        return a;
+    };
+    P3 t = cross(n, P3{1., 0., 0.});
+    if (sqrt((t.x * t.x + t.y * t.y) + t.z * t.z) < 1.e-15) t = cross(n, P3{0., 1., 0.});
+    t = nrm(t);
+    const P3 b = nrm(cross(n, t));
+    B[0] = n.x; B[1] = n.y; B[2] = n.z;
+    B[3] = t.x; B[4] = t.y; B[5] = t.z;
+    B[6] = b.x; B[7] = b.y; B[8] = b.z;
+}
+
 __global__ void k_self_prepare(DevSelf S, int* vslot) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S.nslots) {
@@ -856,6 +878,7 @@ __global__ void k_self_prepare(DevSelf S, int* vslot) {
         st_slot(S.Fc, s, P3{0., 0., 0.});
     }
     if (s < S.nsc) {
+        self_contact_basis(ld_slot(S.normal, s), &S.basis[9 * s]);
         st_slot(S.rem, s, P3{0., 0., 0.});
         st_slot(S.addv, s, P3{0., 0., 0.});
         S.r[3 * s] = 0.; S.r[3 * s + 1] = 0.; S.r[3 * s + 2] = 0.;

@@ -1845,7 +1845,7 @@ void pdfc_sim::self_upload() {
     const int L = n_self_levels;
     const size_t max_int = 18 * (size_t)nsc + 2 * (size_t)L + 4;
     pin_int.grow(max_int);
-    pin_dbl.grow(12 * (size_t)nsc);
+    pin_dbl.grow(6 * (size_t)nsc);
     int* const P = pin_int.p;
     int *A = P, *B = P + nsc, *SA = P + 2 * nsc, *SB = P + 3 * nsc;
     int* lvl_ptr = P + 4 * nsc;
@@ -1853,9 +1853,9 @@ void pdfc_sim::self_upload() {
     int* ent_contact = grp_ptr + (L + 1);
     int* ent_sign = ent_contact + 2 * nsc;
     int* slot_vertex = ent_sign + 2 * nsc;
-    double* basis = pin_dbl.p;
-    double* mu = pin_dbl.p + 9 * (size_t)nsc;
-    double* mass2 = pin_dbl.p + 10 * (size_t)nsc;
+    double* normal = pin_dbl.p;  // the contact bases are built from these on the device
+    double* mu = pin_dbl.p + 3 * (size_t)nsc;
+    double* mass2 = pin_dbl.p + 4 * (size_t)nsc;
     // involved vertices: those of the self contacts, in order of first appearance (the local
     // kernel keeps the last obstacle reaction of every other contact vertex itself)
     std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
@@ -1872,8 +1872,9 @@ void pdfc_sim::self_upload() {
             }
         SA[j] = slot_of[A[j]];
         SB[j] = slot_of[B[j]];
-        const double nr[3] = {p.normal.x, p.normal.y, p.normal.z};
-        pdfc::host_contact_basis(nr, &basis[9 * (size_t)j]);
+        normal[3 * (size_t)j] = p.normal.x;
+        normal[3 * (size_t)j + 1] = p.normal.y;
+        normal[3 * (size_t)j + 2] = p.normal.z;
         mu[j] = self_mu;
         mass2[2 * (size_t)j] = model.mass[p.v];
         mass2[2 * (size_t)j + 1] = model.mass[p.partner];
@@ -1947,7 +1948,7 @@ void pdfc_sim::self_upload() {
         }
     }
     const size_t n_int = (size_t)(slot_ent - P) + 2 * (size_t)nsc;
-    const size_t n_dbl = 12 * (size_t)nsc;
+    const size_t n_dbl = 6 * (size_t)nsc;
     if (n_int > cap_self_int) {
         free_ptr(d_self_int);
         cap_self_int = n_int * 2;
@@ -1963,7 +1964,7 @@ void pdfc_sim::self_upload() {
         free_ptr(d_self_active);
         cap_self_contacts = std::max(cap_self_contacts, (size_t)nsc * 2);
         cap_self_slots = std::max(cap_self_slots, (size_t)nslots * 2);
-        d_self_work = dalloc<double>(15 * cap_self_contacts + 18 * cap_self_slots);
+        d_self_work = dalloc<double>(24 * cap_self_contacts + 18 * cap_self_slots);
         d_self_active = dalloc<int>(cap_self_contacts);
     }
     // staged in pinned memory: the copies run asynchronously; the staging is next written by
@@ -1990,15 +1991,16 @@ void pdfc_sim::self_upload() {
     S.slot_vertex = d_self_int + oSV;
     S.slot_ent_ptr = d_self_int + oSEP;
     S.slot_ent = d_self_int + oSEN;
-    S.basis = d_self_dbl;
-    S.mu = d_self_dbl + 9 * (size_t)nsc;
-    S.mass2 = d_self_dbl + 10 * (size_t)nsc;
+    S.normal = d_self_dbl;
+    S.mu = d_self_dbl + 3 * (size_t)nsc;
+    S.mass2 = d_self_dbl + 4 * (size_t)nsc;
     double* w = d_self_work;
     S.r = w;
     S.rem = w + 3 * cap_self_contacts;
     S.addv = w + 6 * cap_self_contacts;
     S.x2 = w + 9 * cap_self_contacts;
-    double* ws = w + 15 * cap_self_contacts;
+    S.basis = w + 15 * cap_self_contacts;
+    double* ws = w + 24 * cap_self_contacts;
     S.Fc = ws;
     S.CFc = ws + 3 * cap_self_slots;
     S.CFn = ws + 6 * cap_self_slots;
