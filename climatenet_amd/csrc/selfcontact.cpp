@@ -188,24 +188,28 @@ int order_self_contacts(SelfWorkspace& W, const std::function<bool(int)>& is_see
     std::vector<int>& lev_of = W.adj_fill;  // reused: level of each contact
     lev_of.assign(ns, -1);
     int nlev = 0;
-    auto bfs = [&](size_t nstart) {  // frontier already holds the starting vertices
-        int lvl = 0;
-        while (nstart > 0) {
-            if (nlev <= lvl) nlev = lvl + 1;
-            W.next.clear();
-            for (size_t f = 0; f < nstart; ++f) {
-                const int u = W.frontier[f];
-                for (int q = W.adj_ptr[u]; q < W.adj_ptr[u + 1]; ++q) {
-                    const int s = W.adj_s[q];
-                    if (W.visited[s]) continue;
-                    W.visited[s] = 1;
-                    W.next.push_back(W.adj_v[q]);
-                    lev_of[s] = lvl;
-                }
+    // One queue for every search: the frontier of a level is the queue's range between two level
+    // ends, and its vertices are expanded in the order they were discovered (as a frontier list
+    // rebuilt per level would hold them).
+    W.frontier.resize((size_t)nv + ns);
+    int* const queue = W.frontier.data();
+    auto bfs = [&](int nstart) {  // queue[0, nstart) holds the starting vertices
+        int head = 0, tail = nstart, level_end = nstart, lvl = 0;
+        if (nstart > 0 && nlev < 1) nlev = 1;
+        while (head < tail) {
+            const int u = queue[head++];
+            for (int q = W.adj_ptr[u]; q < W.adj_ptr[u + 1]; ++q) {
+                const int s = W.adj_s[q];
+                if (W.visited[s]) continue;
+                W.visited[s] = 1;
+                queue[tail++] = W.adj_v[q];
+                lev_of[s] = lvl;
             }
-            W.frontier.swap(W.next);
-            nstart = W.frontier.size();
-            ++lvl;
+            if (head == level_end && head < tail) {
+                ++lvl;
+                level_end = tail;
+                if (nlev <= lvl) nlev = lvl + 1;
+            }
         }
     };
     auto all_visited = [&](int u) {
@@ -215,15 +219,15 @@ int order_self_contacts(SelfWorkspace& W, const std::function<bool(int)>& is_see
     };
     // the first search starts from every obstacle-contact vertex at once; those outside the
     // graph add nothing, and the order of a frontier changes only the order inside a level
-    W.frontier.clear();
+    int nseed = 0;
     for (int u = 0; u < nv; ++u)
-        if (is_seed(W.verts[u])) W.frontier.push_back(u);
-    bfs(W.frontier.size());
+        if (is_seed(W.verts[u])) queue[nseed++] = u;
+    bfs(nseed);
     for (int pass = 0; pass < 2; ++pass)
         for (int u = 0; u < nv; ++u) {
             const int deg = W.adj_ptr[u + 1] - W.adj_ptr[u];
             if ((pass == 0 ? deg == 1 : deg > 1) && !all_visited(u)) {
-                W.frontier.assign(1, u);
+                queue[0] = u;
                 bfs(1);
             }
         }
