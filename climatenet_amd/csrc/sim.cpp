@@ -329,7 +329,8 @@ struct pdfc_sim {
         if (it != allocs.end()) allocs.erase(it);
         (void)hipFree(p);
     }
-    void self_detect_and_pair(bool write_full, const std::function<void()>& after_download = {});
+    void self_detect_and_pair(bool write_full, const std::function<void()>& behind_copy = {});
+    hipEvent_t ev_xfer = nullptr;  // the detection's copy back has landed
     void self_upload();
 
     template <typename T>
@@ -354,6 +355,7 @@ struct pdfc_sim {
         if (host_nonfinite) (void)hipHostFree(host_nonfinite);
         for (auto& e : ev_ac)
             if (e) (void)hipEventDestroy(e);
+        if (ev_xfer) (void)hipEventDestroy(ev_xfer);
         for (auto& e : step_ev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         for (auto& e : ev)
@@ -1759,9 +1761,10 @@ void pdfc_sim::xfer_alloc(int cap_filt) {
 
 // Device detection, host pairing and level ordering, upload of the step's self
 // contacts (geometry_util.cpp:94-206, SimulationFrictionEstimation.cpp:107-293).
-// after_download runs once the detection results are on the host, before the pairing: work
-// enqueued there keeps the device busy while the host pairs.
-void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>& after_download) {
+// behind_copy enqueues, once, work that runs on the device behind the copy of the detection's
+// results: the host waits for the copy alone (an event), so that work keeps the device busy while
+// the host wakes up and pairs.
+void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>& behind_copy) {
     auto t_host0 = std::chrono::steady_clock::now();
     // One synchronisation in the common case: totals, seed bits and a guessed prefix of the
     // filtered list come back in one copy; a second copy only if the guess was short.
@@ -1777,7 +1780,14 @@ void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>
         else
             pdfc::launch_self_detect(sd, ccnt, seed_bits, stream);
         HIP_CHECK(hipMemcpyAsync(host_xfer.p, d_xfer, xfer_filt + guess * cand, hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
+        if (attempt == 0 && behind_copy) {
+            if (!ev_xfer) HIP_CHECK(hipEventCreateWithFlags(&ev_xfer, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(ev_xfer, stream));
+            behind_copy();
+            HIP_CHECK(hipEventSynchronize(ev_xfer));
+        } else {
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
         const int* tot = host_totals();
         if (single_pass && tot[3]) {  // a vertex has more candidates than slots: two-pass path
             single_pass = false;
@@ -1814,7 +1824,6 @@ void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>
         HIP_CHECK(hipStreamSynchronize(stream));
     }
     filt_guess = nf + nf / 2 + 256;
-    if (after_download) after_download();
     const auto t_pair0 = std::chrono::steady_clock::now();
     pdfc::pair_self_contacts(self_ws, model, reinterpret_cast<const pdfc::SelfCandidate*>(host_xfer.p + xfer_filt), nf,
                              pairs);
@@ -2075,8 +2084,9 @@ void pdfc_sim::step_prologue(StepPhase& ph) {
         throw std::invalid_argument("learned warm start without a network (pdfc_set_mlp)");
     step_event(0);
     // k_step_begin (which also clears the residual records) reads q_n and the state velocity and
-    // writes v^0 and the step's constant, which no detection kernel reads: with self collision it
-    // runs while the host pairs the candidates
+    // writes v^0 and the step's constant, which no detection kernel reads or writes: with self
+    // collision it is enqueued behind the copy of the detection's results and runs while the host
+    // pairs the candidates (a detection run again after it leaves its results as they are)
     auto step_begin = [&] { pdfc::launch_step_begin(dm, qn, vstate, V[0], cvec, a, stream); };
     if (!self_on) step_begin();
     detect();
