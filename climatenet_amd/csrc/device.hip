@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(kBlock) k_local(DevModel M, const double* __re
                                                   double* __restrict__ rhs, const int* __restrict__ ccnt,
                                                   const int* __restrict__ cstart, DevContacts C, StepArgs a,
                                                   int iteration, LocalSelf ls) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int n = M.n;
     double ac_s = 0., ac_m = 0.;
     if (i < n) {
@@ -1015,7 +1015,7 @@ __global__ void __launch_bounds__(64 * W) k_forward(DevFactor Fa, int t0, const 
                                                     double* __restrict__ Y) {
     extern __shared__ double lds[];
     __shared__ double part[W > 1 ? W : 1][3][64];
-    forward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Bv, Y, lds, part);
+    forward_task<T, W>(Fa, Fa.desc[t0 + xcd_block(blockIdx.x, gridDim.x)], Bv, Y, lds, part);
 }
 
 // Backward substitution, one task = (supernode s, 64 columns), W waves:
@@ -1239,7 +1239,9 @@ __global__ void __launch_bounds__(64 * W) k_backward(DevFactor Fa, int t0, const
         }
         return;
     }
-    backward_task<T, W>(Fa, Fa.desc[t0 + blockIdx.x], Y, X, lds, part, fo);
+    // the task workgroups come first; the extra ones above (fold, split finish) keep their order
+    const int ntask = fo.part ? fo.ntasks : (fin.count > 0 ? fin.first : (int)gridDim.x);
+    backward_task<T, W>(Fa, Fa.desc[t0 + xcd_block(blockIdx.x, ntask)], Y, X, lds, part, fo);
 }
 
 // ---------------------------------------------------------------- merged levels

@@ -129,6 +129,15 @@ struct TaskDesc {
     int64_t rng;
 };
 
+// Logical block of physical workgroup b of nb. The dispatcher hands workgroups to the 8 XCDs in
+// turn (workgroup b to XCD b % 8), and each XCD has its own L2; this gives each XCD a contiguous
+// range of logical blocks (b / 8-th of those of XCD b % 8), so that blocks whose gathers touch the
+// same lines (neighbouring vertices, sibling supernodes) share one L2. A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b & 7, k = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + k;
+}
+
 // Kernel-duration timing. While a launch timer is current on the thread and has a family set,
 // the launches made through launch_k record their own start and end (the events that
 // hipExtLaunchKernelGGL attaches to the dispatch itself, so no separate markers are timed) under

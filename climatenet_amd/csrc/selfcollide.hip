@@ -299,7 +299,7 @@ constexpr int kQueryCells = 8;
 __global__ void k_vertex_query_flat(DevSelfDetect D) {
     __shared__ int slot_count[kB / kFlatLanes];
     const int local_v = threadIdx.x / kFlatLanes, sub = threadIdx.x % kFlatLanes;
-    const int v = blockIdx.x * (kB / kFlatLanes) + local_v;
+    const int v = xcd_block(blockIdx.x, gridDim.x) * (kB / kFlatLanes) + local_v;
     if (sub == 0) slot_count[local_v] = 0;
     __syncthreads();
     if (v < D.n) {
