@@ -138,7 +138,7 @@ __device__ __forceinline__ D3 ata_product(const DevModel& M, const double* __res
 // SimulationSpeed::computeRhs, :263-277, hoisted out of the iteration loop).
 __global__ void k_step_begin(DevModel M, const double* __restrict__ qn, const double* __restrict__ vstate,
                              double* __restrict__ v0, double* __restrict__ cvec, StepArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // the A^T A gathers share an L2
     const int n = M.n;
     if (a.record)  // the step's residual records start from zero
         for (int k = i; k < a.ac_cap; k += gridDim.x * blockDim.x) {
