@@ -318,7 +318,6 @@ struct DevSelf {
 
 void launch_fill(double* p, size_t count, double value, hipStream_t st);
 void launch_fill_int(int* p, int count, int value, hipStream_t st);
-void launch_copy_int(int* dst, const int* src, int count, hipStream_t st);
 void launch_scan_int(int* dst, const int* src, int count, int* sums, int* total, int* tail, hipStream_t st);
 void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits, hipStream_t st);
 // Single pass: filtered candidates only (the hot path); falls back to launch_self_detect

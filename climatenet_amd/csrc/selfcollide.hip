@@ -77,10 +77,6 @@ __global__ void k_fill_int(int* p, int count, int value) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = value;
 }
-__global__ void k_copy_int(int* dst, const int* src, int count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) dst[i] = src[i];
-}
 
 // Triangle bounding boxes and the largest box extent (for the cell size). Also clears the
 // slot-overflow flag of the query that follows (the largest extent starts from zero: the
@@ -1073,10 +1069,6 @@ void launch_mesh_build(const DevMeshBuild& B, hipStream_t st) {
 
 void launch_fill_int(int* p, int count, int value, hipStream_t st) {
     if (count > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(count)), dim3(kB), 0, st, p, count, value);
-}
-
-void launch_copy_int(int* dst, const int* src, int count, hipStream_t st) {
-    if (count > 0) hipLaunchKernelGGL(k_copy_int, dim3(nblk(count)), dim3(kB), 0, st, dst, src, count);
 }
 
 void launch_scan_int(int* dst, const int* src, int count, int* sums, int* total, int* tail, hipStream_t st) {
