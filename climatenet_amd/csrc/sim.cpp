@@ -300,11 +300,9 @@ struct pdfc_sim {
     pdfc::SelfWorkspace self_ws;                   // pairing and level ordering arrays
     std::vector<int> level_order, level_ptr;       // pair indices level by level
     std::vector<int> up_gidx, up_gcount, up_gfill, up_gstart;  // upload packing, reused
-    Pinned<int> pin_int;                           // and staged for the copy (read by the copy until
-    Pinned<double> pin_dbl;                        // the next step's detection synchronises)
-    int* d_self_int = nullptr;
-    double* d_self_dbl = nullptr;
-    size_t cap_self_int = 0, cap_self_dbl = 0;
+    Pinned<unsigned char> pin_up;                  // and staged for the one copy (read by the copy until
+    unsigned char* d_self_up = nullptr;            // the next step's detection has synchronised)
+    size_t cap_self_up = 0;
     double* d_self_work = nullptr;                 // r, rem, addv (per contact) + Fc, CF x2, SCF x2 (per slot)
     int* d_self_active = nullptr;
     size_t cap_self_contacts = 0, cap_self_slots = 0;
@@ -1847,24 +1845,26 @@ void pdfc_sim::self_detect_and_pair(bool write_full, const std::function<void()>
 }
 
 // Packs the self contacts sorted by level, the involved-vertex slots and the per-level apply
-// groups straight into the pinned staging buffers, and uploads them in one copy per type.
-// Integer layout: a, b, sa, sb (nsc each), lvl_ptr, grp_ptr (L + 1 each), ent_contact,
-// ent_sign (2 nsc each), slot_vertex (nslots), grp_slot (G), ent_ptr (G + 1).
+// groups straight into one pinned staging buffer, and uploads them in one copy: the doubles
+// (normal (3), mu, the two masses per contact), then the integers: a, b, sa, sb (nsc each),
+// lvl_ptr, grp_ptr (L + 1 each), ent_contact, ent_sign (2 nsc each), slot_vertex (nslots),
+// grp_slot (G), ent_ptr (G + 1), slot_ent_ptr (nslots + 1), slot_ent (2 nsc).
 void pdfc_sim::self_upload() {
     const int L = n_self_levels;
     const size_t max_int = 18 * (size_t)nsc + 2 * (size_t)L + 4;
-    pin_int.grow(max_int);
-    pin_dbl.grow(6 * (size_t)nsc);
-    int* const P = pin_int.p;
+    const size_t dbl_bytes = 6 * (size_t)nsc * sizeof(double);
+    pin_up.grow(dbl_bytes + max_int * sizeof(int));
+    double* const DB = reinterpret_cast<double*>(pin_up.p);
+    int* const P = reinterpret_cast<int*>(pin_up.p + dbl_bytes);
     int *A = P, *B = P + nsc, *SA = P + 2 * nsc, *SB = P + 3 * nsc;
     int* lvl_ptr = P + 4 * nsc;
     int* grp_ptr = lvl_ptr + (L + 1);
     int* ent_contact = grp_ptr + (L + 1);
     int* ent_sign = ent_contact + 2 * nsc;
     int* slot_vertex = ent_sign + 2 * nsc;
-    double* normal = pin_dbl.p;  // the contact bases are built from these on the device
-    double* mu = pin_dbl.p + 3 * (size_t)nsc;
-    double* mass2 = pin_dbl.p + 4 * (size_t)nsc;
+    double* normal = DB;  // the contact bases are built from these on the device
+    double* mu = DB + 3 * (size_t)nsc;
+    double* mass2 = DB + 4 * (size_t)nsc;
     // involved vertices: those of the self contacts, in order of first appearance (the local
     // kernel keeps the last obstacle reaction of every other contact vertex itself)
     std::vector<int>& slot_of = slot_of_scratch;  // -1 everywhere between calls
@@ -1957,16 +1957,11 @@ void pdfc_sim::self_upload() {
         }
     }
     const size_t n_int = (size_t)(slot_ent - P) + 2 * (size_t)nsc;
-    const size_t n_dbl = 6 * (size_t)nsc;
-    if (n_int > cap_self_int) {
-        free_ptr(d_self_int);
-        cap_self_int = n_int * 2;
-        d_self_int = dalloc<int>(cap_self_int);
-    }
-    if (n_dbl > cap_self_dbl) {
-        free_ptr(d_self_dbl);
-        cap_self_dbl = n_dbl * 2;
-        d_self_dbl = dalloc<double>(cap_self_dbl);
+    const size_t n_bytes = dbl_bytes + n_int * sizeof(int);
+    if (n_bytes > cap_self_up) {
+        free_ptr(d_self_up);
+        cap_self_up = n_bytes * 2;
+        d_self_up = dalloc<unsigned char>(cap_self_up);
     }
     if ((size_t)nsc > cap_self_contacts || (size_t)nslots > cap_self_slots) {
         free_ptr(d_self_work);
@@ -1976,10 +1971,11 @@ void pdfc_sim::self_upload() {
         d_self_work = dalloc<double>(24 * cap_self_contacts + 18 * cap_self_slots);
         d_self_active = dalloc<int>(cap_self_contacts);
     }
-    // staged in pinned memory: the copies run asynchronously; the staging is next written by
-    // the next step's upload, after that step's detection has synchronised the stream
-    HIP_CHECK(hipMemcpyAsync(d_self_int, pin_int.p, n_int * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_self_dbl, pin_dbl.p, n_dbl * sizeof(double), hipMemcpyHostToDevice, stream));
+    // staged in pinned memory: the copy runs asynchronously; the staging is next written by the
+    // next step's upload, after that step's detection results (copied behind it) have arrived
+    HIP_CHECK(hipMemcpyAsync(d_self_up, pin_up.p, n_bytes, hipMemcpyHostToDevice, stream));
+    int* const d_self_int = reinterpret_cast<int*>(d_self_up + dbl_bytes);
+    const double* const d_self_dbl = reinterpret_cast<const double*>(d_self_up);
     const size_t oA = 0, oB = nsc, oSA = 2 * (size_t)nsc, oSB = 3 * (size_t)nsc, oL = 4 * (size_t)nsc,
                  oG = oL + L + 1, oEC = oG + L + 1, oES = oEC + 2 * (size_t)nsc, oSV = oES + 2 * (size_t)nsc,
                  oGS = oSV + nslots, oEP = oGS + G, oSEP = oEP + G + 1, oSEN = oSEP + nslots + 1;
