@@ -325,7 +325,6 @@ void launch_self_detect(const DevSelfDetect& D, const int* ccnt, unsigned long l
 void launch_self_detect_filtered(const DevSelfDetect& D, const int* ccnt, unsigned long long* seed_bits,
                                  hipStream_t st);
 void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st);
-void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st);
 void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
                        const StepArgs& a, int iteration, hipStream_t st);
 void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st);
@@ -343,7 +342,9 @@ void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, c
 void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                        int iteration, double* rhs, hipStream_t st);
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st);
-void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st);
+// End of step: the last iteration's self-contact friction residual, and the slot map cleared.
+void launch_self_end(const DevSelf& S, int* vslot, const double* v, int n, const StepArgs& a, int iteration,
+                     hipStream_t st);
 void launch_fill_u64(unsigned long long* p, size_t count, hipStream_t st);
 void launch_step_begin(const DevModel& M, const double* qn, const double* vstate, double* v0, double* cvec,
                        const StepArgs& a, hipStream_t st);

@@ -828,7 +828,9 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused(DevSelf S, int
     if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
 }
 
-__global__ void k_self_ac_final(DevSelf S, const double* __restrict__ v, int n, StepArgs a, int iteration) {
+// End of step: the friction residual of the last iteration's self contacts, and the slot map off
+// (one launch; neither part reads what the other writes).
+__global__ void k_self_end(DevSelf S, int* vslot, const double* __restrict__ v, int n, StepArgs a, int iteration) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     double acs = 0.;
     if (s < S.nsc) {
@@ -838,9 +840,10 @@ __global__ void k_self_ac_final(DevSelf S, const double* __restrict__ v, int n, 
         acs = ac_norm3(P3{pdot(bn, u), pdot(bt, u), pdot(bb, u)}, &S.r[3 * s], S.mu[s]);
     }
     ac_block(acs, acs, &a.ac_sum[iteration], &a.ac_max[iteration]);
+    if (s < S.nslots) vslot[S.slot_vertex[s]] = -1;
 }
 
-// Start of step: slot map on, buffers zeroed. End of step: slot map off.
+// Start of step: slot map on, buffers zeroed (the end of step is k_self_end).
 // computeBasis (SimulationFrictionEstimation.cpp:89-105): n, t = n x e_x (n x e_y when that is
 // shorter than 1e-15), b = n x t, t and b normalised; the operations of the host's and the
 // oracle's, in their order (IEEE square root and division).
@@ -880,10 +883,6 @@ __global__ void k_self_prepare(DevSelf S, int* vslot) {
         S.r[3 * s] = 0.; S.r[3 * s + 1] = 0.; S.r[3 * s + 2] = 0.;
         S.active[s] = 0;
     }
-}
-__global__ void k_self_release(DevSelf S, int* vslot) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S.nslots) vslot[S.slot_vertex[s]] = -1;
 }
 
 inline unsigned nblk(long long c) { return (unsigned)((c + kB - 1) / kB); }
@@ -1109,9 +1108,6 @@ void launch_self_prepare(const DevSelf& S, int* vslot, hipStream_t st) {
     int c = S.nslots > S.nsc ? S.nslots : S.nsc;
     if (c > 0) hipLaunchKernelGGL(k_self_prepare, dim3(nblk(c)), dim3(kB), 0, st, S, vslot);
 }
-void launch_self_release(const DevSelf& S, int* vslot, hipStream_t st) {
-    if (S.nslots > 0) hipLaunchKernelGGL(k_self_release, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, vslot);
-}
 void launch_self_level(const DevSelf& S, int l, int count, const double* v, int n, const double* mass,
                        const StepArgs& a, int iteration, hipStream_t st) {
     if (count > 0) launch_k(k_self_level, dim3(nblk(count)), dim3(kB), 0, st, S, l, v, n, mass, a, iteration);
@@ -1130,8 +1126,10 @@ void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, c
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st) {
     if (S.nslots > 0) launch_k(k_self_finalize, dim3(nblk(S.nslots)), dim3(kB), 0, st, S, rhs, n);
 }
-void launch_self_ac_final(const DevSelf& S, const double* v, int n, const StepArgs& a, int iteration, hipStream_t st) {
-    if (S.nsc > 0) hipLaunchKernelGGL(k_self_ac_final, dim3(nblk(S.nsc)), dim3(kB), 0, st, S, v, n, a, iteration);
+void launch_self_end(const DevSelf& S, int* vslot, const double* v, int n, const StepArgs& a, int iteration,
+                     hipStream_t st) {
+    const int c = S.nslots > S.nsc ? S.nslots : S.nsc;
+    if (c > 0) hipLaunchKernelGGL(k_self_end, dim3(nblk(c)), dim3(kB), 0, st, S, vslot, v, n, a, iteration);
 }
 
 }  // namespace pdfc

@@ -2142,8 +2142,7 @@ void pdfc_sim::step_epilogue(StepPhase& ph, int K, int base, int to_tol, bool fe
     last_records = K;
     pdfc::launch_finish(dm, qn, V[K & 1], vstate, ccnt, cstart, dc, dob, a, K - 1, stream);
     if (nsc > 0) {
-        pdfc::launch_self_ac_final(ds, V[K & 1], n, a, K - 1, stream);
-        pdfc::launch_self_release(ds, vslot, stream);
+        pdfc::launch_self_end(ds, vslot, V[K & 1], n, a, K - 1, stream);
     }
     if (check && to_tol < 0) {
         HIP_CHECK(hipMemcpyAsync(host_ac + (K - 1), ac_sum + (K - 1), sizeof(double), hipMemcpyDeviceToHost, stream));
