@@ -339,8 +339,10 @@ constexpr int kSelfLdsMaxSlots = 2 * kSelfLdsMaxContacts;
 constexpr int kSelfLdsMaxLevels = 255;  // the levels' contact and group offsets are held in LDS too
 void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                            int iteration, double* rhs, hipStream_t st);
-void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
-                       int iteration, double* rhs, hipStream_t st);
+// Levels l0 .. nlev - 1 in one workgroup through global memory, then (finish) the right-hand side.
+void launch_self_fused(const DevSelf& S, int l0, int nlev, const double* v, int n, const double* mass,
+                       const StepArgs& a, int iteration, double* rhs, bool finish, hipStream_t st);
+constexpr int kSelfTailContacts = kSelfFusedThreads;  // a level of at most this many contacts is "small"
 void launch_self_finalize(const DevSelf& S, double* rhs, int n, hipStream_t st);
 // End of step: the last iteration's self-contact friction residual, and the slot map cleared.
 void launch_self_end(const DevSelf& S, int* vslot, const double* v, int n, const StepArgs& a, int iteration,

@@ -807,15 +807,19 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused_lds(DevSelf S,
     }
 }
 
-// All levels of one iteration in one workgroup (few self contacts): each level's updates,
-// a barrier, its applies, a barrier; then the right-hand side. The barriers order the
-// workgroup's global-memory writes and reads as the kernel boundaries of the per-level
-// launches do; the arithmetic per contact, group and vertex is theirs.
-__global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused(DevSelf S, int nlev, const double* __restrict__ v,
-                                                                 int n, const double* __restrict__ mass, StepArgs a,
-                                                                 int iteration, double* __restrict__ rhs) {
+// Levels l0 .. nlev - 1 of one iteration in one workgroup (few self contacts, or the tail of
+// small levels after per-level launches of the large ones): each level's updates, a barrier,
+// its applies, a barrier; then, with finish, the right-hand side (without, k_self_finalize
+// follows over all CUs: past a few thousand involved vertices one workgroup's rounds cost more).
+// The barriers order the workgroup's global-memory writes and reads as the kernel boundaries of
+// the per-level launches do; the arithmetic per contact, group and vertex is theirs.
+__global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused(DevSelf S, int l0, int nlev,
+                                                                 const double* __restrict__ v, int n,
+                                                                 const double* __restrict__ mass, StepArgs a,
+                                                                 int iteration, double* __restrict__ rhs,
+                                                                 int finish) {
     double acs = 0., acm = 0.;
-    for (int l = 0; l < nlev; ++l) {
+    for (int l = l0; l < nlev; ++l) {
         const int s1 = S.lvl_ptr[l + 1];
         for (int s = S.lvl_ptr[l] + (int)threadIdx.x; s < s1; s += kSelfFusedThreads)
             self_contact_update(S, s, v, n, mass, a, iteration, acs, acm);
@@ -824,7 +828,8 @@ __global__ void __launch_bounds__(kSelfFusedThreads) k_self_fused(DevSelf S, int
         for (int g = S.grp_ptr[l] + (int)threadIdx.x; g < g1; g += kSelfFusedThreads) self_group_apply(S, g);
         __syncthreads();
     }
-    for (int s = threadIdx.x; s < S.nslots; s += kSelfFusedThreads) self_slot_finalize(S, s, rhs, n);
+    if (finish)
+        for (int s = threadIdx.x; s < S.nslots; s += kSelfFusedThreads) self_slot_finalize(S, s, rhs, n);
     if (a.record && iteration > 0) ac_block(acs, acm, &a.ac_sum[iteration - 1], &a.ac_max[iteration - 1]);
 }
 
@@ -1115,9 +1120,10 @@ void launch_self_level(const DevSelf& S, int l, int count, const double* v, int 
 void launch_self_apply(const DevSelf& S, int l, int groups, hipStream_t st) {
     if (groups > 0) launch_k(k_self_apply, dim3(nblk(groups)), dim3(kB), 0, st, S, l);
 }
-void launch_self_fused(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
-                       int iteration, double* rhs, hipStream_t st) {
-    launch_k(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, nlev, v, n, mass, a, iteration, rhs);
+void launch_self_fused(const DevSelf& S, int l0, int nlev, const double* v, int n, const double* mass,
+                       const StepArgs& a, int iteration, double* rhs, bool finish, hipStream_t st) {
+    launch_k(k_self_fused, dim3(1), dim3(kSelfFusedThreads), 0, st, S, l0, nlev, v, n, mass, a, iteration, rhs,
+             finish ? 1 : 0);
 }
 void launch_self_fused_lds(const DevSelf& S, int nlev, const double* v, int n, const double* mass, const StepArgs& a,
                            int iteration, double* rhs, hipStream_t st) {

@@ -311,7 +311,10 @@ struct pdfc_sim {
     std::vector<int> lvl_counts, grp_counts;       // per level
     bool keep_full_candidates = false;
     // self-contact levels of an iteration: 2 one workgroup with LDS (the default, when they fit),
-    // 1 one workgroup through global memory, 0 two launches per level (PDFC_SELF_PATH overrides)
+    // 1 one workgroup through global memory (up to kSelfFusedMaxContacts), and beyond that two
+    // launches per large level and one workgroup for the tail of small levels; 0 two launches per
+    // level throughout; 3 the first level by launches and the rest in one workgroup whatever the
+    // counts (PDFC_SELF_PATH overrides; 0 and 3 exist to check the others against)
     int self_path = 2;
     size_t filt_guess = 4096;
     double self_host_us = 0.;   // host time in detection round trips + pairing, accumulated per pdfc_step
@@ -557,7 +560,7 @@ void pdfc_sim::build(const pdfc_scene* scenes, size_t count, const pdfc_params& 
     // dynamic LDS (right-hand side of one supernode) + the 24 KiB partial-sum buffer
     if ((size_t)3 * fac.max_front * sizeof(double) + 16 * 3 * 64 * sizeof(double) > 64 * 1024)
         throw std::invalid_argument("a separator exceeds the substitution kernels' LDS budget");
-    if (const char* e = std::getenv("PDFC_SELF_PATH")) self_path = std::max(0, std::min(2, std::atoi(e)));
+    if (const char* e = std::getenv("PDFC_SELF_PATH")) self_path = std::max(0, std::min(3, std::atoi(e)));
     pdfc::plan_top(fac, cross ? fac.top_bytes : top_mb * 1048576., fp32 ? 4 : 8, top);
 
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -2031,16 +2034,26 @@ void pdfc_sim::launch_iteration(int it, pdfc::StepArgs& a, pdfc::LocalSelf& ls) 
     timed(0, [&] { pdfc::launch_local(dm, qn, vcur, cvec, vnext, ccnt, cstart, dc, a, it, ls, stream); });
     if (nsc > 0) {
         timed(4, [&] {
-            if (self_path >= 2 && nsc <= pdfc::kSelfLdsMaxContacts && ds.nslots <= pdfc::kSelfLdsMaxSlots &&
+            if (self_path == 2 && nsc <= pdfc::kSelfLdsMaxContacts && ds.nslots <= pdfc::kSelfLdsMaxSlots &&
                 n_self_levels <= pdfc::kSelfLdsMaxLevels) {
                 pdfc::launch_self_fused_lds(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
-            } else if (self_path >= 1 && nsc <= pdfc::kSelfFusedMaxContacts) {
-                pdfc::launch_self_fused(ds, n_self_levels, vcur, n, dm.mass, a, it, vnext, stream);
+            } else if ((self_path == 1 || self_path == 2) && nsc <= pdfc::kSelfFusedMaxContacts) {
+                pdfc::launch_self_fused(ds, 0, n_self_levels, vcur, n, dm.mass, a, it, vnext, true, stream);
             } else {
-                for (int l = 0; l < n_self_levels; ++l) {
+                // large levels: two launches each, over all CUs; the tail of levels that are all small
+                // (a long tail of a few contacts each is common once the contact graph has chains):
+                // one workgroup, one launch; the right-hand side of the involved vertices over all CUs
+                int l0 = n_self_levels;
+                if (self_path == 3)
+                    l0 = std::min(1, n_self_levels);
+                else if (self_path != 0)
+                    while (l0 > 0 && lvl_counts[l0 - 1] <= pdfc::kSelfTailContacts) --l0;
+                for (int l = 0; l < l0; ++l) {
                     pdfc::launch_self_level(ds, l, lvl_counts[l], vcur, n, dm.mass, a, it, stream);
                     pdfc::launch_self_apply(ds, l, grp_counts[l], stream);
                 }
+                if (l0 < n_self_levels)
+                    pdfc::launch_self_fused(ds, l0, n_self_levels, vcur, n, dm.mass, a, it, vnext, false, stream);
                 pdfc::launch_self_finalize(ds, vnext, n, stream);
             }
         });

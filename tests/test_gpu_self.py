@@ -188,9 +188,10 @@ def test_double_obstacle_contacts_with_self_contacts(gpu):
 @pytest.mark.parametrize("scene", ["stacked_small", "drape_40k_step_118"])
 def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
     """The self-contact levels of an iteration run as two launches per level (PDFC_SELF_PATH=0),
-    in one workgroup through global memory (1) or in one workgroup with the level-to-level state in
-    LDS (2, the default when it fits). Same arithmetic in the same order: bit-identical states and
-    reactions."""
+    in one workgroup through global memory (1), in one workgroup with the level-to-level state in
+    LDS (2, the default when it fits), or the first level by launches and the rest in one workgroup
+    (3, the split used past the one-workgroup limit). Same arithmetic in the same order:
+    bit-identical states and reactions."""
     if scene == "stacked_small":
         spec, first, steps = stacked_small(16), 30, 20
     else:
@@ -201,7 +202,7 @@ def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
     q0, v0 = base.state()
     monkeypatch.setenv("PDFC_TUNE", base.tuning())
     out, selfs = [], 0
-    for path in ("0", "1", "2"):
+    for path in ("0", "1", "2", "3"):
         monkeypatch.setenv("PDFC_SELF_PATH", path)
         sim = gpu.Simulation(spec)
         sim.set_state(q0, v0)
@@ -214,6 +215,6 @@ def test_self_level_paths_bit_identical(gpu, monkeypatch, scene):
         sim.close()
     print(scene, "self contacts (max over steps)", selfs)
     assert selfs > 0
-    for path, (q, v, r, rs) in zip(("1", "2"), out[1:]):
+    for path, (q, v, r, rs) in zip(("1", "2", "3"), out[1:]):
         assert np.array_equal(q, out[0][0]) and np.array_equal(v, out[0][1]), path
         assert np.array_equal(r, out[0][2]) and np.array_equal(rs, out[0][3]), path
